@@ -1,0 +1,147 @@
+"""ctypes binding of libarlmi.so (the HIP implementation).
+
+The library is loaded from this package directory.  There is no fallback: if the
+shared object is missing, or no GPU is present when a compute entry point is called,
+the call raises.
+"""
+
+import ctypes
+import os
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libarlmi.so")
+
+ARLMI_OK = 0
+ERRORS = {
+    -1: "ARLMI_E_ARG",
+    -2: "ARLMI_E_UV_OFF_GRID",
+    -3: "ARLMI_E_W_RANGE",
+    -4: "ARLMI_E_FREQ_MAP",
+    -5: "ARLMI_E_ODD_SUPPORT",
+    -6: "ARLMI_E_SUBSAMPLE",
+    -7: "ARLMI_E_HIP",
+    -8: "ARLMI_E_UNSUPPORTED",
+}
+MODE_2D = 0
+MODE_WSTACK = 1
+FLAG_DROP_OFF_GRID = 1
+KERNEL_SEPARABLE = 0
+KERNEL_TABLE = 1
+
+# Names exported by the C ABI (include/arlmi.h).
+EXPORTS = (
+    "arlmi_version", "arlmi_last_error", "arlmi_wmax", "arlmi_plan_create", "arlmi_plan_update",
+    "arlmi_plan_stats_get", "arlmi_plan_plane_w", "arlmi_plan_destroy", "arlmi_convolution_mapping",
+    "arlmi_grid", "arlmi_degrid",
+    "arlmi_grids_to_image", "arlmi_image_to_grids", "arlmi_fft2d", "arlmi_w_screen", "arlmi_dft",
+    "arlmi_profile_enable", "arlmi_profile_collect", "arlmi_profile_reset", "arlmi_profile_report",
+)
+
+
+class ArlmiError(RuntimeError):
+    """A libarlmi call failed; ``code`` is the ARLMI_E_* value."""
+
+    def __init__(self, code, message):
+        super().__init__("%s: %s" % (ERRORS.get(code, str(code)), message))
+        self.code = code
+        self.name = ERRORS.get(code, str(code))
+
+
+class UVOffGridError(ArlmiError, AssertionError):
+    """ARL asserts when a visibility maps off the grid (gridding.py:43-46)."""
+
+
+class Geom(ctypes.Structure):
+    _fields_ = [
+        ("npixel", ctypes.c_int32), ("nchan", ctypes.c_int32), ("support", ctypes.c_int32),
+        ("oversampling", ctypes.c_int32), ("nz", ctypes.c_int32), ("nplanes", ctypes.c_int32),
+        ("mode", ctypes.c_int32), ("tile", ctypes.c_int32),
+        ("du", ctypes.c_double), ("dv", ctypes.c_double), ("ddu", ctypes.c_double), ("ddv", ctypes.c_double),
+        ("crpix_uv", ctypes.c_double), ("crpix_d", ctypes.c_double), ("zstep", ctypes.c_double),
+        ("crpix_z", ctypes.c_double), ("wstep_grid", ctypes.c_double), ("crpix_wg", ctypes.c_double),
+        ("freq0", ctypes.c_double), ("dfreq", ctypes.c_double), ("cellsize", ctypes.c_double),
+    ]
+
+
+class PlanStats(ctypes.Structure):
+    _fields_ = [
+        ("nrows", ctypes.c_int64), ("nentries", ctypes.c_int64), ("ngrid_items", ctypes.c_int64),
+        ("ndegrid_items", ctypes.c_int64), ("nmember", ctypes.c_int64), ("ndropped", ctypes.c_int64),
+        ("norphans", ctypes.c_int64), ("ntiles", ctypes.c_int32), ("tile", ctypes.c_int32),
+    ]
+
+
+class Kernel(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32), ("nz", ctypes.c_int32), ("oversampling", ctypes.c_int32),
+        ("support", ctypes.c_int32), ("d_table", ctypes.c_void_p),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Load libarlmi.so (once) and declare its signatures."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libarlmi.so not found at %s; build it with `python -m arlmi._build` "
+                          "(or __graft_entry__.build())" % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i32, i64, dbl = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+    pp = ctypes.POINTER(ctypes.c_void_p)
+    sig = {
+        "arlmi_version": (ctypes.c_int, []),
+        "arlmi_last_error": (ctypes.c_char_p, []),
+        "arlmi_wmax": (ctypes.c_int, [vp, i64, vp, ctypes.POINTER(dbl)]),
+        "arlmi_plan_create": (ctypes.c_int, [ctypes.POINTER(Geom), vp, vp, i64, ctypes.POINTER(dbl), dbl,
+                                             ctypes.c_uint32, vp, pp]),
+        "arlmi_plan_update": (ctypes.c_int, [vp, vp, vp, i64, ctypes.POINTER(dbl), dbl, vp]),
+        "arlmi_plan_stats_get": (ctypes.c_int, [vp, ctypes.POINTER(PlanStats)]),
+        "arlmi_plan_plane_w": (ctypes.c_int, [vp, ctypes.POINTER(dbl), i32]),
+        "arlmi_plan_destroy": (None, [vp]),
+        "arlmi_convolution_mapping": (ctypes.c_int, [ctypes.POINTER(Geom), vp, vp, i64, vp, vp]),
+        "arlmi_grid": (ctypes.c_int, [vp, vp, vp, ctypes.c_int, ctypes.POINTER(Kernel), vp, vp, ctypes.c_int, vp]),
+        "arlmi_degrid": (ctypes.c_int, [vp, vp, ctypes.POINTER(Kernel), vp, ctypes.c_int, vp]),
+        "arlmi_grids_to_image": (ctypes.c_int, [vp, vp, vp, vp, ctypes.c_int, vp, vp, ctypes.c_int, vp]),
+        "arlmi_image_to_grids": (ctypes.c_int, [vp, vp, vp, vp, ctypes.c_int, vp]),
+        "arlmi_fft2d": (ctypes.c_int, [vp, i32, i64, ctypes.c_int, ctypes.c_int, vp]),
+        "arlmi_w_screen": (ctypes.c_int, [i32, dbl, dbl, vp, ctypes.c_int, vp]),
+        "arlmi_dft": (ctypes.c_int, [vp, vp, i64, vp, i32, i32, vp, ctypes.c_int, vp]),
+        "arlmi_profile_enable": (ctypes.c_int, [ctypes.c_int]),
+        "arlmi_profile_collect": (ctypes.c_int, []),
+        "arlmi_profile_reset": (ctypes.c_int, []),
+        "arlmi_profile_report": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc):
+    """Raise ArlmiError (UVOffGridError for off-grid rows) when rc is not ARLMI_OK."""
+    if rc != ARLMI_OK:
+        msg = lib().arlmi_last_error().decode(errors="replace")
+        if rc == -2:
+            raise UVOffGridError(rc, msg)
+        raise ArlmiError(rc, msg)
+    return rc
+
+
+def profile_report():
+    """{kernel: (total_ms, launches)} accumulated since the last reset."""
+    L = lib()
+    L.arlmi_profile_collect()
+    need = L.arlmi_profile_report(None, 0)
+    buf = ctypes.create_string_buffer(need + 16)
+    L.arlmi_profile_report(buf, need + 16)
+    out = {}
+    for line in buf.value.decode().splitlines():
+        name, ms, cnt = line.split()
+        out[name] = (float(ms), int(cnt))
+    return out

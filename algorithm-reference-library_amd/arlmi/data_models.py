@@ -1,0 +1,306 @@
+"""ARL's data models for the Fourier hot path, holding their arrays as device tensors.
+
+Mirrors data_models/memory_data_models.py (Image 349-434, GridData 437-524,
+ConvolutionFunction 527-614, Visibility 745-937, BlockVisibility 940-1094) and the
+parts of their astropy WCS that the hot path reads.  The WCS is a small linear
+stand-in (``WCS``) with the same ``wcs.crpix / cdelt / crval / ctype`` arrays and the
+same values ARL stores; no celestial transformation is implemented (image centre and
+visibility phase centre always coincide here).
+"""
+
+import copy
+
+import numpy
+import torch
+
+__all__ = ["WCS", "PolarisationFrame", "Image", "GridData", "ConvolutionFunction", "Visibility",
+           "BlockVisibility", "Skycomponent", "device", "C_LIGHT"]
+
+C_LIGHT = 299792458.0
+D2R = numpy.pi / 180.0
+
+
+def device():
+    """The GPU the arrays live on (torch's current device)."""
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+class _WcsPrm:
+    def __init__(self, naxis):
+        self.naxis = naxis
+        self.crpix = numpy.zeros(naxis)
+        self.cdelt = numpy.ones(naxis)
+        self.crval = numpy.zeros(naxis)
+        self.ctype = [""] * naxis
+
+
+class WCS:
+    """Linear WCS with the attributes the hot path reads (astropy.wcs.WCS stand-in)."""
+
+    def __init__(self, naxis=4):
+        self.naxis = naxis
+        self.wcs = _WcsPrm(naxis)
+
+    def deepcopy(self):
+        return copy.deepcopy(self)
+
+    def world2pix0(self, axis, world):
+        """Origin-0 pixel of a linear axis: ((world - crval)/cdelt + crpix) - 1."""
+        w = self.wcs
+        return ((numpy.asarray(world) - w.crval[axis]) / w.cdelt[axis] + w.crpix[axis]) - 1.0
+
+    def pix2world0(self, axis, pix):
+        w = self.wcs
+        return w.cdelt[axis] * ((numpy.asarray(pix) + 1.0) - w.crpix[axis]) + w.crval[axis]
+
+
+class PolarisationFrame:
+    """Only Stokes I (npol = 1) is handled by the hot path (data_models/polarisation.py:66-113)."""
+
+    def __init__(self, name="stokesI"):
+        if name != "stokesI":
+            raise NotImplementedError("only stokesI is supported")
+        self.type = name
+        self.npol = 1
+
+    def __eq__(self, other):
+        return isinstance(other, PolarisationFrame) and other.type == self.type
+
+
+class Image:
+    """Image: data [nchan, npol, ny, nx] plus WCS (memory_data_models.py:349-434)."""
+
+    def __init__(self, data=None, wcs=None, polarisation_frame=None):
+        self.data = data
+        self.wcs = wcs
+        self.polarisation_frame = polarisation_frame or PolarisationFrame()
+
+    @property
+    def shape(self):
+        return tuple(self.data.shape)
+
+    @property
+    def nchan(self):
+        return self.data.shape[0]
+
+    @property
+    def npol(self):
+        return self.data.shape[1]
+
+    @property
+    def npixel(self):
+        return self.data.shape[3]
+
+    @property
+    def cellsize(self):
+        """abs(cdelt0) * pi / 180 as ARL reads it back (image/operations.py:389)."""
+        return abs(self.wcs.wcs.cdelt[0]) * numpy.pi / 180.0
+
+    def numpy(self):
+        d = self.data
+        return d.detach().cpu().numpy() if isinstance(d, torch.Tensor) else numpy.asarray(d)
+
+
+class GridData:
+    """uv grid [nchan, npol, nw, ny, nx] complex (memory_data_models.py:437-524)."""
+
+    def __init__(self, data=None, grid_wcs=None, projection_wcs=None, polarisation_frame=None):
+        self.data = data
+        self.grid_wcs = grid_wcs
+        self.projection_wcs = projection_wcs
+        self.polarisation_frame = polarisation_frame or PolarisationFrame()
+
+    @property
+    def shape(self):
+        return tuple(self.data.shape)
+
+
+class ConvolutionFunction:
+    """Oversampled kernel [nchan, npol, nz, os, os, S, S] (memory_data_models.py:527-614).
+
+    ``separable`` optionally holds the real 1-D table k[os][S] with
+    data[..., yo, xo, a, b] == k[yo, a] * k[xo, b] (the PSWF case); the gridding kernels
+    then read 2*S numbers per visibility instead of S*S.
+    """
+
+    def __init__(self, data=None, grid_wcs=None, projection_wcs=None, polarisation_frame=None, separable=None):
+        self.data = data
+        self.grid_wcs = grid_wcs
+        self.projection_wcs = projection_wcs
+        self.polarisation_frame = polarisation_frame or PolarisationFrame()
+        self.separable = separable
+        self._cache = {}
+
+    @property
+    def shape(self):
+        return tuple(self.data.shape)
+
+    @property
+    def nz(self):
+        return self.data.shape[2]
+
+    @property
+    def oversampling(self):
+        return self.data.shape[3]
+
+    @property
+    def support(self):
+        return self.data.shape[5]
+
+    def table(self, precision):
+        """Device table in the kernels' layout for the given precision (cached)."""
+        key = ("tab", precision)
+        if key not in self._cache:
+            cdt = torch.complex64 if precision == 32 else torch.complex128
+            self._cache[key] = torch.as_tensor(self.data)[0, 0].to(device=device(), dtype=cdt).contiguous()
+        return self._cache[key]
+
+    def separable_table(self, precision):
+        key = ("sep", precision)
+        if self.separable is None:
+            return None
+        if key not in self._cache:
+            rdt = torch.float32 if precision == 32 else torch.float64
+            self._cache[key] = torch.as_tensor(self.separable, dtype=torch.float64).to(
+                device=device(), dtype=rdt).contiguous()
+        return self._cache[key]
+
+
+class Visibility:
+    """Row-form visibilities (memory_data_models.py:745-937).
+
+    One row per (time, baseline, channel); uvw in wavelengths.  Columns are held in
+    ``data`` as device tensors: uvw f64 [V,3], vis complex [V,npol], weight and
+    imaging_weight real [V,npol], frequency f64 [V], time, channel_bandwidth,
+    integration_time f64 [V], antenna1/antenna2 int64 [V].  complex64 vis selects the
+    single precision path, complex128 the double precision one.
+    """
+
+    COLUMNS = ("uvw", "vis", "weight", "imaging_weight", "frequency", "time", "channel_bandwidth",
+               "integration_time", "antenna1", "antenna2")
+
+    def __init__(self, data=None, polarisation_frame=None, phasecentre=None):
+        self.data = data or {}
+        self.polarisation_frame = polarisation_frame or PolarisationFrame()
+        self.phasecentre = phasecentre
+
+    @classmethod
+    def from_arrays(cls, uvw, vis=None, weight=None, imaging_weight=None, frequency=None, time=None,
+                    antenna1=None, antenna2=None, channel_bandwidth=None, integration_time=None,
+                    precision=32, dev=None):
+        dev = dev or device()
+        cdt = torch.complex64 if precision == 32 else torch.complex128
+        rdt = torch.float32 if precision == 32 else torch.float64
+        uvw = torch.as_tensor(numpy.asarray(uvw, dtype=numpy.float64)).to(dev)
+        n = uvw.shape[0]
+
+        def col(x, shape, dt, fill):
+            if x is None:
+                return torch.full(shape, fill, dtype=dt, device=dev)
+            t = torch.as_tensor(numpy.asarray(x)).to(device=dev, dtype=dt)
+            return t.reshape(shape).contiguous()
+
+        data = {
+            "uvw": uvw.contiguous(),
+            "vis": col(vis, (n, 1), cdt, 0.0),
+            "weight": col(weight, (n, 1), rdt, 1.0),
+            "imaging_weight": col(imaging_weight if imaging_weight is not None else weight, (n, 1), rdt, 1.0),
+            "frequency": col(frequency, (n,), torch.float64, 1e8),
+            "time": col(time, (n,), torch.float64, 0.0),
+            "channel_bandwidth": col(channel_bandwidth, (n,), torch.float64, 1e6),
+            "integration_time": col(integration_time, (n,), torch.float64, 1.0),
+            "antenna1": col(antenna1, (n,), torch.int64, 0),
+            "antenna2": col(antenna2, (n,), torch.int64, 0),
+        }
+        return cls(data)
+
+    @property
+    def nvis(self):
+        return self.data["uvw"].shape[0]
+
+    @property
+    def npol(self):
+        return self.data["vis"].shape[1]
+
+    @property
+    def uvw(self):
+        return self.data["uvw"]
+
+    @property
+    def u(self):
+        return self.data["uvw"][:, 0]
+
+    @property
+    def v(self):
+        return self.data["uvw"][:, 1]
+
+    @property
+    def w(self):
+        return self.data["uvw"][:, 2]
+
+    @property
+    def vis(self):
+        return self.data["vis"]
+
+    @property
+    def weight(self):
+        return self.data["weight"]
+
+    @property
+    def imaging_weight(self):
+        return self.data["imaging_weight"]
+
+    @property
+    def frequency(self):
+        return self.data["frequency"]
+
+    @property
+    def precision(self):
+        return 32 if self.data["vis"].dtype == torch.complex64 else 64
+
+    def copy(self, zero=False):
+        """copy_visibility (visibility/base.py:33-47)."""
+        new = Visibility({k: v.clone() for k, v in self.data.items()}, self.polarisation_frame, self.phasecentre)
+        if zero:
+            new.data["vis"].zero_()
+        return new
+
+    def with_vis(self, vis):
+        """A shallow copy that shares every column but ``vis``."""
+        d = dict(self.data)
+        d["vis"] = vis
+        return Visibility(d, self.polarisation_frame, self.phasecentre)
+
+
+class BlockVisibility:
+    """Correlator-layout visibilities (memory_data_models.py:940-1094).
+
+    vis [ntimes, nants, nants, nchan, npol] complex, uvw [ntimes, nants, nants, 3] in
+    metres, frequency [nchan], channel_bandwidth [nchan], time [ntimes].
+    """
+
+    def __init__(self, data=None, frequency=None, channel_bandwidth=None, polarisation_frame=None,
+                 phasecentre=None):
+        self.data = data or {}
+        self.frequency = frequency
+        self.channel_bandwidth = channel_bandwidth
+        self.polarisation_frame = polarisation_frame or PolarisationFrame()
+        self.phasecentre = phasecentre
+
+    @property
+    def vis(self):
+        return self.data["vis"]
+
+    @property
+    def uvw(self):
+        return self.data["uvw"]
+
+
+class Skycomponent:
+    """Point component at direction cosines (l, m) with flux [nchan, npol]
+    (memory_data_models.py:617-689; the direction is given in (l, m) directly)."""
+
+    def __init__(self, l, m, flux):
+        self.l = float(l)
+        self.m = float(m)
+        self.flux = numpy.atleast_2d(numpy.asarray(flux, dtype=float))

@@ -1,0 +1,204 @@
+"""invert / predict with ARL's signatures (processing_components/imaging/base.py and
+processing_components/imaging/wstack_single.py), running on the GPU through libarlmi.
+"""
+
+import ctypes
+
+import numpy
+import torch
+
+from . import _lib
+from .data_models import Image, Visibility, Skycomponent, device
+from .kernels import create_pswf_convolutionfunction, create_image
+from .plan import Plan, ptr, stream_handle, grids_buffer
+from .gridding import grid_planes, degrid_planes
+
+__all__ = ["get_parameter", "normalize_sumwt", "shift_vis_to_image", "invert_2d", "predict_2d",
+           "invert_wstack_single", "predict_wstack_single", "predict_skycomponent_visibility",
+           "create_image_from_visibility", "Imager"]
+
+
+def get_parameter(kwargs, key, default=None):
+    """data_models/parameters.py:79-96."""
+    if kwargs is None:
+        return default
+    return kwargs.get(key, default)
+
+
+def normalize_sumwt(im, sumwt):
+    """img[c, p] /= sumwt[c, p] where sumwt > 0, else 0 (imaging/base.py:84-101)."""
+    sw = torch.as_tensor(numpy.asarray(sumwt, dtype=numpy.float64), device=im.data.device)
+    pos = sw > 0.0
+    div = torch.where(pos, sw, torch.ones_like(sw))[:, :, None, None]
+    data = torch.where(pos[:, :, None, None], im.data.to(torch.float64) / div, torch.zeros((), dtype=torch.float64,
+                                                                                           device=im.data.device))
+    im.data = data.to(im.data.dtype)
+    return im
+
+
+def shift_vis_to_image(vis, im, tangent=True, inverse=False):
+    """imaging/base.py:50-81.  Images here are always centred on the visibility phase centre,
+    so the phase rotation ARL would apply is the identity."""
+    return vis
+
+
+def _gcfcf(im, gcfcf, kwargs):
+    if gcfcf is None:
+        return create_pswf_convolutionfunction(im, support=get_parameter(kwargs, "support", 6),
+                                               oversampling=get_parameter(kwargs, "oversampling", 128))
+    return gcfcf
+
+
+def _real_dtype(precision):
+    return torch.float32 if precision == 32 else torch.float64
+
+
+class Imager:
+    """invert and predict for one visibility set, image geometry and kernel.
+
+    The plan (convolution mapping, w-plane membership, tile lists) is built once on the GPU
+    and reused by invert and predict; ``update(vis)`` rebuilds it for new uvw.
+      mode "2d":     planes are the grid w planes (one by default), as invert_2d / predict_2d;
+      mode "wstack": vis_slices w slabs with ARL's slab rule, screens and slab-mean w shift,
+                     as the w-stack list workflows (imaging_serial.py:41-216).
+    """
+
+    def __init__(self, vis, im, gcfcf=None, mode="2d", vis_slices=1, slab_boxes=None, tile=0,
+                 drop_off_grid=False, **kwargs):
+        self.gcf, self.cf = _gcfcf(im, gcfcf, kwargs)
+        self.im = im
+        self.precision = vis.precision
+        if mode == "2d":
+            self.plan = Plan(vis, im, self.cf, _lib.MODE_2D, 1, tile=tile, drop_off_grid=drop_off_grid)
+        elif mode == "wstack":
+            self.plan = Plan(vis, im, self.cf, _lib.MODE_WSTACK, vis_slices, tile=tile, drop_off_grid=drop_off_grid,
+                             slab_boxes=slab_boxes)
+        else:
+            raise ValueError("mode must be '2d' or 'wstack'")
+        self.mode = mode
+        self.dev = device()
+        self.grids = grids_buffer(self.plan, self.precision, self.dev)
+        self.sumwt_planes = torch.empty((self.plan.nplanes, self.plan.nchan), dtype=torch.float64, device=self.dev)
+        self.gcf_dev = self.gcf.data.to(torch.float64).contiguous()
+
+    def update(self, vis, slab_boxes=None):
+        self.plan.update(vis, slab_boxes)
+
+    def invert(self, vis, dopsf=False, normalize=True, imaginary=False, out=None):
+        """Dirty image (or PSF): grid, shifted inverse FFT, screens, gcf and sum-of-weights."""
+        if vis.precision != self.precision:
+            raise TypeError("visibility precision changed")
+        grid_planes(self.plan, vis, self.cf, self.grids, self.sumwt_planes, dopsf=dopsf)
+        n, nchan = self.plan.npixel, self.plan.nchan
+        rdt = _real_dtype(self.precision)
+        re = out if out is not None else torch.empty((nchan, 1, n, n), dtype=rdt, device=self.dev)
+        imo = torch.empty_like(re) if imaginary else None
+        _lib.check(_lib.lib().arlmi_grids_to_image(self.plan.handle, ptr(self.grids), ptr(self.sumwt_planes),
+                                                   ptr(self.gcf_dev), int(bool(normalize)), ptr(re), ptr(imo),
+                                                   self.precision, stream_handle()))
+        return re, imo
+
+    def sumwt(self):
+        """[nchan, 1] float64 (host) sum of weights of the last invert."""
+        return self.sumwt_planes.sum(dim=0).cpu().numpy()[:, None]
+
+    def predict(self, vis, model_data, out=None):
+        """Model visibilities: gcf * conj(screen) * model, shifted FFT, degridding."""
+        rdt = _real_dtype(self.precision)
+        model = model_data.to(device=self.dev, dtype=rdt).contiguous()
+        _lib.check(_lib.lib().arlmi_image_to_grids(self.plan.handle, ptr(model), ptr(self.gcf_dev), ptr(self.grids),
+                                                   self.precision, stream_handle()))
+        vis_out = out if out is not None else torch.empty_like(vis.vis)
+        degrid_planes(self.plan, self.grids, self.cf, vis_out, self.precision)
+        return vis_out
+
+
+def invert_2d(vis, im, dopsf=False, normalize=True, gcfcf=None, **kwargs):
+    """Dirty image by convolutional gridding (imaging/base.py:141-189).
+
+    Returns (Image, sumwt[nchan, npol]), or (re, sumwt, im) with imaginary=True.
+    """
+    imager = Imager(vis, im, gcfcf=gcfcf, mode="2d", **kwargs)
+    imaginary = get_parameter(kwargs, "imaginary", False)
+    re, imo = imager.invert(vis, dopsf=dopsf, normalize=normalize, imaginary=imaginary)
+    sumwt = imager.sumwt()
+    out_re = Image(re, im.wcs.deepcopy(), im.polarisation_frame)
+    if imaginary:
+        return out_re, sumwt, Image(imo, im.wcs.deepcopy(), im.polarisation_frame)
+    return out_re, sumwt
+
+
+def predict_2d(vis, model, gcfcf=None, **kwargs):
+    """Model visibilities by convolutional degridding (imaging/base.py:104-138); returns a new
+    Visibility."""
+    if model is None:
+        return vis
+    imager = Imager(vis, model, gcfcf=gcfcf, mode="2d", **kwargs)
+    return vis.with_vis(imager.predict(vis, model.data))
+
+
+def invert_wstack_single(vis, im, dopsf, normalize=True, remove=True, gcfcf=None, **kwargs):
+    """One w slab: all rows, w shifted by their mean, screen applied (wstack_single.py:67-96)."""
+    if not remove:
+        raise NotImplementedError("remove=False is not supported")
+    imager = Imager(vis, im, gcfcf=gcfcf, mode="wstack", vis_slices=1,
+                    slab_boxes=(numpy.zeros(1), float("inf")), **kwargs)
+    re, _ = imager.invert(vis, dopsf=dopsf, normalize=normalize)
+    return Image(re, im.wcs.deepcopy(), im.polarisation_frame), imager.sumwt()
+
+
+def predict_wstack_single(vis, model, remove=True, gcfcf=None, **kwargs):
+    """One w slab predict: P(conj(b) * I) with w shifted by the mean (wstack_single.py:26-64)."""
+    if not remove:
+        raise NotImplementedError("remove=False is not supported")
+    imager = Imager(vis, model, gcfcf=gcfcf, mode="wstack", vis_slices=1,
+                    slab_boxes=(numpy.zeros(1), float("inf")), **kwargs)
+    return vis.with_vis(imager.predict(vis, model.data))
+
+
+def predict_skycomponent_visibility(vis, sc, chan_map=None):
+    """Add the direct-DFT visibilities of point components (imaging/base.py:192-220), on the GPU.
+
+    sc: Skycomponent or list of them, directions given as (l, m); flux [nchan_img, npol].
+    chan_map: optional int32 device tensor, the image channel of each row.
+    """
+    if sc is None:
+        return vis
+    if isinstance(sc, Skycomponent):
+        sc = [sc]
+    if not sc:
+        return vis
+    nchan = max(c.flux.shape[0] for c in sc)
+    lmf = numpy.zeros((len(sc), 2 + nchan))
+    for i, c in enumerate(sc):
+        lmf[i, 0], lmf[i, 1] = c.l, c.m
+        f = c.flux[:, 0]
+        lmf[i, 2:] = f if len(f) == nchan else f[0]
+    d_lmf = torch.as_tensor(lmf).to(device())
+    out = vis.vis.clone()
+    uvw = vis.data["uvw"]
+    chan = None if chan_map is None else chan_map.to(device=device(), dtype=torch.int32).contiguous()
+    _lib.check(_lib.lib().arlmi_dft(ptr(uvw), ptr(chan), uvw.shape[0], ptr(d_lmf), len(sc), nchan, ptr(out),
+                                    vis.precision, stream_handle()))
+    vis.data["vis"] = out
+    return vis
+
+
+def create_image_from_visibility(vis, **kwargs):
+    """Empty image for a visibility set (imaging/base.py:247-339): default cell is half the
+    critical cell 1/(2 max|u|), capped at the critical cell; MFS unless nchan is given."""
+    npixel = get_parameter(kwargs, "npixel", 512)
+    uvmax = float(torch.max(torch.abs(vis.data["uvw"][:, 0:1])).item())
+    critical = 1.0 / (uvmax * 2.0)
+    cellsize = get_parameter(kwargs, "cellsize", 0.5 * critical)
+    if get_parameter(kwargs, "override_cellsize", True) and cellsize > critical:
+        cellsize = critical
+    freq = vis.data["frequency"]
+    ufreq = numpy.unique(freq.cpu().numpy())
+    frequency = get_parameter(kwargs, "frequency", ufreq)
+    nchan = get_parameter(kwargs, "nchan", len(ufreq))
+    bw = vis.data["channel_bandwidth"][0].item() if "channel_bandwidth" in vis.data else 1e6
+    channel_bandwidth = get_parameter(kwargs, "channel_bandwidth", 0.99999999999 * bw)
+    dtype = torch.float32 if vis.precision == 32 else torch.float64
+    return create_image(npixel=npixel, cellsize=cellsize, frequency=numpy.atleast_1d(frequency),
+                        channel_bandwidth=numpy.atleast_1d(channel_bandwidth), nchan=nchan, dtype=dtype)

@@ -1,0 +1,204 @@
+// Internal declarations shared by the libarlmi translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstdarg>
+#include <cmath>
+
+#include "arlmi.h"
+
+namespace arlmi {
+
+void set_error(const char* fmt, ...);
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+#define ARLMI_HIP(call)                                                                     \
+  do {                                                                                      \
+    hipError_t e_ = (call);                                                                 \
+    if (e_ != hipSuccess) {                                                                 \
+      ::arlmi::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__,  \
+                         __LINE__);                                                         \
+      return ARLMI_E_HIP;                                                                   \
+    }                                                                                       \
+  } while (0)
+
+#define ARLMI_LAUNCH_CHECK()                                                                \
+  do {                                                                                      \
+    hipError_t e_ = hipGetLastError();                                                      \
+    if (e_ != hipSuccess) {                                                                 \
+      ::arlmi::set_error("kernel launch failed: %s (%s:%d)", hipGetErrorString(e_),        \
+                         __FILE__, __LINE__);                                               \
+      return ARLMI_E_HIP;                                                                   \
+    }                                                                                       \
+  } while (0)
+
+template <typename R> struct cplx_t;
+template <> struct cplx_t<float> { using type = float2; };
+template <> struct cplx_t<double> { using type = double2; };
+template <typename R> using cplx = typename cplx_t<R>::type;
+
+// Error bits raised by the mapping kernels (reduced to the first ARLMI_E_* code on the host).
+enum : uint32_t {
+  ERRB_UV_OFF_GRID = 1u,
+  ERRB_W_RANGE = 2u,
+  ERRB_FREQ_MAP = 4u,
+  ERRB_SUBSAMPLE = 8u,
+};
+
+// Packed entry word: lx | ly << 8 | xoff << 16 | yoff << 23 | START << 30 | DEGRID << 31.
+// lx, ly: kernel footprint start relative to the tile origin, plus S (so >= 1).
+constexpr uint32_t ENTRY_START = 1u << 30;   // the tile holding the footprint start (counts sumwt)
+constexpr uint32_t ENTRY_DEGRID = 1u << 31;  // the (row, plane) degridding writes back
+
+__host__ __device__ inline uint32_t pack_entry(int lx, int ly, int xoff, int yoff, bool start, bool degrid) {
+  return (uint32_t)lx | ((uint32_t)ly << 8) | ((uint32_t)xoff << 16) | ((uint32_t)yoff << 23) |
+         (start ? ENTRY_START : 0u) | (degrid ? ENTRY_DEGRID : 0u);
+}
+__host__ __device__ inline int entry_lx(uint32_t p) { return (int)(p & 0xffu); }
+__host__ __device__ inline int entry_ly(uint32_t p) { return (int)((p >> 8) & 0xffu); }
+__host__ __device__ inline int entry_xoff(uint32_t p) { return (int)((p >> 16) & 0x7fu); }
+__host__ __device__ inline int entry_yoff(uint32_t p) { return (int)((p >> 23) & 0x7fu); }
+
+// Result of ARL's convolution_mapping for one row (gridding.py:24-96).
+struct RowMap {
+  int pu, pv;      // nearest grid pixel (numpy.round: half to even)
+  int xoff, yoff;  // kernel sub-sample (numpy.floor; a -1 wraps to os-1 like a numpy index)
+  int chan;        // image channel
+  uint32_t err;    // ERRB_* bits
+};
+
+// Geometry in the form the device code reads it.
+struct DevGeom {
+  int n, nchan, support, os, nz, nplanes, mode, tile, ntx;
+  double du, dv, ddu, ddv, crpix_uv, crpix_d, zstep, crpix_z, wstep_grid, crpix_wg, freq0, dfreq;
+  double cellsize;
+};
+
+// World -> pixel on a linear WCS axis with origin 0, in wcslib's operation order:
+// ((world - crval) / cdelt + crpix) - 1 (crval is 0 on the UU/VV/DUU/DVV/WW axes).
+__device__ inline double world2pix0(double world, double cdelt, double crpix) {
+  return (world / cdelt + crpix) - 1.0;
+}
+
+__device__ inline RowMap map_row(const DevGeom& g, double u, double v, double f, bool have_freq) {
+  RowMap m;
+  m.err = 0;
+  const double x = world2pix0(u, g.du, g.crpix_uv);
+  const double y = world2pix0(v, g.dv, g.crpix_uv);
+  const double pud = rint(x);
+  const double pvd = rint(y);
+  const double half = (double)(g.support / 2);
+  // footprint (pu - S/2 .. pu + S/2 - 1) must lie inside [0, N): also implies 0 <= pu < N
+  if (!(pud - half >= 0.0 && pud + half <= (double)g.n && pvd - half >= 0.0 && pvd + half <= (double)g.n)) {
+    m.err |= ERRB_UV_OFF_GRID;
+    m.pu = m.pv = 0;
+  } else {
+    m.pu = (int)pud;
+    m.pv = (int)pvd;
+  }
+  // pixel -> world (origin 0): cdelt * ((p + 1) - crpix) + crval, crval = 0
+  const double wu_grid = g.du * ((pud + 1.0) - g.crpix_uv) + 0.0;
+  const double wv_grid = g.dv * ((pvd + 1.0) - g.crpix_uv) + 0.0;
+  const double wu_sub = u - wu_grid;
+  const double wv_sub = v - wv_grid;
+  int xo = (int)floor(world2pix0(wu_sub, g.ddu, g.crpix_d));
+  int yo = (int)floor(world2pix0(wv_sub, g.ddv, g.crpix_d));
+  if (xo < 0) xo += g.os;  // numpy negative index
+  if (yo < 0) yo += g.os;
+  if (xo < 0 || xo >= g.os || yo < 0 || yo >= g.os) m.err |= ERRB_SUBSAMPLE;
+  m.xoff = xo;
+  m.yoff = yo;
+  int c = 0;
+  if (have_freq) {
+    const double pf = rint(((f - g.freq0) / g.dfreq + 1.0) - 1.0);
+    if (!(pf >= 0.0 && pf < (double)g.nchan)) {
+      m.err |= ERRB_FREQ_MAP;
+    } else {
+      c = (int)pf;
+    }
+  }
+  m.chan = c;
+  return m;
+}
+
+// Kernel w plane of a (shifted) w value: round((w/zstep + crpix_z) - 1) (gridding.py:70-76).
+__device__ inline int map_wplane(double w, double step, double crpix, int nmax, uint32_t* err) {
+  const double p = rint(world2pix0(w, step, crpix));
+  if (!(p >= 0.0 && p < (double)nmax)) {
+    *err |= ERRB_W_RANGE;
+    return 0;
+  }
+  return (int)p;
+}
+
+int first_error_code(uint32_t bits);
+
+// Exclusive scan of n uint32 values (out has n+1 entries; out[n] = total).  tmp must hold
+// scan_tmp_words(n) words.
+size_t scan_tmp_words(int64_t n);
+int exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tmp, hipStream_t s);
+
+// Device twiddle table exp(-2 pi i k / n), k < n, cached per (n, precision).
+const void* twiddles(int n, int precision);
+
+// Opt-in kernel timing (profile.hip).
+bool profile_on();
+void* profile_begin(const char* name, hipStream_t s);
+void profile_end(void* token, hipStream_t s);
+
+// Launch a kernel, bracketed by timing events when profiling is on.
+#define ARLMI_LAUNCH(name, kernel, grid, block, shmem, stream, ...)          \
+  do {                                                                       \
+    void* tok_ = ::arlmi::profile_begin(name, stream);                       \
+    hipLaunchKernelGGL(kernel, grid, block, shmem, stream, __VA_ARGS__);     \
+    ::arlmi::profile_end(tok_, stream);                                      \
+  } while (0)
+
+}  // namespace arlmi
+
+// The plan object (opaque to C callers).
+struct arlmi_plan {
+  arlmi::DevGeom g;
+  uint32_t flags;
+  int64_t nrows;
+  int ntiles;  // planes * tiles per plane
+  int tpp;     // tiles per plane (nchan * ntx * ntx)
+  // device buffers (capacities in elements)
+  double* d_boxes = nullptr;             // [P]
+  double* d_plane_w = nullptr;           // [P] slab mean w
+  double* d_wsum = nullptr;              // [P]
+  unsigned long long* d_wcnt = nullptr;  // [P]
+  uint32_t* d_cnt = nullptr;             // [ntiles] entries per tile
+  uint32_t* d_cntd = nullptr;            // [ntiles] degrid entries per tile
+  uint32_t* d_off = nullptr;             // [ntiles] first entry of a tile
+  uint32_t* d_nch = nullptr;             // [ntiles] grid chunks per tile
+  uint32_t* d_nchd = nullptr;            // [ntiles] degrid chunks per tile
+  uint32_t* d_nslot = nullptr;           // [ntiles] partial-tile slots per tile
+  uint32_t* d_choff = nullptr;           // [ntiles+1]
+  uint32_t* d_choffd = nullptr;          // [ntiles+1]
+  uint32_t* d_slotoff = nullptr;         // [ntiles+1]
+  uint32_t* d_misc = nullptr;            // [8]: err bits, -, dropped, nmember, nmulti
+  uint32_t* d_bounds = nullptr;          // [P+2] plane segment starts in the record array
+  uint32_t* d_hbase = nullptr;           // [P+1] plane bases in h2
+  uint32_t* d_nbp = nullptr;             // [P] pass-2 blocks per plane
+  uint32_t *d_h1 = nullptr, *d_s1 = nullptr;
+  uint32_t *d_h2 = nullptr, *d_s2 = nullptr;
+  uint32_t* d_scan_tmp = nullptr;
+  void* d_recs = nullptr;                // 16-byte membership records
+  void* d_blk2 = nullptr;                // pass-2 block descriptors
+  uint2* d_entries = nullptr;            // (row, packed)
+  uint16_t* d_entry_z = nullptr;         // kernel w plane per entry (nz > 1)
+  int4* d_gitems = nullptr;              // {tile, start, count, partial slot + 1 or 0}
+  int4* d_ditems = nullptr;              // {tile, start, count, 0}
+  int4* d_multi = nullptr;               // {tile, first slot, slots, 0} for split tiles
+  void* d_partial = nullptr;             // partial tiles of split tiles (grid precision)
+  int64_t cap_h1 = 0, cap_s1 = 0, cap_h2 = 0, cap_s2 = 0, cap_scan = 0, cap_recs = 0, cap_blk2 = 0, cap_entries = 0, cap_z = 0,
+          cap_gitems = 0, cap_ditems = 0, cap_partial_bytes = 0;
+  // host copies
+  int64_t nentries = 0, ngitems = 0, nditems = 0, nmulti = 0, nslots = 0, norphans = 0, ndropped = 0, nmember = 0;
+  int64_t orphan_start = 0;  // orphan rows are records [orphan_start, orphan_start + norphans)
+  int chunk = 4096;
+};

@@ -1,0 +1,774 @@
+// Shifted 2-D FFTs and the image-plane work fused into them.
+//
+// ARL's fft / ifft (processing_library/fourier_transforms/fft_support.py:21-89) are
+// fftshift(fft2(ifftshift(a))).  For even N this equals (-1)^(k+l) * DFT2((-1)^(x+y) a), so
+// the shifts become a sign applied while loading the first pass and while storing the
+// second.  Each pass is an LDS-staged Stockham transform (radix 8, remainder radix 4 or 2 in
+// the last stage): a row pass works on contiguous rows, a column pass stages NB adjacent
+// columns (NB * 8 or 16 bytes per grid row) so that its loads and stores stay coalesced.
+//
+// Fused around the passes:
+//   image -> grids (predict): model * gcf * conj(b_p) for every w plane p, in the row pass
+//     (gridding.py:292-302 with wstack_single.py:50-59, b_p = exp(-2 pi i w_p (1 - n)));
+//   grids -> image (invert): Re(b_p * ifft(G_p)) summed over planes, times gcf, divided by the
+//     sum of weights, in the column pass (gridding.py:270-289, wstack_single.py:93-94,
+//     imaging_serial.py:193-196, imaging/base.py:84-101).
+// The screen phase, the gcf product and the weight division are evaluated in float64.
+#include "arlmi_internal.h"
+
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <vector>
+
+namespace arlmi {
+
+template <typename R>
+__device__ __forceinline__ cplx<R> cmk(R x, R y) {
+  cplx<R> c;
+  c.x = x;
+  c.y = y;
+  return c;
+}
+template <typename C>
+__device__ __forceinline__ C cadd(C a, C b) {
+  C r;
+  r.x = a.x + b.x;
+  r.y = a.y + b.y;
+  return r;
+}
+template <typename C>
+__device__ __forceinline__ C csub(C a, C b) {
+  C r;
+  r.x = a.x - b.x;
+  r.y = a.y - b.y;
+  return r;
+}
+template <typename C>
+__device__ __forceinline__ C cmul(C a, C b) {
+  C r;
+  r.x = a.x * b.x - a.y * b.y;
+  r.y = a.x * b.y + a.y * b.x;
+  return r;
+}
+// multiply by -i (forward) or +i (inverse)
+template <typename R, bool INV>
+__device__ __forceinline__ cplx<R> mul_mi(cplx<R> a) {
+  return INV ? cmk<R>(-a.y, a.x) : cmk<R>(a.y, -a.x);
+}
+
+template <typename R, bool INV>
+__device__ __forceinline__ void dft2(cplx<R>* v) {
+  const cplx<R> a = v[0], b = v[1];
+  v[0] = cadd(a, b);
+  v[1] = csub(a, b);
+}
+
+template <typename R, bool INV>
+__device__ __forceinline__ void dft4(cplx<R>& a0, cplx<R>& a1, cplx<R>& a2, cplx<R>& a3) {
+  const cplx<R> t0 = cadd(a0, a2), t1 = csub(a0, a2), t2 = cadd(a1, a3), t3 = mul_mi<R, INV>(csub(a1, a3));
+  a0 = cadd(t0, t2);
+  a2 = csub(t0, t2);
+  a1 = cadd(t1, t3);
+  a3 = csub(t1, t3);
+}
+
+template <typename R, bool INV>
+__device__ __forceinline__ void dft8(cplx<R>* v) {
+  // even / odd split: E = DFT4(v0, v2, v4, v6), O = DFT4(v1, v3, v5, v7)
+  cplx<R> e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+  cplx<R> o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+  dft4<R, INV>(e0, e1, e2, e3);
+  dft4<R, INV>(o0, o1, o2, o3);
+  const R h = R(0.70710678118654752440084436210484903928);
+  // W8^1 = (1 -+ i)/sqrt2, W8^2 = -+i, W8^3 = (-1 -+ i)/sqrt2
+  const cplx<R> w1o1 = INV ? cmk<R>(h * (o1.x - o1.y), h * (o1.x + o1.y)) : cmk<R>(h * (o1.x + o1.y), h * (o1.y - o1.x));
+  const cplx<R> w2o2 = mul_mi<R, INV>(o2);
+  const cplx<R> w3o3 = INV ? cmk<R>(-h * (o3.x + o3.y), h * (o3.x - o3.y)) : cmk<R>(h * (o3.y - o3.x), -h * (o3.x + o3.y));
+  v[0] = cadd(e0, o0);
+  v[4] = csub(e0, o0);
+  v[1] = cadd(e1, w1o1);
+  v[5] = csub(e1, w1o1);
+  v[2] = cadd(e2, w2o2);
+  v[6] = csub(e2, w2o2);
+  v[3] = cadd(e3, w3o3);
+  v[7] = csub(e3, w3o3);
+}
+
+template <typename R, int RAD, bool INV>
+__device__ __forceinline__ void dft(cplx<R>* v) {
+  if constexpr (RAD == 8) dft8<R, INV>(v);
+  else if constexpr (RAD == 4) dft4<R, INV>(v[0], v[1], v[2], v[3]);
+  else if constexpr (RAD == 2) dft2<R, INV>(v);
+}
+
+// LDS addressing: ROW layout [b][n], COL layout [n][b]
+template <bool COL, int N, int NB>
+__device__ __forceinline__ int lidx(int b, int n) {
+  return COL ? n * NB + b : b * N + n;
+}
+
+// One Stockham stage on transform b held in LDS (in place: read all, barrier, write all).
+template <typename R, int N, int NB, int NT, int RAD, int NS, bool INV, bool COL>
+__device__ __forceinline__ void stage(cplx<R>* lds, int b, int t, const cplx<R>* __restrict__ tw) {
+  constexpr int NBF = N / RAD;
+  constexpr int PER = NBF / NT;
+  static_assert(PER * NT == NBF, "butterflies must divide among threads");
+  cplx<R> v[PER][RAD];
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int j = t + q * NT;
+#pragma unroll
+    for (int r = 0; r < RAD; ++r) v[q][r] = lds[lidx<COL, N, NB>(b, j + r * NBF)];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int j = t + q * NT;
+    const int k = j % NS;
+    if constexpr (NS > 1) {
+#pragma unroll
+      for (int r = 1; r < RAD; ++r) {
+        cplx<R> w = tw[r * k * (N / (NS * RAD))];
+        if (INV) w.y = -w.y;
+        v[q][r] = cmul(v[q][r], w);
+      }
+    }
+    dft<R, RAD, INV>(v[q]);
+    const int idxD = (j / NS) * NS * RAD + k;
+#pragma unroll
+    for (int r = 0; r < RAD; ++r) lds[lidx<COL, N, NB>(b, idxD + r * NS)] = v[q][r];
+  }
+  __syncthreads();
+}
+
+template <typename R, int N, int NB, int NT, int NS, bool INV, bool COL>
+__device__ __forceinline__ void stages(cplx<R>* lds, int b, int t, const cplx<R>* __restrict__ tw) {
+  if constexpr (NS < N) {
+    constexpr int RAD = (N / NS >= 8) ? 8 : (N / NS);
+    stage<R, N, NB, NT, RAD, NS, INV, COL>(lds, b, t, tw);
+    stages<R, N, NB, NT, NS * RAD, INV, COL>(lds, b, t, tw);
+  }
+}
+
+// The twiddle table lives in LDS after the transform data (LDS loads cannot be hoisted
+// across the stage barriers, global ones can, which would pin every stage's twiddles in
+// registers at once).  Callers size dynamic LDS as data + N complex values.
+template <typename R, int N>
+__device__ __forceinline__ cplx<R>* stage_twiddles_at(unsigned char* smem, size_t data_bytes, const cplx<R>* twg) {
+  cplx<R>* tw = reinterpret_cast<cplx<R>*>(smem + data_bytes);
+  for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = twg[i];
+  return tw;
+}
+
+template <int N>
+struct Shape {
+  static constexpr int NT = N >= 16 ? N / 16 : 1;       // threads per transform
+  static constexpr int ROW_NB = (NT >= 256) ? 1 : 256 / NT;
+  // LDS budget: NB transforms plus one twiddle table of N values within 160 KiB
+  template <typename R>
+  static constexpr int col_nb() {
+    constexpr int by_lds = (int)(163840 / (N * sizeof(cplx<R>))) - 1;
+    constexpr int by_thr = 1024 / NT;
+    constexpr int nb = by_lds < by_thr ? by_lds : by_thr;
+    constexpr int c = nb > 16 ? 16 : (nb < 1 ? 1 : nb);
+    return c >= 16 ? 16 : c >= 8 ? 8 : c >= 4 ? 4 : c >= 2 ? 2 : 1;  // a power of two dividing N
+  }
+  // the invert column pass also holds per-pixel accumulators: half the columns, twice the registers
+  template <typename R>
+  static constexpr int inv_nb() {
+    constexpr int nb = col_nb<R>() / 2;
+    return nb < 1 ? 1 : nb;
+  }
+  template <typename R>
+  static constexpr bool fits() {
+    return (size_t)N * sizeof(cplx<R>) * (size_t)(ROW_NB + 1) <= 163840 &&
+           (size_t)N * sizeof(cplx<R>) * (size_t)(col_nb<R>() + 1) <= 163840;
+  }
+};
+
+// w-screen helpers ---------------------------------------------------------
+
+// 1 - sqrt(1 - r^2) for pixel (y, x); r2 >= 1 flags the pixel as outside the unit circle.
+__device__ __forceinline__ double screen_q(int y, int x, int n, double fov, bool* outside) {
+  const double c = (double)(n / 2);
+  const double ly = ((double)y - c) / (double)n;
+  const double mx = ((double)x - c) / (double)n;
+  const double r2 = fov * fov * (ly * ly + mx * mx);
+  *outside = r2 >= 1.0;
+  return *outside ? 0.0 : 1.0 - sqrt(1.0 - r2);
+}
+
+// exp(2 pi i t) for t given in turns, evaluated in float64 up to the range reduction.
+// Single precision: t is reduced to [-1/2, 1/2] in float64, then to an octant
+// f in [-1/8, 1/8] turns, and sin / cos of 2 pi f come from their Taylor series
+// (truncation below 3e-8 on that interval), rotated by the quadrant.
+template <typename R>
+__device__ __forceinline__ cplx<R> expi_turns(double t);
+template <>
+__device__ __forceinline__ float2 expi_turns<float>(double t) {
+  const double red = t - rint(t);                    // [-1/2, 1/2]
+  const float q4 = rintf((float)(red * 4.0));        // quadrant -2..2
+  const float f = (float)(red - 0.25 * (double)q4);  // [-1/8, 1/8]
+  const float th = 6.28318530717958647692f * f;
+  const float t2 = th * th;
+  const float sn = th * (1.0f + t2 * (-1.0f / 6.0f + t2 * (1.0f / 120.0f + t2 * (-1.0f / 5040.0f + t2 * (1.0f / 362880.0f)))));
+  const float cs = 1.0f + t2 * (-0.5f + t2 * (1.0f / 24.0f + t2 * (-1.0f / 720.0f + t2 * (1.0f / 40320.0f))));
+  const int q = ((int)q4) & 3;
+  // rotate (cs, sn) by q quarter turns
+  float c = cs, s = sn;
+  if (q == 1) { c = -sn; s = cs; }
+  else if (q == 2) { c = -cs; s = -sn; }
+  else if (q == 3) { c = sn; s = -cs; }
+  return make_float2(c, s);
+}
+template <>
+__device__ __forceinline__ double2 expi_turns<double>(double t) {
+  double s, c;
+  sincospi(2.0 * (t - rint(t)), &s, &c);
+  return make_double2(c, s);
+}
+
+constexpr double kMinusTwoPi = -2.0 * 3.14159265358979323846264338327950288;
+
+// phase -2 pi w (1 - n) of the w screen, in turns
+__device__ __forceinline__ double screen_turns(double w, double q) { return -(w * q); }
+
+struct FftArgs {
+  void* data;            // grids [planes][chan][N][N] (complex)
+  int nbatch;            // plain mode: number of planes
+  int nplanes, nchan;
+  int in_mod, out_mod;   // apply (-1)^(x+y) on load / store
+  double scale;
+  const void* model;     // [chan][N][N] real (precision R)
+  const double* gcf;     // [chan][N][N]
+  const double* plane_w; // [nplanes] w-screen centres
+  const double* sumwt;   // [nplanes][chan]
+  void* img_re;
+  void* img_im;
+  int normalize, screens, wstack;
+  double fov;            // N * cellsize
+};
+
+// ---------------------------------------------------------------------------
+// row passes
+// ---------------------------------------------------------------------------
+
+template <typename R, int N, bool INV>
+__global__ __launch_bounds__(Shape<N>::NT * Shape<N>::ROW_NB) void k_rows_plain(FftArgs a, const cplx<R>* __restrict__ twg) {
+  constexpr int NT = Shape<N>::NT, NB = Shape<N>::ROW_NB, THR = NT * NB, EPT = N * NB / THR;
+  extern __shared__ __align__(16) unsigned char smem[];
+  cplx<R>* lds = reinterpret_cast<cplx<R>*>(smem);
+  cplx<R>* tw = stage_twiddles_at<R, N>(smem, sizeof(cplx<R>) * (size_t)N * NB, twg);
+  cplx<R>* data = reinterpret_cast<cplx<R>*>(a.data);
+  const int64_t row0 = (int64_t)blockIdx.x * NB;
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    const int i = threadIdx.x + e * THR;
+    const int b = i / N, n = i % N;
+    const int64_t row = row0 + b;
+    cplx<R> v = data[row * N + n];
+    if (a.in_mod && (((int)(row % N) + n) & 1)) v = cmk<R>(-v.x, -v.y);
+    lds[lidx<false, N, NB>(b, n)] = v;
+  }
+  __syncthreads();
+  const int b = threadIdx.x / NT, t = threadIdx.x % NT;
+  stages<R, N, NB, NT, 1, INV, false>(lds, b, t, tw);
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    const int i = threadIdx.x + e * THR;
+    const int bb = i / N, n = i % N;
+    const int64_t row = row0 + bb;
+    cplx<R> v = lds[lidx<false, N, NB>(bb, n)];
+    R sc = (R)a.scale;
+    if (a.out_mod && (((int)(row % N) + n) & 1)) sc = -sc;
+    data[row * N + n] = cmk<R>(v.x * sc, v.y * sc);
+  }
+}
+
+// Predict: grids[p][c][y][:] = rowfft((-1)^(x+y) * model[c][y][x] * gcf[c][y][x] * conj(b_p(x,y)))
+template <typename R, int N>
+__global__ __launch_bounds__(Shape<N>::NT * Shape<N>::ROW_NB) void k_rows_predict(FftArgs a, const cplx<R>* __restrict__ twg) {
+  constexpr int NT = Shape<N>::NT, NB = Shape<N>::ROW_NB, THR = NT * NB, EPT = N * NB / THR;
+  extern __shared__ __align__(16) unsigned char smem[];
+  cplx<R>* lds = reinterpret_cast<cplx<R>*>(smem);
+  cplx<R>* tw = stage_twiddles_at<R, N>(smem, sizeof(cplx<R>) * (size_t)N * NB, twg);
+  cplx<R>* grids = reinterpret_cast<cplx<R>*>(a.data);
+  const R* model = reinterpret_cast<const R*>(a.model);
+  constexpr int RB = N / NB;  // row blocks per channel
+  const int c = blockIdx.x / RB;
+  const int y0 = (blockIdx.x % RB) * NB;
+  R mv[EPT];
+  double q[EPT];
+  bool out[EPT];
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    const int i = threadIdx.x + e * THR;
+    const int b = i / N, x = i % N, y = y0 + b;
+    const size_t gi = ((size_t)c * N + y) * N + x;
+    double m = (double)model[gi] * a.gcf[gi];
+    if ((x + y) & 1) m = -m;
+    mv[e] = (R)m;
+    q[e] = a.screens ? screen_q(y, x, N, a.fov, &out[e]) : 0.0;
+    if (!a.screens) out[e] = false;
+  }
+  const int b = threadIdx.x / NT, t = threadIdx.x % NT;
+  for (int p = 0; p < a.nplanes; ++p) {
+    const double wp = a.screens ? a.plane_w[p] : 0.0;
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const int i = threadIdx.x + e * THR;
+      const int bb = i / N, x = i % N;
+      cplx<R> v;
+      if (!a.screens) {
+        v = cmk<R>(mv[e], R(0));
+      } else if (out[e]) {
+        v = cmk<R>(R(0), R(0));
+      } else {
+        const cplx<R> s = expi_turns<R>(screen_turns(wp, q[e]));  // b; conj(b) = (s.x, -s.y)
+        v = cmk<R>(mv[e] * s.x, -mv[e] * s.y);
+      }
+      lds[lidx<false, N, NB>(bb, x)] = v;
+    }
+    __syncthreads();
+    stages<R, N, NB, NT, 1, false, false>(lds, b, t, tw);
+    cplx<R>* dst = grids + ((size_t)p * a.nchan + c) * N * N;
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const int i = threadIdx.x + e * THR;
+      const int bb = i / N, x = i % N;
+      dst[(size_t)(y0 + bb) * N + x] = lds[lidx<false, N, NB>(bb, x)];
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// column passes
+// ---------------------------------------------------------------------------
+
+template <typename R, int N, bool INV>
+__global__ __launch_bounds__(Shape<N>::NT * Shape<N>::template col_nb<R>()) void k_cols_plain(FftArgs a, const cplx<R>* __restrict__ twg) {
+  constexpr int NT = Shape<N>::NT, NB = Shape<N>::template col_nb<R>(), THR = NT * NB, EPT = N * NB / THR;
+  extern __shared__ __align__(16) unsigned char smem[];
+  cplx<R>* lds = reinterpret_cast<cplx<R>*>(smem);
+  cplx<R>* tw = stage_twiddles_at<R, N>(smem, sizeof(cplx<R>) * (size_t)N * NB, twg);
+  cplx<R>* data = reinterpret_cast<cplx<R>*>(a.data);
+  constexpr int CB = N / NB;
+  const int64_t plane = blockIdx.x / CB;
+  const int x0 = (blockIdx.x % CB) * NB;
+  cplx<R>* base = data + plane * (int64_t)N * N;
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    const int i = threadIdx.x + e * THR;
+    const int n = i / NB, b = i % NB;
+    cplx<R> v = base[(size_t)n * N + x0 + b];
+    if (a.in_mod && ((n + x0 + b) & 1)) v = cmk<R>(-v.x, -v.y);
+    lds[lidx<true, N, NB>(b, n)] = v;
+  }
+  __syncthreads();
+  const int b = threadIdx.x % NB, t = threadIdx.x / NB;
+  stages<R, N, NB, NT, 1, INV, true>(lds, b, t, tw);
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    const int i = threadIdx.x + e * THR;
+    const int n = i / NB, bb = i % NB;
+    const cplx<R> v = lds[lidx<true, N, NB>(bb, n)];
+    R sc = (R)a.scale;
+    if (a.out_mod && ((n + x0 + bb) & 1)) sc = -sc;
+    base[(size_t)n * N + x0 + bb] = cmk<R>(v.x * sc, v.y * sc);
+  }
+}
+
+// Invert: img[c][y][x] = gcf / S_c * sum_p Re(b_p * (-1)^(x+y) * colifft(G_p)).
+// SCREENS: w-stack planes (real part after the screen); otherwise the planes are summed and the
+// imaginary part is kept as well.  Single precision accumulates in float, double in double.
+template <typename R, int N, bool SCREENS>
+__global__ __launch_bounds__(Shape<N>::NT * Shape<N>::template inv_nb<R>()) void k_cols_invert(FftArgs a, const cplx<R>* __restrict__ twg) {
+  constexpr int NT = Shape<N>::NT, NB = Shape<N>::template inv_nb<R>(), THR = NT * NB, EPT = N * NB / THR;
+  extern __shared__ __align__(16) unsigned char smem[];
+  cplx<R>* lds = reinterpret_cast<cplx<R>*>(smem);
+  cplx<R>* tw = stage_twiddles_at<R, N>(smem, sizeof(cplx<R>) * (size_t)N * NB, twg);
+  const cplx<R>* grids = reinterpret_cast<const cplx<R>*>(a.data);
+  constexpr int CB = N / NB;
+  const int c = blockIdx.x / CB;
+  const int x0 = (blockIdx.x % CB) * NB;
+  R acc[EPT];
+  R acci[SCREENS ? 1 : EPT];
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) acc[e] = R(0);
+  if constexpr (!SCREENS) {
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) acci[e] = R(0);
+  }
+  double tot = 0.0;
+  const int b = threadIdx.x % NB, t = threadIdx.x / NB;
+  for (int p = 0; p < a.nplanes; ++p) {
+    const double sw = a.sumwt[(size_t)p * a.nchan + c];
+    tot += sw;
+    if (a.wstack && a.normalize && !(sw > 0.0)) continue;  // normalize_sumwt zeroes this slab's image
+    const cplx<R>* base = grids + ((size_t)p * a.nchan + c) * N * N;
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const int i = threadIdx.x + e * THR;
+      const int n = i / NB, bb = i % NB;
+      lds[lidx<true, N, NB>(bb, n)] = base[(size_t)n * N + x0 + bb];
+    }
+    __syncthreads();
+    stages<R, N, NB, NT, 1, true, true>(lds, b, t, tw);
+    const double wp = SCREENS ? a.plane_w[p] : 0.0;
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const int i = threadIdx.x + e * THR;
+      const int n = i / NB, bb = i % NB;
+      const int x = x0 + bb, y = n;
+      cplx<R> v = lds[lidx<true, N, NB>(bb, n)];
+      if ((x + y) & 1) v = cmk<R>(-v.x, -v.y);
+      if constexpr (SCREENS) {
+        bool outside;
+        const double qq = screen_q(y, x, N, a.fov, &outside);
+        if (!outside) {
+          const cplx<R> s = expi_turns<R>(screen_turns(wp, qq));
+          acc[e] += s.x * v.x - s.y * v.y;
+        }
+      } else {
+        acc[e] += v.x;
+        acci[e] += v.y;
+      }
+    }
+    __syncthreads();
+  }
+  R* img = reinterpret_cast<R*>(a.img_re);
+  R* imi = reinterpret_cast<R*>(a.img_im);
+  const bool zero = a.normalize && !(tot > 0.0);
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    const int i = threadIdx.x + e * THR;
+    const int n = i / NB, bb = i % NB;
+    const size_t gi = ((size_t)c * N + n) * N + x0 + bb;
+    const double g = a.gcf[gi];
+    double re = (double)acc[e] * g, im = SCREENS ? 0.0 : (double)acci[SCREENS ? 0 : e] * g;
+    if (a.normalize) {
+      re = zero ? 0.0 : re / tot;
+      im = zero ? 0.0 : im / tot;
+    }
+    img[gi] = (R)re;
+    if (imi) imi[gi] = (R)im;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// w screen and direct DFT
+// ---------------------------------------------------------------------------
+
+template <typename R>
+__global__ void k_w_screen(int n, double fov, double w, cplx<R>* out) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= (int64_t)n * n) return;
+  const int y = (int)(i / n), x = (int)(i % n);
+  bool outside;
+  const double q = screen_q(y, x, n, fov, &outside);
+  if (outside) {
+    out[i] = cmk<R>(R(0), R(0));
+    return;
+  }
+  double s, c;
+  sincos(kMinusTwoPi * w * q, &s, &c);
+  out[i] = cmk<R>((R)c, (R)s);
+}
+
+template <typename R>
+__global__ void k_dft(const double* __restrict__ uvw, const int32_t* __restrict__ chan, int64_t nrows,
+                      const double* __restrict__ lmf, int ncomp, int nchan, cplx<R>* __restrict__ vis) {
+  extern __shared__ double comp[];
+  const int stride = 2 + nchan;
+  for (int i = threadIdx.x; i < ncomp * stride; i += blockDim.x) comp[i] = lmf[i];
+  __syncthreads();
+  const int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (row >= nrows) return;
+  const double u = uvw[3 * row], v = uvw[3 * row + 1], w = uvw[3 * row + 2];
+  const int ch = chan ? chan[row] : 0;
+  double ar = 0.0, ai = 0.0;
+  for (int k = 0; k < ncomp; ++k) {
+    const double l = comp[k * stride], m = comp[k * stride + 1];
+    const double f = comp[k * stride + 2 + ch];
+    const double nm1 = sqrt(1.0 - l * l - m * m) - 1.0;
+    const double d = u * l + v * m + w * nm1;
+    double s, c;
+    sincos(kMinusTwoPi * d, &s, &c);  // exp(-2 pi i d)
+    ar += f * c;
+    ai += f * s;
+  }
+  cplx<R> o = vis[row];
+  vis[row] = cmk<R>(o.x + (R)ar, o.y + (R)ai);
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+static std::mutex g_tw_mutex;
+static std::map<std::pair<int, int>, void*> g_tw;  // (device*100000 + n, precision) -> table
+
+const void* twiddles(int n, int precision) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  std::lock_guard<std::mutex> lock(g_tw_mutex);
+  const auto key = std::make_pair(dev * 100000 + n, precision);
+  auto it = g_tw.find(key);
+  if (it != g_tw.end()) return it->second;
+  const size_t esz = precision == 32 ? sizeof(float2) : sizeof(double2);
+  std::vector<unsigned char> host(esz * (size_t)n);
+  for (int k = 0; k < n; ++k) {
+    const long double ang = -2.0L * 3.141592653589793238462643383279502884L * (long double)k / (long double)n;
+    const double c = (double)cosl(ang), s = (double)sinl(ang);
+    if (precision == 32) {
+      float2 v = make_float2((float)c, (float)s);
+      memcpy(&host[esz * k], &v, esz);
+    } else {
+      double2 v = make_double2(c, s);
+      memcpy(&host[esz * k], &v, esz);
+    }
+  }
+  void* d = nullptr;
+  if (hipMalloc(&d, host.size()) != hipSuccess) return nullptr;
+  if (hipMemcpy(d, host.data(), host.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(d);
+    return nullptr;
+  }
+  g_tw[key] = d;
+  return d;
+}
+
+template <typename R, int N>
+static int run_fft2d(FftArgs a, int direction, hipStream_t s) {
+  const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(twiddles(N, sizeof(R) == 4 ? 32 : 64));
+  if (!tw) {
+    set_error("twiddle table allocation failed");
+    return ARLMI_E_HIP;
+  }
+  constexpr int NT = Shape<N>::NT, RNB = Shape<N>::ROW_NB, CNB = Shape<N>::template col_nb<R>();
+  const size_t rsh = sizeof(cplx<R>) * (size_t)N * (RNB + 1), csh = sizeof(cplx<R>) * (size_t)N * (CNB + 1);
+  const unsigned rblocks = (unsigned)((int64_t)a.nbatch * N / RNB), cblocks = (unsigned)((int64_t)a.nbatch * N / CNB);
+  FftArgs r = a, c = a;
+  r.in_mod = 1;
+  r.out_mod = 0;
+  r.scale = 1.0;
+  c.in_mod = 0;
+  c.out_mod = 1;
+  c.scale = direction < 0 ? 1.0 / ((double)N * (double)N) : 1.0;
+  if (direction < 0) {
+    ARLMI_LAUNCH("k_rows_plain", (k_rows_plain<R, N, true>), dim3(rblocks), dim3(NT * RNB), rsh, s, r, tw);
+    ARLMI_LAUNCH_CHECK();
+    ARLMI_LAUNCH("k_cols_plain", (k_cols_plain<R, N, true>), dim3(cblocks), dim3(NT * CNB), csh, s, c, tw);
+    ARLMI_LAUNCH_CHECK();
+  } else {
+    ARLMI_LAUNCH("k_rows_plain", (k_rows_plain<R, N, false>), dim3(rblocks), dim3(NT * RNB), rsh, s, r, tw);
+    ARLMI_LAUNCH_CHECK();
+    ARLMI_LAUNCH("k_cols_plain", (k_cols_plain<R, N, false>), dim3(cblocks), dim3(NT * CNB), csh, s, c, tw);
+    ARLMI_LAUNCH_CHECK();
+  }
+  return ARLMI_OK;
+}
+
+template <typename R, int N>
+static int run_grids_to_image(FftArgs a, hipStream_t s) {
+  const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(twiddles(N, sizeof(R) == 4 ? 32 : 64));
+  if (!tw) {
+    set_error("twiddle table allocation failed");
+    return ARLMI_E_HIP;
+  }
+  constexpr int NT = Shape<N>::NT, RNB = Shape<N>::ROW_NB, CNB = Shape<N>::template col_nb<R>();
+  const size_t rsh = sizeof(cplx<R>) * (size_t)N * (RNB + 1), csh = sizeof(cplx<R>) * (size_t)N * (CNB + 1);
+  FftArgs r = a;
+  r.in_mod = 1;
+  r.out_mod = 0;
+  r.scale = 1.0;
+  r.nbatch = a.nplanes * a.nchan;
+  const unsigned rblocks = (unsigned)((int64_t)r.nbatch * N / RNB);
+  ARLMI_LAUNCH("k_rows_plain", (k_rows_plain<R, N, true>), dim3(rblocks), dim3(NT * RNB), rsh, s, r, tw);
+  ARLMI_LAUNCH_CHECK();
+  constexpr int INB = Shape<N>::template inv_nb<R>();
+  const size_t ish = sizeof(cplx<R>) * (size_t)N * (INB + 1);
+  const unsigned cblocks = (unsigned)((int64_t)a.nchan * N / INB);
+  if (a.screens)
+    ARLMI_LAUNCH("k_cols_invert", (k_cols_invert<R, N, true>), dim3(cblocks), dim3(NT * INB), ish, s, a, tw);
+  else
+    ARLMI_LAUNCH("k_cols_invert", (k_cols_invert<R, N, false>), dim3(cblocks), dim3(NT * INB), ish, s, a, tw);
+  ARLMI_LAUNCH_CHECK();
+  return ARLMI_OK;
+}
+
+template <typename R, int N>
+static int run_image_to_grids(FftArgs a, hipStream_t s) {
+  const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(twiddles(N, sizeof(R) == 4 ? 32 : 64));
+  if (!tw) {
+    set_error("twiddle table allocation failed");
+    return ARLMI_E_HIP;
+  }
+  constexpr int NT = Shape<N>::NT, RNB = Shape<N>::ROW_NB, CNB = Shape<N>::template col_nb<R>();
+  const size_t rsh = sizeof(cplx<R>) * (size_t)N * (RNB + 1), csh = sizeof(cplx<R>) * (size_t)N * (CNB + 1);
+  const unsigned rblocks = (unsigned)((int64_t)a.nchan * N / RNB);
+  ARLMI_LAUNCH("k_rows_predict", (k_rows_predict<R, N>), dim3(rblocks), dim3(NT * RNB), rsh, s, a, tw);
+  ARLMI_LAUNCH_CHECK();
+  FftArgs c = a;
+  c.in_mod = 0;
+  c.out_mod = 1;
+  c.scale = 1.0;
+  c.nbatch = a.nplanes * a.nchan;
+  const unsigned cblocks = (unsigned)((int64_t)c.nbatch * N / CNB);
+  ARLMI_LAUNCH("k_cols_plain", (k_cols_plain<R, N, false>), dim3(cblocks), dim3(NT * CNB), csh, s, c, tw);
+  ARLMI_LAUNCH_CHECK();
+  return ARLMI_OK;
+}
+
+enum class Op { FFT, TO_IMAGE, TO_GRIDS };
+
+template <typename R>
+static int dispatch_n(Op op, int n, FftArgs a, int direction, hipStream_t s) {
+#define ARLMI_CASE(NN)                                                  \
+  case NN:                                                              \
+    if constexpr (Shape<NN>::template fits<R>()) {                      \
+      if (op == Op::FFT) return run_fft2d<R, NN>(a, direction, s);      \
+      if (op == Op::TO_IMAGE) return run_grids_to_image<R, NN>(a, s);   \
+      return run_image_to_grids<R, NN>(a, s);                           \
+    }                                                                   \
+    break;
+  switch (n) {
+    ARLMI_CASE(64)
+    ARLMI_CASE(128)
+    ARLMI_CASE(256)
+    ARLMI_CASE(512)
+    ARLMI_CASE(1024)
+    ARLMI_CASE(2048)
+    ARLMI_CASE(4096)
+    ARLMI_CASE(8192)
+    default:
+      break;
+  }
+#undef ARLMI_CASE
+  set_error("FFT size %d not supported at this precision (powers of two 64..8192 single, 64..4096 double)", n);
+  return ARLMI_E_UNSUPPORTED;
+}
+
+static int prec_ok(int precision) {
+  if (precision != 32 && precision != 64) {
+    set_error("precision must be 32 or 64");
+    return ARLMI_E_ARG;
+  }
+  return ARLMI_OK;
+}
+
+}  // namespace arlmi
+
+using namespace arlmi;
+
+extern "C" {
+
+int arlmi_fft2d(void* d_data, int32_t npixel, int64_t nbatch, int direction, int precision, void* stream) {
+  if (int rc = prec_ok(precision)) return rc;
+  if (!d_data || nbatch < 0 || (direction != 1 && direction != -1)) {
+    set_error("arlmi_fft2d: bad argument");
+    return ARLMI_E_ARG;
+  }
+  if (nbatch == 0) return ARLMI_OK;
+  FftArgs a = {};
+  a.data = d_data;
+  a.nbatch = (int)nbatch;
+  return precision == 32 ? dispatch_n<float>(Op::FFT, npixel, a, direction, as_stream(stream))
+                         : dispatch_n<double>(Op::FFT, npixel, a, direction, as_stream(stream));
+}
+
+int arlmi_grids_to_image(const arlmi_plan* p, void* d_grids, const double* d_sumwt, const double* d_gcf, int normalize,
+                         void* d_img_re, void* d_img_im, int precision, void* stream) {
+  if (int rc = prec_ok(precision)) return rc;
+  if (!p || !d_grids || !d_sumwt || !d_gcf || !d_img_re) {
+    set_error("arlmi_grids_to_image: NULL argument");
+    return ARLMI_E_ARG;
+  }
+  if (d_img_im && p->g.mode == ARLMI_MODE_WSTACK) {
+    set_error("the imaginary image is only defined without w-stacking");
+    return ARLMI_E_ARG;
+  }
+  FftArgs a = {};
+  a.data = d_grids;
+  a.nplanes = p->g.nplanes;
+  a.nchan = p->g.nchan;
+  a.gcf = d_gcf;
+  a.plane_w = p->d_plane_w;
+  a.sumwt = d_sumwt;
+  a.img_re = d_img_re;
+  a.img_im = d_img_im;
+  a.normalize = normalize;
+  a.wstack = p->g.mode == ARLMI_MODE_WSTACK;
+  a.screens = a.wstack;
+  a.fov = (double)p->g.n * p->g.cellsize;
+  return precision == 32 ? dispatch_n<float>(Op::TO_IMAGE, p->g.n, a, 0, as_stream(stream))
+                         : dispatch_n<double>(Op::TO_IMAGE, p->g.n, a, 0, as_stream(stream));
+}
+
+int arlmi_image_to_grids(const arlmi_plan* p, const void* d_model, const double* d_gcf, void* d_grids, int precision,
+                         void* stream) {
+  if (int rc = prec_ok(precision)) return rc;
+  if (!p || !d_model || !d_gcf || !d_grids) {
+    set_error("arlmi_image_to_grids: NULL argument");
+    return ARLMI_E_ARG;
+  }
+  FftArgs a = {};
+  a.data = d_grids;
+  a.nplanes = p->g.nplanes;
+  a.nchan = p->g.nchan;
+  a.model = d_model;
+  a.gcf = d_gcf;
+  a.plane_w = p->d_plane_w;
+  a.wstack = p->g.mode == ARLMI_MODE_WSTACK;
+  a.screens = a.wstack;
+  a.fov = (double)p->g.n * p->g.cellsize;
+  return precision == 32 ? dispatch_n<float>(Op::TO_GRIDS, p->g.n, a, 0, as_stream(stream))
+                         : dispatch_n<double>(Op::TO_GRIDS, p->g.n, a, 0, as_stream(stream));
+}
+
+int arlmi_w_screen(int32_t npixel, double cellsize, double w, void* d_out, int precision, void* stream) {
+  if (int rc = prec_ok(precision)) return rc;
+  if (!d_out || npixel <= 0 || (npixel & 1)) {
+    set_error("arlmi_w_screen: bad argument");
+    return ARLMI_E_ARG;
+  }
+  const int64_t tot = (int64_t)npixel * npixel;
+  const unsigned blocks = (unsigned)((tot + 255) / 256);
+  const double fov = (double)npixel * cellsize;
+  if (precision == 32)
+    ARLMI_LAUNCH("k_w_screen", k_w_screen<float>, dim3(blocks), dim3(256), 0, as_stream(stream), npixel, fov, w,
+                       (float2*)d_out);
+  else
+    ARLMI_LAUNCH("k_w_screen", k_w_screen<double>, dim3(blocks), dim3(256), 0, as_stream(stream), npixel, fov, w,
+                       (double2*)d_out);
+  ARLMI_LAUNCH_CHECK();
+  return ARLMI_OK;
+}
+
+int arlmi_dft(const double* d_uvw, const int32_t* d_chan, int64_t nrows, const double* d_lmflux, int32_t ncomp,
+              int32_t nchan, void* d_vis, int precision, void* stream) {
+  if (int rc = prec_ok(precision)) return rc;
+  if (nrows < 0 || ncomp < 0 || nchan <= 0 || (nrows > 0 && (!d_uvw || !d_vis)) || (ncomp > 0 && !d_lmflux)) {
+    set_error("arlmi_dft: bad argument");
+    return ARLMI_E_ARG;
+  }
+  const size_t sh = sizeof(double) * (size_t)ncomp * (2 + nchan);
+  if (sh > 64 * 1024) {
+    set_error("too many components for one call (%d)", ncomp);
+    return ARLMI_E_UNSUPPORTED;
+  }
+  if (nrows == 0 || ncomp == 0) return ARLMI_OK;
+  const unsigned blocks = (unsigned)((nrows + 255) / 256);
+  if (precision == 32)
+    ARLMI_LAUNCH("k_dft", k_dft<float>, dim3(blocks), dim3(256), sh, as_stream(stream), d_uvw, d_chan, nrows, d_lmflux,
+                       ncomp, nchan, (float2*)d_vis);
+  else
+    ARLMI_LAUNCH("k_dft", k_dft<double>, dim3(blocks), dim3(256), sh, as_stream(stream), d_uvw, d_chan, nrows, d_lmflux,
+                       ncomp, nchan, (double2*)d_vis);
+  ARLMI_LAUNCH_CHECK();
+  return ARLMI_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,903 @@
+// Gridding plan: per-row convolution mapping, w-slab membership and slab means, and the
+// tile-sorted entry lists walked by the gridding and degridding kernels.
+//
+// Follows processing_components/griddata/gridding.py:24-96 (convolution_mapping),
+// processing_components/visibility/iterators.py:91-109 (vis_wslice_iter) and
+// processing_components/imaging/wstack_single.py:43,83 (slab mean w).
+//
+// Layout: the grid of every plane (w slab or grid w plane) and image channel is cut into
+// T x T tiles.  A (row, plane) membership produces one entry in every tile its S x S kernel
+// footprint touches (1..4 tiles); the gridding kernel accumulates a tile in LDS and writes it
+// back once, so no pixel needs a global atomic unless the tile is split into several chunks.
+// Within a tile's segment the entries that the degridder uses (footprint-start tile of the
+// row's last plane) come first.
+#include "arlmi_internal.h"
+
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <thread>
+#include <string>
+#include <vector>
+#include <algorithm>
+
+namespace arlmi {
+
+static thread_local std::string g_last_error;
+
+void set_error(const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+}
+
+const char* last_error() { return g_last_error.c_str(); }
+
+int first_error_code(uint32_t bits) {
+  if (bits & ERRB_UV_OFF_GRID) return ARLMI_E_UV_OFF_GRID;
+  if (bits & ERRB_SUBSAMPLE) return ARLMI_E_SUBSAMPLE;
+  if (bits & ERRB_W_RANGE) return ARLMI_E_W_RANGE;
+  if (bits & ERRB_FREQ_MAP) return ARLMI_E_FREQ_MAP;
+  return ARLMI_OK;
+}
+
+// ---------------------------------------------------------------------------
+// per-lane counter updates (inactive lanes do nothing)
+// ---------------------------------------------------------------------------
+
+__device__ inline void cnt_add(uint32_t* counters, uint32_t key, bool active) {
+  if (active) atomicAdd(&counters[key], 1u);
+}
+
+__device__ inline uint32_t cnt_fetch_add(uint32_t* counters, uint32_t key, bool active) {
+  return active ? atomicAdd(&counters[key], 1u) : 0u;
+}
+
+// ---------------------------------------------------------------------------
+// max |w|
+// ---------------------------------------------------------------------------
+
+__global__ void k_wmax(const double* __restrict__ uvw, int64_t n, unsigned long long* out) {
+  double m = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    m = fmax(m, fabs(uvw[3 * i + 2]));
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+  // |w| >= 0 so the IEEE bit pattern orders like the value
+  if (__lane_id() == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
+}
+
+// ---------------------------------------------------------------------------
+// w-slab membership: |w - boxes[k]| < 0.5 * wslice (iterators.py:107-109)
+// ---------------------------------------------------------------------------
+
+struct Members {
+  int n;
+  int k[3];
+};
+
+__device__ inline Members slab_members(double w, const double* boxes, int nb, double wslice) {
+  Members m;
+  m.n = 0;
+  const double halfw = 0.5 * wslice;
+  if (!(wslice > 0.0)) return m;  // all |w| zero: |w - box| < 0 never holds
+  const double kd = fmin(fmax(rint((w - boxes[0]) / wslice), -2.0), (double)nb + 1.0);
+  const int k0 = (int)kd;
+  for (int k = k0 - 1; k <= k0 + 1; ++k) {
+    if (k < 0 || k >= nb) continue;
+    if (fabs(w - boxes[k]) < halfw) m.k[m.n++] = k;
+  }
+  return m;
+}
+
+// Pass A (w-stack): per-slab count and sum of w -> slab means.
+__global__ void k_slab_sums(const double* __restrict__ uvw, int64_t n, const double* __restrict__ boxes, int nb,
+                            double wslice, double* wsum, unsigned long long* wcnt) {
+  extern __shared__ char smem[];
+  double* ls = reinterpret_cast<double*>(smem);
+  unsigned int* lc = reinterpret_cast<unsigned int*>(ls + nb);
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+    ls[i] = 0.0;
+    lc[i] = 0u;
+  }
+  __syncthreads();
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double w = uvw[3 * i + 2];
+    Members m = slab_members(w, boxes, nb, wslice);
+    for (int j = 0; j < m.n; ++j) {
+      atomicAdd(&ls[m.k[j]], w);
+      atomicAdd(&lc[m.k[j]], 1u);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+    if (lc[i]) {
+      atomicAdd(&wsum[i], ls[i]);
+      atomicAdd(&wcnt[i], (unsigned long long)lc[i]);
+    }
+  }
+}
+
+__global__ void k_slab_means(const double* wsum, const unsigned long long* wcnt, double* wmean, int nb) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nb) wmean[i] = wcnt[i] ? wsum[i] / (double)wcnt[i] : 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// Row enumeration shared by the count and scatter passes.
+// ---------------------------------------------------------------------------
+
+struct Enum {
+  bool valid;        // row contributes (mapped and in at least one plane)
+  bool dropped;
+  int nm;            // memberships
+  int plane[3];
+  int zp[3];         // kernel w plane per membership
+  RowMap map;
+};
+
+__device__ inline Enum enumerate_row(const DevGeom& g, const double* __restrict__ uvw, const double* __restrict__ freq,
+                                     int64_t row, const double* __restrict__ boxes, double wslice,
+                                     const double* __restrict__ plane_w, uint32_t flags, uint32_t* err_out) {
+  Enum e;
+  e.valid = false;
+  e.dropped = false;
+  e.nm = 0;
+  const double u = uvw[3 * row + 0], v = uvw[3 * row + 1], w = uvw[3 * row + 2];
+  const double f = freq ? freq[row] : 0.0;
+  uint32_t err = 0;
+  if (g.mode == ARLMI_MODE_WSTACK) {
+    Members m = slab_members(w, boxes, g.nplanes, wslice);
+    if (m.n == 0) return e;  // in no slab: never mapped, never asserted (imaging_serial.py:93-95)
+    for (int j = 0; j < m.n; ++j) {
+      const double ws = w - plane_w[m.k[j]];
+      e.plane[j] = m.k[j];
+      // grid WW axis of a 1-plane GridData (nw = 1, wstep = 1e15) is checked against the kernel's nz
+      (void)map_wplane(ws, g.wstep_grid, g.crpix_wg, g.nz, &err);
+      e.zp[j] = map_wplane(ws, g.zstep, g.crpix_z, g.nz, &err);
+    }
+    e.nm = m.n;
+  } else {
+    // 2-D: grid w plane by rounding (gridding.py:58-65), kernel w plane (70-76)
+    const int pg = map_wplane(w, g.wstep_grid, g.crpix_wg, g.nz, &err);
+    if (pg >= g.nplanes) err |= ERRB_W_RANGE;
+    e.plane[0] = pg < g.nplanes ? pg : 0;
+    e.zp[0] = map_wplane(w, g.zstep, g.crpix_z, g.nz, &err);
+    e.nm = 1;
+  }
+  e.map = map_row(g, u, v, f, freq != nullptr);
+  err |= e.map.err;
+  if ((flags & ARLMI_FLAG_DROP_OFF_GRID) && (e.map.err & ERRB_UV_OFF_GRID)) {
+    e.dropped = true;
+    e.nm = 0;
+    return e;
+  }
+  if (err) {
+    *err_out |= err;
+    e.nm = 0;
+    return e;
+  }
+  e.valid = true;
+  return e;
+}
+
+// ---------------------------------------------------------------------------
+// Binning without global atomics: a two-level counting sort.
+//
+// Pass 1 (rows -> planes): each block of K1 rows counts its memberships per plane in LDS
+// (plus one bin for rows in no plane) and writes the counts to h1[bin][block].  One exclusive
+// scan of h1 gives every (bin, block) its output range; the scatter pass re-enumerates the
+// rows and writes one 16-byte record per membership into it (LDS cursors inside a block).
+//
+// Pass 2 (plane records -> tiles): every plane's record segment is cut into blocks of K2;
+// a block counts its entries per (tile, kind) in LDS, kind 0 = the entry the degridder uses,
+// 1 = the rest, and writes the counts to h2[plane base + (2*tile + kind) * nb_p + block].
+// One exclusive scan of h2 then yields, for every (plane, tile, kind, block), the position of
+// that block's entries in the final tile-sorted entry array (degrid entries first in a tile).
+// ---------------------------------------------------------------------------
+
+constexpr int K1_ROWS = 4096;  // rows per pass-1 block
+constexpr int K2_RECS = 8192;  // records per pass-2 block
+
+struct Rec {
+  uint32_t row;
+  uint32_t uv;   // pu | pv << 16
+  uint32_t aux;  // xoff | yoff << 8 | chan << 16 | last-membership << 24
+  uint32_t z;    // kernel w plane
+};
+
+__global__ void k_p1_count(DevGeom g, const double* __restrict__ uvw, const double* __restrict__ freq, int64_t n,
+                           const double* __restrict__ boxes, double wslice, const double* __restrict__ plane_w,
+                           uint32_t flags, uint32_t* __restrict__ h1, int nb1, uint32_t* misc) {
+  extern __shared__ uint32_t lh[];
+  const int P = g.nplanes;
+  for (int i = threadIdx.x; i < P + 3; i += blockDim.x) lh[i] = 0u;
+  __syncthreads();
+  const int64_t r0 = (int64_t)blockIdx.x * K1_ROWS;
+  const int64_t r1 = r0 + K1_ROWS < n ? r0 + K1_ROWS : n;
+  uint32_t err = 0;
+  for (int64_t row = r0 + threadIdx.x; row < r1; row += blockDim.x) {
+    Enum e = enumerate_row(g, uvw, freq, row, boxes, wslice, plane_w, flags, &err);
+    if (!e.valid) {
+      atomicAdd(&lh[P], 1u);
+      if (e.dropped) atomicAdd(&lh[P + 1], 1u);
+    } else {
+      for (int j = 0; j < e.nm; ++j) atomicAdd(&lh[e.plane[j]], 1u);
+      atomicAdd(&lh[P + 2], (uint32_t)e.nm);
+    }
+  }
+  if (err) atomicOr(&misc[0], err);
+  __syncthreads();
+  for (int i = threadIdx.x; i <= P; i += blockDim.x) h1[(int64_t)i * nb1 + blockIdx.x] = lh[i];
+  if (threadIdx.x == 0) {
+    if (lh[P + 1]) atomicAdd(&misc[2], lh[P + 1]);
+    if (lh[P + 2]) atomicAdd(&misc[3], lh[P + 2]);
+  }
+}
+
+__global__ void k_p1_scatter(DevGeom g, const double* __restrict__ uvw, const double* __restrict__ freq, int64_t n,
+                             const double* __restrict__ boxes, double wslice, const double* __restrict__ plane_w,
+                             uint32_t flags, const uint32_t* __restrict__ s1, int nb1, Rec* __restrict__ recs) {
+  extern __shared__ uint32_t cur[];
+  const int P = g.nplanes;
+  for (int i = threadIdx.x; i <= P; i += blockDim.x) cur[i] = s1[(int64_t)i * nb1 + blockIdx.x];
+  __syncthreads();
+  const int64_t r0 = (int64_t)blockIdx.x * K1_ROWS;
+  const int64_t r1 = r0 + K1_ROWS < n ? r0 + K1_ROWS : n;
+  uint32_t dummy = 0;
+  for (int64_t row = r0 + threadIdx.x; row < r1; row += blockDim.x) {
+    Enum e = enumerate_row(g, uvw, freq, row, boxes, wslice, plane_w, flags, &dummy);
+    if (!e.valid) {
+      const uint32_t pos = atomicAdd(&cur[P], 1u);
+      recs[pos] = Rec{(uint32_t)row, 0u, 0u, 0u};
+      continue;
+    }
+    for (int j = 0; j < e.nm; ++j) {
+      const uint32_t pos = atomicAdd(&cur[e.plane[j]], 1u);
+      Rec r;
+      r.row = (uint32_t)row;
+      r.uv = (uint32_t)e.map.pu | ((uint32_t)e.map.pv << 16);
+      r.aux = (uint32_t)e.map.xoff | ((uint32_t)e.map.yoff << 8) | ((uint32_t)e.map.chan << 16) |
+              ((j == e.nm - 1) ? (1u << 24) : 0u);
+      r.z = (uint32_t)e.zp[j];
+      recs[pos] = r;
+    }
+  }
+}
+
+// gathers s1[i * nb1] for i = 0..P+1 (the plane segment starts and the total)
+__global__ void k_p1_bounds(const uint32_t* __restrict__ s1, int nb1, int P, uint32_t* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i <= P + 1) out[i] = s1[(int64_t)i * nb1];
+}
+
+// Pass-2 block descriptor: plane, block index in the plane, record range, h2 base, blocks in plane.
+struct Blk2 {
+  int plane, b, nb;
+  uint32_t rs, re;
+  uint32_t hb;
+};
+
+__device__ inline void rec_tiles(const DevGeom& g, const Rec& r, int* tx0, int* ty0, int* ntx_t, int* nty_t,
+                                 int* xs, int* ys) {
+  const int pu = (int)(r.uv & 0xffffu), pv = (int)(r.uv >> 16);
+  const int half = g.support / 2;
+  *xs = pu - half;
+  *ys = pv - half;
+  *tx0 = *xs / g.tile;
+  *ty0 = *ys / g.tile;
+  *ntx_t = (*xs + g.support - 1) / g.tile - *tx0 + 1;
+  *nty_t = (*ys + g.support - 1) / g.tile - *ty0 + 1;
+}
+
+__global__ void k_p2_count(DevGeom g, const Rec* __restrict__ recs, const Blk2* __restrict__ blks,
+                           uint32_t* __restrict__ h2) {
+  extern __shared__ uint32_t lh[];
+  const Blk2 bk = blks[blockIdx.x];
+  const int tpp = g.nchan * g.ntx * g.ntx;
+  for (int i = threadIdx.x; i < 2 * tpp; i += blockDim.x) lh[i] = 0u;
+  __syncthreads();
+  for (uint32_t k = bk.rs + threadIdx.x; k < bk.re; k += blockDim.x) {
+    const Rec r = recs[k];
+    int tx0, ty0, ntxt, nty, xs, ys;
+    rec_tiles(g, r, &tx0, &ty0, &ntxt, &nty, &xs, &ys);
+    const int chan = (int)((r.aux >> 16) & 0xffu);
+    const bool last = (r.aux >> 24) & 1u;
+    for (int dy = 0; dy < nty; ++dy)
+      for (int dx = 0; dx < ntxt; ++dx) {
+        const int tl = (chan * g.ntx + ty0 + dy) * g.ntx + tx0 + dx;
+        const int kind = (dx == 0 && dy == 0 && last) ? 0 : 1;
+        atomicAdd(&lh[2 * tl + kind], 1u);
+      }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 2 * tpp; i += blockDim.x) h2[bk.hb + (uint32_t)i * bk.nb + bk.b] = lh[i];
+}
+
+__global__ void k_p2_scatter(DevGeom g, const Rec* __restrict__ recs, const Blk2* __restrict__ blks,
+                             const uint32_t* __restrict__ s2, uint2* __restrict__ entries,
+                             uint16_t* __restrict__ entry_z) {
+  extern __shared__ uint32_t cur[];
+  const Blk2 bk = blks[blockIdx.x];
+  const int tpp = g.nchan * g.ntx * g.ntx;
+  for (int i = threadIdx.x; i < 2 * tpp; i += blockDim.x) cur[i] = s2[bk.hb + (uint32_t)i * bk.nb + bk.b];
+  __syncthreads();
+  for (uint32_t k = bk.rs + threadIdx.x; k < bk.re; k += blockDim.x) {
+    const Rec r = recs[k];
+    int tx0, ty0, ntxt, nty, xs, ys;
+    rec_tiles(g, r, &tx0, &ty0, &ntxt, &nty, &xs, &ys);
+    const int chan = (int)((r.aux >> 16) & 0xffu);
+    const bool last = (r.aux >> 24) & 1u;
+    const int xo = (int)(r.aux & 0xffu), yo = (int)((r.aux >> 8) & 0xffu);
+    for (int dy = 0; dy < nty; ++dy)
+      for (int dx = 0; dx < ntxt; ++dx) {
+        const int tx = tx0 + dx, ty = ty0 + dy;
+        const int tl = (chan * g.ntx + ty) * g.ntx + tx;
+        const bool start = dx == 0 && dy == 0;
+        const int kind = (start && last) ? 0 : 1;
+        const uint32_t pos = atomicAdd(&cur[2 * tl + kind], 1u);
+        entries[pos] = make_uint2(r.row, pack_entry(xs - tx * g.tile + g.support, ys - ty * g.tile + g.support, xo,
+                                                    yo, start, kind == 0));
+        if (entry_z) entry_z[pos] = (uint16_t)r.z;
+      }
+  }
+}
+
+// Per-tile totals from the scanned h2: off (first entry), cnt (entries), cntd (degrid entries).
+__global__ void k_p2_keys(const uint32_t* __restrict__ s2, const uint32_t* __restrict__ hbase,
+                          const uint32_t* __restrict__ nbp, int P, int tpp, uint32_t* cnt, uint32_t* cntd,
+                          uint32_t* off) {
+  const int key = blockIdx.x * blockDim.x + threadIdx.x;
+  if (key >= P * tpp) return;
+  const int p = key / tpp, tl = key - p * tpp;
+  const uint32_t hb = hbase[p], nb = nbp[p];
+  const uint32_t a = s2[hb + (uint32_t)(2 * tl) * nb];
+  const uint32_t d = s2[hb + (uint32_t)(2 * tl + 1) * nb];
+  const uint32_t e = s2[hb + (uint32_t)(2 * tl + 2) * nb];  // next tile's start, or the next plane's base
+  off[key] = a;
+  cntd[key] = d - a;
+  cnt[key] = e - a;
+}
+
+// Chunk counts per tile; tiles with several grid chunks also get partial-tile slots.
+__global__ void k_chunks(const uint32_t* cnt, const uint32_t* cntd, int ntiles, int chunk, uint32_t* nch,
+                         uint32_t* nchd, uint32_t* nslot) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntiles) return;
+  const uint32_t c = cnt[t];
+  const uint32_t ng = c == 0 ? 1u : (c + chunk - 1) / chunk;  // empty tiles still write zeros
+  nch[t] = ng;
+  nchd[t] = (cntd[t] + chunk - 1) / chunk;
+  nslot[t] = ng > 1 ? ng : 0u;
+}
+
+// Work items: grid items {tile, start, count, slot+1 or 0}, degrid items {tile, start, count, 0};
+// tiles split over several grid items are listed with their first partial slot.
+__global__ void k_items(const uint32_t* cnt, const uint32_t* cntd, const uint32_t* off, const uint32_t* choff,
+                        const uint32_t* choffd, const uint32_t* slotoff, int ntiles, int chunk, int4* gitems,
+                        int4* ditems, int4* multi, uint32_t* misc) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntiles) return;
+  const uint32_t c = cnt[t];
+  const uint32_t ng = choff[t + 1] - choff[t];
+  for (uint32_t k = 0; k < ng; ++k) {
+    const uint32_t s = k * chunk;
+    const uint32_t len = c > s ? min((uint32_t)chunk, c - s) : 0u;
+    gitems[choff[t] + k] = make_int4(t, (int)(off[t] + s), (int)len, ng > 1 ? (int)(slotoff[t] + k + 1) : 0);
+  }
+  if (ng > 1) {
+    const uint32_t i = atomicAdd(&misc[4], 1u);  // one per split tile
+    multi[i] = make_int4(t, (int)slotoff[t], (int)ng, 0);
+  }
+  const uint32_t cd = cntd[t];
+  const uint32_t nd = choffd[t + 1] - choffd[t];
+  for (uint32_t k = 0; k < nd; ++k) {
+    const uint32_t s = k * chunk;
+    ditems[choffd[t] + k] = make_int4(t, (int)(off[t] + s), (int)min((uint32_t)chunk, cd - s), 0);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// exclusive scan of uint32 (three-phase: block scans, scan of block sums, add)
+// ---------------------------------------------------------------------------
+
+constexpr int SCAN_BLOCK = 1024;
+
+__global__ void k_scan_block(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* bsum) {
+  __shared__ uint32_t s[SCAN_BLOCK];
+  const int64_t i = blockIdx.x * (int64_t)SCAN_BLOCK + threadIdx.x;
+  const uint32_t x = i < n ? in[i] : 0u;
+  s[threadIdx.x] = x;
+  __syncthreads();
+  for (int o = 1; o < SCAN_BLOCK; o <<= 1) {
+    uint32_t y = threadIdx.x >= o ? s[threadIdx.x - o] : 0u;
+    __syncthreads();
+    s[threadIdx.x] += y;
+    __syncthreads();
+  }
+  if (i < n) out[i] = s[threadIdx.x] - x;
+  if (threadIdx.x == SCAN_BLOCK - 1) bsum[blockIdx.x] = s[threadIdx.x];
+}
+
+__global__ void k_scan_add(uint32_t* out, int64_t n, const uint32_t* boff) {
+  const int64_t i = blockIdx.x * (int64_t)SCAN_BLOCK + threadIdx.x;
+  if (i < n) out[i] += boff[blockIdx.x];
+}
+
+__global__ void k_scan_total(const uint32_t* in, uint32_t* out, int64_t n) {
+  // out[n] = out[n-1] + in[n-1]
+  out[n] = n ? out[n - 1] + in[n - 1] : 0u;
+}
+
+size_t scan_tmp_words(int64_t n) {
+  size_t words = 0;
+  while (n > 1) {
+    int64_t nb = (n + SCAN_BLOCK - 1) / SCAN_BLOCK;
+    words += (size_t)nb + (size_t)nb + 1;
+    n = nb;
+  }
+  return words + 4;
+}
+
+int exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tmp, hipStream_t s) {
+  if (n <= 0) {
+    ARLMI_HIP(hipMemsetAsync(out, 0, sizeof(uint32_t), s));
+    return ARLMI_OK;
+  }
+  const int64_t nb = (n + SCAN_BLOCK - 1) / SCAN_BLOCK;
+  uint32_t* bsum = tmp;
+  uint32_t* boff = tmp + nb;
+  ARLMI_LAUNCH("k_scan_block", k_scan_block, dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, s, in, out, n, bsum);
+  ARLMI_LAUNCH_CHECK();
+  if (nb > 1) {
+    int rc = exclusive_scan_u32(bsum, boff, nb, tmp + 2 * nb + 1, s);
+    if (rc) return rc;
+    ARLMI_LAUNCH("k_scan_add", k_scan_add, dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, s, out, n, boff);
+    ARLMI_LAUNCH_CHECK();
+  }
+  ARLMI_LAUNCH("k_scan_total", k_scan_total, dim3(1), dim3(1), 0, s, in, out, n);
+  ARLMI_LAUNCH_CHECK();
+  return ARLMI_OK;
+}
+
+// convolution_mapping for every row (2-D semantics: grid plane by rounding w), for callers and
+// tests that want the indices themselves: out[row] = {pu, pu_off, pv, pv_off, pwg, pwc, pfreq, err}.
+__global__ void k_map(DevGeom g, const double* __restrict__ uvw, const double* __restrict__ freq, int64_t n,
+                      int32_t* __restrict__ out) {
+  const int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (row >= n) return;
+  const double u = uvw[3 * row], v = uvw[3 * row + 1], w = uvw[3 * row + 2];
+  RowMap m = map_row(g, u, v, freq ? freq[row] : 0.0, freq != nullptr);
+  uint32_t err = m.err;
+  const int pg = map_wplane(w, g.wstep_grid, g.crpix_wg, g.nz, &err);
+  const int pz = map_wplane(w, g.zstep, g.crpix_z, g.nz, &err);
+  int32_t* o = out + 8 * row;
+  const double x = world2pix0(u, g.du, g.crpix_uv), y = world2pix0(v, g.dv, g.crpix_uv);
+  o[0] = (int32_t)rint(x);
+  o[1] = m.xoff;
+  o[2] = (int32_t)rint(y);
+  o[3] = m.yoff;
+  o[4] = pg;
+  o[5] = pz;
+  o[6] = m.chan;
+  o[7] = (int32_t)err;
+}
+
+}  // namespace arlmi
+
+using namespace arlmi;
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+static int checked_geom(const arlmi_geom* in, DevGeom* g) {
+  if (!in) {
+    set_error("geometry is NULL");
+    return ARLMI_E_ARG;
+  }
+  const int n = in->npixel;
+  if (n < 16 || n > 16384 || (n & (n - 1))) {
+    set_error("npixel %d must be a power of two in [16, 16384]", n);
+    return ARLMI_E_UNSUPPORTED;
+  }
+  if (in->support <= 0 || in->support > 64) {
+    set_error("support %d must be in [1, 64]", in->support);
+    return ARLMI_E_UNSUPPORTED;
+  }
+  if (in->support & 1) {
+    set_error("support %d is odd; ARL's slice-add needs an even support", in->support);
+    return ARLMI_E_ODD_SUPPORT;
+  }
+  if (in->oversampling <= 0 || in->oversampling > 128) {
+    set_error("oversampling %d must be in [1, 128]", in->oversampling);
+    return ARLMI_E_UNSUPPORTED;
+  }
+  if (in->nchan <= 0 || in->nz <= 0 || in->nz > 65535 || in->nplanes <= 0) {
+    set_error("nchan %d, nz %d, nplanes %d must be positive", in->nchan, in->nz, in->nplanes);
+    return ARLMI_E_ARG;
+  }
+  if (in->mode != ARLMI_MODE_2D && in->mode != ARLMI_MODE_WSTACK) {
+    set_error("unknown mode %d", in->mode);
+    return ARLMI_E_ARG;
+  }
+  int tile = in->tile ? in->tile : (n >= 64 ? 64 : n);
+  // the pass-2 tile histogram (2 counters per tile of a plane) must fit in LDS
+  while (!in->tile && tile < 128 && tile < n && (int64_t)in->nchan * (n / tile) * (n / tile) * 8 > 128 * 1024) tile *= 2;
+  if (tile != 16 && tile != 32 && tile != 64 && tile != 128) {
+    set_error("tile %d must be 16, 32, 64 or 128", tile);
+    return ARLMI_E_UNSUPPORTED;
+  }
+  if (tile > n) tile = n;
+  if ((int64_t)in->nchan * (n / tile) * (n / tile) * 8 > 160 * 1024) {
+    set_error("%d channels of %d^2 tiles per plane exceed the tile histogram's LDS", in->nchan, n / tile);
+    return ARLMI_E_UNSUPPORTED;
+  }
+  if (in->nchan > 255) {
+    set_error("at most 255 image channels");
+    return ARLMI_E_UNSUPPORTED;
+  }
+  const int64_t ntiles = (int64_t)in->nplanes * in->nchan * (n / tile) * (n / tile);
+  if (ntiles > (1ll << 30)) {
+    set_error("too many tiles (%lld)", (long long)ntiles);
+    return ARLMI_E_UNSUPPORTED;
+  }
+  g->n = n;
+  g->nchan = in->nchan;
+  g->support = in->support;
+  g->os = in->oversampling;
+  g->nz = in->nz;
+  g->nplanes = in->nplanes;
+  g->mode = in->mode;
+  g->tile = tile;
+  g->ntx = n / tile;
+  g->du = in->du;
+  g->dv = in->dv;
+  g->ddu = in->ddu;
+  g->ddv = in->ddv;
+  g->crpix_uv = in->crpix_uv;
+  g->crpix_d = in->crpix_d;
+  g->zstep = in->zstep;
+  g->crpix_z = in->crpix_z;
+  g->wstep_grid = in->wstep_grid;
+  g->crpix_wg = in->crpix_wg;
+  g->freq0 = in->freq0;
+  g->dfreq = in->dfreq;
+  g->cellsize = in->cellsize;
+  return ARLMI_OK;
+}
+
+template <typename T>
+static int grow(T** p, int64_t* cap, int64_t need) {
+  if (need <= *cap && *p) return ARLMI_OK;
+  if (*p) ARLMI_HIP(hipFree(*p));
+  *p = nullptr;
+  const int64_t c = need + need / 8 + 64;
+  ARLMI_HIP(hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * (size_t)c));
+  *cap = c;
+  return ARLMI_OK;
+}
+
+static int grow_bytes(void** p, int64_t* cap, int64_t need) {
+  if (need <= *cap && *p) return ARLMI_OK;
+  if (*p) ARLMI_HIP(hipFree(*p));
+  *p = nullptr;
+  const int64_t c = need + need / 8 + 256;
+  ARLMI_HIP(hipMalloc(p, (size_t)c));
+  *cap = c;
+  return ARLMI_OK;
+}
+
+static void free_plan(arlmi_plan* p) {
+  if (!p) return;
+  void* ptrs[] = {p->d_boxes, p->d_plane_w, p->d_wsum, p->d_wcnt, p->d_cnt, p->d_cntd, p->d_off, p->d_nch,
+                  p->d_nchd, p->d_nslot, p->d_choff, p->d_choffd, p->d_slotoff, p->d_misc, p->d_bounds,
+                  p->d_hbase, p->d_nbp, p->d_h1, p->d_s1, p->d_h2, p->d_s2, p->d_scan_tmp, p->d_recs, p->d_blk2,
+                  p->d_entries, p->d_entry_z, p->d_gitems, p->d_ditems, p->d_multi, p->d_partial};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+}
+
+static int plan_alloc_fixed(arlmi_plan* p) {
+  const int P = p->g.nplanes;
+  const int64_t nt = p->ntiles;
+  ARLMI_HIP(hipMalloc(&p->d_boxes, sizeof(double) * P));
+  ARLMI_HIP(hipMalloc(&p->d_plane_w, sizeof(double) * P));
+  ARLMI_HIP(hipMalloc(&p->d_wsum, sizeof(double) * P));
+  ARLMI_HIP(hipMalloc(&p->d_wcnt, sizeof(unsigned long long) * P));
+  ARLMI_HIP(hipMalloc(&p->d_cnt, sizeof(uint32_t) * nt));
+  ARLMI_HIP(hipMalloc(&p->d_cntd, sizeof(uint32_t) * nt));
+  ARLMI_HIP(hipMalloc(&p->d_off, sizeof(uint32_t) * nt));
+  ARLMI_HIP(hipMalloc(&p->d_nch, sizeof(uint32_t) * nt));
+  ARLMI_HIP(hipMalloc(&p->d_nchd, sizeof(uint32_t) * nt));
+  ARLMI_HIP(hipMalloc(&p->d_nslot, sizeof(uint32_t) * nt));
+  ARLMI_HIP(hipMalloc(&p->d_choff, sizeof(uint32_t) * (nt + 1)));
+  ARLMI_HIP(hipMalloc(&p->d_choffd, sizeof(uint32_t) * (nt + 1)));
+  ARLMI_HIP(hipMalloc(&p->d_slotoff, sizeof(uint32_t) * (nt + 1)));
+  ARLMI_HIP(hipMalloc(&p->d_misc, sizeof(uint32_t) * 8));
+  ARLMI_HIP(hipMalloc(&p->d_bounds, sizeof(uint32_t) * (P + 2)));
+  ARLMI_HIP(hipMalloc(&p->d_hbase, sizeof(uint32_t) * (P + 1)));
+  ARLMI_HIP(hipMalloc(&p->d_nbp, sizeof(uint32_t) * P));
+  ARLMI_HIP(hipMalloc(&p->d_multi, sizeof(int4) * nt));
+  return ARLMI_OK;
+}
+
+static int ensure_scan_tmp(arlmi_plan* p, int64_t n) {
+  return grow(&p->d_scan_tmp, &p->cap_scan, (int64_t)scan_tmp_words(n));
+}
+
+static int plan_build(arlmi_plan* p, const double* uvw, const double* freq, int64_t nrows, const double* h_boxes,
+                      double wslice, hipStream_t s) {
+  DevGeom& g = p->g;
+  const int P = g.nplanes;
+  const int64_t nt = p->ntiles;
+  const int tpp = p->tpp;
+  if (nrows < 0 || nrows > 0x7fffffffll) {
+    set_error("nrows %lld out of range", (long long)nrows);
+    return ARLMI_E_ARG;
+  }
+  if (nrows > 0 && !uvw) {
+    set_error("uvw is NULL");
+    return ARLMI_E_ARG;
+  }
+  p->nrows = nrows;
+  int rc;
+  ARLMI_HIP(hipMemsetAsync(p->d_plane_w, 0, sizeof(double) * P, s));
+  if (g.mode == ARLMI_MODE_WSTACK) {
+    if (!h_boxes) {
+      set_error("w-stack mode needs the slab centres");
+      return ARLMI_E_ARG;
+    }
+    ARLMI_HIP(hipMemcpyAsync(p->d_boxes, h_boxes, sizeof(double) * P, hipMemcpyHostToDevice, s));
+    ARLMI_HIP(hipMemsetAsync(p->d_wsum, 0, sizeof(double) * P, s));
+    ARLMI_HIP(hipMemsetAsync(p->d_wcnt, 0, sizeof(unsigned long long) * P, s));
+    if (nrows > 0) {
+      const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>((nrows + 255) / 256, 1024));
+      const size_t sh = (sizeof(double) + sizeof(unsigned int)) * P;
+      ARLMI_LAUNCH("k_slab_sums", k_slab_sums, dim3(blocks), dim3(256), sh, s, uvw, nrows, p->d_boxes, P, wslice,
+                   p->d_wsum, p->d_wcnt);
+      ARLMI_LAUNCH_CHECK();
+    }
+    ARLMI_LAUNCH("k_slab_means", k_slab_means, dim3((P + 255) / 256), dim3(256), 0, s, p->d_wsum, p->d_wcnt,
+                 p->d_plane_w, P);
+    ARLMI_LAUNCH_CHECK();
+  }
+  ARLMI_HIP(hipMemsetAsync(p->d_misc, 0, sizeof(uint32_t) * 8, s));
+
+  // pass 1: rows -> plane records
+  const int nb1 = (int)std::max<int64_t>(1, (nrows + K1_ROWS - 1) / K1_ROWS);
+  const int64_t n1 = (int64_t)(P + 1) * nb1;
+  if ((rc = grow(&p->d_h1, &p->cap_h1, n1))) return rc;
+  if ((rc = grow(&p->d_s1, &p->cap_s1, n1 + 1))) return rc;
+  if ((rc = ensure_scan_tmp(p, n1))) return rc;
+  ARLMI_LAUNCH("k_p1_count", k_p1_count, dim3(nb1), dim3(256), sizeof(uint32_t) * (P + 3), s, g, uvw, freq, nrows,
+               p->d_boxes, wslice, p->d_plane_w, p->flags, p->d_h1, nb1, p->d_misc);
+  ARLMI_LAUNCH_CHECK();
+  if ((rc = exclusive_scan_u32(p->d_h1, p->d_s1, n1, p->d_scan_tmp, s))) return rc;
+  ARLMI_LAUNCH("k_p1_bounds", k_p1_bounds, dim3((P + 2 + 255) / 256), dim3(256), 0, s, p->d_s1, nb1, P,
+               p->d_bounds);
+  ARLMI_LAUNCH_CHECK();
+  std::vector<uint32_t> bounds(P + 2);
+  uint32_t misc[8];
+  ARLMI_HIP(hipMemcpyAsync(bounds.data(), p->d_bounds, sizeof(uint32_t) * (P + 2), hipMemcpyDeviceToHost, s));
+  ARLMI_HIP(hipMemcpyAsync(misc, p->d_misc, sizeof(misc), hipMemcpyDeviceToHost, s));
+  ARLMI_HIP(hipStreamSynchronize(s));
+  if (misc[0]) {
+    rc = first_error_code(misc[0]);
+    const char* what = rc == ARLMI_E_UV_OFF_GRID ? "uv (or kernel footprint) off the grid"
+                       : rc == ARLMI_E_SUBSAMPLE ? "kernel sub-sample index equals oversampling"
+                       : rc == ARLMI_E_W_RANGE   ? "w plane out of range"
+                                                 : "frequency maps outside the image channels";
+    set_error("convolution mapping failed: %s", what);
+    return rc;
+  }
+  const int64_t nrec = bounds[P + 1];
+  p->ndropped = misc[2];
+  p->nmember = misc[3];
+  p->orphan_start = bounds[P];
+  p->norphans = bounds[P + 1] - bounds[P];
+  if ((rc = grow_bytes(&p->d_recs, &p->cap_recs, (int64_t)sizeof(Rec) * std::max<int64_t>(1, nrec)))) return rc;
+  ARLMI_LAUNCH("k_p1_scatter", k_p1_scatter, dim3(nb1), dim3(256), sizeof(uint32_t) * (P + 1), s, g, uvw, freq,
+               nrows, p->d_boxes, wslice, p->d_plane_w, p->flags, p->d_s1, nb1, (Rec*)p->d_recs);
+  ARLMI_LAUNCH_CHECK();
+
+  // pass 2: plane records -> tile entries
+  std::vector<Blk2> blks;
+  std::vector<uint32_t> hbase(P + 1), nbp(P);
+  uint64_t hb = 0;
+  for (int pl = 0; pl < P; ++pl) {
+    const uint32_t rs = bounds[pl], re = bounds[pl + 1];
+    const int nb = (int)std::max<int64_t>(1, ((int64_t)(re - rs) + K2_RECS - 1) / K2_RECS);
+    hbase[pl] = (uint32_t)hb;
+    nbp[pl] = (uint32_t)nb;
+    for (int b = 0; b < nb; ++b) {
+      Blk2 k;
+      k.plane = pl;
+      k.b = b;
+      k.nb = nb;
+      k.rs = rs + (uint32_t)b * K2_RECS;
+      k.re = std::min<uint32_t>(re, k.rs + K2_RECS);
+      k.hb = (uint32_t)hb;
+      blks.push_back(k);
+    }
+    hb += (uint64_t)2 * tpp * nb;
+  }
+  hbase[P] = (uint32_t)hb;
+  if (hb >= (1ull << 32)) {
+    set_error("tile histogram too large");
+    return ARLMI_E_UNSUPPORTED;
+  }
+  const int64_t n2 = (int64_t)hb;
+  if ((rc = grow_bytes(&p->d_blk2, &p->cap_blk2, (int64_t)(sizeof(Blk2) * blks.size())))) return rc;
+  if ((rc = grow(&p->d_h2, &p->cap_h2, n2 + 1))) return rc;
+  if ((rc = grow(&p->d_s2, &p->cap_s2, n2 + 2))) return rc;
+  ARLMI_HIP(hipMemcpyAsync(p->d_blk2, blks.data(), sizeof(Blk2) * blks.size(), hipMemcpyHostToDevice, s));
+  ARLMI_HIP(hipMemcpyAsync(p->d_hbase, hbase.data(), sizeof(uint32_t) * (P + 1), hipMemcpyHostToDevice, s));
+  ARLMI_HIP(hipMemcpyAsync(p->d_nbp, nbp.data(), sizeof(uint32_t) * P, hipMemcpyHostToDevice, s));
+  if ((rc = ensure_scan_tmp(p, n2 + 1))) return rc;
+  const size_t sh2 = sizeof(uint32_t) * 2 * (size_t)tpp;
+  ARLMI_LAUNCH("k_p2_count", k_p2_count, dim3((unsigned)blks.size()), dim3(512), sh2, s, g, (const Rec*)p->d_recs,
+               (const Blk2*)p->d_blk2, p->d_h2);
+  ARLMI_LAUNCH_CHECK();
+  // a trailing zero so that the last tile's end (hbase[P]) is readable in the scan
+  ARLMI_HIP(hipMemsetAsync(p->d_h2 + n2, 0, sizeof(uint32_t), s));
+  if ((rc = exclusive_scan_u32(p->d_h2, p->d_s2, n2 + 1, p->d_scan_tmp, s))) return rc;
+  ARLMI_LAUNCH("k_p2_keys", k_p2_keys, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, p->d_s2, p->d_hbase,
+               p->d_nbp, P, tpp, p->d_cnt, p->d_cntd, p->d_off);
+  ARLMI_LAUNCH_CHECK();
+  uint32_t total = 0;
+  ARLMI_HIP(hipMemcpyAsync(&total, p->d_s2 + n2, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  ARLMI_HIP(hipStreamSynchronize(s));
+  p->nentries = total;
+  if ((rc = grow(&p->d_entries, &p->cap_entries, std::max<int64_t>(1, total)))) return rc;
+  if (g.nz > 1 && (rc = grow(&p->d_entry_z, &p->cap_z, std::max<int64_t>(1, total)))) return rc;
+  ARLMI_LAUNCH("k_p2_scatter", k_p2_scatter, dim3((unsigned)blks.size()), dim3(512), sh2, s, g,
+               (const Rec*)p->d_recs, (const Blk2*)p->d_blk2, p->d_s2, p->d_entries,
+               g.nz > 1 ? p->d_entry_z : nullptr);
+  ARLMI_LAUNCH_CHECK();
+
+  // work items
+  ARLMI_LAUNCH("k_chunks", k_chunks, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, p->d_cnt, p->d_cntd,
+               (int)nt, p->chunk, p->d_nch, p->d_nchd, p->d_nslot);
+  ARLMI_LAUNCH_CHECK();
+  if ((rc = ensure_scan_tmp(p, nt))) return rc;
+  if ((rc = exclusive_scan_u32(p->d_nch, p->d_choff, nt, p->d_scan_tmp, s))) return rc;
+  if ((rc = exclusive_scan_u32(p->d_nchd, p->d_choffd, nt, p->d_scan_tmp, s))) return rc;
+  if ((rc = exclusive_scan_u32(p->d_nslot, p->d_slotoff, nt, p->d_scan_tmp, s))) return rc;
+  uint32_t tot3[3] = {0, 0, 0};
+  ARLMI_HIP(hipMemcpyAsync(&tot3[0], p->d_choff + nt, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  ARLMI_HIP(hipMemcpyAsync(&tot3[1], p->d_choffd + nt, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  ARLMI_HIP(hipMemcpyAsync(&tot3[2], p->d_slotoff + nt, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  ARLMI_HIP(hipStreamSynchronize(s));
+  p->ngitems = tot3[0];
+  p->nditems = tot3[1];
+  p->nslots = tot3[2];
+  if ((rc = grow(&p->d_gitems, &p->cap_gitems, std::max<int64_t>(1, p->ngitems)))) return rc;
+  if ((rc = grow(&p->d_ditems, &p->cap_ditems, std::max<int64_t>(1, p->nditems)))) return rc;
+  ARLMI_LAUNCH("k_items", k_items, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, p->d_cnt, p->d_cntd,
+               p->d_off, p->d_choff, p->d_choffd, p->d_slotoff, (int)nt, p->chunk, p->d_gitems, p->d_ditems,
+               p->d_multi, p->d_misc);
+  ARLMI_LAUNCH_CHECK();
+  ARLMI_HIP(hipMemcpyAsync(misc, p->d_misc, sizeof(misc), hipMemcpyDeviceToHost, s));
+  ARLMI_HIP(hipStreamSynchronize(s));
+  p->nmulti = misc[4];
+  return ARLMI_OK;
+}
+
+extern "C" {
+
+int arlmi_version(void) { return 100; }
+
+const char* arlmi_last_error(void) { return arlmi::last_error(); }
+
+int arlmi_wmax(const double* d_uvw, int64_t nrows, void* stream, double* wmax_out) {
+  if (!wmax_out || (nrows > 0 && !d_uvw)) {
+    set_error("arlmi_wmax: NULL argument");
+    return ARLMI_E_ARG;
+  }
+  hipStream_t s = as_stream(stream);
+  unsigned long long* d = nullptr;
+  ARLMI_HIP(hipMallocAsync(reinterpret_cast<void**>(&d), sizeof(unsigned long long), s));
+  ARLMI_HIP(hipMemsetAsync(d, 0, sizeof(unsigned long long), s));
+  if (nrows > 0) {
+    const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>((nrows + 255) / 256, 2048));
+    ARLMI_LAUNCH("k_wmax", k_wmax, dim3(blocks), dim3(256), 0, s, d_uvw, nrows, d);
+    ARLMI_LAUNCH_CHECK();
+  }
+  unsigned long long bits = 0;
+  ARLMI_HIP(hipMemcpyAsync(&bits, d, sizeof(bits), hipMemcpyDeviceToHost, s));
+  ARLMI_HIP(hipFreeAsync(d, s));
+  ARLMI_HIP(hipStreamSynchronize(s));
+  double w;
+  memcpy(&w, &bits, sizeof(w));
+  *wmax_out = w;
+  return ARLMI_OK;
+}
+
+int arlmi_plan_create(const arlmi_geom* gin, const double* d_uvw, const double* d_freq, int64_t nrows,
+                      const double* h_boxes, double wslice, uint32_t flags, void* stream, arlmi_plan** out) {
+  if (!out) {
+    set_error("plan_out is NULL");
+    return ARLMI_E_ARG;
+  }
+  *out = nullptr;
+  arlmi_plan* p = new arlmi_plan();
+  int rc = checked_geom(gin, &p->g);
+  if (rc) {
+    delete p;
+    return rc;
+  }
+  p->flags = flags;
+  p->tpp = p->g.nchan * p->g.ntx * p->g.ntx;
+  p->ntiles = p->g.nplanes * p->tpp;
+  rc = plan_alloc_fixed(p);
+  if (!rc) rc = plan_build(p, d_uvw, d_freq, nrows, h_boxes, wslice, as_stream(stream));
+  if (rc) {
+    free_plan(p);
+    delete p;
+    return rc;
+  }
+  *out = p;
+  return ARLMI_OK;
+}
+
+int arlmi_plan_update(arlmi_plan* p, const double* d_uvw, const double* d_freq, int64_t nrows,
+                      const double* h_boxes, double wslice, void* stream) {
+  if (!p) {
+    set_error("plan is NULL");
+    return ARLMI_E_ARG;
+  }
+  return plan_build(p, d_uvw, d_freq, nrows, h_boxes, wslice, as_stream(stream));
+}
+
+int arlmi_plan_stats_get(const arlmi_plan* p, arlmi_plan_stats* out) {
+  if (!p || !out) {
+    set_error("NULL argument");
+    return ARLMI_E_ARG;
+  }
+  out->nrows = p->nrows;
+  out->nentries = p->nentries;
+  out->ngrid_items = p->ngitems;
+  out->ndegrid_items = p->nditems;
+  out->nmember = p->nmember;
+  out->ndropped = p->ndropped;
+  out->norphans = p->norphans;
+  out->ntiles = p->ntiles;
+  out->tile = p->g.tile;
+  return ARLMI_OK;
+}
+
+int arlmi_plan_plane_w(const arlmi_plan* p, double* h_out, int32_t n) {
+  if (!p || !h_out || n < p->g.nplanes) {
+    set_error("arlmi_plan_plane_w: bad argument");
+    return ARLMI_E_ARG;
+  }
+  ARLMI_HIP(hipMemcpy(h_out, p->d_plane_w, sizeof(double) * p->g.nplanes, hipMemcpyDeviceToHost));
+  return ARLMI_OK;
+}
+
+int arlmi_convolution_mapping(const arlmi_geom* gin, const double* d_uvw, const double* d_freq, int64_t nrows,
+                              int32_t* d_out, void* stream) {
+  DevGeom g;
+  int rc = checked_geom(gin, &g);
+  if (rc) return rc;
+  if (nrows < 0 || (nrows > 0 && (!d_uvw || !d_out))) {
+    set_error("arlmi_convolution_mapping: bad argument");
+    return ARLMI_E_ARG;
+  }
+  if (nrows == 0) return ARLMI_OK;
+  ARLMI_LAUNCH("k_map", k_map, dim3((unsigned)((nrows + 255) / 256)), dim3(256), 0, as_stream(stream), g, d_uvw,
+                     d_freq, nrows, d_out);
+  ARLMI_LAUNCH_CHECK();
+  return ARLMI_OK;
+}
+
+void arlmi_plan_destroy(arlmi_plan* p) {
+  if (!p) return;
+  free_plan(p);
+  delete p;
+}
+
+}  // extern "C"

@@ -1,0 +1,244 @@
+#!/usr/bin/env python3
+"""Benchmark: Mvis/s of w-stacked invert_2d + predict_2d at 2048^2 on one MI355X.
+
+Workload (the w-stacked wide-field observation): 256 antennas log-radial to 10 km,
+8 channels 100-150 MHz, 120 integrations over HA -3h..+3h at dec -30 deg, 200 power-law
+point sources; 2048^2 image at 1.5e-4 rad; 64 w slabs; PSWF kernel S=8, os=8.  Rows whose
+kernel footprint leaves the grid are not generated (the kept count is what is processed).
+
+One step = one pass of the hot path over all rows with the inputs resident in HBM:
+  plan rebuild from uvw (convolution mapping, w-slab membership and means, tile binning)
+  + invert (gridding, shifted inverse FFTs, screens, gcf, sum-of-weights normalisation)
+  + predict (model * gcf * conj(screen), shifted FFTs, degridding).
+The same step is timed in the single precision form (complex64 visibilities and grids, float64
+coordinates / phases / correction) and the double precision form; the reported value is the
+single precision form's whole-job Mvis/s (one row = one channel-visibility through one
+invert and one predict).
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "algorithm-reference-library_amd"))
+
+METRIC = "Mvis/s invert+predict at 2048² w-stack, FP32 vs FP64 GPU, ≤1e-5 rel err"
+
+# HBM bytes each kernel must move per launch (algorithmic, not measured), as functions of
+# N (image side), P (planes), V (rows) and E (gridding entries).
+def _kernel_bytes(name, N, P, V, E, prec):
+    c = 8 if prec == 32 else 16
+    r = c // 2
+    if name == "k_rows_plain":
+        return 2 * c * N * N * P
+    if name == "k_cols_plain":
+        return 2 * c * N * N * P
+    if name == "k_cols_invert":
+        return c * N * N * P + 8 * N * N + r * N * N
+    if name == "k_rows_predict":
+        return r * N * N + 8 * N * N + c * N * N * P
+    if name == "k_grid_sep":
+        return E * (8 + c + r) + c * N * N * P
+    if name == "k_degrid_sep":
+        return c * N * N * P + V * (8 + c)
+    return None
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--npixel", type=int, default=2048)
+    ap.add_argument("--slices", type=int, default=64)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-rows", type=int, default=3000)
+    return ap.parse_args()
+
+
+def step(imager, vis, model, out_img, out_vis):
+    imager.update(vis)
+    imager.invert(vis, out=out_img)
+    imager.predict(vis, model, out=out_vis)
+
+
+def time_form(precision, args, dev):
+    import torch
+    import arlmi
+    from arlmi import simulation
+    vis, im, comps = simulation.config2(precision=precision, npixel=args.npixel)
+    torch.cuda.synchronize()
+    gcf, cf = arlmi.create_pswf_convolutionfunction(im, oversampling=8, support=8)
+    imager = arlmi.Imager(vis, im, gcfcf=(gcf, cf), mode="wstack", vis_slices=args.slices)
+    # model image: the sources at their nearest pixels ('Nearest' insertion,
+    # skycomponent/operations.py:441-444)
+    n = args.npixel
+    cell = im.cellsize
+    model = torch.zeros_like(im.data)
+    for c in comps:
+        x = int(round(n // 2 - c.l / cell))
+        y = int(round(n // 2 + c.m / cell))
+        if 0 <= x < n and 0 <= y < n:
+            model[0, 0, y, x] += float(c.flux[0, 0])
+    out_img = torch.empty_like(im.data)
+    out_vis = torch.empty_like(vis.vis)
+    for _ in range(args.warmup):
+        step(imager, vis, model, out_img, out_vis)
+    torch.cuda.synchronize()
+    L = arlmi._lib.lib()
+    L.arlmi_profile_reset()
+    L.arlmi_profile_enable(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(imager, vis, model, out_img, out_vis)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    L.arlmi_profile_enable(0)
+    prof = arlmi._lib.profile_report()
+    st = imager.plan.stats()
+    res = {
+        "ms": (t1 - t0) * 1e3 / args.steps,
+        "rows": vis.nvis,
+        "rows_total": vis.total_rows,
+        "stats": st,
+        "prof": prof,
+        "img": out_img.double().cpu(),
+        "vis": out_vis.to(torch.complex128).cpu(),
+    }
+    del imager, vis, model, out_img, out_vis
+    torch.cuda.empty_cache()
+    return res
+
+
+def cpu_baseline(args):
+    """The oracle (ARL's per-visibility loops, numpy.fft, per-slab screens) on a bounded sample
+    of the same workload on this host, extrapolated to the whole job."""
+    import numpy
+    sys.path.insert(0, ROOT)
+    from oracle import arl_oracle as O
+    rng = numpy.random.default_rng(0)
+    n, cell, S, os_ = args.npixel, 1.5e-4, 8, 8
+    g = O.ImageGeom(n, cell, frequency=1e8, channel_bandwidth=1e9)
+    gcf, cf, cfg = O.create_pswf_convolutionfunction(g, os_, S)
+    du = 1.0 / (n * cell)
+    lim = (n // 2 - S // 2 - 1) * du
+    m = args.cpu_rows
+    d = {"uvw": numpy.stack([rng.uniform(-lim, lim, m), rng.uniform(-lim, lim, m), numpy.zeros(m)], 1),
+         "vis": rng.normal(size=(m, 1)) + 1j * rng.normal(size=(m, 1)), "imaging_weight": numpy.ones((m, 1)),
+         "frequency": numpy.full(m, 1e8)}
+    t0 = time.perf_counter()
+    grid, _ = O.grid_visibility_to_griddata_loop(d, g, cf, cfg)
+    t1 = time.perf_counter()
+    O.degrid_visibility_from_griddata_loop(d, grid, g, cf, cfg)
+    t2 = time.perf_counter()
+    # one slab of image-plane work: ifft + screen (invert), screen + fft (predict)
+    model = numpy.zeros((1, 1, n, n))
+    t3 = time.perf_counter()
+    img = O.fft_griddata_to_image(grid, gcf, imaginary=True)
+    b = O.create_w_term_like(g, 100.0)
+    _ = b.real * img[0] - b.imag * img[1]
+    b2 = O.create_w_term_like(g, 100.0)
+    O.fft_image_to_griddata(b2.real * model, gcf)
+    O.fft_image_to_griddata(b2.imag * model, gcf)
+    t4 = time.perf_counter()
+    per_row = (t2 - t0) / m
+    per_slab = t4 - t3
+    return per_row, per_slab, m, (t4 - t0)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: every box in this setup has a single MI355X, and the "
+              "multi-GPU path (one all-reduce of partial images) is described, not run.", file=sys.stderr)
+        sys.exit(2)
+    import numpy
+    import torch
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    r32 = time_form(32, args, dev)
+    r64 = None if args.no_fp64 else time_form(64, args, dev)
+    N, P = args.npixel, args.slices
+    rows = r32["rows"]
+    value = rows / (r32["ms"] * 1e-3) / 1e6
+    # dominant kernel and its roofline
+    prof = r32["prof"]
+    E = r32["stats"]["nentries"]
+    dom = max(prof.items(), key=lambda kv: kv[1][0])
+    name, (tot_ms, cnt) = dom
+    per_launch_ms = tot_ms / cnt
+    by = _kernel_bytes(name, N, P, rows, E, 32)
+    roof = None
+    if by is not None:
+        ach = by / (per_launch_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "kernel": name, "achieved": round(ach, 1), "peak": 8000.0, "unit": "GB/s",
+                "frac": round(ach / 8000.0, 4), "traffic": None,
+                "bytes_per_launch": by, "avg_launch_ms": round(per_launch_ms, 4)}
+    extra = {}
+    if r64 is not None:
+        ref_img = r64["img"].numpy()
+        ref_vis = r64["vis"].numpy()
+        extra["fp64_mvis_s"] = round(r64["rows"] / (r64["ms"] * 1e-3) / 1e6, 1)
+        extra["fp64_ms_per_step"] = round(r64["ms"], 3)
+        extra["fp32_over_fp64"] = round(value / extra["fp64_mvis_s"], 2)
+        a = r32["img"].numpy()
+        q = N // 4
+        inner = (slice(None), slice(None), slice(q, N - q), slice(q, N - q))
+        extra["rel_image_err_vs_fp64"] = {
+            "l2": float(numpy.linalg.norm(a - ref_img) / numpy.linalg.norm(ref_img)),
+            "max_inner_half": float(numpy.max(numpy.abs(a[inner] - ref_img[inner])) /
+                                    numpy.max(numpy.abs(ref_img[inner]))),
+            "max_full": float(numpy.max(numpy.abs(a - ref_img)) / numpy.max(numpy.abs(ref_img))),
+        }
+        v = r32["vis"].numpy()
+        extra["rel_vis_err_vs_fp64"] = {
+            "l2": float(numpy.linalg.norm(v - ref_vis) / numpy.linalg.norm(ref_vis)),
+            "max": float(numpy.max(numpy.abs(v - ref_vis)) / numpy.max(numpy.abs(ref_vis))),
+        }
+    cpu = None
+    if not args.no_cpu:
+        per_row, per_slab, m, wall = cpu_baseline(args)
+        cpu_s = per_row * rows + per_slab * P
+        cpu = {"value": round(rows / cpu_s / 1e6, 5), "unit": "Mvis/s", "cores": 1, "kind": "port",
+               "sample": "oracle with ARL's per-visibility grid and degrid loops on %d rows (%.2f us/row) plus one "
+                         "slab of %d^2 ifft/fft and screens (%.2f s/slab), extrapolated to %d rows and %d slabs; "
+                         "%.1f s measured" % (m, per_row * 1e6, N, per_slab, rows, P, wall)}
+    kernels = {k: {"ms_per_step": round(v[0] / args.steps, 4), "launches_per_step": v[1] / args.steps}
+               for k, v in sorted(prof.items(), key=lambda kv: -kv[1][0])}
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "Mvis/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r32["ms"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "w-stacked wide field: 256 antennas log-radial to 10 km, 8 channels 100-150 MHz, "
+                        "120 integrations, 200 power-law sources; %d^2 image, cell 1.5e-4 rad, %d w slabs, "
+                        "PSWF S=8 os=8" % (N, P),
+            "rows": rows,
+            "rows_generated_before_uv_selection": r32["rows_total"],
+            "baseline_times": 32640 * 120,
+            "entries": E,
+            **extra,
+            "kernels_fp32": kernels,
+        },
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

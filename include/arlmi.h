@@ -1,0 +1,190 @@
+/*
+ * arlmi.h - C ABI of libarlmi.so, the MI355X (gfx950) implementation of ARL's
+ * Fourier-processing hot path: convolutional gridding / degridding with
+ * oversampled kernels, w-stacking phase screens, the shifted 2-D FFT and the
+ * gridding correction.
+ *
+ * Conventions
+ *   - Every pointer named d_* is DEVICE memory owned by the caller; the library
+ *     keeps no reference to it after the call returns.
+ *   - Work is enqueued on `stream` (a hipStream_t, NULL = default stream) and is
+ *     stream-ordered; calls return once the work is enqueued unless stated.
+ *   - Return value: ARLMI_OK (0) or a negative ARLMI_E_* code; the message of the
+ *     last failure on the calling thread is returned by arlmi_last_error().
+ *   - precision: 32 -> complex64 visibilities / grids, float32 weights and images;
+ *                64 -> complex128 / float64 throughout.  uvw, frequencies, w-slab
+ *     centres, grid-correction images and sums of weights are always float64.
+ *   - Complex numbers are interleaved (re, im) pairs, as numpy/torch store them.
+ *
+ * The ARL interface each entry point replaces is cited per function
+ * (file:line in SKA-ScienceDataProcessor/algorithm-reference-library).  The
+ * deprecated ARL C FFI (deprecated_code/ffiwrappers/include/arlwrap.h:112-113)
+ * exposes the same operation pair as arl_invert_2d / arl_predict_2d; its
+ * global-error-flag convention (arlwrap.py:41-47) is what arlmi_last_error mirrors.
+ */
+#ifndef ARLMI_H
+#define ARLMI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ARLMI_OK 0
+#define ARLMI_E_ARG (-1)          /* invalid argument / shape */
+#define ARLMI_E_UV_OFF_GRID (-2)  /* gridding.py:43-46 (and footprint over the edge, :128, :264) */
+#define ARLMI_E_W_RANGE (-3)      /* gridding.py:64-65, 75-76 */
+#define ARLMI_E_FREQ_MAP (-4)     /* gridding.py:83-84 */
+#define ARLMI_E_ODD_SUPPORT (-5)  /* gridding.py:125-128 needs an even support */
+#define ARLMI_E_SUBSAMPLE (-6)    /* pu_offset == oversampling (gridding.py:53-54 -> IndexError) */
+#define ARLMI_E_HIP (-7)          /* a HIP runtime call failed */
+#define ARLMI_E_UNSUPPORTED (-8)  /* size / layout not handled by this build */
+
+#define ARLMI_MODE_2D 0      /* planes chosen by rounding w onto the grid's WW axis (nw planes, summed) */
+#define ARLMI_MODE_WSTACK 1  /* planes are w slabs |w - box_k| < wslice/2, w shifted by the slab mean */
+
+#define ARLMI_FLAG_DROP_OFF_GRID 1u /* drop (and count) rows whose kernel footprint leaves the grid */
+
+#define ARLMI_KERNEL_SEPARABLE 0 /* real separable table k[os][S]: CF[yo][xo][a][b] = k[yo][a]*k[xo][b] */
+#define ARLMI_KERNEL_TABLE 1     /* full complex table CF[nz][os][os][S][S] (ARL layout, chan 0, pol 0) */
+
+/* Geometry of one imaging problem, carrying the WCS numbers ARL stores
+ * (griddata/operations.py:91-145, griddata/convolution_functions.py:62-147,
+ * processing_library/image/operations.py:33-69).  Fill it with arlmi_geom_init
+ * semantics: every double is computed on the host exactly as ARL computes it. */
+typedef struct arlmi_geom {
+  int32_t npixel;       /* N: image side = grid side (power of two, 64..8192) */
+  int32_t nchan;        /* image channels */
+  int32_t support;      /* S, even */
+  int32_t oversampling; /* os */
+  int32_t nz;           /* kernel w planes (1 for PSWF) */
+  int32_t nplanes;      /* MODE_WSTACK: number of w slabs; MODE_2D: grid WW planes nw */
+  int32_t mode;         /* ARLMI_MODE_* */
+  int32_t tile;         /* LDS tile side T (32 or 64); 0 = library default */
+  double du, dv;        /* UU, VV cdelt = 1/(N*d2r*cdelt_deg) (wavelengths per pixel) */
+  double ddu, ddv;      /* DUU, DVV cdelt = du/os, dv/os */
+  double crpix_uv;      /* N//2 + 1 */
+  double crpix_d;       /* os//2 + 1 */
+  double zstep;         /* kernel WW cdelt */
+  double crpix_z;       /* nz//2 + 1 */
+  double wstep_grid;    /* grid WW cdelt (MODE_2D) */
+  double crpix_wg;      /* nw//2 + 1 (MODE_2D) */
+  double freq0, dfreq;  /* image FREQ crval and cdelt (crpix 1) */
+  double cellsize;      /* abs(cdelt0_deg)*pi/180, the w-screen pixel size (image/operations.py:389) */
+} arlmi_geom;
+
+/* Counts describing a plan (all host values). */
+typedef struct arlmi_plan_stats {
+  int64_t nrows;        /* rows given */
+  int64_t nentries;     /* (row, plane, tile) gridding entries */
+  int64_t ngrid_items;  /* gridding work items (tile chunks) */
+  int64_t ndegrid_items;
+  int64_t nmember;      /* (row, plane) memberships */
+  int64_t ndropped;     /* rows dropped by ARLMI_FLAG_DROP_OFF_GRID */
+  int64_t norphans;     /* rows in no plane (predicted as 0) */
+  int32_t ntiles;       /* planes * nchan * (N/T)^2 */
+  int32_t tile;
+} arlmi_plan_stats;
+
+/* Convolution kernel description. */
+typedef struct arlmi_kernel {
+  int32_t kind;         /* ARLMI_KERNEL_* */
+  int32_t nz, oversampling, support;
+  const void* d_table;  /* SEPARABLE: real [os][S]; TABLE: complex [nz][os][os][S][S]; precision of the call */
+} arlmi_kernel;
+
+typedef struct arlmi_plan arlmi_plan; /* opaque; owns its device buffers */
+
+int arlmi_version(void);
+const char* arlmi_last_error(void);
+
+/* max |w| over rows: numpy.max(numpy.abs(vis.w)) in vis_wslice_iter
+ * (processing_components/visibility/iterators.py:99).  Synchronises `stream`. */
+int arlmi_wmax(const double* d_uvw, int64_t nrows, void* stream, double* wmax_out);
+
+/* Build the gridding plan: per-row convolution mapping (gridding.py:24-96), w-slab
+ * membership (iterators.py:91-109, boxes = numpy.linspace(-wmax, wmax, P) given by the
+ * host in h_boxes, wslice = boxes[1]-boxes[0]), slab mean w (wstack_single.py:43, :83),
+ * and the tile-sorted entry lists the grid/degrid kernels walk.  d_freq may be NULL
+ * (all rows in image channel 0).  Synchronises `stream`.  Returns an ARLMI_E_* code if
+ * a row violates one of ARL's asserts and ARLMI_FLAG_DROP_OFF_GRID does not cover it. */
+int arlmi_plan_create(const arlmi_geom* g, const double* d_uvw, const double* d_freq, int64_t nrows,
+                      const double* h_boxes, double wslice, uint32_t flags, void* stream,
+                      arlmi_plan** plan_out);
+/* Rebuild an existing plan for new uvw (same geometry), reusing its buffers when they fit. */
+int arlmi_plan_update(arlmi_plan* plan, const double* d_uvw, const double* d_freq, int64_t nrows,
+                      const double* h_boxes, double wslice, void* stream);
+int arlmi_plan_stats_get(const arlmi_plan* plan, arlmi_plan_stats* out);
+/* Host copy of the per-plane mean w (numpy.average(vis.w) per slab; 0 in MODE_2D). */
+int arlmi_plan_plane_w(const arlmi_plan* plan, double* h_out, int32_t n);
+void arlmi_plan_destroy(arlmi_plan* plan);
+
+/* convolution_mapping (gridding.py:24-96) for every row with 2-D semantics: d_out int32
+ * [nrows][8] = {pu, pu_offset, pv, pv_offset, pwg, pwc, pfreq, err_bits}; err_bits: 1 uv or
+ * footprint off the grid, 2 w plane out of range, 4 frequency out of range, 8 sub-sample index
+ * equal to the oversampling.  A negative floor() sub-sample is wrapped like a numpy index. */
+int arlmi_convolution_mapping(const arlmi_geom* g, const double* d_uvw, const double* d_freq, int64_t nrows,
+                              int32_t* d_out, void* stream);
+
+/* grid_visibility_to_griddata (gridding.py:99-133) for every plane of the plan:
+ * d_grids[plane][chan][N][N] (complex) = sum conj(CF) * vis * imaging_weight, and
+ * d_sumwt[plane][chan] (float64) = sum imaging_weight.  dopsf replaces vis by 1
+ * (imaging/base.py:162-163).  Both outputs are overwritten. */
+int arlmi_grid(const arlmi_plan* plan, const void* d_vis, const void* d_weight, int dopsf,
+               const arlmi_kernel* kernel, void* d_grids, double* d_sumwt, int precision, void* stream);
+
+/* degrid_visibility_from_griddata (gridding.py:233-267): d_vis_out[row] = sum CF * G
+ * (no conjugation), reading the plane each row belongs to.  Rows in no plane get 0
+ * (imaging_serial.py:92 zero copy); a row in two w slabs keeps the later slab's value
+ * (imaging_serial.py:97). */
+int arlmi_degrid(const arlmi_plan* plan, const void* d_grids, const arlmi_kernel* kernel, void* d_vis_out,
+                 int precision, void* stream);
+
+/* fft_griddata_to_image (gridding.py:270-289) fused with the w-stack combination
+ * (wstack_single.py:93-94), the per-slab sumwt weighting and normalize_sumwt
+ * (imaging_serial.py:193-196, imaging/base.py:84-101):
+ *   img[c] = gcf[c] / S_c * sum_planes Re(b_p * N^2 ifft(G[p][c]))   (b_p = 1 in MODE_2D)
+ *   S_c = sum over planes with sumwt[p][c] > 0 of sumwt[p][c]; img = 0 if S_c <= 0.
+ * normalize = 0 skips the division.  d_img_im (may be NULL, MODE_2D only) receives the
+ * imaginary part (invert_2d imaginary=True).  d_grids is used as scratch (destroyed).
+ * d_gcf: float64 [nchan][N][N]. */
+int arlmi_grids_to_image(const arlmi_plan* plan, void* d_grids, const double* d_sumwt, const double* d_gcf,
+                         int normalize, void* d_img_re, void* d_img_im, int precision, void* stream);
+
+/* fft_image_to_griddata (gridding.py:292-302) fused with the w-stack model screen
+ * (wstack_single.py:50-59): G[p][c] = fft(model[c] * gcf[c] * conj(b_p)); b_p = 1 in MODE_2D. */
+int arlmi_image_to_grids(const arlmi_plan* plan, const void* d_model, const double* d_gcf, void* d_grids,
+                         int precision, void* stream);
+
+/* ARL fft / ifft (processing_library/fourier_transforms/fft_support.py:21-89) in place on
+ * a batch of complex N x N planes: fftshift(fft2(ifftshift(a))) for direction = +1 and
+ * fftshift(ifft2(ifftshift(a))) (1/N^2 normalised) for direction = -1. */
+int arlmi_fft2d(void* d_data, int32_t npixel, int64_t nbatch, int direction, int precision, void* stream);
+
+/* w-term screen (processing_library/fourier_transforms/convolutional_gridding.py:120-188 via
+ * create_w_term_like, image/operations.py:364-408): d_out[N][N] complex =
+ * exp(-2 pi i w (1 - sqrt(1 - r^2))), r^2 = fov^2 ((y-cy)^2 + (x-cx)^2)/N^2, fov = N*cellsize,
+ * centre cx = cy = N//2; 0 where r^2 >= 1. */
+int arlmi_w_screen(int32_t npixel, double cellsize, double w, void* d_out, int precision, void* stream);
+
+/* Direct DFT of point components (predict_skycomponent_visibility, imaging/base.py:207-220;
+ * simulate_point, coordinate_support.py:192-212): d_vis[row] += sum_k flux[k][chan(row)] *
+ * exp(-2 pi i (u l_k + v m_k + w (sqrt(1 - l_k^2 - m_k^2) - 1))).  d_lmflux: float64 [ncomp][2 + nchan]
+ * (l, m, flux per image channel).  d_chan: int32 [nrows] image channel of each row, or NULL (0).
+ * Phases are evaluated in float64 in both precisions. */
+int arlmi_dft(const double* d_uvw, const int32_t* d_chan, int64_t nrows, const double* d_lmflux, int32_t ncomp,
+              int32_t nchan, void* d_vis, int precision, void* stream);
+
+/* Opt-in kernel timing: with profiling on, every library kernel launch is bracketed by two
+ * HIP events on its stream.  collect() waits for them and adds the elapsed times per kernel;
+ * report() writes "kernel total_ms launches" lines into buf and returns the size needed. */
+int arlmi_profile_enable(int on);
+int arlmi_profile_collect(void);
+int arlmi_profile_reset(void);
+int arlmi_profile_report(char* buf, int cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ARLMI_H */

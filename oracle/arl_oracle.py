@@ -1,0 +1,822 @@
+"""FP64 NumPy restatement of ARL's Fourier-processing hot path.
+
+THIS MODULE IS TEST INFRASTRUCTURE (the oracle). Only ``tests/``,
+``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` may
+import it, and only as the checker of the HIP product path or as the timed CPU
+baseline. The product package ``arlmi`` never imports it and never falls back
+to it.
+
+It restates, in complex128/float64 NumPy, the functions of the
+SKA-ScienceDataProcessor/algorithm-reference-library (ARL) hot path.  Each
+function names the reference file:line it follows.  ARL itself cannot be
+imported here (its data models need astropy, which is absent), so the WCS
+arithmetic that ARL delegates to astropy/wcslib is written out explicitly, in
+the same operation order wcslib uses for linear axes:
+
+    world -> pixel (origin 0):  pix = (world - crval) / cdelt + crpix - 1
+    pixel -> world (origin 0):  world = cdelt * ((pix + 1) - crpix) + crval
+
+Pinning (what checks each function; see tests/test_oracle_*.py):
+  grdsf, create_pswf_convolutionfunction   ARL known answers
+      (tests/processing_components/test_griddata_kernels.py:60-101)
+  create_box_convolutionfunction           ARL known answer (:45-58)
+  create_awterm_convolutionfunction (w only), apply_bounding_box_convolutionfunction
+                                           ARL known answers and symmetries (:104-139, :235-296)
+  w_beam, coordinates*                     ARL known answers
+      (tests/processing_library/test_convolutional_gridding.py:21-44) + closed form
+  fft / ifft                               numpy.fft identity + direct DFT on small arrays
+  convolution_mapping, grid / degrid, fft_griddata_to_image, fft_image_to_griddata,
+  invert_2d, predict_2d, normalize_sumwt   brute-force direct sums on tiny inputs, the
+                                           adjoint identity, box == nearest-pixel gridding,
+                                           point-source agreement with the direct DFT
+  vis_wslice_iter, *_wstack_single, w-stack list workflows
+                                           closed form of the slab partition, adjoint
+                                           identity, direct DFT with w != 0
+  simulate_point / predict_skycomponent_visibility
+                                           ARL known answers
+      (tests/processing_library/test_coordinate_support.py:104-123)
+  xyz_to_uvw, xyz_at_latitude, hadec_to_azel
+                                           closed forms (rotation invariants, known angles)
+"""
+
+import numpy
+
+__all__ = [
+    "coordinateBounds", "coordinates", "coordinates2", "coordinates2Offset", "grdsf", "w_beam",
+    "fft", "ifft", "ImageGeom", "create_image_geom", "create_pswf_convolutionfunction",
+    "create_box_convolutionfunction", "create_awterm_convolutionfunction",
+    "apply_bounding_box_convolutionfunction", "calculate_bounding_box_convolutionfunction",
+    "convolution_mapping", "grid_visibility_to_griddata", "grid_visibility_to_griddata_loop",
+    "grid_visibility_to_griddata_fast", "degrid_visibility_from_griddata",
+    "degrid_visibility_from_griddata_loop", "fft_griddata_to_image", "fft_image_to_griddata",
+    "normalize_sumwt", "invert_2d", "predict_2d", "create_w_term_like", "vis_wslice_iter",
+    "invert_wstack_single", "predict_wstack_single", "invert_list_wstack", "predict_list_wstack",
+    "simulate_point", "predict_skycomponent_visibility", "xyz_at_latitude", "xyz_to_uvw",
+    "hadec_to_azel", "create_visibility", "apply_gains_rows",
+]
+
+D2R = numpy.pi / 180.0
+C_LIGHT = 299792458.0  # astropy constants.c in m/s
+
+
+# ---------------------------------------------------------------------------
+# processing_library/fourier_transforms/convolutional_gridding.py
+# ---------------------------------------------------------------------------
+
+def coordinateBounds(npixel):
+    """convolutional_gridding.py:18-34."""
+    if npixel % 2 == 0:
+        return -0.5, 0.5 * (npixel - 2) / npixel
+    return -0.5 * (npixel - 1) / npixel, 0.5 * (npixel - 1) / npixel
+
+
+def coordinates(npixel):
+    """convolutional_gridding.py:37-41: (arange(n) - n//2)/n."""
+    return (numpy.arange(npixel) - npixel // 2) / npixel
+
+
+def coordinates2(npixel):
+    """convolutional_gridding.py:44-50."""
+    return (numpy.mgrid[0:npixel, 0:npixel] - npixel // 2) / npixel
+
+
+def coordinates2Offset(npixel, cx, cy, quadrant=False):
+    """convolutional_gridding.py:53-70."""
+    if cx is None:
+        cx = npixel // 2
+    if cy is None:
+        cy = npixel // 2
+    if not quadrant:
+        mg = numpy.mgrid[0:npixel, 0:npixel]
+    else:
+        mg = numpy.mgrid[0:npixel // 2 + 1, 0:npixel // 2 + 1]
+    return (mg[0] - cy) / npixel, (mg[1] - cx) / npixel
+
+
+_GRDSF_P = numpy.array([[8.203343e-2, -3.644705e-1, 6.278660e-1, -5.335581e-1, 2.312756e-1],
+                        [4.028559e-3, -3.697768e-2, 1.021332e-1, -1.201436e-1, 6.412774e-2]])
+_GRDSF_Q = numpy.array([[1.0000000e0, 8.212018e-1, 2.078043e-1],
+                        [1.0000000e0, 9.599102e-1, 2.918724e-1]])
+
+
+def grdsf(nu):
+    """Schwab's rational PSWF approximation (m=6, alpha=1).
+
+    convolutional_gridding.py:72-117 (coefficients 83-86).  Returns
+    (psi(nu), (1 - nu**2) * psi(nu)); both are 0 for |nu| > 1.
+    """
+    nu = numpy.abs(numpy.asarray(nu, dtype=float))
+    nuend = numpy.zeros_like(nu)
+    part = numpy.zeros(nu.shape, dtype=int)
+    part[(nu >= 0.0) & (nu < 0.75)] = 0
+    part[(nu >= 0.75) & (nu <= 1.0)] = 1
+    nuend[(nu >= 0.0) & (nu < 0.75)] = 0.75
+    nuend[(nu >= 0.75) & (nu <= 1.0)] = 1.0
+    delnusq = nu ** 2 - nuend ** 2
+    top = _GRDSF_P[part, 0]
+    for k in range(1, _GRDSF_P.shape[1]):
+        top += _GRDSF_P[part, k] * numpy.power(delnusq, k)
+    bot = _GRDSF_Q[part, 0]
+    for k in range(1, _GRDSF_Q.shape[1]):
+        bot += _GRDSF_Q[part, k] * numpy.power(delnusq, k)
+    psi = numpy.zeros_like(nu)
+    ok = bot > 0.0
+    psi[ok] = top[ok] / bot[ok]
+    psi[nu > 1.0] = 0.0
+    return psi, (1 - nu ** 2) * psi
+
+
+def w_beam(npixel, field_of_view, w, cx=None, cy=None, remove_shift=False):
+    """W-term phase screen exp(-2 pi i w (1 - sqrt(1 - r^2))), 0 outside the unit circle.
+
+    convolutional_gridding.py:120-188 (the quadrant + reflect-pad form, 172-184).
+    """
+    if cx is None:
+        cx = npixel // 2
+    if cy is None:
+        cy = npixel // 2
+    ly, mx = coordinates2Offset(npixel, cx, cy, quadrant=True)
+    r2 = field_of_view ** 2 * (ly ** 2 + mx ** 2)
+    with numpy.errstate(invalid="ignore"):
+        ph = -2 * numpy.pi * w * (1 - numpy.sqrt(1.0 - r2))
+    numpy.putmask(ph, r2 >= 1.0, 0)
+    cp = numpy.exp(1j * ph)
+    numpy.putmask(cp, r2 >= 1.0, 0 + 0j)
+    numpy.putmask(cp, r2 == 0, 1.0 + 0j)
+    if remove_shift:
+        cp /= cp[-1, -1]
+    cp = numpy.pad(cp, ((0, int(cx) + npixel % 2 - 1), (0, int(cy) + npixel % 2 - 1)), 'reflect')
+    return cp
+
+
+# ---------------------------------------------------------------------------
+# processing_library/fourier_transforms/fft_support.py (numpy branch)
+# ---------------------------------------------------------------------------
+
+def fft(a):
+    """Image -> grid: fftshift(fft2(ifftshift(a))) over the last two axes.
+
+    fft_support.py:21-38 (numpy branch used when pyfftw is absent).
+    """
+    if a.ndim == 4:
+        return numpy.fft.fftshift(numpy.fft.fft2(numpy.fft.ifftshift(a, axes=[2, 3])), axes=[2, 3])
+    if a.ndim == 5:
+        return numpy.fft.fftshift(numpy.fft.fft2(numpy.fft.ifftshift(a, axes=[3, 4])), axes=[3, 4])
+    return numpy.fft.fftshift(numpy.fft.fft2(numpy.fft.ifftshift(a)))
+
+
+def ifft(a):
+    """Grid -> image: fftshift(ifft2(ifftshift(a))) over the last two axes (1/N^2 normalised).
+
+    fft_support.py:55-71.
+    """
+    if a.ndim == 4:
+        return numpy.fft.fftshift(numpy.fft.ifft2(numpy.fft.ifftshift(a, axes=[2, 3])), axes=[2, 3])
+    if a.ndim == 5:
+        return numpy.fft.fftshift(numpy.fft.ifft2(numpy.fft.ifftshift(a, axes=[2, 3, 4])), axes=[2, 3, 4])
+    return numpy.fft.fftshift(numpy.fft.ifft2(numpy.fft.ifftshift(a)))
+
+
+# ---------------------------------------------------------------------------
+# Image / grid geometry (the WCS values ARL stores and the arithmetic astropy
+# performs on them).  processing_library/image/operations.py:33-69,
+# processing_components/imaging/base.py:326-334,
+# processing_components/griddata/operations.py:91-145,
+# processing_components/griddata/convolution_functions.py:62-147.
+# ---------------------------------------------------------------------------
+
+class ImageGeom:
+    """The numbers of ARL's image WCS that the hot path reads.
+
+    cdelt[0] = -cell*180/pi, cdelt[1] = +cell*180/pi (degrees), crpix = N//2+1,
+    FREQ axis: crval = frequency, cdelt = channel_bandwidth, crpix = 1
+    (image/operations.py:61-64; imaging/base.py:328-333).
+    """
+
+    def __init__(self, npixel, cellsize, nchan=1, npol=1, frequency=1e8, channel_bandwidth=1e6):
+        self.npixel = int(npixel)
+        self.nchan = int(nchan)
+        self.npol = int(npol)
+        self.cellsize = float(cellsize)
+        self.cdelt0 = -cellsize * 180.0 / numpy.pi
+        self.cdelt1 = cellsize * 180.0 / numpy.pi
+        self.crpix0 = float(npixel // 2 + 1)
+        self.crpix1 = float(npixel // 2 + 1)
+        self.freq0 = float(frequency)
+        self.dfreq = float(channel_bandwidth)
+
+    @property
+    def shape(self):
+        return (self.nchan, self.npol, self.npixel, self.npixel)
+
+    def du(self):
+        """UU cdelt of the grid: 1/(N * d2r * cdelt0) (griddata/operations.py:134)."""
+        return 1.0 / (self.npixel * D2R * self.cdelt0)
+
+    def dv(self):
+        """VV cdelt of the grid (griddata/operations.py:135)."""
+        return 1.0 / (self.npixel * D2R * self.cdelt1)
+
+    def wcs_cellsize(self):
+        """abs(cdelt0)*pi/180 as create_w_term_like reads it (image/operations.py:389)."""
+        return abs(self.cdelt0) * numpy.pi / 180.0
+
+
+def create_image_geom(npixel, cellsize, nchan=1, frequency=1e8, channel_bandwidth=1e6):
+    return ImageGeom(npixel, cellsize, nchan=nchan, frequency=frequency,
+                     channel_bandwidth=channel_bandwidth)
+
+
+class CFGeom:
+    """WCS numbers of a ConvolutionFunction (convolution_functions.py:115-131)."""
+
+    def __init__(self, geom, oversampling, support, nz=1, zstep=1e15):
+        self.oversampling = int(oversampling)
+        self.support = int(support)
+        self.nz = int(nz)
+        self.zstep = float(zstep)
+        self.crpix_z = float(nz // 2 + 1)
+        self.du = 1.0 / (geom.npixel * D2R * geom.cdelt0)
+        self.dv = 1.0 / (geom.npixel * D2R * geom.cdelt1)
+        self.ddu = self.du / oversampling
+        self.ddv = self.dv / oversampling
+        self.crpix_d = float(oversampling // 2 + 1)
+
+
+# ---------------------------------------------------------------------------
+# processing_components/griddata/kernels.py
+# ---------------------------------------------------------------------------
+
+def create_pswf_convolutionfunction(geom, oversampling=8, support=6):
+    """PSWF gridding kernel and grid-correction image.
+
+    kernels.py:52-97.  Returns (gcf [nchan,npol,N,N] float64,
+    cf [nchan,npol,1,os,os,S,S] complex128, cfgeom).
+    """
+    kernel = numpy.zeros([oversampling, support])
+    for grid in range(support):
+        for subsample in range(oversampling):
+            nu = ((grid - support // 2) - (subsample - oversampling // 2) / oversampling)
+            kernel[subsample, grid] = grdsf([nu / (support // 2)])[1][0]
+    kernel /= numpy.sum(numpy.real(kernel[oversampling // 2, :]))
+    nchan, npol = geom.nchan, geom.npol
+    cf = numpy.zeros([nchan, npol, 1, oversampling, oversampling, support, support]).astype('complex')
+    for y in range(oversampling):
+        for x in range(oversampling):
+            cf[:, :, 0, y, x, :, :] = numpy.outer(kernel[y, :], kernel[x, :])[numpy.newaxis, numpy.newaxis, ...]
+    norm = numpy.sum(numpy.real(cf[0, 0, 0, 0, 0, :, :]))
+    cf /= norm
+    nx = geom.npixel
+    nu = numpy.abs(2.0 * coordinates(nx))
+    gcf1d = grdsf(nu)[0]
+    gcf = numpy.outer(gcf1d, gcf1d)
+    gcf[gcf > 0.0] = gcf.max() / gcf[gcf > 0.0]
+    gcf_data = numpy.zeros(geom.shape)
+    gcf_data[...] = gcf[numpy.newaxis, numpy.newaxis, ...]
+    return gcf_data, cf, CFGeom(geom, oversampling, support)
+
+
+def create_box_convolutionfunction(geom):
+    """Box kernel: delta at (...,2,2) of an S=4, os=1 CF; gcf = 1/sinc^2 (kernels.py:19-49)."""
+    cf = numpy.zeros([geom.nchan, geom.npol, 1, 1, 1, 4, 4], dtype='complex')
+    cf[..., 2, 2] = 1.0 + 0.0j
+    nu = numpy.abs(coordinates(geom.npixel))
+    gcf1d = numpy.sinc(nu)
+    gcf = 1.0 / numpy.outer(gcf1d, gcf1d)
+    gcf_data = numpy.zeros(geom.shape)
+    gcf_data[...] = gcf[numpy.newaxis, numpy.newaxis, ...]
+    return gcf_data, cf, CFGeom(geom, 1, 4)
+
+
+def create_w_term_like(geom, w, remove_shift=False, cellsize_geom=None, npol=1):
+    """Image of the w-term screen centred on crpix (image/operations.py:364-408).
+
+    cellsize = abs(cdelt0)*pi/180 as read back from the WCS (389); the centre is
+    crpix-1 because the phasecentre argument is never a SkyCoord class (391-394).
+    """
+    g = geom if cellsize_geom is None else cellsize_geom
+    cellsize = g.wcs_cellsize()
+    npixel = geom.npixel
+    temp = w_beam(npixel, npixel * cellsize, w=w, cx=geom.crpix0 - 1.0, cy=geom.crpix1 - 1.0,
+                  remove_shift=remove_shift)
+    out = numpy.zeros([geom.nchan, npol, npixel, npixel], dtype='complex')
+    out[:, :, ...] = temp
+    return out
+
+
+def create_awterm_convolutionfunction(geom, nw=1, wstep=1e15, oversampling=8, support=6, use_aaf=True,
+                                      maxsupport=512, planes=None):
+    """W-only AW (w-projection) kernel, make_pb=None (kernels.py:100-200).
+
+    ``planes`` optionally restricts the (expensive) per-plane FFTs to a subset of
+    z indices; the normalisation always uses plane nw//2 (kernels.py:191).
+    Returns (gcf, cf, cfgeom).
+    """
+    nchan, npol, onx = geom.nchan, geom.npol, geom.npixel
+    cfg = CFGeom(geom, oversampling, support, nz=nw, zstep=wstep)
+    if abs(wstep) > 0.0:
+        # wcs_pix2world(range(nw), 0) on the WW axis: cdelt*((z+1) - crpix) + 0
+        w_list = [wstep * ((z + 1.0) - cfg.crpix_z) + 0.0 for z in range(nw)]
+    else:
+        w_list = [0.0]
+    nx = max(maxsupport, 2 * oversampling * support)
+    ny = nx
+    qnx = nx // oversampling
+    qny = ny // oversampling
+    cf = numpy.zeros([nchan, npol, nw, oversampling, oversampling, support, support], dtype='complex')
+    ccell = onx * numpy.abs(D2R * geom.cdelt0) / qnx
+    # sub-image geometry: cdelt0 = -ccell/d2r, crpix = qnx//2+1
+    sub = ImageGeom(qnx, 1.0, nchan=nchan, npol=npol)
+    sub.cdelt0 = -ccell / D2R
+    sub.cdelt1 = +ccell / D2R
+    sub.crpix0 = float(qnx // 2 + 1)
+    sub.crpix1 = float(qny // 2 + 1)
+    if use_aaf:
+        # create_pswf_convolutionfunction(subim, oversampling=1, support=6) gcf only
+        nu = numpy.abs(2.0 * coordinates(qnx))
+        g1 = grdsf(nu)[0]
+        g2 = numpy.outer(g1, g1)
+        g2[g2 > 0.0] = g2.max() / g2[g2 > 0.0]
+        norm = 1.0 / g2[numpy.newaxis, numpy.newaxis, ...]
+    else:
+        norm = 1.0
+    z_todo = range(len(w_list)) if planes is None else sorted(set(list(planes) + [nw // 2]))
+    ycen, xcen = ny // 2, nx // 2
+    for z in z_todo:
+        w = w_list[z]
+        thisplane = create_w_term_like(sub, w, npol=npol)
+        thisplane = thisplane * norm
+        padded = numpy.zeros([nchan, npol, ny, nx], dtype='complex')
+        ystart = ny // 2 - qny // 2
+        xstart = nx // 2 - qnx // 2
+        padded[..., ystart:ystart + qny, xstart:xstart + qnx] = thisplane
+        padded = ifft(padded)  # fft_image on a RA/DEC image uses ifft (image/operations.py:293)
+        for y in range(oversampling):
+            ybeg = y + ycen + (support * oversampling) // 2 - oversampling // 2
+            yend = y + ycen - (support * oversampling) // 2 - oversampling // 2
+            for x in range(oversampling):
+                xbeg = x + xcen + (support * oversampling) // 2 - oversampling // 2
+                xend = x + xcen - (support * oversampling) // 2 - oversampling // 2
+                cf[..., z, y, x, :, :] = padded[..., ybeg:yend:-oversampling, xbeg:xend:-oversampling]
+    cf /= numpy.sum(numpy.real(cf[0, 0, nw // 2, oversampling // 2, oversampling // 2, :, :]))
+    cf = numpy.conjugate(cf)
+    if use_aaf:
+        gcf, _, _ = create_pswf_convolutionfunction(geom, oversampling=1, support=6)
+    else:
+        gcf = numpy.ones(geom.shape)
+    return gcf, cf, cfg
+
+
+def apply_bounding_box_convolutionfunction(cf, fractional_level=1e-4):
+    """Clip a CF to |cf| > level*max (convolution_functions.py:159-184). Returns the data only."""
+    nx = cf.shape[-1]
+    mask = numpy.max(numpy.abs(cf), axis=(0, 1, 2, 3, 4))
+    coords = numpy.argwhere(mask > fractional_level * numpy.max(numpy.abs(cf)))
+    crpx = int(numpy.round(float(nx // 2) + 1.0))
+    crpy = crpx
+    x0, y0 = coords.min(axis=0)
+    dx = crpx - x0
+    dy = crpy - y0
+    x0 -= 1
+    y0 -= 1
+    x1 = crpx + dx - 1
+    y1 = crpy + dy - 1
+    return cf[..., y0:y1, x0:x1]
+
+
+def calculate_bounding_box_convolutionfunction(cf, fractional_level=1e-4):
+    """Per-z bounding boxes (convolution_functions.py:187-207)."""
+    bboxes = []
+    threshold = fractional_level * numpy.max(numpy.abs(cf))
+    for z in range(cf.shape[2]):
+        mask = numpy.max(numpy.abs(cf[:, :, z, ...]), axis=(0, 1, 2, 3))
+        coords = numpy.argwhere(mask > threshold)
+        x0, y0 = coords.min(axis=0)
+        x1, y1 = coords.max(axis=0)
+        bboxes.append((z, (y0, y1), (x0, x1)))
+    return bboxes
+
+
+# ---------------------------------------------------------------------------
+# processing_components/griddata/gridding.py
+# ---------------------------------------------------------------------------
+
+def convolution_mapping(uvw, frequency, geom, cfg, nw_grid=1, wstep_grid=1e15, nchan_cf=None):
+    """Per-visibility grid/kernel indices (gridding.py:24-96).
+
+    Raises AssertionError exactly where ARL asserts (43-46, 64-65, 75-76, 83-84).
+    Returns pu, pu_off, pv, pv_off, pwg, pwc, pfreq (int arrays).
+    """
+    n = geom.npixel
+    du = 1.0 / (n * D2R * geom.cdelt0)
+    dv = 1.0 / (n * D2R * geom.cdelt1)
+    crpu = float(n // 2 + 1)
+    crpv = float(n // 2 + 1)
+    x = (uvw[:, 0] / du + crpu) - 1.0
+    y = (uvw[:, 1] / dv + crpv) - 1.0
+    pu = numpy.round(x).astype('int')
+    pv = numpy.round(y).astype('int')
+    assert numpy.min(pu) >= 0, "U axis underflows: %f" % numpy.min(pu)
+    assert numpy.max(pu) < n, "U axis overflows: %f" % numpy.max(pu)
+    assert numpy.min(pv) >= 0, "V axis underflows: %f" % numpy.min(pv)
+    assert numpy.max(pv) < n, "V axis overflows: %f" % numpy.max(pv)
+    wu_grid = du * ((pu + 1.0) - crpu) + 0.0
+    wv_grid = dv * ((pv + 1.0) - crpv) + 0.0
+    wu_sub = uvw[:, 0] - wu_grid
+    wv_sub = uvw[:, 1] - wv_grid
+    pu_off = numpy.floor((wu_sub / cfg.ddu + cfg.crpix_d) - 1.0).astype('int')
+    pv_off = numpy.floor((wv_sub / cfg.ddv + cfg.crpix_d) - 1.0).astype('int')
+    crpw = float(nw_grid // 2 + 1)
+    pwg = numpy.round((uvw[:, 2] / wstep_grid + crpw) - 1.0).astype('int')
+    assert numpy.min(pwg) >= 0, "W axis underflows"
+    assert numpy.max(pwg) < cfg.nz, "W axis overflows"
+    pwc = numpy.round((uvw[:, 2] / cfg.zstep + cfg.crpix_z) - 1.0).astype('int')
+    assert numpy.min(pwc) >= 0, "W axis underflows"
+    assert numpy.max(pwc) < cfg.nz, "W axis overflows"
+    pfreq = numpy.round(((frequency - geom.freq0) / geom.dfreq + 1.0) - 1.0).astype('int')
+    nchan_cf = geom.nchan if nchan_cf is None else nchan_cf
+    assert numpy.min(pfreq) >= 0, "Frequency axis underflows"
+    assert numpy.max(pfreq) < nchan_cf, "Frequency axis overflows"
+    return pu, pu_off, pv, pv_off, pwg, pwc, pfreq
+
+
+def _check_footprints(pu, pv, n, support):
+    """A footprint crossing the grid edge makes ARL's slice-add fail (gridding.py:128, 264)."""
+    half = support // 2
+    if len(pu) and (numpy.min(pu) - half < 0 or numpy.max(pu) + half > n or
+                    numpy.min(pv) - half < 0 or numpy.max(pv) + half > n):
+        raise ValueError("could not broadcast: kernel footprint crosses the grid edge")
+
+
+def grid_visibility_to_griddata_loop(vis, geom, cf, cfg, nw_grid=1):
+    """ARL's per-visibility gridding loop, unchanged in structure (gridding.py:99-133).
+
+    vis: dict with 'uvw' [V,3] (wavelengths), 'vis' [V,npol], 'imaging_weight' [V,npol],
+    'frequency' [V].  Returns (grid [nchan,npol,nw,N,N] c128, sumwt [nchan,npol]).
+    """
+    nchan, npol, nz, oversampling, _, support, _ = cf.shape
+    sumwt = numpy.zeros([geom.nchan, npol])
+    pu, pu_off, pv, pv_off, pwg, pwc, pfreq = convolution_mapping(vis['uvw'], vis['frequency'], geom, cfg,
+                                                                  nw_grid=nw_grid)
+    _check_footprints(pu, pv, geom.npixel, support)
+    gv, gu = cf.shape[-2:]
+    grid = numpy.zeros([geom.nchan, npol, nw_grid, geom.npixel, geom.npixel], dtype='complex')
+    cfc = numpy.conjugate(cf)
+    du, dv = gu // 2, gv // 2
+    vw = vis['vis'] * vis['imaging_weight']
+    for v, vwt, chan, uu, uuf, vv, vvf, zzg, zzc in zip(vw, vis['imaging_weight'], pfreq, pu, pu_off, pv,
+                                                         pv_off, pwg, pwc):
+        grid[chan, :, zzg, (vv - dv):(vv + dv), (uu - du):(uu + du)] += \
+            cfc[chan, :, zzc, vvf, uuf, :, :] * v[:, numpy.newaxis, numpy.newaxis]
+        sumwt[chan, :] += vwt
+    return grid, sumwt
+
+
+def grid_visibility_to_griddata(vis, geom, cf, cfg, nw_grid=1, chunk=1 << 16):
+    """Vectorised form of grid_visibility_to_griddata_loop (gridding.py:99-133).
+
+    numpy.add.at adds the contributions to each pixel unbuffered in input order,
+    i.e. in visibility order, with the same complex products as the loop, so the
+    result is bitwise identical to the loop form.
+    """
+    nchan, npol, nz, oversampling, _, support, _ = cf.shape
+    n = geom.npixel
+    pu, pu_off, pv, pv_off, pwg, pwc, pfreq = convolution_mapping(vis['uvw'], vis['frequency'], geom, cfg,
+                                                                  nw_grid=nw_grid)
+    _check_footprints(pu, pv, n, support)
+    cfc = numpy.conjugate(cf)
+    half = support // 2
+    vw = vis['vis'] * vis['imaging_weight']
+    size = geom.nchan * npol * nw_grid * n * n
+    re = numpy.zeros(size)
+    im = numpy.zeros(size)
+    a = numpy.arange(support)
+    nvis = len(pu)
+    for s0 in range(0, nvis, chunk):
+        sl = slice(s0, min(nvis, s0 + chunk))
+        for pol in range(npol):
+            base = (((pfreq[sl] * npol + pol) * nw_grid + pwg[sl]) * n)
+            rows = (pv[sl] - half)[:, None] + a[None, :]            # [v, S]
+            cols = (pu[sl] - half)[:, None] + a[None, :]            # [v, S]
+            idx = ((base[:, None, None] + rows[:, :, None]) * n + cols[:, None, :])
+            vals = cfc[pfreq[sl], pol, pwc[sl], pv_off[sl], pu_off[sl], :, :] * vw[sl, pol][:, None, None]
+            numpy.add.at(re, idx.ravel(), vals.real.ravel())
+            numpy.add.at(im, idx.ravel(), vals.imag.ravel())
+    grid = (re + 1j * im).reshape(geom.nchan, npol, nw_grid, n, n)
+    sumwt = numpy.zeros([geom.nchan, npol])
+    for pol in range(npol):
+        sumwt[:, pol] = numpy.bincount(pfreq, weights=vis['imaging_weight'][:, pol], minlength=geom.nchan)
+    return grid, sumwt
+
+
+def grid_visibility_to_griddata_fast(vis, geom, cfg, nw_grid=1):
+    """Nearest-pixel gridding with no kernel (gridding.py:136-158)."""
+    n = geom.npixel
+    npol = vis['vis'].shape[1]
+    pu, pu_off, pv, pv_off, pwg, pwc, pfreq = convolution_mapping(vis['uvw'], vis['frequency'], geom, cfg,
+                                                                  nw_grid=nw_grid)
+    grid = numpy.zeros([geom.nchan, npol, nw_grid, n, n], dtype='complex')
+    sumwt = numpy.zeros([geom.nchan, npol])
+    for v, vwt, chan, xx, yy, zzg in zip(vis['vis'], vis['imaging_weight'], pfreq, pu, pv, pwg):
+        grid[chan, :, zzg, yy, xx] += v * vwt
+        sumwt[chan, :] += vwt
+    return grid, sumwt
+
+
+def degrid_visibility_from_griddata_loop(vis, grid, geom, cf, cfg):
+    """ARL's per-visibility degridding loop (gridding.py:233-267); no conjugation."""
+    nchan, npol, nz, oversampling, _, support, _ = cf.shape
+    nw_grid = grid.shape[2]
+    pu, pu_off, pv, pv_off, pwg, pwc, pfreq = convolution_mapping(vis['uvw'], vis['frequency'], geom, cfg,
+                                                                  nw_grid=nw_grid)
+    _check_footprints(pu, pv, geom.npixel, support)
+    gv, gu = cf.shape[-2:]
+    du, dv = gu // 2, gv // 2
+    out = numpy.zeros_like(vis['vis'], dtype='complex')
+    for ivis in range(len(pu)):
+        chan, uu, uuf, vv, vvf, zzg, zzc = (pfreq[ivis], pu[ivis], pu_off[ivis], pv[ivis], pv_off[ivis],
+                                            pwg[ivis], pwc[ivis])
+        out[ivis, :] += numpy.einsum('ijk,ijk->i', grid[chan, :, zzg, (vv - dv):(vv + dv), (uu - du):(uu + du)],
+                                     cf[chan, :, zzc, vvf, uuf, :, :])
+    return out
+
+
+def degrid_visibility_from_griddata(vis, grid, geom, cf, cfg, chunk=1 << 15):
+    """Vectorised form of degrid_visibility_from_griddata_loop (gridding.py:233-267)."""
+    nchan, npol, nz, oversampling, _, support, _ = cf.shape
+    n = geom.npixel
+    nw_grid = grid.shape[2]
+    pu, pu_off, pv, pv_off, pwg, pwc, pfreq = convolution_mapping(vis['uvw'], vis['frequency'], geom, cfg,
+                                                                  nw_grid=nw_grid)
+    _check_footprints(pu, pv, n, support)
+    half = support // 2
+    a = numpy.arange(support)
+    out = numpy.zeros(vis['vis'].shape, dtype='complex')
+    flat = grid.reshape(-1)
+    nvis = len(pu)
+    for s0 in range(0, nvis, chunk):
+        sl = slice(s0, min(nvis, s0 + chunk))
+        for pol in range(npol):
+            base = (((pfreq[sl] * npol + pol) * nw_grid + pwg[sl]) * n)
+            rows = (pv[sl] - half)[:, None] + a[None, :]
+            cols = (pu[sl] - half)[:, None] + a[None, :]
+            idx = ((base[:, None, None] + rows[:, :, None]) * n + cols[:, None, :])
+            patch = flat[idx]
+            k = cf[pfreq[sl], pol, pwc[sl], pv_off[sl], pu_off[sl], :, :]
+            out[sl, pol] = numpy.sum(patch * k, axis=(1, 2))
+    return out
+
+
+def fft_griddata_to_image(grid, gcf, imaginary=False):
+    """img = ifft(sum_z G) * gcf * nx * ny; real (and imaginary) part (gridding.py:270-289)."""
+    projected = numpy.sum(grid, axis=2)
+    _, _, ny, nx = projected.shape
+    im_data = ifft(projected) * gcf * float(nx) * float(ny)
+    if imaginary:
+        return im_data.real, im_data.imag
+    return im_data.real
+
+
+def fft_image_to_griddata(im, gcf, nw_grid=1):
+    """G[:, :, z] = fft(im * gcf) for every z (gridding.py:292-302)."""
+    g = fft(im * gcf)
+    return numpy.repeat(g[:, :, numpy.newaxis, ...], nw_grid, axis=2)
+
+
+# ---------------------------------------------------------------------------
+# processing_components/imaging/base.py
+# ---------------------------------------------------------------------------
+
+def normalize_sumwt(im, sumwt):
+    """img[c,p] /= sumwt[c,p] if > 0 else 0 (imaging/base.py:84-101)."""
+    im = numpy.array(im, copy=True)
+    nchan, npol, _, _ = im.shape
+    for chan in range(nchan):
+        for pol in range(npol):
+            if sumwt[chan, pol] > 0.0:
+                im[chan, pol, :, :] = im[chan, pol, :, :] / sumwt[chan, pol]
+            else:
+                im[chan, pol, :, :] = 0.0
+    return im
+
+
+def _copy_vis(vis):
+    return {k: numpy.array(v, copy=True) for k, v in vis.items()}
+
+
+def invert_2d(vis, geom, gcf, cf, cfg, dopsf=False, normalize=True, imaginary=False, loop=False):
+    """Dirty image by convolutional gridding (imaging/base.py:141-189).
+
+    The image centre equals the visibility phase centre for every workload built
+    here, so shift_vis_to_image (base.py:50-81) is the identity and is omitted.
+    """
+    svis = _copy_vis(vis)
+    if dopsf:
+        svis['vis'][...] = 1.0 + 0.0j
+    gridder = grid_visibility_to_griddata_loop if loop else grid_visibility_to_griddata
+    grid, sumwt = gridder(svis, geom, cf, cfg)
+    if imaginary:
+        re, im = fft_griddata_to_image(grid, gcf, imaginary=True)
+        if normalize:
+            re = normalize_sumwt(re, sumwt)
+            im = normalize_sumwt(im, sumwt)
+        return re, sumwt, im
+    re = fft_griddata_to_image(grid, gcf)
+    if normalize:
+        re = normalize_sumwt(re, sumwt)
+    return re, sumwt
+
+
+def predict_2d(vis, model, geom, gcf, cf, cfg, loop=False):
+    """Model visibilities by convolutional degridding (imaging/base.py:104-138). Returns vis array."""
+    grid = fft_image_to_griddata(model, gcf)
+    degridder = degrid_visibility_from_griddata_loop if loop else degrid_visibility_from_griddata
+    return degridder(vis, grid, geom, cf, cfg)
+
+
+# ---------------------------------------------------------------------------
+# w-stacking: processing_components/imaging/wstack_single.py,
+# processing_components/visibility/iterators.py,
+# workflows/serial/imaging/imaging_serial.py
+# ---------------------------------------------------------------------------
+
+def vis_wslice_iter(w, vis_slices=1):
+    """Boolean row masks |w - box| < step/2 over boxes = linspace(-wmax, wmax, P) (iterators.py:91-109)."""
+    wmaxabs = numpy.max(numpy.abs(w))
+    boxes = numpy.linspace(-wmaxabs, +wmaxabs, vis_slices)
+    if vis_slices > 1:
+        wstack = boxes[1] - boxes[0]
+    else:
+        wstack = 2 * wmaxabs
+    for box in boxes:
+        yield numpy.abs(w - box) < 0.5 * wstack
+
+
+def _rows(vis, rows):
+    return {k: numpy.array(v[rows], copy=True) for k, v in vis.items()}
+
+
+def invert_wstack_single(vis, geom, gcf, cf, cfg, dopsf=False, normalize=True, loop=False):
+    """One w slab: shift w by the slab mean, invert, rotate by the screen (wstack_single.py:67-96)."""
+    vis = _copy_vis(vis)
+    w_average = numpy.average(vis['uvw'][:, 2])
+    vis['uvw'][..., 2] -= w_average
+    re, sumwt, im = invert_2d(vis, geom, gcf, cf, cfg, dopsf=dopsf, normalize=normalize, imaginary=True,
+                              loop=loop)
+    b = create_w_term_like(geom, w_average)
+    return b.real * re - b.imag * im, sumwt
+
+
+def predict_wstack_single(vis, model, geom, gcf, cf, cfg, loop=False):
+    """One w slab: P(Re b * I) - 1j P(Im b * I) with w shifted by the slab mean (wstack_single.py:26-64)."""
+    vis = _copy_vis(vis)
+    w_average = numpy.average(vis['uvw'][:, 2])
+    vis['uvw'][..., 2] -= w_average
+    b = create_w_term_like(geom, w_average)
+    v_re = predict_2d(vis, b.real * model, geom, gcf, cf, cfg, loop=loop)
+    v_im = predict_2d(vis, b.imag * model, geom, gcf, cf, cfg, loop=loop)
+    return v_re - 1j * v_im
+
+
+def invert_list_wstack(vis, geom, gcf, cf, cfg, vis_slices, dopsf=False, normalize=True, loop=False):
+    """Serial w-stack invert of one vis list (imaging_serial.py:181-197).
+
+    Each slab's normalised image is re-weighted by its sumwt, summed, and the
+    sum normalised by the total sumwt.
+    """
+    result = numpy.zeros(geom.shape)
+    result_sumwt = numpy.zeros([geom.nchan, geom.npol])
+    for rows in vis_wslice_iter(vis['uvw'][:, 2], vis_slices):
+        if numpy.sum(rows) == 0:
+            continue
+        img, sumwt = invert_wstack_single(_rows(vis, rows), geom, gcf, cf, cfg, dopsf=dopsf, normalize=normalize,
+                                          loop=loop)
+        result += sumwt[:, :, numpy.newaxis, numpy.newaxis] * img
+        result_sumwt += sumwt
+    return normalize_sumwt(result, result_sumwt), result_sumwt
+
+
+def predict_list_wstack(vis, model, geom, gcf, cf, cfg, vis_slices, loop=False):
+    """Serial w-stack predict of one vis list (imaging_serial.py:86-98).
+
+    Rows in no slab stay zero; a row in two slabs keeps the later slab's value.
+    """
+    out = numpy.zeros(vis['vis'].shape, dtype='complex')
+    for rows in vis_wslice_iter(vis['uvw'][:, 2], vis_slices):
+        if numpy.sum(rows) == 0:
+            continue
+        out[rows, ...] = predict_wstack_single(_rows(vis, rows), model, geom, gcf, cf, cfg, loop=loop)
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Direct DFT: processing_library/util/coordinate_support.py:192-212,
+# processing_components/imaging/base.py:192-244
+# ---------------------------------------------------------------------------
+
+def simulate_point(dist_uvw, l, m):
+    """exp(-2 pi i uvw . (l, m, sqrt(1-l^2-m^2)-1)) (coordinate_support.py:192-212)."""
+    s = numpy.array([l, m, numpy.sqrt(1 - l ** 2 - m ** 2) - 1.0])
+    return numpy.exp(-2j * numpy.pi * numpy.dot(dist_uvw, s))
+
+
+def predict_skycomponent_visibility(vis, components, chan_map=None):
+    """Add the direct-DFT visibilities of point components (imaging/base.py:207-220).
+
+    components: list of (l, m, flux) with flux either scalar or [nchan_img, npol].
+    chan_map: image channel of each row (get_frequency_map); default all 0.
+    """
+    out = numpy.array(vis['vis'], copy=True)
+    nvis, npol = out.shape
+    if chan_map is None:
+        chan_map = numpy.zeros(nvis, dtype=int)
+    for (l, m, flux) in components:
+        phasor = simulate_point(vis['uvw'], l, m)
+        f = numpy.atleast_2d(numpy.asarray(flux, dtype=float))
+        if f.shape == (1, 1):
+            out += f[0, 0] * phasor[:, numpy.newaxis]
+        else:
+            out += f[chan_map, :] * phasor[:, numpy.newaxis]
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Array geometry: processing_library/util/coordinate_support.py:37-85, 290-313;
+# processing_components/visibility/base.py:50-150
+# ---------------------------------------------------------------------------
+
+def xyz_at_latitude(local_xyz, lat):
+    """Local XYZ -> celestial XYZ (coordinate_support.py:37-56)."""
+    x, y, z = numpy.hsplit(local_xyz, 3)
+    lat2 = numpy.pi / 2 - lat
+    y2 = -z * numpy.sin(lat2) + y * numpy.cos(lat2)
+    z2 = z * numpy.cos(lat2) + y * numpy.sin(lat2)
+    return numpy.hstack([x, y2, z2])
+
+
+def xyz_to_uvw(xyz, ha, dec):
+    """Rotate earth XYZ to UVW for (ha, dec) (coordinate_support.py:59-85)."""
+    x, y, z = numpy.hsplit(xyz, 3)
+    u = x * numpy.cos(ha) - y * numpy.sin(ha)
+    v0 = x * numpy.sin(ha) + y * numpy.cos(ha)
+    w = z * numpy.sin(dec) - v0 * numpy.cos(dec)
+    v = z * numpy.cos(dec) + v0 * numpy.sin(dec)
+    return numpy.hstack([u, v, w])
+
+
+def hadec_to_azel(ha, dec, latitude):
+    """(coordinate_support.py:290-313)."""
+    coslat, sinlat = numpy.cos(latitude), numpy.sin(latitude)
+    cosdec, sindec = numpy.cos(dec), numpy.sin(dec)
+    cosha, sinha = numpy.cos(ha), numpy.sin(ha)
+    az = numpy.arctan2(- cosdec * sinha, (coslat * sindec - sinlat * cosdec * cosha))
+    el = numpy.arcsin(sinlat * sindec + coslat * cosdec * cosha)
+    return az, el
+
+
+def create_visibility(ants_xyz, times, frequency, channel_bandwidth, dec, zerow=False):
+    """Visibility rows ordered (time, a1, a2>a1, chan); uvw = (pos[a2]-pos[a1]) * f/c.
+
+    visibility/base.py:50-150 with elevation_limit=None.  ants_xyz are celestial
+    XYZ (already rotated by xyz_at_latitude).  Weights are 1 (base.py:120).
+    """
+    nch = len(frequency)
+    nants = ants_xyz.shape[0]
+    rows_uvw, rows_freq, rows_a1, rows_a2, rows_time = [], [], [], [], []
+    for ha in times:
+        ant_pos = xyz_to_uvw(ants_xyz, ha, dec)
+        a1, a2 = numpy.triu_indices(nants, k=1)
+        bl = ant_pos[a2, :] - ant_pos[a1, :]
+        for_chan = []
+        for ch in range(nch):
+            k = frequency[ch] / C_LIGHT
+            for_chan.append(bl * k)
+        uvw = numpy.stack(for_chan, axis=1).reshape(-1, 3)
+        rows_uvw.append(uvw)
+        rows_freq.append(numpy.tile(frequency, len(a1)))
+        rows_a1.append(numpy.repeat(a1, nch))
+        rows_a2.append(numpy.repeat(a2, nch))
+        rows_time.append(numpy.full(len(a1) * nch, ha * 43200.0 / numpy.pi))
+    uvw = numpy.concatenate(rows_uvw)
+    if zerow:
+        uvw[:, 2] = 0.0
+    nrows = uvw.shape[0]
+    return {
+        'uvw': uvw,
+        'frequency': numpy.concatenate(rows_freq),
+        'antenna1': numpy.concatenate(rows_a1),
+        'antenna2': numpy.concatenate(rows_a2),
+        'time': numpy.concatenate(rows_time),
+        'vis': numpy.zeros([nrows, 1], dtype='complex'),
+        'imaging_weight': numpy.ones([nrows, 1]),
+    }
+
+
+def apply_gains_rows(vis_rows, antenna1, antenna2, gain):
+    """Scalar gain convention V_row = g[antenna2] * conj(g[antenna1]) * V.
+
+    calibration/operations.py:127-137 writes applied[t, i, j] = V[t, i, j] * g_i * conj(g_j)
+    on the BlockVisibility; rows come from the [t, a2, a1] triangle (coalesce.py:413-416),
+    so in row form i = antenna2, j = antenna1.  gain: [nants] or [nrows-aligned] complex.
+    """
+    return vis_rows * (gain[antenna2] * numpy.conjugate(gain[antenna1]))[:, numpy.newaxis]
