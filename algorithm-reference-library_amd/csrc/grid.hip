@@ -5,11 +5,14 @@
 // degrid_visibility_from_griddata (gridding.py:233-267):
 //   V = sum_ab G[chan, :, plane, pv-S/2+a, pu-S/2+b] * CF[chan, :, z, yoff, xoff, a, b]
 //
-// One work group owns one tile chunk.  Gridding: the tile plus an S-wide margin on every side
-// is accumulated in LDS with workgroup-scope float atomics (one entry per lane), then the
-// T x T core is stored to HBM with one coalesced pass (global atomics only for tiles that
-// were split into several chunks).  Degridding: the tile plus an S-wide halo is staged in LDS
-// once and every lane gathers the footprint of its own entry from it.
+// One work group owns one tile chunk.  Gridding: every entry whose kernel footprint touches the
+// tile is listed in it; one wavefront adds one entry at a time, one kernel tap per lane, into a
+// T x T float64 accumulator tile in LDS (taps outside the tile are masked), and the tile is
+// stored to HBM in one coalesced pass (split tiles store partial tiles, summed in slot order
+// by k_reduce_tiles; no global atomics).  The accumulators are float64 in both precisions:
+// on gfx950 an LDS ds_add_f64 costs ~9 cycles per wavefront instruction and ds_add_f32 ~194.
+// Degridding: the tile plus an S-wide halo is staged in LDS once and every lane gathers the
+// footprint of its own entry from it.
 #include "arlmi_internal.h"
 
 #include <algorithm>
@@ -84,44 +87,64 @@ __device__ __forceinline__ uint32_t bcast(uint32_t x, int j) {
   return (uint32_t)__builtin_amdgcn_readlane((int)x, j);
 }
 
-// Writes the T x T core: to the grid, or (slot > 0) to partial tile slot-1 of a split tile.
+// Writes the T x T accumulator tile: to the grid, or (slot > 0) to partial tile slot-1 of a
+// split tile.
 template <typename R>
-__device__ inline void store_core_planar(const R* lre, const R* lim, int WS, int M, int T, int N, const TilePos& tp,
-                                         int slot, cplx<R>* __restrict__ grids, cplx<R>* __restrict__ partial) {
+__device__ inline void store_tile(const double* are, const double* aim, int WS, int T, int N, const TilePos& tp,
+                                  int slot, cplx<R>* __restrict__ grids, cplx<R>* __restrict__ partial) {
   if (slot == 0) {
     cplx<R>* g = grids + (size_t)tp.plane * N * N + (size_t)(tp.ty * T) * N + (size_t)tp.tx * T;
     for (int i = threadIdx.x; i < T * T; i += blockDim.x) {
       const int y = i / T, x = i - (i / T) * T;
-      const int o = (y + M) * WS + x + M;
-      g[(size_t)y * N + x] = mk<R>(lre[o], lim[o]);
+      const int o = y * WS + x;
+      g[(size_t)y * N + x] = mk<R>((R)are[o], (R)aim[o]);
     }
   } else {
     cplx<R>* d = partial + (size_t)(slot - 1) * T * T;
     for (int i = threadIdx.x; i < T * T; i += blockDim.x) {
       const int y = i / T, x = i - (i / T) * T;
-      const int o = (y + M) * WS + x + M;
-      d[i] = mk<R>(lre[o], lim[o]);
+      const int o = y * WS + x;
+      d[i] = mk<R>((R)are[o], (R)aim[o]);
     }
   }
 }
 
+__device__ inline void lds_add_f64(double* p, double v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 // Sums the partial tiles of every split tile, in slot order, into the grid.
+// Grid: (split tiles, T*T/256 pixel blocks); each thread sums one pixel over the tile's slots
+// in slot order (four loads in flight).
 template <typename R>
 __global__ void k_reduce_tiles(const int4* __restrict__ multi, const cplx<R>* __restrict__ partial,
                                cplx<R>* __restrict__ grids, int T, int ntx, int N) {
   const int4 m = multi[blockIdx.x];
+  const int i = blockIdx.y * blockDim.x + threadIdx.x;
+  if (i >= T * T) return;
   const TilePos tp = tile_pos(m.x, ntx);
-  cplx<R>* g = grids + (size_t)tp.plane * N * N + (size_t)(tp.ty * T) * N + (size_t)tp.tx * T;
-  const cplx<R>* src = partial + (size_t)m.y * T * T;
-  for (int i = threadIdx.x; i < T * T; i += blockDim.x) {
-    R re = R(0), im = R(0);
-    for (int k = 0; k < m.z; ++k) {
-      const cplx<R> v = src[(size_t)k * T * T + i];
-      re += v.x;
-      im += v.y;
-    }
-    g[(size_t)(i / T) * N + (i % T)] = mk<R>(re, im);
+  const size_t tt = (size_t)T * T;
+  const cplx<R>* src = partial + (size_t)m.y * tt + i;
+  R re = R(0), im = R(0);
+  int k = 0;
+  for (; k + 4 <= m.z; k += 4) {
+    const cplx<R> a = src[(size_t)k * tt], b = src[(size_t)(k + 1) * tt], c = src[(size_t)(k + 2) * tt],
+                  d = src[(size_t)(k + 3) * tt];
+    re += a.x;
+    im += a.y;
+    re += b.x;
+    im += b.y;
+    re += c.x;
+    im += c.y;
+    re += d.x;
+    im += d.y;
   }
+  for (; k < m.z; ++k) {
+    const cplx<R> a = src[(size_t)k * tt];
+    re += a.x;
+    im += a.y;
+  }
+  grids[(size_t)tp.plane * N * N + (size_t)(tp.ty * T + i / T) * N + (size_t)tp.tx * T + (i % T)] = mk<R>(re, im);
 }
 
 template <typename R, int S>
@@ -134,26 +157,24 @@ __global__ __launch_bounds__(256) void k_grid_sep(const int4* __restrict__ items
   __shared__ double red[16];
   constexpr int TAPS = S * S;
   constexpr int TPL = (TAPS + 63) / 64;  // taps per lane
-  const int W = T + 2 * S;
-  const int WS = lds_stride(W);
-  R* lre = reinterpret_cast<R*>(smem);
-  R* lim = lre + W * WS;
-  R* lk = lim + W * WS;
+  const int WS = lds_stride(T);
+  double* are = reinterpret_cast<double*>(smem);
+  double* aim = are + T * WS;
+  R* lk = reinterpret_cast<R*>(aim + T * WS);
   const int4 it = items[blockIdx.x];
-  for (int i = threadIdx.x; i < W * WS; i += blockDim.x) {
-    lre[i] = R(0);
-    lim[i] = R(0);
+  for (int i = threadIdx.x; i < T * WS; i += blockDim.x) {
+    are[i] = 0.0;
+    aim[i] = 0.0;
   }
   for (int i = threadIdx.x; i < os * S; i += blockDim.x) lk[i] = ktab[i];
   __syncthreads();
   const int lane = __lane_id(), wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
-  int ta[TPL], tb[TPL], toff[TPL];
+  int ta[TPL], tb[TPL];
 #pragma unroll
   for (int g = 0; g < TPL; ++g) {
     const int t = lane + 64 * g;
     ta[g] = t < TAPS ? t / S : 0;
     tb[g] = t < TAPS ? t % S : 0;
-    toff[g] = ta[g] * WS + tb[g];
   }
   double swt = 0.0;
   for (int base = wave * 64; base < it.z; base += nwave * 64) {
@@ -177,22 +198,24 @@ __global__ __launch_bounds__(256) void k_grid_sep(const int4* __restrict__ items
     for (int j = 0; j < cnt; ++j) {
       const uint32_t p = bcast(pk, j);
       const R jr = bcast(vr, j), ji = bcast(vi, j);
-      const int sb = entry_ly(p) * WS + entry_lx(p);
+      const int cx = entry_lx(p) - S, cy = entry_ly(p) - S;  // footprint origin in tile coordinates
       const R* ky = lk + entry_yoff(p) * S;
       const R* kx = lk + entry_xoff(p) * S;
 #pragma unroll
       for (int g = 0; g < TPL; ++g) {
-        if (TAPS % 64 == 0 || lane + 64 * g < TAPS) {
+        const int x = cx + tb[g], y = cy + ta[g];
+        if ((TAPS % 64 == 0 || lane + 64 * g < TAPS) && (unsigned)x < (unsigned)T && (unsigned)y < (unsigned)T) {
           const R c = ky[ta[g]] * kx[tb[g]];
-          lds_add(lre + sb + toff[g], c * jr);
-          lds_add(lim + sb + toff[g], c * ji);
+          const int o = y * WS + x;
+          lds_add_f64(are + o, (double)(c * jr));
+          lds_add_f64(aim + o, (double)(c * ji));
         }
       }
     }
   }
   __syncthreads();
   const TilePos tp = tile_pos(it.x, ntx);
-  store_core_planar<R>(lre, lim, WS, S, T, N, tp, it.w, grids, partial);
+  store_tile<R>(are, aim, WS, T, N, tp, it.w, grids, partial);
   block_add_double(swt, &sumwt[tp.plane], red);
 }
 
@@ -209,14 +232,13 @@ __global__ __launch_bounds__(256) void k_grid_tab(const int4* __restrict__ items
                                                   double* __restrict__ sumwt, int T, int ntx, int N) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double red[16];
-  const int W = T + 2 * S;
-  const int WS = lds_stride(W);
-  R* lre = reinterpret_cast<R*>(smem);
-  R* lim = lre + W * WS;
+  const int WS = lds_stride(T);
+  double* are = reinterpret_cast<double*>(smem);
+  double* aim = are + T * WS;
   const int4 it = items[blockIdx.x];
-  for (int i = threadIdx.x; i < W * WS; i += blockDim.x) {
-    lre[i] = R(0);
-    lim[i] = R(0);
+  for (int i = threadIdx.x; i < T * WS; i += blockDim.x) {
+    are[i] = 0.0;
+    aim[i] = 0.0;
   }
   __syncthreads();
   const int lane = __lane_id(), wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
@@ -245,20 +267,24 @@ __global__ __launch_bounds__(256) void k_grid_tab(const int4* __restrict__ items
       const uint32_t p = bcast(pk, j);
       const int z = (int)bcast(zz, j);
       const R jr = bcast(vr, j), ji = bcast(vi, j);
-      const int sb = entry_ly(p) * WS + entry_lx(p);
+      const int cx = entry_lx(p) - S, cy = entry_ly(p) - S;
       const cplx<R>* k = tab + ((size_t)(z * os + entry_yoff(p)) * os + entry_xoff(p)) * taps;
       for (int t = lane; t < taps; t += 64) {
-        const cplx<R> c = k[t];
         const int a = t / S, b = t - a * S;
-        // conj(c) * v
-        lds_add(lre + sb + a * WS + b, c.x * jr + c.y * ji);
-        lds_add(lim + sb + a * WS + b, c.x * ji - c.y * jr);
+        const int x = cx + b, y = cy + a;
+        if ((unsigned)x < (unsigned)T && (unsigned)y < (unsigned)T) {
+          const cplx<R> c = k[t];
+          const int o = y * WS + x;
+          // conj(c) * v
+          lds_add_f64(are + o, (double)(c.x * jr + c.y * ji));
+          lds_add_f64(aim + o, (double)(c.x * ji - c.y * jr));
+        }
       }
     }
   }
   __syncthreads();
   const TilePos tp = tile_pos(it.x, ntx);
-  store_core_planar<R>(lre, lim, WS, S, T, N, tp, it.w, grids, partial);
+  store_tile<R>(are, aim, WS, T, N, tp, it.w, grids, partial);
   block_add_double(swt, &sumwt[tp.plane], red);
 }
 
@@ -385,8 +411,7 @@ template <typename R, int S>
 static int launch_grid_sep(const arlmi_plan* p, const void* vis, const void* wt, int dopsf, const void* ktab,
                            void* grids, cplx<R>* partial, double* sumwt, hipStream_t s) {
   const DevGeom& g = p->g;
-  const int W = g.tile + 2 * S;
-  const size_t sh = sizeof(R) * (2 * (size_t)W * lds_stride(W) + (size_t)g.os * S);
+  const size_t sh = sizeof(double) * 2 * (size_t)g.tile * lds_stride(g.tile) + sizeof(R) * (size_t)g.os * S;
   if (sh > lds_limit()) {
     set_error("gridding tile needs %zu B of LDS", sh);
     return ARLMI_E_UNSUPPORTED;
@@ -433,8 +458,7 @@ static int grid_launch(const arlmi_plan* p, const void* vis, const void* wt, int
     set_error("separable kernel support %d not instantiated; pass the full table", g.support);
     return ARLMI_E_UNSUPPORTED;
   }
-  const int W = g.tile + 2 * g.support;
-  const size_t sh = sizeof(R) * 2 * (size_t)W * lds_stride(W);
+  const size_t sh = sizeof(double) * 2 * (size_t)g.tile * lds_stride(g.tile);
   if (sh > lds_limit()) {
     set_error("gridding tile (T=%d, S=%d) needs %zu B of LDS; use a smaller tile", g.tile, g.support, sh);
     return ARLMI_E_UNSUPPORTED;
@@ -458,8 +482,8 @@ static int grid_impl(const arlmi_plan* p, const void* vis, const void* wt, int d
   rc = grid_launch<R>(p, vis, wt, dopsf, k, grids, partial, sumwt, s);
   if (rc) return rc;
   if (p->nmulti > 0) {
-    ARLMI_LAUNCH("k_reduce_tiles", k_reduce_tiles<R>, dim3((unsigned)p->nmulti), dim3(256), 0, s, p->d_multi,
-                 partial, (cplx<R>*)grids, g.tile, g.ntx, g.n);
+    ARLMI_LAUNCH("k_reduce_tiles", k_reduce_tiles<R>, dim3((unsigned)p->nmulti, (g.tile * g.tile + 255) / 256),
+                 dim3(256), 0, s, p->d_multi, partial, (cplx<R>*)grids, g.tile, g.ntx, g.n);
     ARLMI_LAUNCH_CHECK();
   }
   return ARLMI_OK;
