@@ -63,6 +63,16 @@ __host__ __device__ inline int entry_xoff(uint32_t p) { return (int)((p >> 16) &
 __host__ __device__ inline int entry_yoff(uint32_t p) { return (int)((p >> 23) & 0x7fu); }
 
 // Result of ARL's convolution_mapping for one row (gridding.py:24-96).
+// Work groups of one launch are dispatched round-robin over the 8 XCDs of an MI355X, each
+// with its own L2.  xcd_block maps the hardware block index to a logical one under which
+// XCD k runs the k-th contiguous eighth of the logical blocks, so logical neighbours that
+// read the same cache lines (adjacent column groups, overlapping tile halos) share an L2.
+__device__ inline int xcd_block(int bid, int n) {
+  const int per = n >> 3;
+  if (bid >= (per << 3)) return bid;
+  return (bid & 7) * per + (bid >> 3);
+}
+
 struct RowMap {
   int pu, pv;      // nearest grid pixel (numpy.round: half to even)
   int xoff, yoff;  // kernel sub-sample (numpy.floor; a -1 wraps to os-1 like a numpy index)

@@ -355,8 +355,9 @@ __global__ __launch_bounds__(Shape<N>::NT * Shape<N>::template col_nb<R>()) void
   cplx<R>* tw = stage_twiddles_at<R, N>(smem, sizeof(cplx<R>) * (size_t)N * NB, twg);
   cplx<R>* data = reinterpret_cast<cplx<R>*>(a.data);
   constexpr int CB = N / NB;
-  const int64_t plane = blockIdx.x / CB;
-  const int x0 = (blockIdx.x % CB) * NB;
+  const int bid = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t plane = bid / CB;
+  const int x0 = (bid % CB) * NB;
   cplx<R>* base = data + plane * (int64_t)N * N;
 #pragma unroll
   for (int e = 0; e < EPT; ++e) {
@@ -391,8 +392,9 @@ __global__ __launch_bounds__(Shape<N>::NT * Shape<N>::template inv_nb<R>()) void
   cplx<R>* tw = stage_twiddles_at<R, N>(smem, sizeof(cplx<R>) * (size_t)N * NB, twg);
   const cplx<R>* grids = reinterpret_cast<const cplx<R>*>(a.data);
   constexpr int CB = N / NB;
-  const int c = blockIdx.x / CB;
-  const int x0 = (blockIdx.x % CB) * NB;
+  const int bid = xcd_block(blockIdx.x, gridDim.x);
+  const int c = bid / CB;
+  const int x0 = (bid % CB) * NB;
   R acc[EPT];
   R acci[SCREENS ? 1 : EPT];
 #pragma unroll

@@ -45,6 +45,11 @@ __device__ inline TilePos tile_pos(int tile, int ntx) {
   return t;
 }
 
+// Work-group size of the gridders: with the ~75 KiB tile of T=64, two groups of 16 waves
+// share a CU; the add loop is latency-bound, and 32 waves per CU measured 18% faster than 16
+// and 37% faster than 8.
+constexpr int GRID_THREADS = 1024;
+
 __device__ inline void block_add_double(double v, double* dst, double* scratch) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
   const int wid = threadIdx.x >> 6;
@@ -87,25 +92,22 @@ __device__ __forceinline__ uint32_t bcast(uint32_t x, int j) {
   return (uint32_t)__builtin_amdgcn_readlane((int)x, j);
 }
 
-// Writes the T x T accumulator tile: to the grid, or (slot > 0) to partial tile slot-1 of a
+// Writes the T x T accumulator tile (planar float64: real parts from acc[0], imaginary parts
+// from acc[im_off], row stride WS): to the grid, or (slot > 0) to partial tile slot-1 of a
 // split tile.
 template <typename R>
-__device__ inline void store_tile(const double* are, const double* aim, int WS, int T, int N, const TilePos& tp,
-                                  int slot, cplx<R>* __restrict__ grids, cplx<R>* __restrict__ partial) {
-  if (slot == 0) {
-    cplx<R>* g = grids + (size_t)tp.plane * N * N + (size_t)(tp.ty * T) * N + (size_t)tp.tx * T;
-    for (int i = threadIdx.x; i < T * T; i += blockDim.x) {
-      const int y = i / T, x = i - (i / T) * T;
-      const int o = y * WS + x;
-      g[(size_t)y * N + x] = mk<R>((R)are[o], (R)aim[o]);
-    }
-  } else {
-    cplx<R>* d = partial + (size_t)(slot - 1) * T * T;
-    for (int i = threadIdx.x; i < T * T; i += blockDim.x) {
-      const int y = i / T, x = i - (i / T) * T;
-      const int o = y * WS + x;
-      d[i] = mk<R>((R)are[o], (R)aim[o]);
-    }
+__device__ inline void store_tile(const double* acc, int WS, int T, int N, const TilePos& tp, int slot,
+                                  cplx<R>* __restrict__ grids, cplx<R>* __restrict__ partial, int im_off) {
+  cplx<R>* g = grids + (size_t)tp.plane * N * N + (size_t)(tp.ty * T) * N + (size_t)tp.tx * T;
+  cplx<R>* d = partial + (size_t)(slot > 0 ? slot - 1 : 0) * T * T;
+  for (int i = threadIdx.x; i < T * T; i += blockDim.x) {
+    const int y = i / T, x = i - (i / T) * T;
+    const int o = y * WS + x;
+    const cplx<R> c = mk<R>((R)acc[o], (R)acc[im_off + o]);
+    if (slot == 0)
+      g[(size_t)y * N + x] = c;
+    else
+      d[i] = c;
   }
 }
 
@@ -147,41 +149,97 @@ __global__ void k_reduce_tiles(const int4* __restrict__ multi, const cplx<R>* __
   grids[(size_t)tp.plane * N * N + (size_t)(tp.ty * T + i / T) * N + (size_t)tp.tx * T + (i % T)] = mk<R>(re, im);
 }
 
+// Per-entry values a wavefront broadcasts while it walks its 64 entries:
+//   sa: footprint origin as an accumulator index, cy*WS + cx + SA_BIAS, and bit 31 set when
+//       the whole S x S footprint lies inside the tile (no masking needed);
+//   ko: offsets of the entry's kernel rows in the LDS kernel table, yoff*S | xoff*S << 16.
+constexpr int SA_BIAS = 1 << 20;
+constexpr uint32_t SA_INNER = 0x80000000u;
+
+// Kernel values of the taps this lane owns, for the sub-sample offsets in ko.
+template <typename R, int TPL>
+__device__ __forceinline__ void sep_taps(const R* lk, uint32_t ko, const int* ta, const int* tb, R* ky, R* kx) {
+#pragma unroll
+  for (int g = 0; g < TPL; ++g) {
+    ky[g] = lk[(int)(ko & 0xffffu) + ta[g]];
+    kx[g] = lk[(int)(ko >> 16) + tb[g]];
+  }
+}
+
+// Adds (jr, ji) times this lane's taps of one entry into the planar float64 accumulator tile
+// (real parts acc[0, TW + 64), imaginary parts from acc[TW + 64]; the last 64 of each are a
+// trash row).  Inner footprints take the unmasked path (one uniform branch); taps
+// of an edge footprint outside the tile add 0 to this lane's own slot of the trash row:
+// both paths issue the same two LDS adds per tap, so the wait counts the compiler
+// derives for the software-pipelined kernel reads stay exact.
+template <typename R, int S, int TPL>
+__device__ __forceinline__ void sep_add(double* acc, int T, int TW, int lane, uint32_t sa, uint32_t pk,
+                                        const int* ta, const int* tb, const int* loff, const R* ky, const R* kx,
+                                        R jr, R ji) {
+  constexpr int TAPS = S * S;
+  const int sb = (int)(sa & ~SA_INNER) - SA_BIAS;
+  if (sa & SA_INNER) {
+#pragma unroll
+    for (int g = 0; g < TPL; ++g) {
+      const bool ok = TAPS % 64 == 0 || lane + 64 * g < TAPS;
+      const R c = ok ? ky[g] * kx[g] : R(0);
+      const int o = ok ? sb + loff[g] : TW + lane;
+      lds_add_f64(acc + o, (double)(c * jr));
+      lds_add_f64(acc + (TW + 64) + o, (double)(c * ji));
+    }
+  } else {
+    const int cx = entry_lx(pk) - S, cy = entry_ly(pk) - S;
+#pragma unroll
+    for (int g = 0; g < TPL; ++g) {
+      const int x = cx + tb[g], y = cy + ta[g];
+      const bool ok =
+          (TAPS % 64 == 0 || lane + 64 * g < TAPS) && (unsigned)x < (unsigned)T && (unsigned)y < (unsigned)T;
+      const R c = ok ? ky[g] * kx[g] : R(0);
+      const int o = ok ? sb + loff[g] : TW + lane;
+      lds_add_f64(acc + o, (double)(c * jr));
+      lds_add_f64(acc + (TW + 64) + o, (double)(c * ji));
+    }
+  }
+}
+
 template <typename R, int S>
-__global__ __launch_bounds__(256) void k_grid_sep(const int4* __restrict__ items, const uint2* __restrict__ entries,
-                                                  const cplx<R>* __restrict__ vis, const R* __restrict__ wt,
-                                                  int dopsf, const R* __restrict__ ktab, int os,
-                                                  cplx<R>* __restrict__ grids, cplx<R>* __restrict__ partial,
-                                                  double* __restrict__ sumwt, int T, int ntx, int N) {
+__global__ __launch_bounds__(GRID_THREADS) void k_grid_sep(const int4* __restrict__ items,
+                                                           const uint2* __restrict__ entries,
+                                                           const cplx<R>* __restrict__ vis, const R* __restrict__ wt,
+                                                           int dopsf, const R* __restrict__ ktab, int os,
+                                                           cplx<R>* __restrict__ grids, cplx<R>* __restrict__ partial,
+                                                           double* __restrict__ sumwt, int T, int ntx, int N) {
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ double red[16];
+  __shared__ double red[GRID_THREADS / 64];
   constexpr int TAPS = S * S;
   constexpr int TPL = (TAPS + 63) / 64;  // taps per lane
   const int WS = lds_stride(T);
-  double* are = reinterpret_cast<double*>(smem);
-  double* aim = are + T * WS;
-  R* lk = reinterpret_cast<R*>(aim + T * WS);
+  const int TW = T * WS;
+  double* acc = reinterpret_cast<double*>(smem);  // [2][TW + 64]: real, imaginary (tile + trash row)
+  R* lk = reinterpret_cast<R*>(acc + 2 * (TW + 64));
   const int4 it = items[blockIdx.x];
-  for (int i = threadIdx.x; i < T * WS; i += blockDim.x) {
-    are[i] = 0.0;
-    aim[i] = 0.0;
-  }
+  for (int i = threadIdx.x; i < 2 * (TW + 64); i += blockDim.x) acc[i] = 0.0;
   for (int i = threadIdx.x; i < os * S; i += blockDim.x) lk[i] = ktab[i];
   __syncthreads();
   const int lane = __lane_id(), wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
-  int ta[TPL], tb[TPL];
+  int ta[TPL], tb[TPL], loff[TPL];
 #pragma unroll
   for (int g = 0; g < TPL; ++g) {
     const int t = lane + 64 * g;
     ta[g] = t < TAPS ? t / S : 0;
     tb[g] = t < TAPS ? t % S : 0;
+    loff[g] = ta[g] * WS + tb[g];
   }
   double swt = 0.0;
-  for (int base = wave * 64; base < it.z; base += nwave * 64) {
+  // each wave takes one contiguous share of the item's entries, so every wave of the group
+  // has work even for small items
+  const int per = __builtin_amdgcn_readfirstlane((it.z + nwave - 1) / nwave);
+  const int e0 = min(it.z, wave * per), e1 = min(it.z, e0 + per);
+  for (int base = e0; base < e1; base += 64) {
     const int e = base + lane;
-    uint32_t pk = 0;
+    uint32_t pk = 0;  // lanes past the end hold entry 0 of the tile origin: offsets in range
     R vr = R(0), vi = R(0);
-    if (e < it.z) {
+    if (e < e1) {
       const uint2 en = entries[it.y + e];
       pk = en.y;
       const R w = wt[en.x];
@@ -194,28 +252,38 @@ __global__ __launch_bounds__(256) void k_grid_sep(const int4* __restrict__ items
       }
       if (pk & ENTRY_START) swt += (double)w;
     }
-    const int cnt = min(64, it.z - base);
-    for (int j = 0; j < cnt; ++j) {
-      const uint32_t p = bcast(pk, j);
-      const R jr = bcast(vr, j), ji = bcast(vi, j);
-      const int cx = entry_lx(p) - S, cy = entry_ly(p) - S;  // footprint origin in tile coordinates
-      const R* ky = lk + entry_yoff(p) * S;
-      const R* kx = lk + entry_xoff(p) * S;
+    const int cx = entry_lx(pk) - S, cy = entry_ly(pk) - S;
+    const bool inner = cx >= 0 && cy >= 0 && cx + S <= T && cy + S <= T;
+    const uint32_t sav = (uint32_t)(cy * WS + cx + SA_BIAS) | (inner ? SA_INNER : 0u);
+    const uint32_t kov = (uint32_t)(entry_yoff(pk) * S) | ((uint32_t)(entry_xoff(pk) * S) << 16);
+    const int cnt = __builtin_amdgcn_readfirstlane(min(64, e1 - base));
+    // Software pipeline over two register sets A and B: the kernel values of entry j+1 are read
+    // from LDS before the adds of entry j are issued.  LDS operations retire in order, so the
+    // wait for a set's reads never includes adds.  The prologue issues A reads, 2*TPL adds of
+    // zero to the trash row, then B reads: the same pattern of outstanding operations as the
+    // loop's back edge, so the wait counts at the loop head are exact on both paths.
+    R kya[TPL], kxa[TPL], kyb[TPL], kxb[TPL];
+    sep_taps<R, TPL>(lk, bcast(kov, 0), ta, tb, kya, kxa);
 #pragma unroll
-      for (int g = 0; g < TPL; ++g) {
-        const int x = cx + tb[g], y = cy + ta[g];
-        if ((TAPS % 64 == 0 || lane + 64 * g < TAPS) && (unsigned)x < (unsigned)T && (unsigned)y < (unsigned)T) {
-          const R c = ky[ta[g]] * kx[tb[g]];
-          const int o = y * WS + x;
-          lds_add_f64(are + o, (double)(c * jr));
-          lds_add_f64(aim + o, (double)(c * ji));
-        }
-      }
+    for (int g = 0; g < TPL; ++g) {
+      lds_add_f64(acc + TW + lane, 0.0);
+      lds_add_f64(acc + (TW + 64) + TW + lane, 0.0);
+    }
+    sep_taps<R, TPL>(lk, bcast(kov, 1), ta, tb, kyb, kxb);
+    for (int j = 0;; j += 2) {
+      sep_add<R, S, TPL>(acc, T, TW, lane, bcast(sav, j), bcast(pk, j), ta, tb, loff, kya, kxa, bcast(vr, j),
+                         bcast(vi, j));
+      if (j + 1 >= cnt) break;
+      sep_taps<R, TPL>(lk, bcast(kov, (j + 2) & 63), ta, tb, kya, kxa);
+      sep_add<R, S, TPL>(acc, T, TW, lane, bcast(sav, j + 1), bcast(pk, j + 1), ta, tb, loff, kyb, kxb,
+                         bcast(vr, j + 1), bcast(vi, j + 1));
+      if (j + 2 >= cnt) break;
+      sep_taps<R, TPL>(lk, bcast(kov, (j + 3) & 63), ta, tb, kyb, kxb);
     }
   }
   __syncthreads();
   const TilePos tp = tile_pos(it.x, ntx);
-  store_tile<R>(are, aim, WS, T, N, tp, it.w, grids, partial);
+  store_tile<R>(acc, WS, T, N, tp, it.w, grids, partial, TW + 64);
   block_add_double(swt, &sumwt[tp.plane], red);
 }
 
@@ -223,32 +291,76 @@ __global__ __launch_bounds__(256) void k_grid_sep(const int4* __restrict__ items
 // gridding, full complex kernel table (AW / box / any CF), runtime support
 // ---------------------------------------------------------------------------
 
-template <typename R>
-__global__ __launch_bounds__(256) void k_grid_tab(const int4* __restrict__ items, const uint2* __restrict__ entries,
-                                                  const uint16_t* __restrict__ entry_z,
-                                                  const cplx<R>* __restrict__ vis, const R* __restrict__ wt,
-                                                  int dopsf, const cplx<R>* __restrict__ tab, int os, int S,
-                                                  cplx<R>* __restrict__ grids, cplx<R>* __restrict__ partial,
-                                                  double* __restrict__ sumwt, int T, int ntx, int N) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ double red[16];
-  const int WS = lds_stride(T);
-  double* are = reinterpret_cast<double*>(smem);
-  double* aim = are + T * WS;
-  const int4 it = items[blockIdx.x];
-  for (int i = threadIdx.x; i < T * WS; i += blockDim.x) {
-    are[i] = 0.0;
-    aim[i] = 0.0;
+// Complex kernel values of this lane's taps (t = lane + 64 q) of one entry, from the table
+// slice at kbase (L2-resident: the whole table is read by every tile).
+template <typename R, int MT>
+__device__ __forceinline__ void tab_taps(const cplx<R>* __restrict__ tab, uint32_t kbase, int taps, int lane,
+                                         cplx<R>* k) {
+#pragma unroll
+  for (int q = 0; q < MT; ++q) {
+    const int t = lane + 64 * q;
+    k[q] = t < taps ? tab[(size_t)kbase + t] : mk<R>(R(0), R(0));
   }
+}
+
+// conj(c) * (jr + i ji) for this lane's taps of one entry, into the interleaved accumulators;
+// same masking and trash-row scheme as sep_add.
+template <typename R, int MT>
+__device__ __forceinline__ void tab_add(double* acc, int T, int TW, int S, int taps, int lane, uint32_t sa,
+                                        uint32_t pk, const int* ta, const int* tb, const int* loff,
+                                        const cplx<R>* k, R jr, R ji) {
+  const int sb = (int)(sa & ~SA_INNER) - SA_BIAS;
+  const bool inner = (sa & SA_INNER) != 0;
+  const int cx = entry_lx(pk) - S, cy = entry_ly(pk) - S;
+#pragma unroll
+  for (int q = 0; q < MT; ++q) {
+    const int x = cx + tb[q], y = cy + ta[q];
+    const bool ok = lane + 64 * q < taps && (inner || ((unsigned)x < (unsigned)T && (unsigned)y < (unsigned)T));
+    const int o = ok ? sb + loff[q] : TW + lane;
+    const R cr = ok ? k[q].x : R(0), ci = ok ? k[q].y : R(0);
+    lds_add_f64(acc + o, (double)(cr * jr + ci * ji));
+    lds_add_f64(acc + (TW + 64) + o, (double)(cr * ji - ci * jr));
+  }
+}
+
+// Gridding with a full complex kernel table (AW / box / any CF).  MT = taps per lane held in
+// registers (S*S <= 64*MT); the next entry's kernel values are loaded from L2 while the current
+// entry's adds run.  MT = 0: any support, taps loaded and added one group of 64 at a time.
+template <typename R, int MT>
+__global__ __launch_bounds__(GRID_THREADS) void k_grid_tab(const int4* __restrict__ items,
+                                                           const uint2* __restrict__ entries,
+                                                           const uint16_t* __restrict__ entry_z,
+                                                           const cplx<R>* __restrict__ vis, const R* __restrict__ wt,
+                                                           int dopsf, const cplx<R>* __restrict__ tab, int os, int S,
+                                                           cplx<R>* __restrict__ grids, cplx<R>* __restrict__ partial,
+                                                           double* __restrict__ sumwt, int T, int ntx, int N) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double red[GRID_THREADS / 64];
+  const int WS = lds_stride(T);
+  const int TW = T * WS;
+  double* acc = reinterpret_cast<double*>(smem);  // [2][TW + 64]: real, imaginary (tile + trash row)
+  const int4 it = items[blockIdx.x];
+  for (int i = threadIdx.x; i < 2 * (TW + 64); i += blockDim.x) acc[i] = 0.0;
   __syncthreads();
   const int lane = __lane_id(), wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
   const int taps = S * S;
+  constexpr int NQ = MT > 0 ? MT : 1;
+  int ta[NQ], tb[NQ], loff[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const int t = lane + 64 * q;
+    ta[q] = t < taps ? t / S : 0;
+    tb[q] = t < taps ? t % S : 0;
+    loff[q] = ta[q] * WS + tb[q];
+  }
   double swt = 0.0;
-  for (int base = wave * 64; base < it.z; base += nwave * 64) {
+  const int per = __builtin_amdgcn_readfirstlane((it.z + nwave - 1) / nwave);
+  const int e0 = min(it.z, wave * per), e1 = min(it.z, e0 + per);
+  for (int base = e0; base < e1; base += 64) {
     const int e = base + lane;
     uint32_t pk = 0, zz = 0;
     R vr = R(0), vi = R(0);
-    if (e < it.z) {
+    if (e < e1) {
       const uint2 en = entries[it.y + e];
       pk = en.y;
       zz = entry_z ? (uint32_t)entry_z[it.y + e] : 0u;
@@ -262,29 +374,48 @@ __global__ __launch_bounds__(256) void k_grid_tab(const int4* __restrict__ items
       }
       if (pk & ENTRY_START) swt += (double)w;
     }
-    const int cnt = min(64, it.z - base);
-    for (int j = 0; j < cnt; ++j) {
-      const uint32_t p = bcast(pk, j);
-      const int z = (int)bcast(zz, j);
-      const R jr = bcast(vr, j), ji = bcast(vi, j);
-      const int cx = entry_lx(p) - S, cy = entry_ly(p) - S;
-      const cplx<R>* k = tab + ((size_t)(z * os + entry_yoff(p)) * os + entry_xoff(p)) * taps;
-      for (int t = lane; t < taps; t += 64) {
-        const int a = t / S, b = t - a * S;
-        const int x = cx + b, y = cy + a;
-        if ((unsigned)x < (unsigned)T && (unsigned)y < (unsigned)T) {
-          const cplx<R> c = k[t];
-          const int o = y * WS + x;
-          // conj(c) * v
-          lds_add_f64(are + o, (double)(c.x * jr + c.y * ji));
-          lds_add_f64(aim + o, (double)(c.x * ji - c.y * jr));
+    const int cx = entry_lx(pk) - S, cy = entry_ly(pk) - S;
+    const bool inner = cx >= 0 && cy >= 0 && cx + S <= T && cy + S <= T;
+    const uint32_t sav = (uint32_t)(cy * WS + cx + SA_BIAS) | (inner ? SA_INNER : 0u);
+    const uint32_t kbv = (uint32_t)(((int)zz * os + entry_yoff(pk)) * os + entry_xoff(pk)) * (uint32_t)taps;
+    const int cnt = __builtin_amdgcn_readfirstlane(min(64, e1 - base));
+    if constexpr (MT > 0) {
+      cplx<R> ka[MT], kb[MT];
+      tab_taps<R, MT>(tab, bcast(kbv, 0), taps, lane, ka);
+      tab_taps<R, MT>(tab, bcast(kbv, 1), taps, lane, kb);
+      for (int j = 0;; j += 2) {
+        tab_add<R, MT>(acc, T, TW, S, taps, lane, bcast(sav, j), bcast(pk, j), ta, tb, loff, ka, bcast(vr, j),
+                       bcast(vi, j));
+        if (j + 1 >= cnt) break;
+        tab_taps<R, MT>(tab, bcast(kbv, (j + 2) & 63), taps, lane, ka);
+        tab_add<R, MT>(acc, T, TW, S, taps, lane, bcast(sav, j + 1), bcast(pk, j + 1), ta, tb, loff, kb,
+                       bcast(vr, j + 1), bcast(vi, j + 1));
+        if (j + 2 >= cnt) break;
+        tab_taps<R, MT>(tab, bcast(kbv, (j + 3) & 63), taps, lane, kb);
+      }
+    } else {
+      for (int j = 0; j < cnt; ++j) {
+        const uint32_t p = bcast(pk, j);
+        const int sb = (int)(bcast(sav, j) & ~SA_INNER) - SA_BIAS;
+        const cplx<R>* k = tab + bcast(kbv, j);
+        const R jr = bcast(vr, j), ji = bcast(vi, j);
+        const int jx = entry_lx(p) - S, jy = entry_ly(p) - S;
+        for (int t = lane; t < taps; t += 64) {
+          const int a = t / S, b = t - a * S;
+          const int x = jx + b, y = jy + a;
+          if ((unsigned)x < (unsigned)T && (unsigned)y < (unsigned)T) {
+            const cplx<R> c = k[t];
+            const int o = sb + a * WS + b;
+            lds_add_f64(acc + o, (double)(c.x * jr + c.y * ji));
+            lds_add_f64(acc + (TW + 64) + o, (double)(c.x * ji - c.y * jr));
+          }
         }
       }
     }
   }
   __syncthreads();
   const TilePos tp = tile_pos(it.x, ntx);
-  store_tile<R>(are, aim, WS, T, N, tp, it.w, grids, partial);
+  store_tile<R>(acc, WS, T, N, tp, it.w, grids, partial, TW + 64);
   block_add_double(swt, &sumwt[tp.plane], red);
 }
 
@@ -411,13 +542,13 @@ template <typename R, int S>
 static int launch_grid_sep(const arlmi_plan* p, const void* vis, const void* wt, int dopsf, const void* ktab,
                            void* grids, cplx<R>* partial, double* sumwt, hipStream_t s) {
   const DevGeom& g = p->g;
-  const size_t sh = sizeof(double) * 2 * (size_t)g.tile * lds_stride(g.tile) + sizeof(R) * (size_t)g.os * S;
+  const size_t sh = sizeof(double) * 2 * ((size_t)g.tile * lds_stride(g.tile) + 64) + sizeof(R) * (size_t)g.os * S;
   if (sh > lds_limit()) {
     set_error("gridding tile needs %zu B of LDS", sh);
     return ARLMI_E_UNSUPPORTED;
   }
   if (p->ngitems == 0) return ARLMI_OK;
-  ARLMI_LAUNCH("k_grid_sep", (k_grid_sep<R, S>), dim3((unsigned)p->ngitems), dim3(256), sh, s, p->d_gitems,
+  ARLMI_LAUNCH("k_grid_sep", (k_grid_sep<R, S>), dim3((unsigned)p->ngitems), dim3(GRID_THREADS), sh, s, p->d_gitems,
                p->d_entries, (const cplx<R>*)vis, (const R*)wt, dopsf, (const R*)ktab, g.os, (cplx<R>*)grids, partial,
                sumwt, g.tile, g.ntx, g.n);
   ARLMI_LAUNCH_CHECK();
@@ -458,15 +589,28 @@ static int grid_launch(const arlmi_plan* p, const void* vis, const void* wt, int
     set_error("separable kernel support %d not instantiated; pass the full table", g.support);
     return ARLMI_E_UNSUPPORTED;
   }
-  const size_t sh = sizeof(double) * 2 * (size_t)g.tile * lds_stride(g.tile);
+  const size_t sh = sizeof(double) * 2 * ((size_t)g.tile * lds_stride(g.tile) + 64);
   if (sh > lds_limit()) {
     set_error("gridding tile (T=%d, S=%d) needs %zu B of LDS; use a smaller tile", g.tile, g.support, sh);
     return ARLMI_E_UNSUPPORTED;
   }
   if (p->ngitems == 0) return ARLMI_OK;
-  ARLMI_LAUNCH("k_grid_tab", k_grid_tab<R>, dim3((unsigned)p->ngitems), dim3(256), sh, s, p->d_gitems, p->d_entries,
-               g.nz > 1 ? p->d_entry_z : nullptr, (const cplx<R>*)vis, (const R*)wt, dopsf,
-               (const cplx<R>*)k->d_table, g.os, g.support, (cplx<R>*)grids, partial, sumwt, g.tile, g.ntx, g.n);
+  const int taps = g.support * g.support;
+  const uint16_t* ez = g.nz > 1 ? p->d_entry_z : nullptr;
+  const cplx<R>* tab = (const cplx<R>*)k->d_table;
+#define ARLMI_GRID_TAB(MT)                                                                                        \
+  ARLMI_LAUNCH("k_grid_tab", (k_grid_tab<R, MT>), dim3((unsigned)p->ngitems), dim3(GRID_THREADS), sh, s,        \
+               p->d_gitems, p->d_entries, ez, (const cplx<R>*)vis, (const R*)wt, dopsf, tab, g.os, g.support,     \
+               (cplx<R>*)grids, partial, sumwt, g.tile, g.ntx, g.n)
+  // (a variant holding 8 taps per lane gave wrong sums in the single precision build and is
+  // not used: supports above 16 take the loop without prefetch)
+  if (taps <= 64)
+    ARLMI_GRID_TAB(1);
+  else if (taps <= 256)
+    ARLMI_GRID_TAB(4);
+  else
+    ARLMI_GRID_TAB(0);
+#undef ARLMI_GRID_TAB
   ARLMI_LAUNCH_CHECK();
   return ARLMI_OK;
 }

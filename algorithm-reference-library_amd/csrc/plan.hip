@@ -81,14 +81,21 @@ struct Members {
 __device__ inline Members slab_members(double w, const double* boxes, int nb, double wslice) {
   Members m;
   m.n = 0;
+  m.k[0] = m.k[1] = m.k[2] = 0;
   const double halfw = 0.5 * wslice;
   if (!(wslice > 0.0)) return m;  // all |w| zero: |w - box| < 0 never holds
   const double kd = fmin(fmax(rint((w - boxes[0]) / wslice), -2.0), (double)nb + 1.0);
   const int k0 = (int)kd;
-  for (int k = k0 - 1; k <= k0 + 1; ++k) {
-    if (k < 0 || k >= nb) continue;
-    if (fabs(w - boxes[k]) < halfw) m.k[m.n++] = k;
-  }
+  // the three candidate slabs around the nearest one, compacted with selects (no indexed
+  // writes, so the result stays in registers)
+  const int c0 = k0 - 1, c1 = k0, c2 = k0 + 1;
+  const bool v0 = c0 >= 0 && c0 < nb && fabs(w - boxes[c0]) < halfw;
+  const bool v1 = c1 >= 0 && c1 < nb && fabs(w - boxes[c1]) < halfw;
+  const bool v2 = c2 >= 0 && c2 < nb && fabs(w - boxes[c2]) < halfw;
+  m.n = (int)v0 + (int)v1 + (int)v2;
+  m.k[0] = v0 ? c0 : (v1 ? c1 : c2);
+  m.k[1] = (v0 && v1) ? c1 : c2;
+  m.k[2] = c2;
   return m;
 }
 
@@ -106,9 +113,12 @@ __global__ void k_slab_sums(const double* __restrict__ uvw, int64_t n, const dou
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const double w = uvw[3 * i + 2];
     Members m = slab_members(w, boxes, nb, wslice);
-    for (int j = 0; j < m.n; ++j) {
-      atomicAdd(&ls[m.k[j]], w);
-      atomicAdd(&lc[m.k[j]], 1u);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      if (j < m.n) {
+        atomicAdd(&ls[m.k[j]], w);
+        atomicAdd(&lc[m.k[j]], 1u);
+      }
     }
   }
   __syncthreads();
@@ -145,18 +155,24 @@ __device__ inline Enum enumerate_row(const DevGeom& g, const double* __restrict_
   e.valid = false;
   e.dropped = false;
   e.nm = 0;
+  e.plane[0] = e.plane[1] = e.plane[2] = 0;
+  e.zp[0] = e.zp[1] = e.zp[2] = 0;
   const double u = uvw[3 * row + 0], v = uvw[3 * row + 1], w = uvw[3 * row + 2];
   const double f = freq ? freq[row] : 0.0;
   uint32_t err = 0;
   if (g.mode == ARLMI_MODE_WSTACK) {
     Members m = slab_members(w, boxes, g.nplanes, wslice);
     if (m.n == 0) return e;  // in no slab: never mapped, never asserted (imaging_serial.py:93-95)
-    for (int j = 0; j < m.n; ++j) {
-      const double ws = w - plane_w[m.k[j]];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
       e.plane[j] = m.k[j];
-      // grid WW axis of a 1-plane GridData (nw = 1, wstep = 1e15) is checked against the kernel's nz
-      (void)map_wplane(ws, g.wstep_grid, g.crpix_wg, g.nz, &err);
-      e.zp[j] = map_wplane(ws, g.zstep, g.crpix_z, g.nz, &err);
+      e.zp[j] = 0;
+      if (j < m.n) {
+        const double ws = w - plane_w[m.k[j]];
+        // grid WW axis of a 1-plane GridData (nw = 1, wstep = 1e15) is checked against the kernel's nz
+        (void)map_wplane(ws, g.wstep_grid, g.crpix_wg, g.nz, &err);
+        e.zp[j] = map_wplane(ws, g.zstep, g.crpix_z, g.nz, &err);
+      }
     }
     e.nm = m.n;
   } else {
@@ -224,7 +240,9 @@ __global__ void k_p1_count(DevGeom g, const double* __restrict__ uvw, const doub
       atomicAdd(&lh[P], 1u);
       if (e.dropped) atomicAdd(&lh[P + 1], 1u);
     } else {
-      for (int j = 0; j < e.nm; ++j) atomicAdd(&lh[e.plane[j]], 1u);
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        if (j < e.nm) atomicAdd(&lh[e.plane[j]], 1u);
       atomicAdd(&lh[P + 2], (uint32_t)e.nm);
     }
   }
@@ -254,7 +272,9 @@ __global__ void k_p1_scatter(DevGeom g, const double* __restrict__ uvw, const do
       recs[pos] = Rec{(uint32_t)row, 0u, 0u, 0u};
       continue;
     }
-    for (int j = 0; j < e.nm; ++j) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      if (j >= e.nm) break;
       const uint32_t pos = atomicAdd(&cur[e.plane[j]], 1u);
       Rec r;
       r.row = (uint32_t)row;

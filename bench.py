@@ -56,8 +56,42 @@ def parse():
     ap.add_argument("--slices", type=int, default=64)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-rows", type=int, default=3000)
+    ap.add_argument("--cpu-rows", type=int, default=300000)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 1 GiB device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
+
+
+COPY_PROBE_BYTES = 1 << 30
+
+
+def copy_probe():
+    """A plain elementwise copy of COPY_PROBE_BYTES (float32 src -> dst) three times: under
+    rocprofv3 --pmc its FETCH_SIZE / WRITE_SIZE per dispatch give the factor that turns those
+    counters into HBM bytes."""
+    import torch
+    n = COPY_PROBE_BYTES // 4
+    a = torch.ones(n, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        torch.mul(a, 1.0, out=b)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": COPY_PROBE_BYTES, "launches": 3}))
+
+
+def measured_traffic(kernel, N, P):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE + WRITE_SIZE
+    summary for this workload (profiles/traffic_wstack_2048.json), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic_wstack_%d.json" % N)
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("npixel") != N or t.get("slices") != P:
+        return None
+    k = t.get("kernels", {}).get(kernel)
+    return None if k is None else k.get("hbm_bytes_per_launch")
 
 
 def step(imager, vis, model, out_img, out_vis):
@@ -153,6 +187,9 @@ def cpu_baseline(args):
 
 def main():
     args = parse()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: every box in this setup has a single MI355X, and the "
               "multi-GPU path (one all-reduce of partial images) is described, not run.", file=sys.stderr)
@@ -177,7 +214,7 @@ def main():
     if by is not None:
         ach = by / (per_launch_ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "kernel": name, "achieved": round(ach, 1), "peak": 8000.0, "unit": "GB/s",
-                "frac": round(ach / 8000.0, 4), "traffic": None,
+                "frac": round(ach / 8000.0, 4), "traffic": measured_traffic(name, N, P),
                 "bytes_per_launch": by, "avg_launch_ms": round(per_launch_ms, 4)}
     extra = {}
     if r64 is not None:

@@ -83,7 +83,7 @@ def test_convolution_mapping_bitexact(gpu, os_, S):
 
 
 @pytest.mark.parametrize("precision", [64, 32])
-@pytest.mark.parametrize("S", [6, 8])
+@pytest.mark.parametrize("S", [4, 6, 8, 10, 12, 16])
 def test_grid_parity(gpu, precision, S):
     """grid_visibility_to_griddata: GPU grid vs oracle grid (relative to peak), sumwt."""
     import arlmi
@@ -234,11 +234,13 @@ def test_awterm_kernel_parity(gpu):
 
 
 @pytest.mark.parametrize("precision", [64, 32])
-def test_wprojection_grid_degrid_parity(gpu, precision):
-    """Gridding/degridding with a complex multi-plane (AW) kernel table."""
+@pytest.mark.parametrize("S", [8, 16, 22, 32])
+def test_wprojection_grid_degrid_parity(gpu, precision, S):
+    """Gridding/degridding with a complex multi-plane (AW) kernel table; the supports cover
+    every taps-per-lane variant of the table gridder (64, 256, 512 taps and the generic loop)."""
     import arlmi
     rng = numpy.random.default_rng(9)
-    n, cell, S = 256, 5e-4, 16
+    n, cell = 256, 5e-4
     d = random_rows(rng, 8000, n, cell, S, wmax=80.0)
     im = device_image(n, cell, precision)
     gcf, cf = arlmi.create_awterm_convolutionfunction(im, nw=9, wstep=20.0, oversampling=4, support=S)
