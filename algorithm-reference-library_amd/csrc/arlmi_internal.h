@@ -205,6 +205,8 @@ struct arlmi_plan {
   int4* d_ditems = nullptr;              // {tile, start, count, 0}
   int4* d_multi = nullptr;               // {tile, first slot, slots, 0} for split tiles
   void* d_partial = nullptr;             // partial tiles of split tiles (grid precision)
+  void* d_img_part = nullptr;            // per-plane-group partial images of the invert row pass
+  int64_t cap_img_part_bytes = 0;
   int64_t cap_h1 = 0, cap_s1 = 0, cap_h2 = 0, cap_s2 = 0, cap_scan = 0, cap_recs = 0, cap_blk2 = 0, cap_entries = 0, cap_z = 0,
           cap_gitems = 0, cap_ditems = 0, cap_partial_bytes = 0;
   // host copies

@@ -10,12 +10,17 @@
 // Fused around the passes:
 //   image -> grids (predict): model * gcf * conj(b_p) for every w plane p, in the row pass
 //     (gridding.py:292-302 with wstack_single.py:50-59, b_p = exp(-2 pi i w_p (1 - n)));
-//   grids -> image (invert): Re(b_p * ifft(G_p)) summed over planes, times gcf, divided by the
-//     sum of weights, in the column pass (gridding.py:270-289, wstack_single.py:93-94,
-//     imaging_serial.py:193-196, imaging/base.py:84-101).
+//     the columns follow.
+//   grids -> image (invert): the columns first (in place), then a row pass that forms
+//     Re(b_p * ifft(G_p)) and sums it over planes (gridding.py:270-289, wstack_single.py:93-94,
+//     imaging_serial.py:193-196); with several plane groups each group writes a partial image
+//     and a finishing pass adds them; the gcf product and the division by the sum of weights
+//     (imaging/base.py:84-101) close the last pass.  Both fused passes read and write whole
+//     rows, and the planes are split into groups so that a launch has thousands of blocks.
 // The screen phase, the gcf product and the weight division are evaluated in float64.
 #include "arlmi_internal.h"
 
+#include <algorithm>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -248,6 +253,8 @@ struct FftArgs {
   void* img_im;
   int normalize, screens, wstack;
   double fov;            // N * cellsize
+  int ppg;               // planes per plane group (fused row passes)
+  void* part;            // invert: partial images [groups][chan][N][N] (x2 with the imaginary part)
 };
 
 // ---------------------------------------------------------------------------
@@ -287,6 +294,7 @@ __global__ __launch_bounds__(Shape<N>::NT * Shape<N>::ROW_NB) void k_rows_plain(
 }
 
 // Predict: grids[p][c][y][:] = rowfft((-1)^(x+y) * model[c][y][x] * gcf[c][y][x] * conj(b_p(x,y)))
+// Grid: (row blocks of every channel, plane groups).
 template <typename R, int N>
 __global__ __launch_bounds__(Shape<N>::NT * Shape<N>::ROW_NB) void k_rows_predict(FftArgs a, const cplx<R>* __restrict__ twg) {
   constexpr int NT = Shape<N>::NT, NB = Shape<N>::ROW_NB, THR = NT * NB, EPT = N * NB / THR;
@@ -313,7 +321,8 @@ __global__ __launch_bounds__(Shape<N>::NT * Shape<N>::ROW_NB) void k_rows_predic
     if (!a.screens) out[e] = false;
   }
   const int b = threadIdx.x / NT, t = threadIdx.x % NT;
-  for (int p = 0; p < a.nplanes; ++p) {
+  const int p0 = blockIdx.y * a.ppg, p1 = min(a.nplanes, p0 + a.ppg);  // this block's plane group
+  for (int p = p0; p < p1; ++p) {
     const double wp = a.screens ? a.plane_w[p] : 0.0;
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
@@ -381,56 +390,60 @@ __global__ __launch_bounds__(Shape<N>::NT * Shape<N>::template col_nb<R>()) void
   }
 }
 
-// Invert: img[c][y][x] = gcf / S_c * sum_p Re(b_p * (-1)^(x+y) * colifft(G_p)).
-// SCREENS: w-stack planes (real part after the screen); otherwise the planes are summed and the
-// imaginary part is kept as well.  Single precision accumulates in float, double in double.
+// Invert, after the in-place column pass: for plane group g (blockIdx.y) and the NB rows of this
+// block, acc = sum_p Re(b_p * (-1)^(x+y) * rowifft(G_p)) over the group's planes.  With one group
+// the block writes img = acc * gcf / S_c (imaging_serial.py:193-196 with normalize_sumwt,
+// imaging/base.py:84-101); otherwise it writes acc to its partial image for k_invert_finish.
+// SCREENS: w-stack planes (real part after the screen); otherwise the planes are summed and
+// the imaginary part is kept as well.  Single precision accumulates in float, double in double.
 template <typename R, int N, bool SCREENS>
-__global__ __launch_bounds__(Shape<N>::NT * Shape<N>::template inv_nb<R>()) void k_cols_invert(FftArgs a, const cplx<R>* __restrict__ twg) {
-  constexpr int NT = Shape<N>::NT, NB = Shape<N>::template inv_nb<R>(), THR = NT * NB, EPT = N * NB / THR;
+__global__ __launch_bounds__(Shape<N>::NT * Shape<N>::ROW_NB) void k_rows_invert(FftArgs a, const cplx<R>* __restrict__ twg) {
+  constexpr int NT = Shape<N>::NT, NB = Shape<N>::ROW_NB, THR = NT * NB, EPT = N * NB / THR;
   extern __shared__ __align__(16) unsigned char smem[];
   cplx<R>* lds = reinterpret_cast<cplx<R>*>(smem);
   cplx<R>* tw = stage_twiddles_at<R, N>(smem, sizeof(cplx<R>) * (size_t)N * NB, twg);
   const cplx<R>* grids = reinterpret_cast<const cplx<R>*>(a.data);
-  constexpr int CB = N / NB;
-  const int bid = xcd_block(blockIdx.x, gridDim.x);
-  const int c = bid / CB;
-  const int x0 = (bid % CB) * NB;
+  constexpr int RB = N / NB;  // row blocks per channel
+  const int c = blockIdx.x / RB;
+  const int y0 = (blockIdx.x % RB) * NB;
+  const int G = gridDim.y;
+  const int p0 = blockIdx.y * a.ppg, p1 = min(a.nplanes, p0 + a.ppg);
   R acc[EPT];
   R acci[SCREENS ? 1 : EPT];
+  double q[SCREENS ? EPT : 1];
+  bool out[SCREENS ? EPT : 1];
 #pragma unroll
-  for (int e = 0; e < EPT; ++e) acc[e] = R(0);
-  if constexpr (!SCREENS) {
-#pragma unroll
-    for (int e = 0; e < EPT; ++e) acci[e] = R(0);
+  for (int e = 0; e < EPT; ++e) {
+    acc[e] = R(0);
+    if constexpr (!SCREENS) {
+      acci[e] = R(0);
+    } else {
+      const int i = threadIdx.x + e * THR;
+      q[e] = screen_q(y0 + i / N, i % N, N, a.fov, &out[e]);
+    }
   }
-  double tot = 0.0;
-  const int b = threadIdx.x % NB, t = threadIdx.x / NB;
-  for (int p = 0; p < a.nplanes; ++p) {
+  const int b = threadIdx.x / NT, t = threadIdx.x % NT;
+  for (int p = p0; p < p1; ++p) {
     const double sw = a.sumwt[(size_t)p * a.nchan + c];
-    tot += sw;
     if (a.wstack && a.normalize && !(sw > 0.0)) continue;  // normalize_sumwt zeroes this slab's image
-    const cplx<R>* base = grids + ((size_t)p * a.nchan + c) * N * N;
+    const cplx<R>* base = grids + ((size_t)p * a.nchan + c) * N * N + (size_t)y0 * N;
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
       const int i = threadIdx.x + e * THR;
-      const int n = i / NB, bb = i % NB;
-      lds[lidx<true, N, NB>(bb, n)] = base[(size_t)n * N + x0 + bb];
+      lds[lidx<false, N, NB>(i / N, i % N)] = base[i];
     }
     __syncthreads();
-    stages<R, N, NB, NT, 1, true, true>(lds, b, t, tw);
+    stages<R, N, NB, NT, 1, true, false>(lds, b, t, tw);
     const double wp = SCREENS ? a.plane_w[p] : 0.0;
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
       const int i = threadIdx.x + e * THR;
-      const int n = i / NB, bb = i % NB;
-      const int x = x0 + bb, y = n;
-      cplx<R> v = lds[lidx<true, N, NB>(bb, n)];
+      const int bb = i / N, x = i % N, y = y0 + bb;
+      cplx<R> v = lds[lidx<false, N, NB>(bb, x)];
       if ((x + y) & 1) v = cmk<R>(-v.x, -v.y);
       if constexpr (SCREENS) {
-        bool outside;
-        const double qq = screen_q(y, x, N, a.fov, &outside);
-        if (!outside) {
-          const cplx<R> s = expi_turns<R>(screen_turns(wp, qq));
+        if (!out[e]) {
+          const cplx<R> s = expi_turns<R>(screen_turns(wp, q[e]));
           acc[e] += s.x * v.x - s.y * v.y;
         }
       } else {
@@ -440,14 +453,27 @@ __global__ __launch_bounds__(Shape<N>::NT * Shape<N>::template inv_nb<R>()) void
     }
     __syncthreads();
   }
+  if (G > 1) {
+    R* part = reinterpret_cast<R*>(a.part);
+    const size_t gsz = (size_t)a.nchan * N * N;
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const int i = threadIdx.x + e * THR;
+      const size_t gi = ((size_t)c * N + y0 + i / N) * N + i % N;
+      part[(size_t)blockIdx.y * gsz + gi] = acc[e];
+      if constexpr (!SCREENS) part[(size_t)(G + blockIdx.y) * gsz + gi] = acci[e];
+    }
+    return;
+  }
+  double tot = 0.0;
+  for (int pp = 0; pp < a.nplanes; ++pp) tot += a.sumwt[(size_t)pp * a.nchan + c];
   R* img = reinterpret_cast<R*>(a.img_re);
   R* imi = reinterpret_cast<R*>(a.img_im);
   const bool zero = a.normalize && !(tot > 0.0);
 #pragma unroll
   for (int e = 0; e < EPT; ++e) {
     const int i = threadIdx.x + e * THR;
-    const int n = i / NB, bb = i % NB;
-    const size_t gi = ((size_t)c * N + n) * N + x0 + bb;
+    const size_t gi = ((size_t)c * N + y0 + i / N) * N + i % N;
     const double g = a.gcf[gi];
     double re = (double)acc[e] * g, im = SCREENS ? 0.0 : (double)acci[SCREENS ? 0 : e] * g;
     if (a.normalize) {
@@ -457,6 +483,42 @@ __global__ __launch_bounds__(Shape<N>::NT * Shape<N>::template inv_nb<R>()) void
     img[gi] = (R)re;
     if (imi) imi[gi] = (R)im;
   }
+}
+
+// Adds the G partial images in group order (float64), then applies gcf and the weight
+// normalisation as k_rows_invert does for a single group.  Grid: (pixel blocks, channels).
+template <typename R>
+__global__ __launch_bounds__(256) void k_invert_finish(FftArgs a, int n, int G) {
+  __shared__ double stot;
+  const int c = blockIdx.y;
+  if (threadIdx.x < 64) {
+    double sw = 0.0;
+    for (int p = threadIdx.x; p < a.nplanes; p += 64) sw += a.sumwt[(size_t)p * a.nchan + c];
+    for (int o = 32; o > 0; o >>= 1) sw += __shfl_xor(sw, o);
+    if (threadIdx.x == 0) stot = sw;
+  }
+  __syncthreads();
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= (int64_t)n * n) return;
+  const size_t gsz = (size_t)a.nchan * n * n;
+  const size_t gi = (size_t)c * n * n + (size_t)i;
+  const R* part = reinterpret_cast<const R*>(a.part);
+  double re = 0.0, im = 0.0;
+  for (int g = 0; g < G; ++g) {
+    re += (double)part[(size_t)g * gsz + gi];
+    if (!a.screens) im += (double)part[(size_t)(G + g) * gsz + gi];
+  }
+  const double tot = stot;
+  const double gc = a.gcf[gi];
+  re *= gc;
+  im *= gc;
+  if (a.normalize) {
+    const bool zero = !(tot > 0.0);
+    re = zero ? 0.0 : re / tot;
+    im = zero ? 0.0 : im / tot;
+  }
+  reinterpret_cast<R*>(a.img_re)[gi] = (R)re;
+  if (a.img_im) reinterpret_cast<R*>(a.img_im)[gi] = (R)im;
 }
 
 // ---------------------------------------------------------------------------
@@ -573,6 +635,13 @@ static int run_fft2d(FftArgs a, int direction, hipStream_t s) {
   return ARLMI_OK;
 }
 
+// Plane groups for the fused row passes: enough blocks to fill the GPU several times over.
+static void plane_groups(int rowblocks, int nplanes, int* ppg, int* groups) {
+  const int want = std::max(1, std::min(nplanes, std::min(16, (8192 + rowblocks - 1) / rowblocks)));
+  *ppg = (nplanes + want - 1) / want;
+  *groups = (nplanes + *ppg - 1) / *ppg;
+}
+
 template <typename R, int N>
 static int run_grids_to_image(FftArgs a, hipStream_t s) {
   const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(twiddles(N, sizeof(R) == 4 ? 32 : 64));
@@ -582,22 +651,30 @@ static int run_grids_to_image(FftArgs a, hipStream_t s) {
   }
   constexpr int NT = Shape<N>::NT, RNB = Shape<N>::ROW_NB, CNB = Shape<N>::template col_nb<R>();
   const size_t rsh = sizeof(cplx<R>) * (size_t)N * (RNB + 1), csh = sizeof(cplx<R>) * (size_t)N * (CNB + 1);
-  FftArgs r = a;
-  r.in_mod = 1;
-  r.out_mod = 0;
-  r.scale = 1.0;
-  r.nbatch = a.nplanes * a.nchan;
-  const unsigned rblocks = (unsigned)((int64_t)r.nbatch * N / RNB);
-  ARLMI_LAUNCH("k_rows_plain", (k_rows_plain<R, N, true>), dim3(rblocks), dim3(NT * RNB), rsh, s, r, tw);
+  FftArgs c = a;
+  c.in_mod = 1;
+  c.out_mod = 0;
+  c.scale = 1.0;
+  c.nbatch = a.nplanes * a.nchan;
+  const unsigned cblocks = (unsigned)((int64_t)c.nbatch * N / CNB);
+  ARLMI_LAUNCH("k_cols_plain", (k_cols_plain<R, N, true>), dim3(cblocks), dim3(NT * CNB), csh, s, c, tw);
   ARLMI_LAUNCH_CHECK();
-  constexpr int INB = Shape<N>::template inv_nb<R>();
-  const size_t ish = sizeof(cplx<R>) * (size_t)N * (INB + 1);
-  const unsigned cblocks = (unsigned)((int64_t)a.nchan * N / INB);
+  const int rowblocks = a.nchan * (N / RNB);
+  int G = 1;
+  plane_groups(rowblocks, a.nplanes, &a.ppg, &G);
+  if (G > 1 && !a.part) G = 1, a.ppg = a.nplanes;
   if (a.screens)
-    ARLMI_LAUNCH("k_cols_invert", (k_cols_invert<R, N, true>), dim3(cblocks), dim3(NT * INB), ish, s, a, tw);
+    ARLMI_LAUNCH("k_rows_invert", (k_rows_invert<R, N, true>), dim3((unsigned)rowblocks, (unsigned)G), dim3(NT * RNB),
+                 rsh, s, a, tw);
   else
-    ARLMI_LAUNCH("k_cols_invert", (k_cols_invert<R, N, false>), dim3(cblocks), dim3(NT * INB), ish, s, a, tw);
+    ARLMI_LAUNCH("k_rows_invert", (k_rows_invert<R, N, false>), dim3((unsigned)rowblocks, (unsigned)G),
+                 dim3(NT * RNB), rsh, s, a, tw);
   ARLMI_LAUNCH_CHECK();
+  if (G > 1) {
+    ARLMI_LAUNCH("k_invert_finish", k_invert_finish<R>, dim3((unsigned)(((int64_t)N * N + 255) / 256), (unsigned)a.nchan),
+                 dim3(256), 0, s, a, N, G);
+    ARLMI_LAUNCH_CHECK();
+  }
   return ARLMI_OK;
 }
 
@@ -610,8 +687,11 @@ static int run_image_to_grids(FftArgs a, hipStream_t s) {
   }
   constexpr int NT = Shape<N>::NT, RNB = Shape<N>::ROW_NB, CNB = Shape<N>::template col_nb<R>();
   const size_t rsh = sizeof(cplx<R>) * (size_t)N * (RNB + 1), csh = sizeof(cplx<R>) * (size_t)N * (CNB + 1);
-  const unsigned rblocks = (unsigned)((int64_t)a.nchan * N / RNB);
-  ARLMI_LAUNCH("k_rows_predict", (k_rows_predict<R, N>), dim3(rblocks), dim3(NT * RNB), rsh, s, a, tw);
+  const int rowblocks = a.nchan * (N / RNB);
+  int G = 1;
+  plane_groups(rowblocks, a.nplanes, &a.ppg, &G);
+  ARLMI_LAUNCH("k_rows_predict", (k_rows_predict<R, N>), dim3((unsigned)rowblocks, (unsigned)G), dim3(NT * RNB), rsh,
+               s, a, tw);
   ARLMI_LAUNCH_CHECK();
   FftArgs c = a;
   c.in_mod = 0;
@@ -705,6 +785,22 @@ int arlmi_grids_to_image(const arlmi_plan* p, void* d_grids, const double* d_sum
   a.wstack = p->g.mode == ARLMI_MODE_WSTACK;
   a.screens = a.wstack;
   a.fov = (double)p->g.n * p->g.cellsize;
+  // partial images of the plane groups live with the plan (grown on demand)
+  const int n = p->g.n, nt = n >= 16 ? n / 16 : 1, rnb = nt >= 256 ? 1 : 256 / nt;
+  int ppg = 1, G = 1;
+  plane_groups(a.nchan * (n / rnb), a.nplanes, &ppg, &G);
+  if (G > 1) {
+    arlmi_plan* mp = const_cast<arlmi_plan*>(p);
+    const int64_t need = (int64_t)G * (a.screens ? 1 : 2) * a.nchan * (int64_t)n * n * (precision / 8);
+    if (need > mp->cap_img_part_bytes || !mp->d_img_part) {
+      if (mp->d_img_part) ARLMI_HIP(hipFree(mp->d_img_part));
+      mp->d_img_part = nullptr;
+      mp->cap_img_part_bytes = 0;
+      ARLMI_HIP(hipMalloc(&mp->d_img_part, (size_t)need));
+      mp->cap_img_part_bytes = need;
+    }
+    a.part = mp->d_img_part;
+  }
   return precision == 32 ? dispatch_n<float>(Op::TO_IMAGE, p->g.n, a, 0, as_stream(stream))
                          : dispatch_n<double>(Op::TO_IMAGE, p->g.n, a, 0, as_stream(stream));
 }

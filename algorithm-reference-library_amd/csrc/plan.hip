@@ -615,7 +615,8 @@ static void free_plan(arlmi_plan* p) {
   void* ptrs[] = {p->d_boxes, p->d_plane_w, p->d_wsum, p->d_wcnt, p->d_cnt, p->d_cntd, p->d_off, p->d_nch,
                   p->d_nchd, p->d_nslot, p->d_choff, p->d_choffd, p->d_slotoff, p->d_misc, p->d_bounds,
                   p->d_hbase, p->d_nbp, p->d_h1, p->d_s1, p->d_h2, p->d_s2, p->d_scan_tmp, p->d_recs, p->d_blk2,
-                  p->d_entries, p->d_entry_z, p->d_gitems, p->d_ditems, p->d_multi, p->d_partial};
+                  p->d_entries, p->d_entry_z, p->d_gitems, p->d_ditems, p->d_multi, p->d_partial,
+                  p->d_img_part};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
 }
