@@ -56,7 +56,7 @@ def parse():
     ap.add_argument("--slices", type=int, default=64)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-rows", type=int, default=300000)
+    ap.add_argument("--cpu-rows", type=int, default=1000000)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
