@@ -25,6 +25,7 @@ ERRORS = {
 MODE_2D = 0
 MODE_WSTACK = 1
 FLAG_DROP_OFF_GRID = 1
+FLAG_SPARSE_GRIDS = 2
 KERNEL_SEPARABLE = 0
 KERNEL_TABLE = 1
 

@@ -64,15 +64,18 @@ class Imager:
     """
 
     def __init__(self, vis, im, gcfcf=None, mode="2d", vis_slices=1, slab_boxes=None, tile=0,
-                 drop_off_grid=False, **kwargs):
+                 drop_off_grid=False, sparse=True, **kwargs):
         self.gcf, self.cf = _gcfcf(im, gcfcf, kwargs)
         self.im = im
         self.precision = vis.precision
+        # the imager's grids never leave it, so they can be sparse (tiles no row touches are
+        # neither written nor read)
         if mode == "2d":
-            self.plan = Plan(vis, im, self.cf, _lib.MODE_2D, 1, tile=tile, drop_off_grid=drop_off_grid)
+            self.plan = Plan(vis, im, self.cf, _lib.MODE_2D, 1, tile=tile, drop_off_grid=drop_off_grid,
+                             sparse=sparse)
         elif mode == "wstack":
             self.plan = Plan(vis, im, self.cf, _lib.MODE_WSTACK, vis_slices, tile=tile, drop_off_grid=drop_off_grid,
-                             slab_boxes=slab_boxes)
+                             slab_boxes=slab_boxes, sparse=sparse)
         else:
             raise ValueError("mode must be '2d' or 'wstack'")
         self.mode = mode

@@ -65,12 +65,15 @@ __host__ __device__ inline int entry_yoff(uint32_t p) { return (int)((p >> 23) &
 // Result of ARL's convolution_mapping for one row (gridding.py:24-96).
 // Work groups of one launch are dispatched round-robin over the 8 XCDs of an MI355X, each
 // with its own L2.  xcd_block maps the hardware block index to a logical one under which
-// XCD k runs the k-th contiguous eighth of the logical blocks, so logical neighbours that
-// read the same cache lines (adjacent column groups, overlapping tile halos) share an L2.
-__device__ inline int xcd_block(int bid, int n) {
-  const int per = n >> 3;
-  if (bid >= (per << 3)) return bid;
-  return (bid & 7) * per + (bid >> 3);
+// every run of G consecutive logical blocks (neighbours that read the same cache lines, such
+// as adjacent column groups) is dispatched to one XCD in consecutive slots, while successive
+// runs go round-robin over the XCDs (so uneven work along the logical order stays balanced).
+__device__ inline int xcd_block(int bid, int n, int G) {
+  const int span = 8 * G;
+  const int full = (n / span) * span;
+  if (bid >= full) return bid;
+  const int xcd = bid & 7, slot = bid >> 3;
+  return ((slot / G) * 8 + xcd) * G + slot % G;
 }
 
 struct RowMap {
@@ -182,6 +185,10 @@ struct arlmi_plan {
   double* d_wsum = nullptr;              // [P]
   unsigned long long* d_wcnt = nullptr;  // [P]
   uint32_t* d_cnt = nullptr;             // [ntiles] entries per tile
+  // occupancy bit masks (sparse grids, at most 32 tiles per side), pc = plane * nchan + chan:
+  uint32_t* d_tilemask = nullptr;        // [pc][tx]: bit ty set if tile (pc, ty, tx) has entries
+  uint32_t* d_colmask = nullptr;         // [pc]: bit tx set if tile column (pc, tx) has entries
+  void* d_zeros = nullptr;               // 1 KiB of zeros (masked-out loads of the FFT passes)
   uint32_t* d_cntd = nullptr;            // [ntiles] degrid entries per tile
   uint32_t* d_off = nullptr;             // [ntiles] first entry of a tile
   uint32_t* d_nch = nullptr;             // [ntiles] grid chunks per tile

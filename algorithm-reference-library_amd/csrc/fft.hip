@@ -185,10 +185,16 @@ struct Shape {
     constexpr int nb = col_nb<R>() / 2;
     return nb < 1 ? 1 : nb;
   }
+  // the predict row pass (screens per plane, stores): N/8 threads per row, one radix-8
+  // butterfly per thread and stage, fewer elements (and registers) per thread than the
+  // plain passes
+  static constexpr int FNT = N >= 8 ? N / 8 : 1;
+  static constexpr int FNB = FNT >= 512 ? 1 : 512 / FNT;
   template <typename R>
   static constexpr bool fits() {
     return (size_t)N * sizeof(cplx<R>) * (size_t)(ROW_NB + 1) <= 163840 &&
-           (size_t)N * sizeof(cplx<R>) * (size_t)(col_nb<R>() + 1) <= 163840;
+           (size_t)N * sizeof(cplx<R>) * (size_t)(col_nb<R>() + 1) <= 163840 &&
+           (size_t)N * sizeof(cplx<R>) * (size_t)(FNB + 1) <= 163840;
   }
 };
 
@@ -255,7 +261,24 @@ struct FftArgs {
   double fov;            // N * cellsize
   int ppg;               // planes per plane group (fused row passes)
   void* part;            // invert: partial images [groups][chan][N][N] (x2 with the imaginary part)
+  // sparse grids (at most 32 tiles per side): tiles of side `tile`, ntx per side; for
+  // plane-channel pc, colmask[pc] has bit tx set if tile column tx holds entries and
+  // tilemask[pc * ntx + tx] bit ty if tile (ty, tx) does; zero_fill: tiles without entries were
+  // never written and read as 0; zeros: 64 zero entries
+  const uint32_t* colmask;
+  const uint32_t* tilemask;
+  const void* zeros;
+  int zero_fill;
+  int tile, ntx;
 };
+
+// Masked-out loads read zeros from a small buffer instead of the grid (lane l reads entry
+// l & 63, 16 bytes at most per entry), so that every load of a pass is issued unconditionally
+// and the loads of a thread stay in flight together.
+template <typename R>
+__device__ __forceinline__ const cplx<R>* zero_entry(const void* zeros) {
+  return reinterpret_cast<const cplx<R>*>(zeros) + (threadIdx.x & 63);
+}
 
 // ---------------------------------------------------------------------------
 // row passes
@@ -296,8 +319,8 @@ __global__ __launch_bounds__(Shape<N>::NT * Shape<N>::ROW_NB) void k_rows_plain(
 // Predict: grids[p][c][y][:] = rowfft((-1)^(x+y) * model[c][y][x] * gcf[c][y][x] * conj(b_p(x,y)))
 // Grid: (row blocks of every channel, plane groups).
 template <typename R, int N>
-__global__ __launch_bounds__(Shape<N>::NT * Shape<N>::ROW_NB) void k_rows_predict(FftArgs a, const cplx<R>* __restrict__ twg) {
-  constexpr int NT = Shape<N>::NT, NB = Shape<N>::ROW_NB, THR = NT * NB, EPT = N * NB / THR;
+__global__ __launch_bounds__(Shape<N>::FNT * Shape<N>::FNB) void k_rows_predict(FftArgs a, const cplx<R>* __restrict__ twg) {
+  constexpr int NT = Shape<N>::FNT, NB = Shape<N>::FNB, THR = NT * NB, EPT = N * NB / THR;
   extern __shared__ __align__(16) unsigned char smem[];
   cplx<R>* lds = reinterpret_cast<cplx<R>*>(smem);
   cplx<R>* tw = stage_twiddles_at<R, N>(smem, sizeof(cplx<R>) * (size_t)N * NB, twg);
@@ -342,11 +365,12 @@ __global__ __launch_bounds__(Shape<N>::NT * Shape<N>::ROW_NB) void k_rows_predic
     __syncthreads();
     stages<R, N, NB, NT, 1, false, false>(lds, b, t, tw);
     cplx<R>* dst = grids + ((size_t)p * a.nchan + c) * N * N;
+    const uint32_t cm = a.colmask ? a.colmask[(size_t)p * a.nchan + c] : ~0u;
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
       const int i = threadIdx.x + e * THR;
       const int bb = i / N, x = i % N;
-      dst[(size_t)(y0 + bb) * N + x] = lds[lidx<false, N, NB>(bb, x)];
+      if ((cm >> (x / a.tile)) & 1u) dst[(size_t)(y0 + bb) * N + x] = lds[lidx<false, N, NB>(bb, x)];
     }
     __syncthreads();
   }
@@ -364,15 +388,25 @@ __global__ __launch_bounds__(Shape<N>::NT * Shape<N>::template col_nb<R>()) void
   cplx<R>* tw = stage_twiddles_at<R, N>(smem, sizeof(cplx<R>) * (size_t)N * NB, twg);
   cplx<R>* data = reinterpret_cast<cplx<R>*>(a.data);
   constexpr int CB = N / NB;
-  const int bid = xcd_block(blockIdx.x, gridDim.x);
+  // column groups sharing a 128-byte line run on one XCD
+  constexpr int LG = (NB * (int)sizeof(cplx<R>)) >= 128 ? 1 : 128 / (NB * (int)sizeof(cplx<R>));
+  const int bid = xcd_block(blockIdx.x, gridDim.x, LG);
   const int64_t plane = bid / CB;
   const int x0 = (bid % CB) * NB;
+  uint32_t tm = ~0u;
+  if (a.tilemask) {  // sparse grids: skip a column group whose tile column holds no entries
+    tm = a.tilemask[(size_t)plane * a.ntx + x0 / a.tile];  // NB divides the tile side
+    if (!tm) return;
+    if (!a.zero_fill) tm = ~0u;
+  }
   cplx<R>* base = data + plane * (int64_t)N * N;
 #pragma unroll
   for (int e = 0; e < EPT; ++e) {
     const int i = threadIdx.x + e * THR;
     const int n = i / NB, b = i % NB;
-    cplx<R> v = base[(size_t)n * N + x0 + b];
+    // a tile never written (sparse grids) reads as 0
+    const cplx<R>* src = ((tm >> (n / a.tile)) & 1u) ? base + (size_t)n * N + x0 + b : zero_entry<R>(a.zeros);
+    cplx<R> v = *src;
     if (a.in_mod && ((n + x0 + b) & 1)) v = cmk<R>(-v.x, -v.y);
     lds[lidx<true, N, NB>(b, n)] = v;
   }
@@ -427,10 +461,13 @@ __global__ __launch_bounds__(Shape<N>::NT * Shape<N>::ROW_NB) void k_rows_invert
     const double sw = a.sumwt[(size_t)p * a.nchan + c];
     if (a.wstack && a.normalize && !(sw > 0.0)) continue;  // normalize_sumwt zeroes this slab's image
     const cplx<R>* base = grids + ((size_t)p * a.nchan + c) * N * N + (size_t)y0 * N;
+    const uint32_t cm = a.colmask ? a.colmask[(size_t)p * a.nchan + c] : ~0u;
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
       const int i = threadIdx.x + e * THR;
-      lds[lidx<false, N, NB>(i / N, i % N)] = base[i];
+      // sparse grids: a skipped tile column reads as 0
+      const cplx<R>* src = ((cm >> ((i % N) / a.tile)) & 1u) ? base + i : zero_entry<R>(a.zeros);
+      lds[lidx<false, N, NB>(i / N, i % N)] = *src;
     }
     __syncthreads();
     stages<R, N, NB, NT, 1, true, false>(lds, b, t, tw);
@@ -687,10 +724,12 @@ static int run_image_to_grids(FftArgs a, hipStream_t s) {
   }
   constexpr int NT = Shape<N>::NT, RNB = Shape<N>::ROW_NB, CNB = Shape<N>::template col_nb<R>();
   const size_t rsh = sizeof(cplx<R>) * (size_t)N * (RNB + 1), csh = sizeof(cplx<R>) * (size_t)N * (CNB + 1);
-  const int rowblocks = a.nchan * (N / RNB);
+  constexpr int FNT = Shape<N>::FNT, FNB = Shape<N>::FNB;
+  const size_t fsh = sizeof(cplx<R>) * (size_t)N * (FNB + 1);
+  const int rowblocks = a.nchan * (N / FNB);
   int G = 1;
   plane_groups(rowblocks, a.nplanes, &a.ppg, &G);
-  ARLMI_LAUNCH("k_rows_predict", (k_rows_predict<R, N>), dim3((unsigned)rowblocks, (unsigned)G), dim3(NT * RNB), rsh,
+  ARLMI_LAUNCH("k_rows_predict", (k_rows_predict<R, N>), dim3((unsigned)rowblocks, (unsigned)G), dim3(FNT * FNB), fsh,
                s, a, tw);
   ARLMI_LAUNCH_CHECK();
   FftArgs c = a;
@@ -785,8 +824,16 @@ int arlmi_grids_to_image(const arlmi_plan* p, void* d_grids, const double* d_sum
   a.wstack = p->g.mode == ARLMI_MODE_WSTACK;
   a.screens = a.wstack;
   a.fov = (double)p->g.n * p->g.cellsize;
+  if (p->flags & ARLMI_FLAG_SPARSE_GRIDS) {
+    a.colmask = p->d_colmask;
+    a.tilemask = p->d_tilemask;
+    a.zeros = p->d_zeros;
+    a.zero_fill = 1;
+    a.tile = p->g.tile;
+    a.ntx = p->g.ntx;
+  }
   // partial images of the plane groups live with the plan (grown on demand)
-  const int n = p->g.n, nt = n >= 16 ? n / 16 : 1, rnb = nt >= 256 ? 1 : 256 / nt;
+  const int n = p->g.n, nt = n >= 16 ? n / 16 : 1, rnb = nt >= 256 ? 1 : 256 / nt;  // Shape<n>::NT, ROW_NB
   int ppg = 1, G = 1;
   plane_groups(a.nchan * (n / rnb), a.nplanes, &ppg, &G);
   if (G > 1) {
@@ -822,6 +869,14 @@ int arlmi_image_to_grids(const arlmi_plan* p, const void* d_model, const double*
   a.wstack = p->g.mode == ARLMI_MODE_WSTACK;
   a.screens = a.wstack;
   a.fov = (double)p->g.n * p->g.cellsize;
+  if (p->flags & ARLMI_FLAG_SPARSE_GRIDS) {  // only the tile columns the degridder reads
+    a.colmask = p->d_colmask;
+    a.tilemask = p->d_tilemask;
+    a.zeros = p->d_zeros;
+    a.zero_fill = 0;
+    a.tile = p->g.tile;
+    a.ntx = p->g.ntx;
+  }
   return precision == 32 ? dispatch_n<float>(Op::TO_GRIDS, p->g.n, a, 0, as_stream(stream))
                          : dispatch_n<double>(Op::TO_GRIDS, p->g.n, a, 0, as_stream(stream));
 }

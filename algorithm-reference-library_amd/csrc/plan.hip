@@ -382,15 +382,40 @@ __global__ void k_p2_keys(const uint32_t* __restrict__ s2, const uint32_t* __res
 }
 
 // Chunk counts per tile; tiles with several grid chunks also get partial-tile slots.
-__global__ void k_chunks(const uint32_t* cnt, const uint32_t* cntd, int ntiles, int chunk, uint32_t* nch,
+__global__ void k_chunks(const uint32_t* cnt, const uint32_t* cntd, int ntiles, int chunk, int sparse, uint32_t* nch,
                          uint32_t* nchd, uint32_t* nslot) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ntiles) return;
   const uint32_t c = cnt[t];
-  const uint32_t ng = c == 0 ? 1u : (c + chunk - 1) / chunk;  // empty tiles still write zeros
+  // empty tiles still write zeros, unless the grids are sparse
+  const uint32_t ng = c == 0 ? (sparse ? 0u : 1u) : (c + chunk - 1) / chunk;
   nch[t] = ng;
   nchd[t] = (cntd[t] + chunk - 1) / chunk;
   nslot[t] = ng > 1 ? ng : 0u;
+}
+
+// Occupancy masks of sparse grids (pc = plane * nchan + chan, ntx <= 32):
+// tilemask[pc * ntx + tx] bit ty: tile (pc, ty, tx) has entries;
+// colmask[pc] bit tx: tile column tx of pc has a tile with entries.
+// k_tilemask: one thread per (pc, tx); k_colmask: one thread per pc.
+__global__ void k_tilemask(const uint32_t* cnt, int npc, int ntx, uint32_t* tilemask) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= (int64_t)npc * ntx) return;
+  const int tx = (int)(i % ntx);
+  const uint32_t* c = cnt + (i / ntx) * ntx * ntx + tx;
+  uint32_t m = 0;
+  for (int ty = 0; ty < ntx; ++ty)
+    if (c[(size_t)ty * ntx] != 0u) m |= 1u << ty;
+  tilemask[i] = m;
+}
+
+__global__ void k_colmask(const uint32_t* tilemask, int npc, int ntx, uint32_t* colmask) {
+  const int pc = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pc >= npc) return;
+  uint32_t col = 0;
+  for (int tx = 0; tx < ntx; ++tx)
+    if (tilemask[(size_t)pc * ntx + tx]) col |= 1u << tx;
+  colmask[pc] = col;
 }
 
 // Work items: grid items {tile, start, count, slot+1 or 0}, degrid items {tile, start, count, 0};
@@ -612,7 +637,7 @@ static int grow_bytes(void** p, int64_t* cap, int64_t need) {
 
 static void free_plan(arlmi_plan* p) {
   if (!p) return;
-  void* ptrs[] = {p->d_boxes, p->d_plane_w, p->d_wsum, p->d_wcnt, p->d_cnt, p->d_cntd, p->d_off, p->d_nch,
+  void* ptrs[] = {p->d_boxes, p->d_plane_w, p->d_wsum, p->d_wcnt, p->d_cnt, p->d_tilemask, p->d_colmask, p->d_zeros, p->d_cntd, p->d_off, p->d_nch,
                   p->d_nchd, p->d_nslot, p->d_choff, p->d_choffd, p->d_slotoff, p->d_misc, p->d_bounds,
                   p->d_hbase, p->d_nbp, p->d_h1, p->d_s1, p->d_h2, p->d_s2, p->d_scan_tmp, p->d_recs, p->d_blk2,
                   p->d_entries, p->d_entry_z, p->d_gitems, p->d_ditems, p->d_multi, p->d_partial,
@@ -629,6 +654,13 @@ static int plan_alloc_fixed(arlmi_plan* p) {
   ARLMI_HIP(hipMalloc(&p->d_wsum, sizeof(double) * P));
   ARLMI_HIP(hipMalloc(&p->d_wcnt, sizeof(unsigned long long) * P));
   ARLMI_HIP(hipMalloc(&p->d_cnt, sizeof(uint32_t) * nt));
+  if (p->flags & ARLMI_FLAG_SPARSE_GRIDS) {
+    const int64_t npc = nt / ((int64_t)p->g.ntx * p->g.ntx);
+    ARLMI_HIP(hipMalloc(&p->d_tilemask, sizeof(uint32_t) * (size_t)(npc * p->g.ntx)));
+    ARLMI_HIP(hipMalloc(&p->d_colmask, sizeof(uint32_t) * (size_t)npc));
+    ARLMI_HIP(hipMalloc(&p->d_zeros, 1024));
+    ARLMI_HIP(hipMemset(p->d_zeros, 0, 1024));
+  }
   ARLMI_HIP(hipMalloc(&p->d_cntd, sizeof(uint32_t) * nt));
   ARLMI_HIP(hipMalloc(&p->d_off, sizeof(uint32_t) * nt));
   ARLMI_HIP(hipMalloc(&p->d_nch, sizeof(uint32_t) * nt));
@@ -781,8 +813,18 @@ static int plan_build(arlmi_plan* p, const double* uvw, const double* freq, int6
 
   // work items
   ARLMI_LAUNCH("k_chunks", k_chunks, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, p->d_cnt, p->d_cntd,
-               (int)nt, p->chunk, p->d_nch, p->d_nchd, p->d_nslot);
+               (int)nt, p->chunk, (p->flags & ARLMI_FLAG_SPARSE_GRIDS) ? 1 : 0, p->d_nch, p->d_nchd, p->d_nslot);
   ARLMI_LAUNCH_CHECK();
+  if (p->flags & ARLMI_FLAG_SPARSE_GRIDS) {
+    const int npc = (int)(nt / ((int64_t)g.ntx * g.ntx));
+    const int64_t nm = (int64_t)npc * g.ntx;
+    ARLMI_LAUNCH("k_tilemask", k_tilemask, dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, s, p->d_cnt, npc, g.ntx,
+                 p->d_tilemask);
+    ARLMI_LAUNCH_CHECK();
+    ARLMI_LAUNCH("k_colmask", k_colmask, dim3((unsigned)((npc + 255) / 256)), dim3(256), 0, s, p->d_tilemask, npc,
+                 g.ntx, p->d_colmask);
+    ARLMI_LAUNCH_CHECK();
+  }
   if ((rc = ensure_scan_tmp(p, nt))) return rc;
   if ((rc = exclusive_scan_u32(p->d_nch, p->d_choff, nt, p->d_scan_tmp, s))) return rc;
   if ((rc = exclusive_scan_u32(p->d_nchd, p->d_choffd, nt, p->d_scan_tmp, s))) return rc;
@@ -851,6 +893,9 @@ int arlmi_plan_create(const arlmi_geom* gin, const double* d_uvw, const double* 
     return rc;
   }
   p->flags = flags;
+  // the occupancy of a tile column fits one 32-bit word: beyond 32 tiles per side the grids
+  // stay dense
+  if (p->g.ntx > 32) p->flags &= ~ARLMI_FLAG_SPARSE_GRIDS;
   p->tpp = p->g.nchan * p->g.ntx * p->g.ntx;
   p->ntiles = p->g.nplanes * p->tpp;
   rc = plan_alloc_fixed(p);

@@ -45,6 +45,13 @@ extern "C" {
 #define ARLMI_MODE_WSTACK 1  /* planes are w slabs |w - box_k| < wslice/2, w shifted by the slab mean */
 
 #define ARLMI_FLAG_DROP_OFF_GRID 1u /* drop (and count) rows whose kernel footprint leaves the grid */
+/* Sparse grids (no counterpart in ARL, whose grids are dense numpy arrays): arlmi_grid leaves the
+ * T x T tiles that no row touches unwritten; arlmi_grids_to_image reads only touched tiles (the
+ * others count as zero) and skips tile columns with no touched tile; arlmi_image_to_grids writes
+ * only the tile columns that hold touched tiles, which are all that arlmi_degrid reads.  Grids
+ * of such a plan are meant to pass between these calls, not to be read by the caller.  Plans
+ * with more than 32 tiles per side ignore the flag (dense grids). */
+#define ARLMI_FLAG_SPARSE_GRIDS 2u
 
 #define ARLMI_KERNEL_SEPARABLE 0 /* real separable table k[os][S]: CF[yo][xo][a][b] = k[yo][a]*k[xo][b] */
 #define ARLMI_KERNEL_TABLE 1     /* full complex table CF[nz][os][os][S][S] (ARL layout, chan 0, pol 0) */
@@ -145,7 +152,8 @@ int arlmi_degrid(const arlmi_plan* plan, const void* d_grids, const arlmi_kernel
  * (wstack_single.py:93-94), the per-slab sumwt weighting and normalize_sumwt
  * (imaging_serial.py:193-196, imaging/base.py:84-101):
  *   img[c] = gcf[c] / S_c * sum_planes Re(b_p * N^2 ifft(G[p][c]))   (b_p = 1 in MODE_2D)
- *   S_c = sum over planes with sumwt[p][c] > 0 of sumwt[p][c]; img = 0 if S_c <= 0.
+ *   S_c = sum over all planes of sumwt[p][c]; img = 0 if S_c <= 0; a plane with
+ *   sumwt[p][c] <= 0 contributes nothing (normalize_sumwt zeroes its slab image).
  * normalize = 0 skips the division.  d_img_im (may be NULL, MODE_2D only) receives the
  * imaginary part (invert_2d imaginary=True).  d_grids is used as scratch (destroyed).
  * d_gcf: float64 [nchan][N][N]. */
