@@ -182,8 +182,8 @@ struct arlmi_plan {
   // device buffers (capacities in elements)
   double* d_boxes = nullptr;             // [P]
   double* d_plane_w = nullptr;           // [P] slab mean w
-  double* d_wsum = nullptr;              // [P]
-  unsigned long long* d_wcnt = nullptr;  // [P]
+  double* d_wsum = nullptr;              // [blocks][P] partial slab sums of w
+  uint32_t* d_wcnt = nullptr;            // [blocks][P] partial slab counts
   uint32_t* d_cnt = nullptr;             // [ntiles] entries per tile
   // occupancy bit masks (sparse grids, at most 32 tiles per side), pc = plane * nchan + chan:
   uint32_t* d_tilemask = nullptr;        // [pc][tx]: bit ty set if tile (pc, ty, tx) has entries

@@ -60,13 +60,27 @@ __device__ inline uint32_t cnt_fetch_add(uint32_t* counters, uint32_t key, bool 
 // max |w|
 // ---------------------------------------------------------------------------
 
-__global__ void k_wmax(const double* __restrict__ uvw, int64_t n, unsigned long long* out) {
-  double m = 0.0;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    m = fmax(m, fabs(uvw[3 * i + 2]));
+// Per-block max |w| into part[blockIdx.x] (the host takes the max of the partials: no device
+// atomics on one address).
+__global__ __launch_bounds__(256) void k_wmax(const double* __restrict__ uvw, int64_t n, double* part) {
+  __shared__ double red[4];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  double m = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;
+  for (; i + 3 * stride < n; i += 4 * stride) {  // four loads in flight per thread
+    const double a = uvw[3 * i + 2], b = uvw[3 * (i + stride) + 2], c = uvw[3 * (i + 2 * stride) + 2],
+                 d = uvw[3 * (i + 3 * stride) + 2];
+    m = fmax(m, fabs(a));
+    m1 = fmax(m1, fabs(b));
+    m2 = fmax(m2, fabs(c));
+    m3 = fmax(m3, fabs(d));
+  }
+  for (; i < n; i += stride) m = fmax(m, fabs(uvw[3 * i + 2]));
+  m = fmax(fmax(m, m1), fmax(m2, m3));
   for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
-  // |w| >= 0 so the IEEE bit pattern orders like the value
-  if (__lane_id() == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
+  if (__lane_id() == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
 }
 
 // ---------------------------------------------------------------------------
@@ -99,9 +113,11 @@ __device__ inline Members slab_members(double w, const double* boxes, int nb, do
   return m;
 }
 
-// Pass A (w-stack): per-slab count and sum of w -> slab means.
+// Pass A (w-stack): per-slab count and sum of w -> slab means.  Each of at most SLAB_BLOCKS
+// blocks writes its partial sums; k_slab_means adds them.
+constexpr int SLAB_BLOCKS = 2048;
 __global__ void k_slab_sums(const double* __restrict__ uvw, int64_t n, const double* __restrict__ boxes, int nb,
-                            double wslice, double* wsum, unsigned long long* wcnt) {
+                            double wslice, double* wsum, uint32_t* wcnt) {
   extern __shared__ char smem[];
   double* ls = reinterpret_cast<double*>(smem);
   unsigned int* lc = reinterpret_cast<unsigned int*>(ls + nb);
@@ -110,29 +126,62 @@ __global__ void k_slab_sums(const double* __restrict__ uvw, int64_t n, const dou
     lc[i] = 0u;
   }
   __syncthreads();
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const double w = uvw[3 * i + 2];
-    Members m = slab_members(w, boxes, nb, wslice);
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i0 < n; i0 += 4 * stride) {
+    double wv[4];
 #pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      if (j < m.n) {
-        atomicAdd(&ls[m.k[j]], w);
-        atomicAdd(&lc[m.k[j]], 1u);
+    for (int u = 0; u < 4; ++u) {  // four loads in flight per thread
+      const int64_t i = i0 + u * stride;
+      wv[u] = i < n ? uvw[3 * i + 2] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (i0 + u * stride >= n) break;
+      const double w = wv[u];
+      Members m = slab_members(w, boxes, nb, wslice);
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        if (j < m.n) {
+          atomicAdd(&ls[m.k[j]], w);
+          atomicAdd(&lc[m.k[j]], 1u);
+        }
       }
     }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
-    if (lc[i]) {
-      atomicAdd(&wsum[i], ls[i]);
-      atomicAdd(&wcnt[i], (unsigned long long)lc[i]);
-    }
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {  // this block's partial sums
+    wsum[(size_t)blockIdx.x * nb + i] = ls[i];
+    wcnt[(size_t)blockIdx.x * nb + i] = lc[i];
   }
 }
 
-__global__ void k_slab_means(const double* wsum, const unsigned long long* wcnt, double* wmean, int nb) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nb) wmean[i] = wcnt[i] ? wsum[i] / (double)wcnt[i] : 0.0;
+// Slab k (blockIdx.x): sums the nblk per-block partials in a fixed order (deterministic),
+// mean = sum / count.
+__global__ __launch_bounds__(256) void k_slab_means(const double* wsum, const uint32_t* wcnt, int nblk, double* wmean,
+                                                    int nb) {
+  __shared__ double rs[4];
+  __shared__ unsigned long long rc[4];
+  const int k = blockIdx.x;
+  double sum = 0.0;
+  unsigned long long cnt = 0;
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+    sum += wsum[(size_t)b * nb + k];
+    cnt += wcnt[(size_t)b * nb + k];
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    sum += __shfl_xor(sum, o);
+    cnt += __shfl_xor(cnt, o);
+  }
+  if (__lane_id() == 0) {
+    rs[threadIdx.x >> 6] = sum;
+    rc[threadIdx.x >> 6] = cnt;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double t = (rs[0] + rs[1]) + (rs[2] + rs[3]);
+    const unsigned long long c = rc[0] + rc[1] + rc[2] + rc[3];
+    wmean[k] = c ? t / (double)c : 0.0;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -651,8 +700,8 @@ static int plan_alloc_fixed(arlmi_plan* p) {
   const int64_t nt = p->ntiles;
   ARLMI_HIP(hipMalloc(&p->d_boxes, sizeof(double) * P));
   ARLMI_HIP(hipMalloc(&p->d_plane_w, sizeof(double) * P));
-  ARLMI_HIP(hipMalloc(&p->d_wsum, sizeof(double) * P));
-  ARLMI_HIP(hipMalloc(&p->d_wcnt, sizeof(unsigned long long) * P));
+  ARLMI_HIP(hipMalloc(&p->d_wsum, sizeof(double) * P * SLAB_BLOCKS));
+  ARLMI_HIP(hipMalloc(&p->d_wcnt, sizeof(uint32_t) * P * SLAB_BLOCKS));
   ARLMI_HIP(hipMalloc(&p->d_cnt, sizeof(uint32_t) * nt));
   if (p->flags & ARLMI_FLAG_SPARSE_GRIDS) {
     const int64_t npc = nt / ((int64_t)p->g.ntx * p->g.ntx);
@@ -704,16 +753,17 @@ static int plan_build(arlmi_plan* p, const double* uvw, const double* freq, int6
       return ARLMI_E_ARG;
     }
     ARLMI_HIP(hipMemcpyAsync(p->d_boxes, h_boxes, sizeof(double) * P, hipMemcpyHostToDevice, s));
-    ARLMI_HIP(hipMemsetAsync(p->d_wsum, 0, sizeof(double) * P, s));
-    ARLMI_HIP(hipMemsetAsync(p->d_wcnt, 0, sizeof(unsigned long long) * P, s));
+    const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((nrows + 1023) / 1024, SLAB_BLOCKS));
     if (nrows > 0) {
-      const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>((nrows + 255) / 256, 1024));
       const size_t sh = (sizeof(double) + sizeof(unsigned int)) * P;
-      ARLMI_LAUNCH("k_slab_sums", k_slab_sums, dim3(blocks), dim3(256), sh, s, uvw, nrows, p->d_boxes, P, wslice,
-                   p->d_wsum, p->d_wcnt);
+      ARLMI_LAUNCH("k_slab_sums", k_slab_sums, dim3((unsigned)blocks), dim3(256), sh, s, uvw, nrows, p->d_boxes, P,
+                   wslice, p->d_wsum, p->d_wcnt);
       ARLMI_LAUNCH_CHECK();
+    } else {
+      ARLMI_HIP(hipMemsetAsync(p->d_wsum, 0, sizeof(double) * P, s));
+      ARLMI_HIP(hipMemsetAsync(p->d_wcnt, 0, sizeof(uint32_t) * P, s));
     }
-    ARLMI_LAUNCH("k_slab_means", k_slab_means, dim3((P + 255) / 256), dim3(256), 0, s, p->d_wsum, p->d_wcnt,
+    ARLMI_LAUNCH("k_slab_means", k_slab_means, dim3((unsigned)P), dim3(256), 0, s, p->d_wsum, p->d_wcnt, blocks,
                  p->d_plane_w, P);
     ARLMI_LAUNCH_CHECK();
   }
@@ -861,20 +911,19 @@ int arlmi_wmax(const double* d_uvw, int64_t nrows, void* stream, double* wmax_ou
     return ARLMI_E_ARG;
   }
   hipStream_t s = as_stream(stream);
-  unsigned long long* d = nullptr;
-  ARLMI_HIP(hipMallocAsync(reinterpret_cast<void**>(&d), sizeof(unsigned long long), s));
-  ARLMI_HIP(hipMemsetAsync(d, 0, sizeof(unsigned long long), s));
+  const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((nrows + 1023) / 1024, 1024));
+  double* d = nullptr;
+  ARLMI_HIP(hipMallocAsync(reinterpret_cast<void**>(&d), sizeof(double) * blocks, s));
+  std::vector<double> part(blocks, 0.0);
   if (nrows > 0) {
-    const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>((nrows + 255) / 256, 2048));
-    ARLMI_LAUNCH("k_wmax", k_wmax, dim3(blocks), dim3(256), 0, s, d_uvw, nrows, d);
+    ARLMI_LAUNCH("k_wmax", k_wmax, dim3((unsigned)blocks), dim3(256), 0, s, d_uvw, nrows, d);
     ARLMI_LAUNCH_CHECK();
+    ARLMI_HIP(hipMemcpyAsync(part.data(), d, sizeof(double) * blocks, hipMemcpyDeviceToHost, s));
   }
-  unsigned long long bits = 0;
-  ARLMI_HIP(hipMemcpyAsync(&bits, d, sizeof(bits), hipMemcpyDeviceToHost, s));
   ARLMI_HIP(hipFreeAsync(d, s));
   ARLMI_HIP(hipStreamSynchronize(s));
-  double w;
-  memcpy(&w, &bits, sizeof(w));
+  double w = 0.0;
+  for (double v : part) w = std::max(w, v);
   *wmax_out = w;
   return ARLMI_OK;
 }
