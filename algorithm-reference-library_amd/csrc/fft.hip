@@ -107,14 +107,25 @@ __device__ __forceinline__ void dft(cplx<R>* v) {
   else if constexpr (RAD == 2) dft2<R, INV>(v);
 }
 
-// LDS addressing: ROW layout [b][n], COL layout [n][b]
-template <bool COL, int N, int NB>
+// LDS layouts of NB transforms of length N:
+//   LAY_ROW    [b][n];
+//   LAY_ROWPAD [b][n] with one pad element after every 8 (the Stockham writes of the first
+//              stages go to stride-8 positions, which would otherwise hit the same banks; the
+//              extra index arithmetic costs more than it saves in the VALU-heavy invert row pass);
+//   LAY_COL    [n][b].
+constexpr int LAY_ROW = 0, LAY_ROWPAD = 1, LAY_COL = 2;
+template <int LAY, int N, int NB>
 __device__ __forceinline__ int lidx(int b, int n) {
-  return COL ? n * NB + b : b * N + n;
+  return LAY == LAY_COL ? n * NB + b : LAY == LAY_ROWPAD ? b * (N + N / 8) + n + (n >> 3) : b * N + n;
+}
+// elements of LDS a layout of NB transforms occupies
+template <int LAY, int N>
+__host__ __device__ constexpr size_t lds_elems(int nb) {
+  return LAY == LAY_ROWPAD ? (size_t)nb * (N + N / 8) : (size_t)N * nb;
 }
 
 // One Stockham stage on transform b held in LDS (in place: read all, barrier, write all).
-template <typename R, int N, int NB, int NT, int RAD, int NS, bool INV, bool COL>
+template <typename R, int N, int NB, int NT, int RAD, int NS, bool INV, int LAY>
 __device__ __forceinline__ void stage(cplx<R>* lds, int b, int t, const cplx<R>* __restrict__ tw) {
   constexpr int NBF = N / RAD;
   constexpr int PER = NBF / NT;
@@ -124,7 +135,7 @@ __device__ __forceinline__ void stage(cplx<R>* lds, int b, int t, const cplx<R>*
   for (int q = 0; q < PER; ++q) {
     const int j = t + q * NT;
 #pragma unroll
-    for (int r = 0; r < RAD; ++r) v[q][r] = lds[lidx<COL, N, NB>(b, j + r * NBF)];
+    for (int r = 0; r < RAD; ++r) v[q][r] = lds[lidx<LAY, N, NB>(b, j + r * NBF)];
   }
   __syncthreads();
 #pragma unroll
@@ -142,17 +153,17 @@ __device__ __forceinline__ void stage(cplx<R>* lds, int b, int t, const cplx<R>*
     dft<R, RAD, INV>(v[q]);
     const int idxD = (j / NS) * NS * RAD + k;
 #pragma unroll
-    for (int r = 0; r < RAD; ++r) lds[lidx<COL, N, NB>(b, idxD + r * NS)] = v[q][r];
+    for (int r = 0; r < RAD; ++r) lds[lidx<LAY, N, NB>(b, idxD + r * NS)] = v[q][r];
   }
   __syncthreads();
 }
 
-template <typename R, int N, int NB, int NT, int NS, bool INV, bool COL>
+template <typename R, int N, int NB, int NT, int NS, bool INV, int LAY>
 __device__ __forceinline__ void stages(cplx<R>* lds, int b, int t, const cplx<R>* __restrict__ tw) {
   if constexpr (NS < N) {
     constexpr int RAD = (N / NS >= 8) ? 8 : (N / NS);
-    stage<R, N, NB, NT, RAD, NS, INV, COL>(lds, b, t, tw);
-    stages<R, N, NB, NT, NS * RAD, INV, COL>(lds, b, t, tw);
+    stage<R, N, NB, NT, RAD, NS, INV, LAY>(lds, b, t, tw);
+    stages<R, N, NB, NT, NS * RAD, INV, LAY>(lds, b, t, tw);
   }
 }
 
@@ -192,9 +203,9 @@ struct Shape {
   static constexpr int FNB = FNT >= 512 ? 1 : 512 / FNT;
   template <typename R>
   static constexpr bool fits() {
-    return (size_t)N * sizeof(cplx<R>) * (size_t)(ROW_NB + 1) <= 163840 &&
+    return sizeof(cplx<R>) * (lds_elems<LAY_ROWPAD, N>(ROW_NB) + N) <= 163840 &&
            (size_t)N * sizeof(cplx<R>) * (size_t)(col_nb<R>() + 1) <= 163840 &&
-           (size_t)N * sizeof(cplx<R>) * (size_t)(FNB + 1) <= 163840;
+           sizeof(cplx<R>) * (lds_elems<LAY_ROWPAD, N>(FNB) + N) <= 163840;
   }
 };
 
@@ -289,7 +300,7 @@ __global__ __launch_bounds__(Shape<N>::NT * Shape<N>::ROW_NB) void k_rows_plain(
   constexpr int NT = Shape<N>::NT, NB = Shape<N>::ROW_NB, THR = NT * NB, EPT = N * NB / THR;
   extern __shared__ __align__(16) unsigned char smem[];
   cplx<R>* lds = reinterpret_cast<cplx<R>*>(smem);
-  cplx<R>* tw = stage_twiddles_at<R, N>(smem, sizeof(cplx<R>) * (size_t)N * NB, twg);
+  cplx<R>* tw = stage_twiddles_at<R, N>(smem, sizeof(cplx<R>) * lds_elems<LAY_ROWPAD, N>(NB), twg);
   cplx<R>* data = reinterpret_cast<cplx<R>*>(a.data);
   const int64_t row0 = (int64_t)blockIdx.x * NB;
 #pragma unroll
@@ -299,17 +310,17 @@ __global__ __launch_bounds__(Shape<N>::NT * Shape<N>::ROW_NB) void k_rows_plain(
     const int64_t row = row0 + b;
     cplx<R> v = data[row * N + n];
     if (a.in_mod && (((int)(row % N) + n) & 1)) v = cmk<R>(-v.x, -v.y);
-    lds[lidx<false, N, NB>(b, n)] = v;
+    lds[lidx<LAY_ROWPAD, N, NB>(b, n)] = v;
   }
   __syncthreads();
   const int b = threadIdx.x / NT, t = threadIdx.x % NT;
-  stages<R, N, NB, NT, 1, INV, false>(lds, b, t, tw);
+  stages<R, N, NB, NT, 1, INV, LAY_ROWPAD>(lds, b, t, tw);
 #pragma unroll
   for (int e = 0; e < EPT; ++e) {
     const int i = threadIdx.x + e * THR;
     const int bb = i / N, n = i % N;
     const int64_t row = row0 + bb;
-    cplx<R> v = lds[lidx<false, N, NB>(bb, n)];
+    cplx<R> v = lds[lidx<LAY_ROWPAD, N, NB>(bb, n)];
     R sc = (R)a.scale;
     if (a.out_mod && (((int)(row % N) + n) & 1)) sc = -sc;
     data[row * N + n] = cmk<R>(v.x * sc, v.y * sc);
@@ -323,7 +334,7 @@ __global__ __launch_bounds__(Shape<N>::FNT * Shape<N>::FNB) void k_rows_predict(
   constexpr int NT = Shape<N>::FNT, NB = Shape<N>::FNB, THR = NT * NB, EPT = N * NB / THR;
   extern __shared__ __align__(16) unsigned char smem[];
   cplx<R>* lds = reinterpret_cast<cplx<R>*>(smem);
-  cplx<R>* tw = stage_twiddles_at<R, N>(smem, sizeof(cplx<R>) * (size_t)N * NB, twg);
+  cplx<R>* tw = stage_twiddles_at<R, N>(smem, sizeof(cplx<R>) * lds_elems<LAY_ROWPAD, N>(NB), twg);
   cplx<R>* grids = reinterpret_cast<cplx<R>*>(a.data);
   const R* model = reinterpret_cast<const R*>(a.model);
   constexpr int RB = N / NB;  // row blocks per channel
@@ -360,17 +371,17 @@ __global__ __launch_bounds__(Shape<N>::FNT * Shape<N>::FNB) void k_rows_predict(
         const cplx<R> s = expi_turns<R>(screen_turns(wp, q[e]));  // b; conj(b) = (s.x, -s.y)
         v = cmk<R>(mv[e] * s.x, -mv[e] * s.y);
       }
-      lds[lidx<false, N, NB>(bb, x)] = v;
+      lds[lidx<LAY_ROWPAD, N, NB>(bb, x)] = v;
     }
     __syncthreads();
-    stages<R, N, NB, NT, 1, false, false>(lds, b, t, tw);
+    stages<R, N, NB, NT, 1, false, LAY_ROWPAD>(lds, b, t, tw);
     cplx<R>* dst = grids + ((size_t)p * a.nchan + c) * N * N;
     const uint32_t cm = a.colmask ? a.colmask[(size_t)p * a.nchan + c] : ~0u;
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
       const int i = threadIdx.x + e * THR;
       const int bb = i / N, x = i % N;
-      if ((cm >> (x / a.tile)) & 1u) dst[(size_t)(y0 + bb) * N + x] = lds[lidx<false, N, NB>(bb, x)];
+      if ((cm >> (x / a.tile)) & 1u) dst[(size_t)(y0 + bb) * N + x] = lds[lidx<LAY_ROWPAD, N, NB>(bb, x)];
     }
     __syncthreads();
   }
@@ -408,16 +419,16 @@ __global__ __launch_bounds__(Shape<N>::NT * Shape<N>::template col_nb<R>()) void
     const cplx<R>* src = ((tm >> (n / a.tile)) & 1u) ? base + (size_t)n * N + x0 + b : zero_entry<R>(a.zeros);
     cplx<R> v = *src;
     if (a.in_mod && ((n + x0 + b) & 1)) v = cmk<R>(-v.x, -v.y);
-    lds[lidx<true, N, NB>(b, n)] = v;
+    lds[lidx<LAY_COL, N, NB>(b, n)] = v;
   }
   __syncthreads();
   const int b = threadIdx.x % NB, t = threadIdx.x / NB;
-  stages<R, N, NB, NT, 1, INV, true>(lds, b, t, tw);
+  stages<R, N, NB, NT, 1, INV, LAY_COL>(lds, b, t, tw);
 #pragma unroll
   for (int e = 0; e < EPT; ++e) {
     const int i = threadIdx.x + e * THR;
     const int n = i / NB, bb = i % NB;
-    const cplx<R> v = lds[lidx<true, N, NB>(bb, n)];
+    const cplx<R> v = lds[lidx<LAY_COL, N, NB>(bb, n)];
     R sc = (R)a.scale;
     if (a.out_mod && ((n + x0 + bb) & 1)) sc = -sc;
     base[(size_t)n * N + x0 + bb] = cmk<R>(v.x * sc, v.y * sc);
@@ -435,7 +446,7 @@ __global__ __launch_bounds__(Shape<N>::NT * Shape<N>::ROW_NB) void k_rows_invert
   constexpr int NT = Shape<N>::NT, NB = Shape<N>::ROW_NB, THR = NT * NB, EPT = N * NB / THR;
   extern __shared__ __align__(16) unsigned char smem[];
   cplx<R>* lds = reinterpret_cast<cplx<R>*>(smem);
-  cplx<R>* tw = stage_twiddles_at<R, N>(smem, sizeof(cplx<R>) * (size_t)N * NB, twg);
+  cplx<R>* tw = stage_twiddles_at<R, N>(smem, sizeof(cplx<R>) * lds_elems<LAY_ROW, N>(NB), twg);
   const cplx<R>* grids = reinterpret_cast<const cplx<R>*>(a.data);
   constexpr int RB = N / NB;  // row blocks per channel
   const int c = blockIdx.x / RB;
@@ -467,16 +478,16 @@ __global__ __launch_bounds__(Shape<N>::NT * Shape<N>::ROW_NB) void k_rows_invert
       const int i = threadIdx.x + e * THR;
       // sparse grids: a skipped tile column reads as 0
       const cplx<R>* src = ((cm >> ((i % N) / a.tile)) & 1u) ? base + i : zero_entry<R>(a.zeros);
-      lds[lidx<false, N, NB>(i / N, i % N)] = *src;
+      lds[lidx<LAY_ROW, N, NB>(i / N, i % N)] = *src;
     }
     __syncthreads();
-    stages<R, N, NB, NT, 1, true, false>(lds, b, t, tw);
+    stages<R, N, NB, NT, 1, true, LAY_ROW>(lds, b, t, tw);
     const double wp = SCREENS ? a.plane_w[p] : 0.0;
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
       const int i = threadIdx.x + e * THR;
       const int bb = i / N, x = i % N, y = y0 + bb;
-      cplx<R> v = lds[lidx<false, N, NB>(bb, x)];
+      cplx<R> v = lds[lidx<LAY_ROW, N, NB>(bb, x)];
       if ((x + y) & 1) v = cmk<R>(-v.x, -v.y);
       if constexpr (SCREENS) {
         if (!out[e]) {
@@ -649,7 +660,7 @@ static int run_fft2d(FftArgs a, int direction, hipStream_t s) {
     return ARLMI_E_HIP;
   }
   constexpr int NT = Shape<N>::NT, RNB = Shape<N>::ROW_NB, CNB = Shape<N>::template col_nb<R>();
-  const size_t rsh = sizeof(cplx<R>) * (size_t)N * (RNB + 1), csh = sizeof(cplx<R>) * (size_t)N * (CNB + 1);
+  const size_t rsh = sizeof(cplx<R>) * (lds_elems<LAY_ROWPAD, N>(RNB) + N), csh = sizeof(cplx<R>) * (size_t)N * (CNB + 1);
   const unsigned rblocks = (unsigned)((int64_t)a.nbatch * N / RNB), cblocks = (unsigned)((int64_t)a.nbatch * N / CNB);
   FftArgs r = a, c = a;
   r.in_mod = 1;
@@ -687,7 +698,8 @@ static int run_grids_to_image(FftArgs a, hipStream_t s) {
     return ARLMI_E_HIP;
   }
   constexpr int NT = Shape<N>::NT, RNB = Shape<N>::ROW_NB, CNB = Shape<N>::template col_nb<R>();
-  const size_t rsh = sizeof(cplx<R>) * (size_t)N * (RNB + 1), csh = sizeof(cplx<R>) * (size_t)N * (CNB + 1);
+  const size_t rsh = sizeof(cplx<R>) * (lds_elems<LAY_ROWPAD, N>(RNB) + N), csh = sizeof(cplx<R>) * (size_t)N * (CNB + 1);
+  const size_t ish = sizeof(cplx<R>) * (lds_elems<LAY_ROW, N>(RNB) + N);
   FftArgs c = a;
   c.in_mod = 1;
   c.out_mod = 0;
@@ -702,10 +714,10 @@ static int run_grids_to_image(FftArgs a, hipStream_t s) {
   if (G > 1 && !a.part) G = 1, a.ppg = a.nplanes;
   if (a.screens)
     ARLMI_LAUNCH("k_rows_invert", (k_rows_invert<R, N, true>), dim3((unsigned)rowblocks, (unsigned)G), dim3(NT * RNB),
-                 rsh, s, a, tw);
+                 ish, s, a, tw);
   else
     ARLMI_LAUNCH("k_rows_invert", (k_rows_invert<R, N, false>), dim3((unsigned)rowblocks, (unsigned)G),
-                 dim3(NT * RNB), rsh, s, a, tw);
+                 dim3(NT * RNB), ish, s, a, tw);
   ARLMI_LAUNCH_CHECK();
   if (G > 1) {
     ARLMI_LAUNCH("k_invert_finish", k_invert_finish<R>, dim3((unsigned)(((int64_t)N * N + 255) / 256), (unsigned)a.nchan),
@@ -723,9 +735,9 @@ static int run_image_to_grids(FftArgs a, hipStream_t s) {
     return ARLMI_E_HIP;
   }
   constexpr int NT = Shape<N>::NT, RNB = Shape<N>::ROW_NB, CNB = Shape<N>::template col_nb<R>();
-  const size_t rsh = sizeof(cplx<R>) * (size_t)N * (RNB + 1), csh = sizeof(cplx<R>) * (size_t)N * (CNB + 1);
+  const size_t rsh = sizeof(cplx<R>) * (lds_elems<LAY_ROWPAD, N>(RNB) + N), csh = sizeof(cplx<R>) * (size_t)N * (CNB + 1);
   constexpr int FNT = Shape<N>::FNT, FNB = Shape<N>::FNB;
-  const size_t fsh = sizeof(cplx<R>) * (size_t)N * (FNB + 1);
+  const size_t fsh = sizeof(cplx<R>) * (lds_elems<LAY_ROWPAD, N>(FNB) + N);
   const int rowblocks = a.nchan * (N / FNB);
   int G = 1;
   plane_groups(rowblocks, a.nplanes, &a.ppg, &G);
