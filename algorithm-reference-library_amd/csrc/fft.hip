@@ -124,20 +124,24 @@ __host__ __device__ constexpr size_t lds_elems(int nb) {
   return LAY == LAY_ROWPAD ? (size_t)nb * (N + N / 8) : (size_t)N * nb;
 }
 
-// One Stockham stage on transform b held in LDS (in place: read all, barrier, write all).
-template <typename R, int N, int NB, int NT, int RAD, int NS, bool INV, int LAY>
-__device__ __forceinline__ void stage(cplx<R>* lds, int b, int t, const cplx<R>* __restrict__ tw) {
-  constexpr int NBF = N / RAD;
-  constexpr int PER = NBF / NT;
-  static_assert(PER * NT == NBF, "butterflies must divide among threads");
-  cplx<R> v[PER][RAD];
-#pragma unroll
-  for (int q = 0; q < PER; ++q) {
-    const int j = t + q * NT;
-#pragma unroll
-    for (int r = 0; r < RAD; ++r) v[q][r] = lds[lidx<LAY, N, NB>(b, j + r * NBF)];
-  }
-  __syncthreads();
+// Stockham stages.  Stage NS (1, 8, 64, ...) of radix RAD: butterfly j = t + q*NT of transform b
+// reads elements j + r*(N/RAD), applies twiddles exp(-+2 pi i r k / (NS*RAD)) (k = j % NS) and a
+// radix-RAD DFT, and writes elements (j / NS)*NS*RAD + k + r*NS.  The first stage (NS = 1, radix
+// 8) reads straight from the caller's loader and the last one (NS = NSL, the largest power of 8
+// below N, radix N / NSL) hands its outputs, element t + q*NT + r*NSL, to the caller's storer:
+// only the stages between go through LDS twice.
+
+template <int N>
+__host__ __device__ constexpr int last_ns() {
+  int ns = 1;
+  while (ns * 8 < N) ns *= 8;
+  return ns;
+}
+
+template <typename R, int N, int NT, int RAD, int NS, bool INV>
+__device__ __forceinline__ void stage_math(cplx<R> (&v)[(N / RAD) / NT][RAD], int t, const cplx<R>* __restrict__ tw) {
+  constexpr int PER = (N / RAD) / NT;
+  static_assert(PER * NT * RAD == N, "butterflies must divide among threads");
 #pragma unroll
   for (int q = 0; q < PER; ++q) {
     const int j = t + q * NT;
@@ -151,19 +155,73 @@ __device__ __forceinline__ void stage(cplx<R>* lds, int b, int t, const cplx<R>*
       }
     }
     dft<R, RAD, INV>(v[q]);
-    const int idxD = (j / NS) * NS * RAD + k;
+  }
+}
+
+template <typename R, int N, int NB, int NT, int RAD, int NS, int LAY>
+__device__ __forceinline__ void stage_store_lds(cplx<R>* lds, int b, int t, const cplx<R> (&v)[(N / RAD) / NT][RAD]) {
+  constexpr int PER = (N / RAD) / NT;
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int j = t + q * NT;
+    const int idxD = (j / NS) * NS * RAD + j % NS;
 #pragma unroll
     for (int r = 0; r < RAD; ++r) lds[lidx<LAY, N, NB>(b, idxD + r * NS)] = v[q][r];
   }
-  __syncthreads();
 }
 
+template <typename R, int N, int NB, int NT, int RAD, int LAY>
+__device__ __forceinline__ void stage_load_lds(const cplx<R>* lds, int b, int t, cplx<R> (&v)[(N / RAD) / NT][RAD]) {
+  constexpr int PER = (N / RAD) / NT, NBF = N / RAD;
+#pragma unroll
+  for (int q = 0; q < PER; ++q)
+#pragma unroll
+    for (int r = 0; r < RAD; ++r) v[q][r] = lds[lidx<LAY, N, NB>(b, t + q * NT + r * NBF)];
+}
+
+// the radix-8 stages between the first and the last: read, barrier, compute, write, barrier
 template <typename R, int N, int NB, int NT, int NS, bool INV, int LAY>
-__device__ __forceinline__ void stages(cplx<R>* lds, int b, int t, const cplx<R>* __restrict__ tw) {
-  if constexpr (NS < N) {
-    constexpr int RAD = (N / NS >= 8) ? 8 : (N / NS);
-    stage<R, N, NB, NT, RAD, NS, INV, LAY>(lds, b, t, tw);
-    stages<R, N, NB, NT, NS * RAD, INV, LAY>(lds, b, t, tw);
+__device__ __forceinline__ void mid_stages(cplx<R>* lds, int b, int t, const cplx<R>* __restrict__ tw) {
+  if constexpr (NS < last_ns<N>()) {
+    cplx<R> v[(N / 8) / NT][8];
+    stage_load_lds<R, N, NB, NT, 8, LAY>(lds, b, t, v);
+    __syncthreads();
+    stage_math<R, N, NT, 8, NS, INV>(v, t, tw);
+    stage_store_lds<R, N, NB, NT, 8, NS, LAY>(lds, b, t, v);
+    __syncthreads();
+    mid_stages<R, N, NB, NT, NS * 8, INV, LAY>(lds, b, t, tw);
+  }
+}
+
+// One length-N transform per b over the work group: ld(q, r, n) returns input element
+// n = t + q*NT + r*N/8, st(q, r, n, v) receives output element n = t + q*NT + r*NSL.  The pass
+// ends right after its last LDS reads: a caller that reuses the LDS must synchronise first.
+template <typename R, int N, int NB, int NT, bool INV, int LAY, typename LD, typename ST>
+__device__ __forceinline__ void fft_pass(cplx<R>* lds, int b, int t, const cplx<R>* __restrict__ tw, LD&& ld,
+                                         ST&& st) {
+  constexpr int NSL = last_ns<N>(), RADL = N / NSL;
+  static_assert(N >= 64, "at least two stages");
+  {
+    constexpr int PER = (N / 8) / NT;
+    cplx<R> v[PER][8];
+#pragma unroll
+    for (int q = 0; q < PER; ++q)
+#pragma unroll
+      for (int r = 0; r < 8; ++r) v[q][r] = ld(q, r, t + q * NT + r * (N / 8));
+    stage_math<R, N, NT, 8, 1, INV>(v, t, tw);
+    stage_store_lds<R, N, NB, NT, 8, 1, LAY>(lds, b, t, v);
+  }
+  __syncthreads();
+  mid_stages<R, N, NB, NT, 8, INV, LAY>(lds, b, t, tw);
+  {
+    constexpr int PER = (N / RADL) / NT;
+    cplx<R> v[PER][RADL];
+    stage_load_lds<R, N, NB, NT, RADL, LAY>(lds, b, t, v);
+    stage_math<R, N, NT, RADL, NSL, INV>(v, t, tw);
+#pragma unroll
+    for (int q = 0; q < PER; ++q)
+#pragma unroll
+      for (int r = 0; r < RADL; ++r) st(q, r, t + q * NT + r * NSL, v[q][r]);
   }
 }
 
@@ -297,41 +355,35 @@ __device__ __forceinline__ const cplx<R>* zero_entry(const void* zeros) {
 
 template <typename R, int N, bool INV>
 __global__ __launch_bounds__(Shape<N>::NT * Shape<N>::ROW_NB) void k_rows_plain(FftArgs a, const cplx<R>* __restrict__ twg) {
-  constexpr int NT = Shape<N>::NT, NB = Shape<N>::ROW_NB, THR = NT * NB, EPT = N * NB / THR;
+  constexpr int NT = Shape<N>::NT, NB = Shape<N>::ROW_NB;
   extern __shared__ __align__(16) unsigned char smem[];
   cplx<R>* lds = reinterpret_cast<cplx<R>*>(smem);
   cplx<R>* tw = stage_twiddles_at<R, N>(smem, sizeof(cplx<R>) * lds_elems<LAY_ROWPAD, N>(NB), twg);
-  cplx<R>* data = reinterpret_cast<cplx<R>*>(a.data);
-  const int64_t row0 = (int64_t)blockIdx.x * NB;
-#pragma unroll
-  for (int e = 0; e < EPT; ++e) {
-    const int i = threadIdx.x + e * THR;
-    const int b = i / N, n = i % N;
-    const int64_t row = row0 + b;
-    cplx<R> v = data[row * N + n];
-    if (a.in_mod && (((int)(row % N) + n) & 1)) v = cmk<R>(-v.x, -v.y);
-    lds[lidx<LAY_ROWPAD, N, NB>(b, n)] = v;
-  }
-  __syncthreads();
   const int b = threadIdx.x / NT, t = threadIdx.x % NT;
-  stages<R, N, NB, NT, 1, INV, LAY_ROWPAD>(lds, b, t, tw);
-#pragma unroll
-  for (int e = 0; e < EPT; ++e) {
-    const int i = threadIdx.x + e * THR;
-    const int bb = i / N, n = i % N;
-    const int64_t row = row0 + bb;
-    cplx<R> v = lds[lidx<LAY_ROWPAD, N, NB>(bb, n)];
-    R sc = (R)a.scale;
-    if (a.out_mod && (((int)(row % N) + n) & 1)) sc = -sc;
-    data[row * N + n] = cmk<R>(v.x * sc, v.y * sc);
-  }
+  const int64_t row = (int64_t)blockIdx.x * NB + b;
+  cplx<R>* data = reinterpret_cast<cplx<R>*>(a.data) + row * N;
+  const int ym = (int)(row % N);
+  fft_pass<R, N, NB, NT, INV, LAY_ROWPAD>(
+      lds, b, t, tw,
+      [&](int, int, int n) {
+        cplx<R> v = data[n];
+        if (a.in_mod && ((ym + n) & 1)) v = cmk<R>(-v.x, -v.y);
+        return v;
+      },
+      [&](int, int, int n, cplx<R> v) {
+        R sc = (R)a.scale;
+        if (a.out_mod && ((ym + n) & 1)) sc = -sc;
+        data[n] = cmk<R>(v.x * sc, v.y * sc);
+      });
 }
 
 // Predict: grids[p][c][y][:] = rowfft((-1)^(x+y) * model[c][y][x] * gcf[c][y][x] * conj(b_p(x,y)))
-// Grid: (row blocks of every channel, plane groups).
+// Grid: (row blocks of every channel, plane groups).  The model term and the screen's q of
+// the elements each thread feeds to the first stage are computed once, before the plane loop.
 template <typename R, int N>
 __global__ __launch_bounds__(Shape<N>::FNT * Shape<N>::FNB) void k_rows_predict(FftArgs a, const cplx<R>* __restrict__ twg) {
-  constexpr int NT = Shape<N>::FNT, NB = Shape<N>::FNB, THR = NT * NB, EPT = N * NB / THR;
+  constexpr int NT = Shape<N>::FNT, NB = Shape<N>::FNB;
+  constexpr int PER = (N / 8) / NT;
   extern __shared__ __align__(16) unsigned char smem[];
   cplx<R>* lds = reinterpret_cast<cplx<R>*>(smem);
   cplx<R>* tw = stage_twiddles_at<R, N>(smem, sizeof(cplx<R>) * lds_elems<LAY_ROWPAD, N>(NB), twg);
@@ -339,50 +391,37 @@ __global__ __launch_bounds__(Shape<N>::FNT * Shape<N>::FNB) void k_rows_predict(
   const R* model = reinterpret_cast<const R*>(a.model);
   constexpr int RB = N / NB;  // row blocks per channel
   const int c = blockIdx.x / RB;
-  const int y0 = (blockIdx.x % RB) * NB;
-  R mv[EPT];
-  double q[EPT];
-  bool out[EPT];
-#pragma unroll
-  for (int e = 0; e < EPT; ++e) {
-    const int i = threadIdx.x + e * THR;
-    const int b = i / N, x = i % N, y = y0 + b;
-    const size_t gi = ((size_t)c * N + y) * N + x;
-    double m = (double)model[gi] * a.gcf[gi];
-    if ((x + y) & 1) m = -m;
-    mv[e] = (R)m;
-    q[e] = a.screens ? screen_q(y, x, N, a.fov, &out[e]) : 0.0;
-    if (!a.screens) out[e] = false;
-  }
   const int b = threadIdx.x / NT, t = threadIdx.x % NT;
+  const int y = (blockIdx.x % RB) * NB + b;
+  R mv[PER][8];
+  double qs[PER][8];
+#pragma unroll
+  for (int q = 0; q < PER; ++q)
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int x = t + q * NT + r * (N / 8);
+      const size_t gi = ((size_t)c * N + y) * N + x;
+      double m = (double)model[gi] * a.gcf[gi];
+      if ((x + y) & 1) m = -m;
+      bool outside = false;
+      qs[q][r] = a.screens ? screen_q(y, x, N, a.fov, &outside) : 0.0;
+      mv[q][r] = outside ? R(0) : (R)m;
+    }
   const int p0 = blockIdx.y * a.ppg, p1 = min(a.nplanes, p0 + a.ppg);  // this block's plane group
   for (int p = p0; p < p1; ++p) {
     const double wp = a.screens ? a.plane_w[p] : 0.0;
-#pragma unroll
-    for (int e = 0; e < EPT; ++e) {
-      const int i = threadIdx.x + e * THR;
-      const int bb = i / N, x = i % N;
-      cplx<R> v;
-      if (!a.screens) {
-        v = cmk<R>(mv[e], R(0));
-      } else if (out[e]) {
-        v = cmk<R>(R(0), R(0));
-      } else {
-        const cplx<R> s = expi_turns<R>(screen_turns(wp, q[e]));  // b; conj(b) = (s.x, -s.y)
-        v = cmk<R>(mv[e] * s.x, -mv[e] * s.y);
-      }
-      lds[lidx<LAY_ROWPAD, N, NB>(bb, x)] = v;
-    }
-    __syncthreads();
-    stages<R, N, NB, NT, 1, false, LAY_ROWPAD>(lds, b, t, tw);
-    cplx<R>* dst = grids + ((size_t)p * a.nchan + c) * N * N;
+    cplx<R>* dst = grids + ((size_t)p * a.nchan + c) * N * N + (size_t)y * N;
     const uint32_t cm = a.colmask ? a.colmask[(size_t)p * a.nchan + c] : ~0u;
-#pragma unroll
-    for (int e = 0; e < EPT; ++e) {
-      const int i = threadIdx.x + e * THR;
-      const int bb = i / N, x = i % N;
-      if ((cm >> (x / a.tile)) & 1u) dst[(size_t)(y0 + bb) * N + x] = lds[lidx<LAY_ROWPAD, N, NB>(bb, x)];
-    }
+    fft_pass<R, N, NB, NT, false, LAY_ROWPAD>(
+        lds, b, t, tw,
+        [&](int q, int r, int) {
+          if (!a.screens) return cmk<R>(mv[q][r], R(0));
+          const cplx<R> s = expi_turns<R>(screen_turns(wp, qs[q][r]));  // b; conj(b) = (s.x, -s.y)
+          return cmk<R>(mv[q][r] * s.x, -mv[q][r] * s.y);
+        },
+        [&](int, int, int x, cplx<R> v) {
+          if ((cm >> (x / a.tile)) & 1u) dst[x] = v;
+        });
     __syncthreads();
   }
 }
@@ -393,11 +432,10 @@ __global__ __launch_bounds__(Shape<N>::FNT * Shape<N>::FNB) void k_rows_predict(
 
 template <typename R, int N, bool INV>
 __global__ __launch_bounds__(Shape<N>::NT * Shape<N>::template col_nb<R>()) void k_cols_plain(FftArgs a, const cplx<R>* __restrict__ twg) {
-  constexpr int NT = Shape<N>::NT, NB = Shape<N>::template col_nb<R>(), THR = NT * NB, EPT = N * NB / THR;
+  constexpr int NT = Shape<N>::NT, NB = Shape<N>::template col_nb<R>();
   extern __shared__ __align__(16) unsigned char smem[];
   cplx<R>* lds = reinterpret_cast<cplx<R>*>(smem);
   cplx<R>* tw = stage_twiddles_at<R, N>(smem, sizeof(cplx<R>) * (size_t)N * NB, twg);
-  cplx<R>* data = reinterpret_cast<cplx<R>*>(a.data);
   constexpr int CB = N / NB;
   // column groups sharing a 128-byte line run on one XCD
   constexpr int LG = (NB * (int)sizeof(cplx<R>)) >= 128 ? 1 : 128 / (NB * (int)sizeof(cplx<R>));
@@ -410,29 +448,23 @@ __global__ __launch_bounds__(Shape<N>::NT * Shape<N>::template col_nb<R>()) void
     if (!tm) return;
     if (!a.zero_fill) tm = ~0u;
   }
-  cplx<R>* base = data + plane * (int64_t)N * N;
-#pragma unroll
-  for (int e = 0; e < EPT; ++e) {
-    const int i = threadIdx.x + e * THR;
-    const int n = i / NB, b = i % NB;
-    // a tile never written (sparse grids) reads as 0
-    const cplx<R>* src = ((tm >> (n / a.tile)) & 1u) ? base + (size_t)n * N + x0 + b : zero_entry<R>(a.zeros);
-    cplx<R> v = *src;
-    if (a.in_mod && ((n + x0 + b) & 1)) v = cmk<R>(-v.x, -v.y);
-    lds[lidx<LAY_COL, N, NB>(b, n)] = v;
-  }
-  __syncthreads();
   const int b = threadIdx.x % NB, t = threadIdx.x / NB;
-  stages<R, N, NB, NT, 1, INV, LAY_COL>(lds, b, t, tw);
-#pragma unroll
-  for (int e = 0; e < EPT; ++e) {
-    const int i = threadIdx.x + e * THR;
-    const int n = i / NB, bb = i % NB;
-    const cplx<R> v = lds[lidx<LAY_COL, N, NB>(bb, n)];
-    R sc = (R)a.scale;
-    if (a.out_mod && ((n + x0 + bb) & 1)) sc = -sc;
-    base[(size_t)n * N + x0 + bb] = cmk<R>(v.x * sc, v.y * sc);
-  }
+  const int x = x0 + b;
+  cplx<R>* col = reinterpret_cast<cplx<R>*>(a.data) + plane * (int64_t)N * N + x;
+  fft_pass<R, N, NB, NT, INV, LAY_COL>(
+      lds, b, t, tw,
+      [&](int, int, int n) {
+        // a tile never written (sparse grids) reads as 0
+        const cplx<R>* src = ((tm >> (n / a.tile)) & 1u) ? col + (size_t)n * N : zero_entry<R>(a.zeros);
+        cplx<R> v = *src;
+        if (a.in_mod && ((n + x) & 1)) v = cmk<R>(-v.x, -v.y);
+        return v;
+      },
+      [&](int, int, int n, cplx<R> v) {
+        R sc = (R)a.scale;
+        if (a.out_mod && ((n + x) & 1)) sc = -sc;
+        col[(size_t)n * N] = cmk<R>(v.x * sc, v.y * sc);
+      });
 }
 
 // Invert, after the in-place column pass: for plane group g (blockIdx.y) and the NB rows of this
@@ -441,76 +473,78 @@ __global__ __launch_bounds__(Shape<N>::NT * Shape<N>::template col_nb<R>()) void
 // imaging/base.py:84-101); otherwise it writes acc to its partial image for k_invert_finish.
 // SCREENS: w-stack planes (real part after the screen); otherwise the planes are summed and
 // the imaginary part is kept as well.  Single precision accumulates in float, double in double.
+// Each thread accumulates the elements the last stage hands it (x = t + q*NT + r*NSL).
 template <typename R, int N, bool SCREENS>
 __global__ __launch_bounds__(Shape<N>::NT * Shape<N>::ROW_NB) void k_rows_invert(FftArgs a, const cplx<R>* __restrict__ twg) {
-  constexpr int NT = Shape<N>::NT, NB = Shape<N>::ROW_NB, THR = NT * NB, EPT = N * NB / THR;
+  constexpr int NT = Shape<N>::NT, NB = Shape<N>::ROW_NB;
+  constexpr int NSL = last_ns<N>(), RADL = N / NSL, PER = NSL / NT;
   extern __shared__ __align__(16) unsigned char smem[];
   cplx<R>* lds = reinterpret_cast<cplx<R>*>(smem);
   cplx<R>* tw = stage_twiddles_at<R, N>(smem, sizeof(cplx<R>) * lds_elems<LAY_ROW, N>(NB), twg);
   const cplx<R>* grids = reinterpret_cast<const cplx<R>*>(a.data);
   constexpr int RB = N / NB;  // row blocks per channel
   const int c = blockIdx.x / RB;
-  const int y0 = (blockIdx.x % RB) * NB;
+  const int b = threadIdx.x / NT, t = threadIdx.x % NT;
+  const int y = (blockIdx.x % RB) * NB + b;
   const int G = gridDim.y;
   const int p0 = blockIdx.y * a.ppg, p1 = min(a.nplanes, p0 + a.ppg);
-  R acc[EPT];
-  R acci[SCREENS ? 1 : EPT];
-  double q[SCREENS ? EPT : 1];
-  bool out[SCREENS ? EPT : 1];
+  R acc[PER][RADL];
+  R acci[SCREENS ? 1 : PER][SCREENS ? 1 : RADL];
+  double qs[SCREENS ? PER : 1][SCREENS ? RADL : 1];
+  uint32_t outside = 0;  // bit q * RADL + r: element outside the unit circle (no screen)
+  static_assert(PER * RADL <= 32, "outside mask");
 #pragma unroll
-  for (int e = 0; e < EPT; ++e) {
-    acc[e] = R(0);
-    if constexpr (!SCREENS) {
-      acci[e] = R(0);
-    } else {
-      const int i = threadIdx.x + e * THR;
-      q[e] = screen_q(y0 + i / N, i % N, N, a.fov, &out[e]);
+  for (int q = 0; q < PER; ++q)
+#pragma unroll
+    for (int r = 0; r < RADL; ++r) {
+      acc[q][r] = R(0);
+      if constexpr (!SCREENS) {
+        acci[q][r] = R(0);
+      } else {
+        bool o;
+        qs[q][r] = screen_q(y, t + q * NT + r * NSL, N, a.fov, &o);
+        if (o) outside |= 1u << (q * RADL + r);
+      }
     }
-  }
-  const int b = threadIdx.x / NT, t = threadIdx.x % NT;
   for (int p = p0; p < p1; ++p) {
     const double sw = a.sumwt[(size_t)p * a.nchan + c];
     if (a.wstack && a.normalize && !(sw > 0.0)) continue;  // normalize_sumwt zeroes this slab's image
-    const cplx<R>* base = grids + ((size_t)p * a.nchan + c) * N * N + (size_t)y0 * N;
+    const cplx<R>* row = grids + ((size_t)p * a.nchan + c) * N * N + (size_t)y * N;
     const uint32_t cm = a.colmask ? a.colmask[(size_t)p * a.nchan + c] : ~0u;
-#pragma unroll
-    for (int e = 0; e < EPT; ++e) {
-      const int i = threadIdx.x + e * THR;
-      // sparse grids: a skipped tile column reads as 0
-      const cplx<R>* src = ((cm >> ((i % N) / a.tile)) & 1u) ? base + i : zero_entry<R>(a.zeros);
-      lds[lidx<LAY_ROW, N, NB>(i / N, i % N)] = *src;
-    }
-    __syncthreads();
-    stages<R, N, NB, NT, 1, true, LAY_ROW>(lds, b, t, tw);
     const double wp = SCREENS ? a.plane_w[p] : 0.0;
-#pragma unroll
-    for (int e = 0; e < EPT; ++e) {
-      const int i = threadIdx.x + e * THR;
-      const int bb = i / N, x = i % N, y = y0 + bb;
-      cplx<R> v = lds[lidx<LAY_ROW, N, NB>(bb, x)];
-      if ((x + y) & 1) v = cmk<R>(-v.x, -v.y);
-      if constexpr (SCREENS) {
-        if (!out[e]) {
-          const cplx<R> s = expi_turns<R>(screen_turns(wp, q[e]));
-          acc[e] += s.x * v.x - s.y * v.y;
-        }
-      } else {
-        acc[e] += v.x;
-        acci[e] += v.y;
-      }
-    }
+    fft_pass<R, N, NB, NT, true, LAY_ROW>(
+        lds, b, t, tw,
+        [&](int, int, int n) {
+          // sparse grids: a skipped tile column reads as 0
+          const cplx<R>* src = ((cm >> (n / a.tile)) & 1u) ? row + n : zero_entry<R>(a.zeros);
+          return *src;
+        },
+        [&](int q, int r, int x, cplx<R> v) {
+          if ((x + y) & 1) v = cmk<R>(-v.x, -v.y);
+          if constexpr (SCREENS) {
+            if (!((outside >> (q * RADL + r)) & 1u)) {
+              const cplx<R> s = expi_turns<R>(screen_turns(wp, qs[q][r]));
+              acc[q][r] += s.x * v.x - s.y * v.y;
+            }
+          } else {
+            acc[q][r] += v.x;
+            acci[q][r] += v.y;
+          }
+        });
     __syncthreads();
   }
+  const size_t rowoff = ((size_t)c * N + y) * N;
   if (G > 1) {
     R* part = reinterpret_cast<R*>(a.part);
     const size_t gsz = (size_t)a.nchan * N * N;
 #pragma unroll
-    for (int e = 0; e < EPT; ++e) {
-      const int i = threadIdx.x + e * THR;
-      const size_t gi = ((size_t)c * N + y0 + i / N) * N + i % N;
-      part[(size_t)blockIdx.y * gsz + gi] = acc[e];
-      if constexpr (!SCREENS) part[(size_t)(G + blockIdx.y) * gsz + gi] = acci[e];
-    }
+    for (int q = 0; q < PER; ++q)
+#pragma unroll
+      for (int r = 0; r < RADL; ++r) {
+        const size_t gi = rowoff + t + q * NT + r * NSL;
+        part[(size_t)blockIdx.y * gsz + gi] = acc[q][r];
+        if constexpr (!SCREENS) part[(size_t)(G + blockIdx.y) * gsz + gi] = acci[q][r];
+      }
     return;
   }
   double tot = 0.0;
@@ -519,18 +553,21 @@ __global__ __launch_bounds__(Shape<N>::NT * Shape<N>::ROW_NB) void k_rows_invert
   R* imi = reinterpret_cast<R*>(a.img_im);
   const bool zero = a.normalize && !(tot > 0.0);
 #pragma unroll
-  for (int e = 0; e < EPT; ++e) {
-    const int i = threadIdx.x + e * THR;
-    const size_t gi = ((size_t)c * N + y0 + i / N) * N + i % N;
-    const double g = a.gcf[gi];
-    double re = (double)acc[e] * g, im = SCREENS ? 0.0 : (double)acci[SCREENS ? 0 : e] * g;
-    if (a.normalize) {
-      re = zero ? 0.0 : re / tot;
-      im = zero ? 0.0 : im / tot;
+  for (int q = 0; q < PER; ++q)
+#pragma unroll
+    for (int r = 0; r < RADL; ++r) {
+      const size_t gi = rowoff + t + q * NT + r * NSL;
+      const double g = a.gcf[gi];
+      double re = (double)acc[q][r] * g;
+      double im = 0.0;
+      if constexpr (!SCREENS) im = (double)acci[q][r] * g;
+      if (a.normalize) {
+        re = zero ? 0.0 : re / tot;
+        im = zero ? 0.0 : im / tot;
+      }
+      img[gi] = (R)re;
+      if (imi) imi[gi] = (R)im;
     }
-    img[gi] = (R)re;
-    if (imi) imi[gi] = (R)im;
-  }
 }
 
 // Adds the G partial images in group order (float64), then applies gcf and the weight
