@@ -7,7 +7,7 @@ PyTorch-ROCm tensors holding device memory.
 
 from . import _lib
 from .data_models import (WCS, PolarisationFrame, Image, GridData, ConvolutionFunction, Visibility, BlockVisibility,
-                          Skycomponent)
+                          GainTable, Skycomponent)
 from .kernels import (create_image, create_convolutionfunction_from_image, create_pswf_convolutionfunction,
                       create_box_convolutionfunction, create_awterm_convolutionfunction,
                       apply_bounding_box_convolutionfunction, calculate_bounding_box_convolutionfunction)
@@ -20,5 +20,9 @@ from .workflows import (imaging_contexts, imaging_context, invert_list_serial_wo
                         predict_list_serial_workflow, sum_invert_results)
 from .fft_support import fft, ifft
 from .plan import Plan
+from .coalesce import convert_blockvisibility_to_visibility, convert_blocks, decoalesce_visibility
+from .calibration import (create_gaintable_from_blockvisibility, apply_gaintable, gain_row_indices, apply_gains_rows,
+                          apply_gains_rows_adjoint)
+from .differentiable import PredictFunction, GainsFunction, predict_with_gains, visibility_loss
 
 __version__ = "0.1.0"

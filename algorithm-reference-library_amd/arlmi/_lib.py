@@ -35,6 +35,7 @@ EXPORTS = (
     "arlmi_plan_stats_get", "arlmi_plan_plane_w", "arlmi_plan_destroy", "arlmi_convolution_mapping",
     "arlmi_grid", "arlmi_degrid",
     "arlmi_grids_to_image", "arlmi_image_to_grids", "arlmi_fft2d", "arlmi_w_screen", "arlmi_dft",
+    "arlmi_block_to_rows", "arlmi_apply_gaintable", "arlmi_apply_gains_rows", "arlmi_apply_gains_rows_adjoint",
     "arlmi_profile_enable", "arlmi_profile_collect", "arlmi_profile_reset", "arlmi_profile_report",
 )
 
@@ -79,6 +80,25 @@ class Kernel(ctypes.Structure):
     ]
 
 
+class Blocks(ctypes.Structure):
+    _fields_ = [
+        ("ntimes", ctypes.c_int32), ("nants", ctypes.c_int32), ("nchan", ctypes.c_int32),
+        ("reserved", ctypes.c_int32), ("d_vis", ctypes.c_void_p), ("d_uvw", ctypes.c_void_p),
+        ("d_weight", ctypes.c_void_p), ("d_imaging_weight", ctypes.c_void_p), ("d_time", ctypes.c_void_p),
+        ("d_integration_time", ctypes.c_void_p), ("d_frequency", ctypes.c_void_p),
+        ("d_channel_bandwidth", ctypes.c_void_p),
+    ]
+
+
+class Rows(ctypes.Structure):
+    _fields_ = [
+        ("d_uvw", ctypes.c_void_p), ("d_vis", ctypes.c_void_p), ("d_weight", ctypes.c_void_p),
+        ("d_imaging_weight", ctypes.c_void_p), ("d_time", ctypes.c_void_p), ("d_integration_time", ctypes.c_void_p),
+        ("d_frequency", ctypes.c_void_p), ("d_channel_bandwidth", ctypes.c_void_p), ("d_antenna1", ctypes.c_void_p),
+        ("d_antenna2", ctypes.c_void_p),
+    ]
+
+
 _lib = None
 
 
@@ -111,6 +131,10 @@ def lib():
         "arlmi_fft2d": (ctypes.c_int, [vp, i32, i64, ctypes.c_int, ctypes.c_int, vp]),
         "arlmi_w_screen": (ctypes.c_int, [i32, dbl, dbl, vp, ctypes.c_int, vp]),
         "arlmi_dft": (ctypes.c_int, [vp, vp, i64, vp, i32, i32, vp, ctypes.c_int, vp]),
+        "arlmi_block_to_rows": (ctypes.c_int, [ctypes.POINTER(Blocks), ctypes.POINTER(Rows), vp, ctypes.c_int, vp]),
+        "arlmi_apply_gaintable": (ctypes.c_int, [vp, i32, i32, i32, vp, vp, vp, i32, ctypes.c_int, ctypes.c_int, vp]),
+        "arlmi_apply_gains_rows": (ctypes.c_int, [i64, vp, vp, vp, i64, vp, vp, ctypes.c_int, vp]),
+        "arlmi_apply_gains_rows_adjoint": (ctypes.c_int, [i64, vp, vp, vp, i64, vp, vp, vp, vp, ctypes.c_int, vp]),
         "arlmi_profile_enable": (ctypes.c_int, [ctypes.c_int]),
         "arlmi_profile_collect": (ctypes.c_int, []),
         "arlmi_profile_reset": (ctypes.c_int, []),

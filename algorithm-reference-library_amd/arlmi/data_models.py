@@ -14,7 +14,7 @@ import numpy
 import torch
 
 __all__ = ["WCS", "PolarisationFrame", "Image", "GridData", "ConvolutionFunction", "Visibility",
-           "BlockVisibility", "Skycomponent", "device", "C_LIGHT"]
+           "BlockVisibility", "GainTable", "Skycomponent", "device", "C_LIGHT"]
 
 C_LIGHT = 299792458.0
 D2R = numpy.pi / 180.0
@@ -279,6 +279,8 @@ class BlockVisibility:
     metres, frequency [nchan], channel_bandwidth [nchan], time [ntimes].
     """
 
+    COLUMNS = ("vis", "uvw", "weight", "imaging_weight", "time", "integration_time")
+
     def __init__(self, data=None, frequency=None, channel_bandwidth=None, polarisation_frame=None,
                  phasecentre=None):
         self.data = data or {}
@@ -287,6 +289,59 @@ class BlockVisibility:
         self.polarisation_frame = polarisation_frame or PolarisationFrame()
         self.phasecentre = phasecentre
 
+    @classmethod
+    def from_arrays(cls, uvw, vis=None, weight=None, imaging_weight=None, time=None, integration_time=None,
+                    frequency=None, channel_bandwidth=None, precision=32, dev=None):
+        """Device columns from arrays: uvw [ntimes, nants, nants, 3] metres; vis, weight and
+        imaging_weight [ntimes, nants, nants, nchan, 1] (vis 0, weights 1 when omitted)."""
+        dev = dev or device()
+        cdt = torch.complex64 if precision == 32 else torch.complex128
+        rdt = torch.float32 if precision == 32 else torch.float64
+        uvw = torch.as_tensor(numpy.asarray(uvw, dtype=numpy.float64)).to(dev).contiguous()
+        ntimes, nants = uvw.shape[0], uvw.shape[1]
+        frequency = numpy.atleast_1d(numpy.asarray(frequency, dtype=numpy.float64))
+        nchan = len(frequency)
+        if channel_bandwidth is None:
+            channel_bandwidth = numpy.ones(nchan)
+        shape = (ntimes, nants, nants, nchan, 1)
+
+        def col(x, dt, fill):
+            if x is None:
+                return torch.full(shape, fill, dtype=dt, device=dev)
+            return torch.as_tensor(numpy.asarray(x)).to(device=dev, dtype=dt).reshape(shape).contiguous()
+
+        t = numpy.zeros(ntimes) if time is None else numpy.asarray(time, dtype=numpy.float64)
+        it = numpy.ones(ntimes) if integration_time is None else numpy.asarray(integration_time, dtype=numpy.float64)
+        data = {
+            "uvw": uvw,
+            "vis": col(vis, cdt, 0.0),
+            "weight": col(weight, rdt, 1.0),
+            "imaging_weight": col(imaging_weight, rdt, 1.0),
+            "time": torch.as_tensor(t).to(dev),
+            "integration_time": torch.as_tensor(it).to(dev),
+        }
+        return cls(data, frequency=frequency, channel_bandwidth=numpy.asarray(channel_bandwidth, dtype=numpy.float64))
+
+    @property
+    def nants(self):
+        return self.data["uvw"].shape[1]
+
+    @property
+    def nchan(self):
+        return self.data["vis"].shape[3]
+
+    @property
+    def ntimes(self):
+        return self.data["uvw"].shape[0]
+
+    @property
+    def time(self):
+        return self.data["time"]
+
+    @property
+    def precision(self):
+        return 32 if self.data["vis"].dtype == torch.complex64 else 64
+
     @property
     def vis(self):
         return self.data["vis"]
@@ -294,6 +349,31 @@ class BlockVisibility:
     @property
     def uvw(self):
         return self.data["uvw"]
+
+
+class GainTable:
+    """Scalar antenna gains (memory_data_models.py:132-234): gain [ntimes, nants, nchan, 1, 1]
+    complex, weight the same shape (float64), time and interval [ntimes] (float64), frequency
+    [nchan].  The convention is V_obs = g_i conj(g_j) V_model for the [t, i, j] element."""
+
+    def __init__(self, gain, time, interval, weight=None, frequency=None):
+        self.gain = gain
+        self.time = numpy.asarray(time, dtype=numpy.float64)
+        self.interval = numpy.asarray(interval, dtype=numpy.float64)
+        self.weight = weight
+        self.frequency = frequency
+
+    @property
+    def ntimes(self):
+        return self.gain.shape[0]
+
+    @property
+    def nants(self):
+        return self.gain.shape[1]
+
+    @property
+    def nchan(self):
+        return self.gain.shape[2]
 
 
 class Skycomponent:

@@ -101,6 +101,23 @@ class Imager:
                                                    self.precision, stream_handle()))
         return re, imo
 
+    def adjoint(self, vis, vis_data, out=None):
+        """Adjoint of predict for a real image: Re(sum over planes of screen * gcf * F^H grid(v)),
+        which is invert with unit weights and no normalisation (F^H = N^2 ifft2 with the shifts,
+        grid with the conjugated kernel).  vis_data: [V, 1] complex rows of ``vis``."""
+        ones = self._unit_weights(vis)
+        v = vis.with_vis(vis_data.to(vis.vis.dtype).contiguous())
+        v.data["imaging_weight"] = ones
+        re, _ = self.invert(v, dopsf=False, normalize=False, out=out)
+        return re
+
+    def _unit_weights(self, vis):
+        w = getattr(self, "_ones", None)
+        if w is None or w.shape[0] != vis.nvis:
+            w = torch.ones((vis.nvis, 1), dtype=_real_dtype(self.precision), device=self.dev)
+            self._ones = w
+        return w
+
     def sumwt(self):
         """[nchan, 1] float64 (host) sum of weights of the last invert."""
         return self.sumwt_planes.sum(dim=0).cpu().numpy()[:, None]

@@ -184,6 +184,81 @@ int arlmi_w_screen(int32_t npixel, double cellsize, double w, void* d_out, int p
 int arlmi_dft(const double* d_uvw, const int32_t* d_chan, int64_t nrows, const double* d_lmflux, int32_t ncomp,
               int32_t nchan, void* d_vis, int precision, void* stream);
 
+/* Columns of a BlockVisibility (data_models/memory_data_models.py:940-1094), npol = 1.
+ * Complex and real columns are in the call's precision. */
+typedef struct arlmi_blocks {
+  int32_t ntimes, nants, nchan, reserved;
+  const void* d_vis;                  /* [ntimes][nants][nants][nchan] complex */
+  const double* d_uvw;                /* [ntimes][nants][nants][3] metres */
+  const void* d_weight;               /* [ntimes][nants][nants][nchan] real, or NULL */
+  const void* d_imaging_weight;       /* [ntimes][nants][nants][nchan] real, or NULL */
+  const double* d_time;               /* [ntimes], or NULL */
+  const double* d_integration_time;   /* [ntimes], or NULL */
+  const double* d_frequency;          /* [nchan] Hz (required) */
+  const double* d_channel_bandwidth;  /* [nchan], or NULL */
+} arlmi_blocks;
+
+/* Columns of row-form Visibility (memory_data_models.py:745-937), npol = 1.  A NULL pointer
+ * skips that column (as does a NULL input column of arlmi_blocks). */
+typedef struct arlmi_rows {
+  double* d_uvw;                /* [V][3] wavelengths */
+  void* d_vis;                  /* [V] complex */
+  void* d_weight;               /* [V] real */
+  void* d_imaging_weight;       /* [V] real */
+  double* d_time;               /* [V] */
+  double* d_integration_time;   /* [V] */
+  double* d_frequency;          /* [V] */
+  double* d_channel_bandwidth;  /* [V] */
+  int64_t* d_antenna1;          /* [V] */
+  int64_t* d_antenna2;          /* [V] */
+} arlmi_rows;
+
+/* convert_blockvisibility_to_visibility / convert_blocks (processing_components/visibility/
+ * coalesce.py:85-109, 331-444), no averaging.  Rows are the lower triangle [t, a2, a1] with
+ * a1 < a2 of every integration, in numpy.argwhere order (t, a2, a1, chan) (:413-416):
+ *   row = (t * nbl + a2 (a2 - 1) / 2 + a1) * nchan + chan,   nbl = nants (nants - 1) / 2,
+ * so V = ntimes * nbl * nchan.  antenna1 = a1, antenna2 = a2; uvw = uvw_m[t, a2, a1] * (f / c)
+ * with c = 299792458 m/s, evaluated in that order (:435-437), which makes it bit-identical to
+ * ARL's float64 result.  d_cindex (int64 [ntimes][nants][nants][nchan], or NULL) receives the row
+ * of every lower-triangle element and 0 elsewhere (:418). */
+int arlmi_block_to_rows(const arlmi_blocks* blocks, const arlmi_rows* rows, int64_t* d_cindex, int precision,
+                        void* stream);
+
+/* apply_gaintable with scalar gains (processing_components/calibration/operations.py:79-164,
+ * scalar branch :127-140), in place on a BlockVisibility's vis [ntimes][nants][nants][nchan]:
+ *   vis[t][i][j][ch] *= lg[s][i][ch] * conj(lg[s][j][ch]),   s = d_gain_slot[t],
+ * with lg = g (inverse = 0) or lg = 1/g where |g| > 0 and 1 elsewhere (inverse = 1, :129-131), and
+ * vis set to 0 where gain_weight[s][i][ch] * gain_weight[s][j][ch] == 0 (:137-138).  Every
+ * (i, j) pair, both triangles and the diagonal, is processed as in ARL; weights are not
+ * changed (ARL's appliedwt is computed but never stored back, :164).  d_gain: complex
+ * [ngain_times][nants][nchan]; d_gain_weight: float64, same shape, or NULL (all 1).  d_gain_slot:
+ * int32 [ntimes], the gain-table row whose time matches each integration (:112-113); an
+ * integration whose slot lies outside [0, ngain_times) is left unchanged. */
+int arlmi_apply_gaintable(void* d_vis, int32_t ntimes, int32_t nants, int32_t nchan, const void* d_gain,
+                          const double* d_gain_weight, const int32_t* d_gain_slot, int32_t ngain_times,
+                          int inverse, int precision, void* stream);
+
+/* The same scalar gain convention on rows (operations.py:135-137 seen through convert_blocks'
+ * (a2, a1) rows, so the un-conjugated gain is antenna2's):
+ *   d_out[r] = d_in[r] * g[d_gidx2[r]] * conj(g[d_gidx1[r]]),
+ * d_gidx1 / d_gidx2: int32 [nrows], index into d_gain (complex [ngain], flattened by the caller,
+ * e.g. (slot * nants + antenna) * nchan + chan) of the gain of antenna1 / antenna2.  A row with an
+ * index outside [0, ngain) gets 0.  d_out may equal d_in. */
+int arlmi_apply_gains_rows(int64_t nrows, const int32_t* d_gidx1, const int32_t* d_gidx2, const void* d_gain,
+                           int64_t ngain, const void* d_in, void* d_out, int precision, void* stream);
+
+/* Adjoint of arlmi_apply_gains_rows for a real loss L (no ARL counterpart: ARL has no
+ * gradients; this is what gradient-based calibration through the convention above needs).
+ * Gradients are complex numbers dL/dRe + i dL/dIm.  With G = d_grad_out and s = g[gidx2] conj(g[gidx1]):
+ *   d_grad_in[r]            = G[r] * conj(s_r)                      (precision complex, or NULL)
+ *   d_grad_gain[gidx2[r]]  += G[r] * g[gidx1[r]] * conj(d_in[r])
+ *   d_grad_gain[gidx1[r]]  += conj(G[r]) * g[gidx2[r]] * d_in[r]    (complex128 [ngain], or NULL)
+ * d_grad_gain is accumulated into (the caller zeroes it); rows with an index outside [0, ngain)
+ * contribute nothing and get d_grad_in = 0. */
+int arlmi_apply_gains_rows_adjoint(int64_t nrows, const int32_t* d_gidx1, const int32_t* d_gidx2, const void* d_gain,
+                                   int64_t ngain, const void* d_in, const void* d_grad_out, void* d_grad_in,
+                                   void* d_grad_gain, int precision, void* stream);
+
 /* Opt-in kernel timing: with profiling on, every library kernel launch is bracketed by two
  * HIP events on its stream.  collect() waits for them and adds the elapsed times per kernel;
  * report() writes "kernel total_ms launches" lines into buf and returns the size needed. */

@@ -37,6 +37,15 @@ Pinning (what checks each function; see tests/test_oracle_*.py):
       (tests/processing_library/test_coordinate_support.py:104-123)
   xyz_to_uvw, xyz_at_latitude, hadec_to_azel
                                            closed forms (rotation invariants, known angles)
+  create_blockvisibility, convert_blocks, decoalesce_visibility
+                                           the element loop of convert_blocks, the independent
+                                           row construction of create_visibility (same rows, same
+                                           bits, other order), cindex round trips
+  apply_gaintable, apply_gains_rows        ARL's own assertions (test_calibration_operations.py:
+                                           92-135: apply changes, apply + inverse = identity to
+                                           1e-12, zero gains + inverse = no change) and block/row
+                                           agreement through convert_blocks
+  apply_gains_rows_adjoint                 central finite differences and the dot-product identity
 """
 
 import numpy
@@ -52,7 +61,8 @@ __all__ = [
     "normalize_sumwt", "invert_2d", "predict_2d", "create_w_term_like", "vis_wslice_iter",
     "invert_wstack_single", "predict_wstack_single", "invert_list_wstack", "predict_list_wstack",
     "simulate_point", "predict_skycomponent_visibility", "xyz_at_latitude", "xyz_to_uvw",
-    "hadec_to_azel", "create_visibility", "apply_gains_rows",
+    "hadec_to_azel", "create_visibility", "apply_gains_rows", "create_blockvisibility", "convert_blocks",
+    "convert_blocks_loop", "decoalesce_visibility", "apply_gaintable", "apply_gains_rows_adjoint",
 ]
 
 D2R = numpy.pi / 180.0
@@ -820,3 +830,153 @@ def apply_gains_rows(vis_rows, antenna1, antenna2, gain):
     so in row form i = antenna2, j = antenna1.  gain: [nants] or [nrows-aligned] complex.
     """
     return vis_rows * (gain[antenna2] * numpy.conjugate(gain[antenna1]))[:, numpy.newaxis]
+
+
+# ---------------------------------------------------------------------------
+# BlockVisibility layout: visibility/base.py:153-250, visibility/coalesce.py:85-143, 331-444
+# ---------------------------------------------------------------------------
+
+def create_blockvisibility(ants_xyz, times, frequency, channel_bandwidth, dec, zerow=False, integration_time=1.0):
+    """BlockVisibility columns (visibility/base.py:153-250, no elevation limit): vis
+    [ntimes, nants, nants, nchan, 1] zeros, uvw [ntimes, nants, nants, 3] metres with
+    uvw[t, a2, a1] = pos[a2] - pos[a1] and uvw[t, a1, a2] = -(that), weights 1."""
+    nants = ants_xyz.shape[0]
+    nch = len(frequency)
+    ntimes = len(times)
+    uvw = numpy.zeros([ntimes, nants, nants, 3])
+    rtimes = numpy.zeros(ntimes)
+    for it, ha in enumerate(times):
+        ant_pos = xyz_to_uvw(ants_xyz, ha, dec)
+        rtimes[it] = ha * 43200.0 / numpy.pi
+        for a1 in range(nants):
+            for a2 in range(a1 + 1, nants):
+                uvw[it, a2, a1, :] = ant_pos[a2, :] - ant_pos[a1, :]
+                uvw[it, a1, a2, :] = ant_pos[a1, :] - ant_pos[a2, :]
+    if zerow:
+        uvw[..., 2] = 0.0
+    shape = [ntimes, nants, nants, nch, 1]
+    return {
+        'vis': numpy.zeros(shape, dtype='complex'),
+        'uvw': uvw,
+        'weight': numpy.ones(shape),
+        'imaging_weight': numpy.ones(shape),
+        'time': rtimes,
+        'integration_time': numpy.full(ntimes, integration_time),
+        'frequency': numpy.asarray(frequency, dtype=float),
+        'channel_bandwidth': numpy.asarray(channel_bandwidth, dtype=float),
+    }
+
+
+def convert_blocks(bvis):
+    """Rows of the lower triangle (coalesce.py:331-444), vectorised as ARL does it: boolean
+    masks of [t, a2, a1] with a1 < a2, numpy.argwhere order (t, a2, a1, chan),
+    uvw = uvw_m * (f / c)."""
+    vis, uvw = bvis['vis'], bvis['uvw']
+    ntimes, nant, _, nchan, npol = vis.shape
+    frequency, bw = bvis['frequency'], bvis['channel_bandwidth']
+    cnvis = ntimes * nant * (nant - 1) * nchan // 2
+    rowgrid = numpy.arange(ntimes * nant * nant * nchan).reshape(ntimes, nant, nant, nchan)
+    cindex = numpy.zeros(rowgrid.size, dtype='int')
+    lower = numpy.tril(numpy.ones((nant, nant), dtype=bool), k=-1)
+    mask_rowgrid = numpy.broadcast_to(lower[None, :, :, None], rowgrid.shape)
+    mask_uvw = numpy.broadcast_to(lower[None, :, :, None], uvw.shape)
+    mask_vis = numpy.broadcast_to(lower[None, :, :, None, None], vis.shape)
+    rg = numpy.argwhere(mask_rowgrid)
+    ca2, ca1 = rg[:, 1], rg[:, 2]
+    cindex[rowgrid[rg[:, 0], rg[:, 1], rg[:, 2], rg[:, 3]]] = numpy.arange(cnvis)
+    nbl = nant * (nant - 1) // 2
+    cfrequency = numpy.tile(frequency, ntimes * nbl)
+    cbw = numpy.tile(bw, ntimes * nbl)
+    ctime = numpy.repeat(bvis['time'], nchan * nbl)
+    cint = numpy.repeat(bvis['integration_time'], nchan * nbl)
+    cuvw = numpy.tile(uvw[mask_uvw].reshape(-1, 3), nchan).reshape(-1, 3)
+    freq = numpy.repeat(cfrequency, 3).reshape(-1, 3)
+    cuvw[..., :] *= freq[:] / C_LIGHT
+    return {
+        'vis': vis[mask_vis].reshape(-1, npol),
+        'uvw': cuvw,
+        'weight': bvis['weight'][mask_vis].reshape(-1, npol),
+        'imaging_weight': bvis['imaging_weight'][mask_vis].reshape(-1, npol),
+        'time': ctime, 'frequency': cfrequency, 'channel_bandwidth': cbw, 'integration_time': cint,
+        'antenna1': ca1, 'antenna2': ca2, 'cindex': cindex,
+    }
+
+
+def convert_blocks_loop(bvis):
+    """The same rows by an explicit loop over (t, a2, a1 < a2, chan) (the row semantics of
+    coalesce.py:386-407 in argwhere order)."""
+    vis, uvw = bvis['vis'], bvis['uvw']
+    ntimes, nant, _, nchan, npol = vis.shape
+    rows = {k: [] for k in ('vis', 'uvw', 'weight', 'imaging_weight', 'time', 'frequency', 'channel_bandwidth',
+                            'integration_time', 'antenna1', 'antenna2')}
+    cindex = numpy.zeros(ntimes * nant * nant * nchan, dtype='int')
+    row = 0
+    for t in range(ntimes):
+        for a2 in range(nant):
+            for a1 in range(a2):
+                for ch in range(nchan):
+                    f = bvis['frequency'][ch]
+                    rows['vis'].append(vis[t, a2, a1, ch])
+                    rows['uvw'].append(uvw[t, a2, a1] * (f / C_LIGHT))
+                    rows['weight'].append(bvis['weight'][t, a2, a1, ch])
+                    rows['imaging_weight'].append(bvis['imaging_weight'][t, a2, a1, ch])
+                    rows['time'].append(bvis['time'][t])
+                    rows['integration_time'].append(bvis['integration_time'][t])
+                    rows['frequency'].append(f)
+                    rows['channel_bandwidth'].append(bvis['channel_bandwidth'][ch])
+                    rows['antenna1'].append(a1)
+                    rows['antenna2'].append(a2)
+                    cindex[((t * nant + a2) * nant + a1) * nchan + ch] = row
+                    row += 1
+    out = {k: numpy.array(v) for k, v in rows.items()}
+    out['cindex'] = cindex
+    return out
+
+
+def decoalesce_visibility(rows, bvis):
+    """Block element i takes row cindex[i] (coalesce.py:112-143), for vis and both weights."""
+    out = {k: numpy.array(v, copy=True) for k, v in bvis.items()}
+    shape = bvis['vis'].shape
+    for k in ('vis', 'weight', 'imaging_weight'):
+        out[k] = rows[k].reshape(-1)[rows['cindex']].reshape(shape)
+    return out
+
+
+def apply_gaintable(bvis_vis, gain, gain_weight, slots, inverse=False):
+    """Scalar branch of apply_gaintable (calibration/operations.py:79-164, 127-140) on a
+    [ntimes, nants, nants, nchan, 1] vis array, returning a new array.  gain, gain_weight:
+    [ngain_times, nants, nchan]; slots[t]: gain row of integration t, or -1 (no gain row matched:
+    the integration is left as it is)."""
+    applied = numpy.array(bvis_vis, copy=True)
+    ntimes, nant, _, nchan, _ = applied.shape
+    for t in range(ntimes):
+        s = slots[t]
+        if s < 0:
+            continue
+        g = gain[s]
+        if inverse:
+            lgain = numpy.ones_like(g)
+            nz = numpy.abs(g) > 0.0
+            lgain[nz] = 1.0 / g[nz]
+        else:
+            lgain = g
+        clgain = numpy.conjugate(lgain)
+        for chan in range(nchan):
+            smueller = numpy.outer(lgain[:, chan], clgain[:, chan])
+            applied[t, :, :, chan, 0] = bvis_vis[t, :, :, chan, 0] * smueller
+            antantwt = numpy.outer(gain_weight[s, :, chan], gain_weight[s, :, chan])
+            applied[t, :, :, chan, 0][antantwt == 0.0] = 0.0
+    return applied
+
+
+def apply_gains_rows_adjoint(vis_rows, grad_out, idx1, idx2, gain_flat):
+    """Gradients of a real loss through out = vis * g[idx2] * conj(g[idx1]), as complex numbers
+    dL/dRe + i dL/dIm: (grad_vis, grad_gain)."""
+    g1, g2 = gain_flat[idx1], gain_flat[idx2]
+    v = vis_rows.reshape(-1)
+    G = grad_out.reshape(-1)
+    grad_vis = (G * numpy.conjugate(g2 * numpy.conjugate(g1))).reshape(vis_rows.shape)
+    grad_gain = numpy.zeros(gain_flat.shape, dtype=complex)
+    numpy.add.at(grad_gain, idx2, G * g1 * numpy.conjugate(v))
+    numpy.add.at(grad_gain, idx1, numpy.conjugate(G) * g2 * v)
+    return grad_vis, grad_gain
