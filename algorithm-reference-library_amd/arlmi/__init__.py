@@ -24,5 +24,6 @@ from .coalesce import convert_blockvisibility_to_visibility, convert_blocks, dec
 from .calibration import (create_gaintable_from_blockvisibility, apply_gaintable, gain_row_indices, apply_gains_rows,
                           apply_gains_rows_adjoint)
 from .differentiable import PredictFunction, GainsFunction, predict_with_gains, visibility_loss
+from .batch import Autotuner, BatchImager
 
 __version__ = "0.1.0"

@@ -64,7 +64,7 @@ class Imager:
     """
 
     def __init__(self, vis, im, gcfcf=None, mode="2d", vis_slices=1, slab_boxes=None, tile=0,
-                 drop_off_grid=False, sparse=True, **kwargs):
+                 drop_off_grid=False, sparse=True, chunk=0, **kwargs):
         self.gcf, self.cf = _gcfcf(im, gcfcf, kwargs)
         self.im = im
         self.precision = vis.precision
@@ -72,10 +72,10 @@ class Imager:
         # neither written nor read)
         if mode == "2d":
             self.plan = Plan(vis, im, self.cf, _lib.MODE_2D, 1, tile=tile, drop_off_grid=drop_off_grid,
-                             sparse=sparse)
+                             sparse=sparse, chunk=chunk)
         elif mode == "wstack":
             self.plan = Plan(vis, im, self.cf, _lib.MODE_WSTACK, vis_slices, tile=tile, drop_off_grid=drop_off_grid,
-                             slab_boxes=slab_boxes, sparse=sparse)
+                             slab_boxes=slab_boxes, sparse=sparse, chunk=chunk)
         else:
             raise ValueError("mode must be '2d' or 'wstack'")
         self.mode = mode

@@ -93,7 +93,7 @@ class Plan:
     """
 
     def __init__(self, vis, im, cf, mode=_lib.MODE_2D, nplanes=1, nw_grid=1, wstep_grid=1e15, tile=0,
-                 drop_off_grid=False, slab_boxes=None, use_freq=None, sparse=False):
+                 drop_off_grid=False, slab_boxes=None, use_freq=None, sparse=False, chunk=0):
         self.L = _lib.lib()
         self.mode = mode
         self.nplanes = nplanes
@@ -104,6 +104,9 @@ class Plan:
         self.nchan = self.geom.nchan
         self.use_freq = use_freq
         self._build(vis, slab_boxes, create=True)
+        if chunk:
+            _lib.check(self.L.arlmi_plan_set_chunk(self.handle, int(chunk)))
+            self._build(vis, slab_boxes, create=False)
 
     def _freq_ptr(self, vis):
         use = self.use_freq if self.use_freq is not None else (self.nchan > 1 or self.geom.dfreq != 0.0)

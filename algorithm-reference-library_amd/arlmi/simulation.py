@@ -16,8 +16,8 @@ import torch
 from .data_models import Visibility, Skycomponent, C_LIGHT, device
 from .kernels import create_image
 
-__all__ = ["site_xyz", "uvw_rows", "make_observation", "config1", "config2", "power_law_sources",
-           "uniform_disk", "log_radial", "LATITUDE"]
+__all__ = ["site_xyz", "uvw_rows", "make_observation", "config1", "config2", "config3", "config4",
+           "power_law_sources", "uniform_disk", "log_radial", "gaussian_blobs", "block_uvw", "LATITUDE"]
 
 LATITUDE = -26.82 * numpy.pi / 180.0  # site latitude used by every workload
 
@@ -165,3 +165,120 @@ def config2(precision=32, seed=2, npixel=2048, cellsize=1.5e-4, support=8, nsour
     dtype = torch.float32 if precision == 32 else torch.float64
     im = create_image(npixel=npixel, cellsize=cellsize, frequency=[100e6], channel_bandwidth=[1e9], dtype=dtype)
     return vis, im, comps
+
+
+def config3(nobs=64, seed=3, precision=32, dev=None, max_rows=4_000_000, min_rows=10_000):
+    """Batching: nobs independent observations, each with a seeded antenna count in 32-512,
+    channel count in 1-16 and integration count in 10-240, redrawn until its row count lies in
+    [min_rows, max_rows]; image side from {256, 512, 1024, 2048}, PSWF support from {6, 8, 10},
+    8-64 w slabs; arrays log-radial to 10 km, channels over 100-150 MHz, HA -1h..+1h at dec
+    -30 deg, cell half the critical cell (imaging/base.py:303-318), MFS image.  Visibilities are
+    seeded noise (the workload shape, not the sky, is what is measured).  Returns a list of
+    dicts: vis, im, support, vis_slices, rows_generated."""
+    from .imaging import create_image_from_visibility
+    rng = numpy.random.default_rng(seed)
+    dev = dev or device()
+    out = []
+    while len(out) < nobs:
+        nants = int(rng.integers(32, 513))
+        nchan = int(rng.integers(1, 17))
+        nint = int(rng.integers(10, 241))
+        rows = nint * nants * (nants - 1) // 2 * nchan
+        if not (min_rows <= rows <= max_rows):
+            continue
+        npixel = int(rng.choice([256, 512, 1024, 2048]))
+        support = int(rng.choice([6, 8, 10]))
+        slices = int(rng.integers(8, 65))
+        enu = log_radial(rng, nants, 10000.0)
+        ha = numpy.linspace(-numpy.pi / 12, numpy.pi / 12, nint)
+        freq = numpy.linspace(100e6, 150e6, nchan) if nchan > 1 else numpy.array([100e6])
+        vis = make_observation(enu, ha, freq, -30.0 * numpy.pi / 180.0, precision=precision, select=False, dev=dev)
+        tmpl = create_image_from_visibility(vis, npixel=npixel)
+        cell = float(tmpl.cellsize)
+        keep = _select_uv(vis.data["uvw"], npixel, cell, support)
+        if not bool(keep.all()):
+            vis = Visibility({k: v[keep].contiguous() for k, v in vis.data.items()})
+        n = vis.nvis
+        noise = torch.randn((n, 2), dtype=torch.float64, device=dev,
+                            generator=torch.Generator(device=dev).manual_seed(int(rng.integers(1 << 30))))
+        vis.data["vis"] = torch.complex(noise[:, 0], noise[:, 1]).to(vis.vis.dtype)[:, None].contiguous()
+        dtype = torch.float32 if precision == 32 else torch.float64
+        im = create_image(npixel=npixel, cellsize=cell, frequency=[freq[0]], channel_bandwidth=[1e9], dtype=dtype)
+        out.append({"vis": vis, "im": im, "support": support, "vis_slices": slices, "rows_generated": rows})
+    return out
+
+
+def gaussian_blobs(rng, npixel, nblobs, fwhm_min, fwhm_max, flux_min=0.1, flux_max=1.0, margin=None):
+    """[1, 1, N, N] float64 sum of circular Gaussians with FWHM U(fwhm_min, fwhm_max) pixels."""
+    margin = margin if margin is not None else npixel // 4
+    img = numpy.zeros((npixel, npixel))
+    y, x = numpy.mgrid[0:npixel, 0:npixel]
+    for _ in range(nblobs):
+        cy, cx = rng.uniform(margin, npixel - margin, 2)
+        sig = rng.uniform(fwhm_min, fwhm_max) / (2.0 * math.sqrt(2.0 * math.log(2.0)))
+        r = max(1, int(5 * sig))
+        iy, ix = int(cy), int(cx)
+        sl = (slice(iy - r, iy + r + 1), slice(ix - r, ix + r + 1))
+        img[sl] += rng.uniform(flux_min, flux_max) * numpy.exp(-((y[sl] - cy) ** 2 + (x[sl] - cx) ** 2) / (2 * sig ** 2))
+    return img[None, None]
+
+
+def block_uvw(xyz, hour_angles, dec, dev=None):
+    """BlockVisibility uvw [ntimes, nants, nants, 3] metres, uvw[t, i, j] = pos[i] - pos[j]
+    (visibility/base.py:224-233), float64 on the device."""
+    dev = dev or device()
+    X = torch.as_tensor(xyz, dtype=torch.float64, device=dev)
+    ha = torch.as_tensor(numpy.asarray(hour_angles, dtype=numpy.float64), device=dev)
+    x, y, z = X[:, 0], X[:, 1], X[:, 2]
+    ch, sh = torch.cos(ha)[:, None], torch.sin(ha)[:, None]
+    u = x[None, :] * ch - y[None, :] * sh
+    v0 = x[None, :] * sh + y[None, :] * ch
+    w = z[None, :] * math.sin(dec) - v0 * math.cos(dec)
+    v = z[None, :] * math.cos(dec) + v0 * math.sin(dec)
+    pos = torch.stack([u, v, w], dim=-1)  # [nt, nants, 3]
+    return (pos[:, :, None, :] - pos[:, None, :, :]).contiguous()
+
+
+def config4(seed=4, precision=32, npixel=1024, dev=None):
+    """Differentiation: 128 antennas uniform in a 6 km (diameter) disk, 4 channels 120-140 MHz, 60
+    integrations over HA -1h..+1h at dec -30 deg, as a BlockVisibility converted to rows on the
+    GPU (8,128 baselines x 60 x 4 = 1,950,720 rows); a 1024^2 image of 50 Gaussian blobs (FWHM 2-6
+    pixels) at half the critical cell; gains amplitude 1 +- 0.1 (uniform), phase U(-pi/4, pi/4) per
+    antenna per channel (one gain-table row per integration, all equal).  Returns a dict: bvis,
+    vis, im, image (numpy [1, 1, N, N]), gaintable."""
+    from .coalesce import convert_blockvisibility_to_visibility
+    from .calibration import create_gaintable_from_blockvisibility
+    from .data_models import BlockVisibility
+    from .imaging import create_image_from_visibility
+    rng = numpy.random.default_rng(seed)
+    dev = dev or device()
+    nants, nchan, nint = 128, 4, 60
+    enu = uniform_disk(rng, nants, 3000.0)
+    ha = numpy.linspace(-numpy.pi / 12, numpy.pi / 12, nint)
+    freq = numpy.linspace(120e6, 140e6, nchan)
+    dec = -30.0 * numpy.pi / 180.0
+    uvw = block_uvw(site_xyz(enu), ha, dec, dev=dev)
+    cdt = torch.complex64 if precision == 32 else torch.complex128
+    rdt = torch.float32 if precision == 32 else torch.float64
+    shape = (nint, nants, nants, nchan, 1)
+    data = {
+        "uvw": uvw,
+        "vis": torch.zeros(shape, dtype=cdt, device=dev),
+        "weight": torch.ones(shape, dtype=rdt, device=dev),
+        "imaging_weight": torch.ones(shape, dtype=rdt, device=dev),
+        "time": torch.as_tensor(ha * 43200.0 / numpy.pi).to(dev),
+        "integration_time": torch.full((nint,), 10.0, dtype=torch.float64, device=dev),
+    }
+    bvis = BlockVisibility(data, frequency=freq, channel_bandwidth=numpy.full(nchan, 5e6))
+    vis = convert_blockvisibility_to_visibility(bvis)
+    tmpl = create_image_from_visibility(vis, npixel=npixel)
+    dtype = torch.float32 if precision == 32 else torch.float64
+    im = create_image(npixel=npixel, cellsize=float(tmpl.cellsize), frequency=[freq[0]], channel_bandwidth=[1e9],
+                      dtype=dtype)
+    image = gaussian_blobs(rng, npixel, 50, 2.0, 6.0)
+    gt = create_gaintable_from_blockvisibility(bvis)
+    amp = rng.uniform(0.9, 1.1, (nants, nchan))
+    phase = rng.uniform(-numpy.pi / 4, numpy.pi / 4, (nants, nchan))
+    g = (amp * numpy.exp(1j * phase))[None, :, :, None, None].repeat(gt.ntimes, axis=0)
+    gt.gain = torch.as_tensor(g).to(device=dev, dtype=cdt)
+    return {"bvis": bvis, "vis": vis, "im": im, "image": image, "gaintable": gt}

@@ -523,6 +523,12 @@ __global__ void k_zero_rows(const uint32_t* __restrict__ rows, int stride, int64
   if (i < n) vis[rows[i * stride]] = mk<R>(R(0), R(0));
 }
 
+// Zero n doubles (a kernel rather than a memset node, so that captured graphs replay it).
+__global__ void k_zero_f64(double* __restrict__ p, int64_t n) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
@@ -627,7 +633,9 @@ template <typename R>
 static int grid_impl(const arlmi_plan* p, const void* vis, const void* wt, int dopsf, const arlmi_kernel* k,
                      void* grids, double* sumwt, hipStream_t s) {
   const DevGeom& g = p->g;
-  ARLMI_HIP(hipMemsetAsync(sumwt, 0, sizeof(double) * (size_t)g.nplanes * g.nchan, s));
+  const int64_t nsw = (int64_t)g.nplanes * g.nchan;
+  ARLMI_LAUNCH("k_zero_f64", k_zero_f64, dim3((unsigned)((nsw + 255) / 256)), dim3(256), 0, s, sumwt, nsw);
+  ARLMI_LAUNCH_CHECK();
   cplx<R>* partial = nullptr;
   int rc = partial_buffer<R>(p, &partial);
   if (rc) return rc;

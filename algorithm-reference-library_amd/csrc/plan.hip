@@ -967,6 +967,15 @@ int arlmi_plan_update(arlmi_plan* p, const double* d_uvw, const double* d_freq, 
   return plan_build(p, d_uvw, d_freq, nrows, h_boxes, wslice, as_stream(stream));
 }
 
+int arlmi_plan_set_chunk(arlmi_plan* p, int32_t chunk) {
+  if (!p || chunk < 64 || chunk > (1 << 20)) {
+    set_error("chunk %d outside [64, 1048576]", chunk);
+    return ARLMI_E_ARG;
+  }
+  p->chunk = chunk;
+  return ARLMI_OK;
+}
+
 int arlmi_plan_stats_get(const arlmi_plan* p, arlmi_plan_stats* out) {
   if (!p || !out) {
     set_error("NULL argument");

@@ -57,6 +57,10 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=1000000)
+    ap.add_argument("--workload", choices=("wstack", "batch", "gradient"), default="wstack",
+                    help="wstack: the headline line (default); batch: 64 uneven observations replayed from one "
+                         "captured graph; gradient: value and gradient of ||V - g_p conj(g_q) predict(I)||^2")
+    ap.add_argument("--nobs", type=int, default=64)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
@@ -185,6 +189,211 @@ def cpu_baseline(args):
     return per_row, per_slab, m, (t4 - t0)
 
 
+def bench_batch(args):
+    """Many uneven observations: ARL-style sequential calls (plan per call) vs persistent plans
+    eager vs one captured graph with autotuned tile and chunk."""
+    import numpy
+    import torch
+    import arlmi
+    from arlmi.simulation import config3
+    t0 = time.perf_counter()
+    obs = config3(nobs=args.nobs, seed=3)
+    rng = numpy.random.default_rng(33)
+    tasks = []
+    for o in obs:
+        n = o["im"].data.shape[-1]
+        gcf, cf = arlmi.create_pswf_convolutionfunction(o["im"], oversampling=8, support=o["support"])
+        model = numpy.zeros((1, 1, n, n))
+        model[0, 0, rng.integers(n // 4, 3 * n // 4, 20), rng.integers(n // 4, 3 * n // 4, 20)] = 1.0
+        tasks.append({"vis": o["vis"], "im": o["im"], "gcfcf": (gcf, cf), "mode": "wstack",
+                      "vis_slices": o["vis_slices"], "model": torch.as_tensor(model).to("cuda", torch.float32)})
+    rows = sum(t["vis"].nvis for t in tasks)
+    print("batch: %d observations, %d rows, generated in %.1f s" % (len(tasks), rows, time.perf_counter() - t0),
+          file=sys.stderr, flush=True)
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - a) / reps
+
+    def sequential():
+        for t in tasks:
+            arlmi.invert_list_serial_workflow([t["vis"]], [t["im"]], context="wstack", vis_slices=t["vis_slices"],
+                                              gcfcf=[t["gcfcf"]])
+            mim = arlmi.Image(t["model"], t["im"].wcs)
+            arlmi.predict_list_serial_workflow([t["vis"]], [mim], context="wstack", vis_slices=t["vis_slices"],
+                                               gcfcf=[t["gcfcf"]])
+
+    t_seq = timed(sequential, 1)
+    print("batch: sequential calls %.1f ms" % (t_seq * 1e3), file=sys.stderr, flush=True)
+    eager = arlmi.BatchImager(tasks, graph=False)
+    t_eager = timed(eager.run, max(1, args.steps // 2))
+    del eager
+    torch.cuda.empty_cache()
+    print("batch: persistent plans, eager %.1f ms" % (t_eager * 1e3), file=sys.stderr, flush=True)
+    tuner = arlmi.Autotuner(reps=2)
+    t1 = time.perf_counter()
+    batch = arlmi.BatchImager(tasks, autotuner=tuner, graph=True)
+    t_tune = time.perf_counter() - t1
+    for _ in range(args.warmup):
+        batch.run()
+    torch.cuda.synchronize()
+    a = time.perf_counter()
+    for _ in range(args.steps):
+        batch.run()
+    torch.cuda.synchronize()
+    t_graph = (time.perf_counter() - a) / args.steps
+    print("batch: graph %.1f ms (tuning and capture %.1f s)" % (t_graph * 1e3, t_tune), file=sys.stderr, flush=True)
+    choices = {}
+    for c in batch.choices:
+        choices["%d/%d" % c] = choices.get("%d/%d" % c, 0) + 1
+    out = {
+        "metric": "Mvis/s invert+predict, 64 batched observations, one captured graph",
+        "value": round(rows / t_graph / 1e6, 1),
+        "unit": "Mvis/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_graph * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "%d observations: 32-512 antennas, 1-16 channels, 10-240 integrations (10k-4M rows each), "
+                        "N in {256..2048}, PSWF S in {6,8,10}, 8-64 w slabs, complex64" % len(tasks),
+            "rows": rows,
+            "sequential_calls_mvis_s": round(rows / t_seq / 1e6, 1),
+            "persistent_plans_eager_mvis_s": round(rows / t_eager / 1e6, 1),
+            "graph_over_sequential_calls": round(t_seq / t_graph, 2),
+            "graph_over_eager": round(t_eager / t_graph, 3),
+            "autotune_and_capture_s": round(t_tune, 1),
+            "tile_chunk_choices": choices,
+            "per_observation": [[t["im"].data.shape[-1], t["gcfcf"][1].support, t["vis_slices"], t["vis"].nvis]
+                                for t in tasks],
+        },
+    }
+    print(json.dumps(out))
+
+
+def bench_gradient(args):
+    """Value and gradient (image and gains) of ||V - g_p conj(g_q) predict(I)||^2 on the
+    differentiation workload, float32, with central finite differences of the float64 form at
+    200 random coordinates (step 1e-3) and the float32 gradients against the float64 ones."""
+    import numpy
+    import torch
+    import arlmi
+    from arlmi.simulation import config4
+    P = 16
+
+    def setup(precision):
+        c = config4(precision=precision)
+        vis, im = c["vis"], c["im"]
+        gcf, cf = arlmi.create_pswf_convolutionfunction(im, oversampling=8, support=8)
+        imager = arlmi.Imager(vis, im, gcfcf=(gcf, cf), mode="wstack", vis_slices=P)
+        i1, i2 = arlmi.gain_row_indices(vis, c["gaintable"])
+        dt = torch.float32 if precision == 32 else torch.float64
+        truth = torch.as_tensor(c["image"]).to("cuda", dt)
+        gtrue = c["gaintable"].gain
+        with torch.no_grad():
+            obs = arlmi.predict_with_gains(truth, gtrue, imager, vis, i1, i2)
+        noise = torch.as_tensor(numpy.random.default_rng(44).normal(0, 1e-3, (vis.nvis, 2))).to("cuda")
+        vis.data["vis"] = (obs + torch.complex(noise[:, 0], noise[:, 1]).to(obs.dtype)[:, None]).contiguous()
+        # the point the gradient is taken at: a perturbed truth
+        rng = numpy.random.default_rng(45)
+        img0 = (truth * 0.8).contiguous()
+        g0 = (gtrue * torch.as_tensor(numpy.exp(1j * rng.normal(0, 0.1, tuple(gtrue.shape)))).to(gtrue.device,
+                                                                                              gtrue.dtype)).contiguous()
+        return vis, imager, i1, i2, img0, g0
+
+    def grad(vis, imager, i1, i2, img0, g0):
+        img = img0.clone().requires_grad_(True)
+        g = g0.clone().requires_grad_(True)
+        loss = arlmi.visibility_loss(img, g, imager, vis, i1, i2)
+        loss.backward()
+        return loss.detach(), img.grad, g.grad
+
+    vis, imager, i1, i2, img0, g0 = setup(32)
+    rows = vis.nvis
+    for _ in range(args.warmup):
+        grad(vis, imager, i1, i2, img0, g0)
+    torch.cuda.synchronize()
+    a = time.perf_counter()
+    for _ in range(args.steps):
+        loss32, gi32, gg32 = grad(vis, imager, i1, i2, img0, g0)
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - a) / args.steps
+    gi32, gg32 = gi32.double().cpu(), gg32.to(torch.complex128).cpu()
+    del vis, imager
+    torch.cuda.empty_cache()
+    # float64: finite differences
+    vis, imager, i1, i2, img0, g0 = setup(64)
+    loss64, gi, gg = grad(vis, imager, i1, i2, img0, g0)
+    rng = numpy.random.default_rng(46)
+    n = img0.shape[-1]
+
+    def L(x, g):
+        with torch.no_grad():
+            return float(arlmi.visibility_loss(x, g, imager, vis, i1, i2))
+
+    h = 1e-3
+    errs_i, errs_g = [], []
+    si = float(gi.abs().max())
+    sg = float(gg.abs().max())
+    for _ in range(100):
+        y, x = rng.integers(n // 4, 3 * n // 4, 2)
+        xp, xm = img0.clone(), img0.clone()
+        xp[0, 0, y, x] += h
+        xm[0, 0, y, x] -= h
+        fd = (L(xp, g0) - L(xm, g0)) / (2 * h)
+        errs_i.append(abs(fd - float(gi[0, 0, y, x])) / si)
+    flat = g0.reshape(-1)
+    gflat = gg.reshape(-1)
+    for _ in range(50):
+        k = int(rng.integers(0, flat.numel()))
+        for unit in (1.0, 1j):
+            gp, gm = flat.clone(), flat.clone()
+            gp[k] += h * unit
+            gm[k] -= h * unit
+            fd = (L(img0, gp.reshape(g0.shape)) - L(img0, gm.reshape(g0.shape))) / (2 * h)
+            an = float(gflat[k].real if unit == 1.0 else gflat[k].imag)
+            errs_g.append(abs(fd - an) / sg)
+    gi64, gg64 = gi.cpu(), gg.cpu()
+    out = {
+        "metric": "Mvis/s value+gradient of ||V - g_p conj(g_q) predict(I)||^2 (image and gains)",
+        "value": round(rows / t / 1e6, 1),
+        "unit": "Mvis/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "128 antennas in a 6 km disk, 4 channels 120-140 MHz, 60 integrations (BlockVisibility "
+                        "converted on the GPU), 1024^2 image of 50 Gaussian blobs, %d w slabs, PSWF S=8 os=8, "
+                        "gains amplitude 1+-0.1, phase U(-pi/4, pi/4)" % P,
+            "rows": rows,
+            "fd_step": h,
+            "fd_points": len(errs_i) + len(errs_g),
+            "fd_max_err_image_rel_to_peak_fp64": max(errs_i),
+            "fd_max_err_gain_rel_to_peak_fp64": max(errs_g),
+            "fp32_vs_fp64_grad_image_l2": float(torch.linalg.norm(gi32 - gi64) / torch.linalg.norm(gi64)),
+            "fp32_vs_fp64_grad_gain_l2": float(torch.linalg.norm(gg32 - gg64) / torch.linalg.norm(gg64)),
+            "loss_fp32": float(loss32), "loss_fp64": float(loss64),
+        },
+    }
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.copy_probe:
@@ -194,6 +403,16 @@ def main():
         print("bench.py measures one GPU only: every box in this setup has a single MI355X, and the "
               "multi-GPU path (one all-reduce of partial images) is described, not run.", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batch":
+        import torch
+        torch.cuda.set_device(0)
+        bench_batch(args)
+        return
+    if args.workload == "gradient":
+        import torch
+        torch.cuda.set_device(0)
+        bench_gradient(args)
+        return
     import numpy
     import torch
     dev = torch.device("cuda", 0)

@@ -15,6 +15,10 @@
  *                64 -> complex128 / float64 throughout.  uvw, frequencies, w-slab
  *     centres, grid-correction images and sums of weights are always float64.
  *   - Complex numbers are interleaved (re, im) pairs, as numpy/torch store them.
+ *   - Graph capture: once a plan has served one call in a precision, arlmi_grid, arlmi_degrid,
+ *     arlmi_grids_to_image and arlmi_image_to_grids on it allocate nothing, record no memset
+ *     and never synchronise, so a sequence of them can be captured into a HIP graph and
+ *     replayed (plan creation and update synchronise and cannot be captured).
  *
  * The ARL interface each entry point replaces is cited per function
  * (file:line in SKA-ScienceDataProcessor/algorithm-reference-library).  The
@@ -122,6 +126,9 @@ int arlmi_plan_create(const arlmi_geom* g, const double* d_uvw, const double* d_
 /* Rebuild an existing plan for new uvw (same geometry), reusing its buffers when they fit. */
 int arlmi_plan_update(arlmi_plan* plan, const double* d_uvw, const double* d_freq, int64_t nrows,
                       const double* h_boxes, double wslice, void* stream);
+/* Work partition (no counterpart in ARL): gridding / degridding work items hold at most `chunk`
+ * entries of one tile (default 4096).  Takes effect at the next arlmi_plan_update. */
+int arlmi_plan_set_chunk(arlmi_plan* plan, int32_t chunk);
 int arlmi_plan_stats_get(const arlmi_plan* plan, arlmi_plan_stats* out);
 /* Host copy of the per-plane mean w (numpy.average(vis.w) per slab; 0 in MODE_2D). */
 int arlmi_plan_plane_w(const arlmi_plan* plan, double* h_out, int32_t n);
