@@ -25,5 +25,7 @@ from .calibration import (create_gaintable_from_blockvisibility, apply_gaintable
                           apply_gains_rows_adjoint)
 from .differentiable import PredictFunction, GainsFunction, predict_with_gains, visibility_loss
 from .batch import Autotuner, BatchImager
+from .weighting import (grid_weight_to_griddata, griddata_merge_weights, griddata_reweight, weight_visibility,
+                        weight_list_serial_workflow)
 
 __version__ = "0.1.0"

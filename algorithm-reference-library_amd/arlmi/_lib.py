@@ -35,7 +35,7 @@ EXPORTS = (
     "arlmi_plan_set_chunk", "arlmi_plan_stats_get", "arlmi_plan_plane_w", "arlmi_plan_destroy", "arlmi_convolution_mapping",
     "arlmi_grid", "arlmi_degrid",
     "arlmi_grids_to_image", "arlmi_image_to_grids", "arlmi_fft2d", "arlmi_w_screen", "arlmi_dft",
-    "arlmi_block_to_rows", "arlmi_apply_gaintable", "arlmi_apply_gains_rows", "arlmi_apply_gains_rows_adjoint",
+    "arlmi_grid_weights", "arlmi_reweight", "arlmi_block_to_rows", "arlmi_apply_gaintable", "arlmi_apply_gains_rows", "arlmi_apply_gains_rows_adjoint",
     "arlmi_profile_enable", "arlmi_profile_collect", "arlmi_profile_reset", "arlmi_profile_report",
 )
 
@@ -132,6 +132,8 @@ def lib():
         "arlmi_fft2d": (ctypes.c_int, [vp, i32, i64, ctypes.c_int, ctypes.c_int, vp]),
         "arlmi_w_screen": (ctypes.c_int, [i32, dbl, dbl, vp, ctypes.c_int, vp]),
         "arlmi_dft": (ctypes.c_int, [vp, vp, i64, vp, i32, i32, vp, ctypes.c_int, vp]),
+        "arlmi_grid_weights": (ctypes.c_int, [ctypes.POINTER(Geom), vp, vp, i64, vp, vp, vp, ctypes.c_int, vp]),
+        "arlmi_reweight": (ctypes.c_int, [ctypes.POINTER(Geom), vp, vp, i64, vp, vp, ctypes.c_int, vp]),
         "arlmi_block_to_rows": (ctypes.c_int, [ctypes.POINTER(Blocks), ctypes.POINTER(Rows), vp, ctypes.c_int, vp]),
         "arlmi_apply_gaintable": (ctypes.c_int, [vp, i32, i32, i32, vp, vp, vp, i32, ctypes.c_int, ctypes.c_int, vp]),
         "arlmi_apply_gains_rows": (ctypes.c_int, [i64, vp, vp, vp, i64, vp, vp, ctypes.c_int, vp]),

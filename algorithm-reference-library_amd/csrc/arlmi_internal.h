@@ -149,6 +149,9 @@ __device__ inline int map_wplane(double w, double step, double crpix, int nmax, 
 
 int first_error_code(uint32_t bits);
 
+// Validates a caller's geometry and converts it to the device form.
+int checked_geom(const arlmi_geom* in, DevGeom* g);
+
 // Exclusive scan of n uint32 values (out has n+1 entries; out[n] = total).  tmp must hold
 // scan_tmp_words(n) words.
 size_t scan_tmp_words(int64_t n);

@@ -587,7 +587,7 @@ using namespace arlmi;
 // host side
 // ---------------------------------------------------------------------------
 
-static int checked_geom(const arlmi_geom* in, DevGeom* g) {
+int arlmi::checked_geom(const arlmi_geom* in, DevGeom* g) {
   if (!in) {
     set_error("geometry is NULL");
     return ARLMI_E_ARG;

@@ -435,7 +435,25 @@ def main():
         roof = {"bound": "hbm", "kernel": name, "achieved": round(ach, 1), "peak": 8000.0, "unit": "GB/s",
                 "frac": round(ach / 8000.0, 4), "traffic": measured_traffic(name, N, P),
                 "bytes_per_launch": by, "avg_launch_ms": round(per_launch_ms, 4)}
-    extra = {}
+    # Whole-step roofline: sum over phases of max(HBM, LDS, FLOP) time with the algorithmic
+    # counts of the row and plane work (float32): gridding reads 36 B/row and does 16 S^2 B of
+    # LDS read-modify-write per row; degridding reads 24 B/row, writes 8 B/row and reads 8 S^2 B
+    # of LDS per row; the plane work (grid write-back, two-pass FFTs, screens, gcf, model) moves
+    # 108 N^2 B per plane and runs 2 x 5 N^2 log2(N^2) FFT flops per plane; taps cost 4 S^2 flops
+    # per row and direction.
+    S = 8
+    bw_hbm, bw_lds, fl = 8.0e12, 78.6e12, 157.3e12
+    nm = r32["stats"]["nmember"]
+    t_grid = max(36.0 * nm / bw_hbm, 16.0 * S * S * nm / bw_lds, 4.0 * S * S * nm / fl)
+    t_degrid = max(32.0 * nm / bw_hbm, 8.0 * S * S * nm / bw_lds, 4.0 * S * S * nm / fl)
+    t_planes = max(108.0 * N * N * P / bw_hbm, 2 * 5.0 * N * N * numpy.log2(N * N) * P / fl)
+    t_plan = 24.0 * rows / bw_hbm  # the mapping reads uvw once
+    t_roof = t_grid + t_degrid + t_planes + t_plan
+    step_roof = {"t_roof_ms": round(t_roof * 1e3, 3), "frac": round(t_roof / (r32["ms"] * 1e-3), 3),
+                 "grid_ms": round(t_grid * 1e3, 3), "degrid_ms": round(t_degrid * 1e3, 3),
+                 "planes_ms": round(t_planes * 1e3, 3), "mapping_ms": round(t_plan * 1e3, 3),
+                 "peaks": "HBM 8 TB/s, LDS 78.6 TB/s (128 B/clk/CU), FP32 157.3 TFLOP/s"}
+    extra = {"step_roofline": step_roof}
     if r64 is not None:
         ref_img = r64["img"].numpy()
         ref_vis = r64["vis"].numpy()

@@ -191,6 +191,25 @@ int arlmi_w_screen(int32_t npixel, double cellsize, double w, void* d_out, int p
 int arlmi_dft(const double* d_uvw, const int32_t* d_chan, int64_t nrows, const double* d_lmflux, int32_t ncomp,
               int32_t nchan, void* d_vis, int precision, void* stream);
 
+/* grid_weight_to_griddata (processing_components/griddata/gridding.py:161-180): every row's
+ * imaging weight is added at its nearest grid pixel, d_density[chan][plane][pv][pu] (float64, N x N
+ * per channel and grid w plane, zeroed by the call), and to d_sumwt[chan] (float64, zeroed).
+ * Rows are mapped as convolution_mapping maps them (gridding.py:40-85; MODE_2D geometry, planes =
+ * the grid's w planes), with its asserts: 0 <= pu, pv < N (the kernel footprint is not checked,
+ * since weights are not convolved), grid and kernel w planes below nz, channel in range; a row
+ * that fails them returns the matching ARLMI_E_* code.  Synchronises `stream`.  The float64 sums
+ * are reproducible to rounding (the order of the adds varies). */
+int arlmi_grid_weights(const arlmi_geom* g, const double* d_uvw, const double* d_freq, int64_t nrows,
+                       const void* d_weight, double* d_density, double* d_sumwt, int precision, void* stream);
+
+/* griddata_reweight (gridding.py:209-230), the uniform weighting of weight_visibility
+ * (processing_components/imaging/weighting.py:19-40): d_weight[row] /= d_density at the row's pixel
+ * where that density is non-zero (in place, in float64, rounded to the call's precision).  The
+ * mapping is checked for every row before any weight changes, as ARL asserts first.
+ * Synchronises `stream`. */
+int arlmi_reweight(const arlmi_geom* g, const double* d_uvw, const double* d_freq, int64_t nrows,
+                   const double* d_density, void* d_weight, int precision, void* stream);
+
 /* Columns of a BlockVisibility (data_models/memory_data_models.py:940-1094), npol = 1.
  * Complex and real columns are in the call's precision. */
 typedef struct arlmi_blocks {

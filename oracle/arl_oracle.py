@@ -46,6 +46,12 @@ Pinning (what checks each function; see tests/test_oracle_*.py):
                                            1e-12, zero gains + inverse = no change) and block/row
                                            agreement through convert_blocks
   apply_gains_rows_adjoint                 central finite differences and the dot-product identity
+  grid_weight_to_griddata, griddata_reweight, weight_visibility
+                                           a direct per-pixel count on integer-valued weights, and
+                                           the uniform-weighting identity (reweighted weights grid
+                                           to exactly 1 on every occupied pixel); ARL's own test of
+                                           it (test_griddata_gridding.py:283-295) needs LOWBD2 data
+                                           and is not reproducible here
 """
 
 import numpy
@@ -63,6 +69,7 @@ __all__ = [
     "simulate_point", "predict_skycomponent_visibility", "xyz_at_latitude", "xyz_to_uvw",
     "hadec_to_azel", "create_visibility", "apply_gains_rows", "create_blockvisibility", "convert_blocks",
     "convert_blocks_loop", "decoalesce_visibility", "apply_gaintable", "apply_gains_rows_adjoint",
+    "grid_weight_to_griddata", "griddata_merge_weights", "griddata_reweight", "weight_visibility",
 ]
 
 D2R = numpy.pi / 180.0
@@ -980,3 +987,50 @@ def apply_gains_rows_adjoint(vis_rows, grad_out, idx1, idx2, gain_flat):
     numpy.add.at(grad_gain, idx2, G * g1 * numpy.conjugate(v))
     numpy.add.at(grad_gain, idx1, numpy.conjugate(G) * g2 * v)
     return grad_vis, grad_gain
+
+
+# ---------------------------------------------------------------------------
+# Weighting: processing_components/griddata/gridding.py:161-230,
+# processing_components/imaging/weighting.py:19-40
+# ---------------------------------------------------------------------------
+
+def grid_weight_to_griddata(vis, geom, cfg, nw_grid=1):
+    """Imaging weights added at the nearest grid pixel, row by row (gridding.py:161-180).
+    Returns (grid [nchan, npol, nw, N, N] complex128, sumwt [nchan, npol])."""
+    npol = vis['imaging_weight'].shape[1]
+    pu, _, pv, _, pwg, _, pfreq = convolution_mapping(vis['uvw'], vis['frequency'], geom, cfg, nw_grid=nw_grid)
+    grid = numpy.zeros([geom.nchan, npol, nw_grid, geom.npixel, geom.npixel], dtype='complex')
+    sumwt = numpy.zeros([geom.nchan, npol])
+    for vwt, chan, xx, yy, zzg in zip(vis['imaging_weight'], pfreq, pu, pv, pwg):
+        grid[chan, :, zzg, yy, xx] += vwt
+        sumwt[chan, :] += vwt
+    return grid, sumwt
+
+
+def griddata_merge_weights(gd_list):
+    """Sum of the (grid, sumwt) pairs (gridding.py:183-206)."""
+    grid = numpy.array(gd_list[len(gd_list) // 2][0], copy=True)
+    sumwt = numpy.array(gd_list[len(gd_list) // 2][1], copy=True)
+    for i, g in enumerate(gd_list):
+        if i != len(gd_list) // 2:
+            grid += g[0]
+            sumwt += g[1]
+    return grid, sumwt
+
+
+def griddata_reweight(vis, grid, geom, cfg, nw_grid=1):
+    """imaging_weight /= Re(grid) at the row's pixel where that is non-zero (gridding.py:209-230);
+    returns the new imaging_weight array.  (ARL's test `numpy.real(g).all() > 0.0` compares a
+    bool with 0, so it is true exactly when every polarisation's density is non-zero.)"""
+    pu, _, pv, _, pwg, _, pfreq = convolution_mapping(vis['uvw'], vis['frequency'], geom, cfg, nw_grid=nw_grid)
+    wt = numpy.array(vis['imaging_weight'], dtype=float, copy=True)
+    for vwt, chan, xx, yy, zzg in zip(wt, pfreq, pu, pv, pwg):
+        if numpy.real(grid[chan, :, zzg, yy, xx]).all() > 0.0:
+            vwt /= numpy.real(grid[chan, :, zzg, yy, xx])
+    return wt
+
+
+def weight_visibility(vis, geom, cfg):
+    """Uniform weighting (imaging/weighting.py:19-40): returns the new imaging_weight."""
+    grid, _ = grid_weight_to_griddata(vis, geom, cfg)
+    return griddata_reweight(vis, grid, geom, cfg)

@@ -161,3 +161,59 @@ def test_gain_adjoint_finite_differences():
     ax = x * g[i2] * numpy.conjugate(g[i1])
     aty, _ = O.apply_gains_rows_adjoint(v, y, i1, i2, g)
     assert abs(numpy.vdot(ax, y).real - numpy.vdot(x, aty).real) < 1e-10 * n
+
+
+# --- weighting (gridding.py:161-230) --------------------------------------------------------
+
+def _weight_rows(seed, n=64, nvis=3000, cell=0.01):
+    rng = numpy.random.default_rng(seed)
+    du = 1.0 / (n * cell)
+    lim = (n // 2 - 1) * du
+    uvw = numpy.zeros((nvis, 3))
+    # few distinct pixels so that many rows share one
+    uvw[:, 0] = rng.integers(-8, 8, nvis) * du + rng.uniform(-0.3, 0.3, nvis) * du
+    uvw[:, 1] = rng.integers(-8, 8, nvis) * du + rng.uniform(-0.3, 0.3, nvis) * du
+    return {'uvw': uvw, 'frequency': numpy.full(nvis, 1e8),
+            'imaging_weight': rng.integers(1, 5, (nvis, 1)).astype(float),
+            'vis': numpy.zeros((nvis, 1), dtype=complex)}
+
+
+def test_grid_weight_counts_per_pixel():
+    """Integer weights: the density equals a direct count per nearest pixel, sumwt the total."""
+    d = _weight_rows(1)
+    g = O.ImageGeom(64, 0.01)
+    _, _, cfg = O.create_pswf_convolutionfunction(g, 8, 8)
+    grid, sumwt = O.grid_weight_to_griddata(d, g, cfg)
+    x = numpy.round(64 // 2 + d["uvw"][:, 0] / g.du()).astype(int)
+    y = numpy.round(64 // 2 + d["uvw"][:, 1] / g.dv()).astype(int)
+    direct = numpy.zeros((64, 64))
+    numpy.add.at(direct, (y, x), d['imaging_weight'][:, 0])
+    assert numpy.array_equal(grid[0, 0, 0].real, direct)
+    assert numpy.all(grid.imag == 0)
+    assert sumwt[0, 0] == d['imaging_weight'].sum()
+
+
+def test_uniform_weighting_identity():
+    """After uniform reweighting the weights of every occupied pixel sum to 1, and weights are
+    only ever divided by their own pixel's density."""
+    d = _weight_rows(2)
+    g = O.ImageGeom(64, 0.01)
+    _, _, cfg = O.create_pswf_convolutionfunction(g, 8, 8)
+    new = O.weight_visibility(d, g, cfg)
+    d2 = dict(d)
+    d2['imaging_weight'] = new
+    grid2, _ = O.grid_weight_to_griddata(d2, g, cfg)
+    occ = grid2[0, 0, 0].real != 0
+    assert numpy.allclose(grid2[0, 0, 0].real[occ], 1.0, rtol=0, atol=1e-14)
+    assert occ.sum() <= 256
+    # zero weights leave a pixel empty and are not divided (no division by zero)
+    d3 = dict(d)
+    w = d['imaging_weight'].copy()
+    w[:] = 0.0
+    d3['imaging_weight'] = w
+    grid3, s3 = O.grid_weight_to_griddata(d3, g, cfg)
+    assert numpy.all(O.griddata_reweight(d3, grid3, g, cfg) == 0.0) and s3[0, 0] == 0.0
+    # merging two lists doubles the density
+    gm, sm = O.griddata_merge_weights([O.grid_weight_to_griddata(d, g, cfg)] * 2)
+    grid, sumwt = O.grid_weight_to_griddata(d, g, cfg)
+    assert numpy.array_equal(gm, 2 * grid) and numpy.array_equal(sm, 2 * sumwt)
