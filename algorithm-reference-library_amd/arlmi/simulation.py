@@ -16,7 +16,7 @@ import torch
 from .data_models import Visibility, Skycomponent, C_LIGHT, device
 from .kernels import create_image
 
-__all__ = ["site_xyz", "uvw_rows", "make_observation", "config1", "config2", "config3", "config4",
+__all__ = ["site_xyz", "uvw_rows", "make_observation", "config1", "config2", "config3", "config4", "config5",
            "power_law_sources", "uniform_disk", "log_radial", "gaussian_blobs", "block_uvw", "LATITUDE"]
 
 LATITUDE = -26.82 * numpy.pi / 180.0  # site latitude used by every workload
@@ -282,3 +282,33 @@ def config4(seed=4, precision=32, npixel=1024, dev=None):
     g = (amp * numpy.exp(1j * phase))[None, :, :, None, None].repeat(gt.ntimes, axis=0)
     gt.gain = torch.as_tensor(g).to(device=dev, dtype=cdt)
     return {"bvis": bvis, "vis": vis, "im": im, "image": image, "gaintable": gt}
+
+
+def core_spiral(rng, ncore=133, narm=64, core_radius=1000.0, rmax=80000.0, arms=3):
+    """Dense core (uniform disk) plus log-spiral arms out to rmax (metres, local ENU)."""
+    core = uniform_disk(rng, ncore, core_radius)
+    k = numpy.arange(narm)
+    arm = k % arms
+    r = core_radius * (rmax / core_radius) ** ((k // arms + 1) / math.ceil(narm / arms))
+    phi = 2 * numpy.pi * arm / arms + 0.8 * numpy.log(r / core_radius) + rng.normal(0, 0.02, narm)
+    spiral = numpy.stack([r * numpy.cos(phi), r * numpy.sin(phi), numpy.zeros(narm)], axis=1)
+    return numpy.concatenate([core, spiral])
+
+
+def config5(precision=32, seed=5, integrations=13, first=0, npixel=8192, cellsize=3e-6, support=8, dev=None):
+    """Streaming wide band: 197 dishes (133 in a 1 km core, 64 on three log-spiral arms to 80 km),
+    64 channels 0.95-1.76 GHz, 480 integrations over HA -4h..+4h at dec -30 deg; one chunk of
+    `integrations` consecutive integrations starting at `first` (13 integrations = 16.06 M rows
+    before the on-grid selection), 8192^2 image at 3e-6 rad.  Rows whose kernel footprint would
+    leave the grid are not generated (vis.total_rows vs vis.nvis).  Visibilities are zero (the
+    caller fills them)."""
+    rng = numpy.random.default_rng(seed)
+    enu = core_spiral(rng)
+    ha_all = numpy.linspace(-numpy.pi / 3, numpy.pi / 3, 480)
+    ha = ha_all[first:first + integrations]
+    freq = numpy.linspace(0.95e9, 1.76e9, 64)
+    vis = make_observation(enu, ha, freq, -30.0 * numpy.pi / 180.0, precision=precision, npixel=npixel,
+                           cellsize=cellsize, support=support, select=True, dev=dev)
+    dtype = torch.float32 if precision == 32 else torch.float64
+    im = create_image(npixel=npixel, cellsize=cellsize, frequency=[freq[0]], channel_bandwidth=[1e10], dtype=dtype)
+    return vis, im
