@@ -96,6 +96,14 @@ __device__ inline double world2pix0(double world, double cdelt, double crpix) {
   return (world / cdelt + crpix) - 1.0;
 }
 
+// round((x / step + crpix) - 1) for an integral crpix.  When |x| <= |step| / 4 the quotient lies
+// in [-1/4, 1/4] (division is monotone and 1/4 is exact), so the rounded pixel is crpix - 1
+// without dividing; otherwise the division is done as wcslib does it.  Identical results.
+__device__ inline double round_pix0(double x, double step, double crpix) {
+  if (fabs(x) <= 0.25 * fabs(step) && step != 0.0 && crpix == rint(crpix)) return crpix - 1.0;
+  return rint(world2pix0(x, step, crpix));
+}
+
 __device__ inline RowMap map_row(const DevGeom& g, double u, double v, double f, bool have_freq) {
   RowMap m;
   m.err = 0;
@@ -126,7 +134,7 @@ __device__ inline RowMap map_row(const DevGeom& g, double u, double v, double f,
   m.yoff = yo;
   int c = 0;
   if (have_freq) {
-    const double pf = rint(((f - g.freq0) / g.dfreq + 1.0) - 1.0);
+    const double pf = round_pix0(f - g.freq0, g.dfreq, 1.0);
     if (!(pf >= 0.0 && pf < (double)g.nchan)) {
       m.err |= ERRB_FREQ_MAP;
     } else {
@@ -139,7 +147,7 @@ __device__ inline RowMap map_row(const DevGeom& g, double u, double v, double f,
 
 // Kernel w plane of a (shifted) w value: round((w/zstep + crpix_z) - 1) (gridding.py:70-76).
 __device__ inline int map_wplane(double w, double step, double crpix, int nmax, uint32_t* err) {
-  const double p = rint(world2pix0(w, step, crpix));
+  const double p = round_pix0(w, step, crpix);
   if (!(p >= 0.0 && p < (double)nmax)) {
     *err |= ERRB_W_RANGE;
     return 0;

@@ -92,13 +92,16 @@ struct Members {
   int k[3];
 };
 
-__device__ inline Members slab_members(double w, const double* boxes, int nb, double wslice) {
+// inv_wslice = 1 / wslice only places the candidate window (a rounding difference moves the
+// nearest slab by at most one, and the window covers its neighbours); membership itself is the
+// exact strict test.
+__device__ inline Members slab_members(double w, const double* boxes, int nb, double wslice, double inv_wslice) {
   Members m;
   m.n = 0;
   m.k[0] = m.k[1] = m.k[2] = 0;
   const double halfw = 0.5 * wslice;
   if (!(wslice > 0.0)) return m;  // all |w| zero: |w - box| < 0 never holds
-  const double kd = fmin(fmax(rint((w - boxes[0]) / wslice), -2.0), (double)nb + 1.0);
+  const double kd = fmin(fmax(rint((w - boxes[0]) * inv_wslice), -2.0), (double)nb + 1.0);
   const int k0 = (int)kd;
   // the three candidate slabs around the nearest one, compacted with selects (no indexed
   // writes, so the result stays in registers)
@@ -120,13 +123,16 @@ __global__ void k_slab_sums(const double* __restrict__ uvw, int64_t n, const dou
                             double wslice, double* wsum, uint32_t* wcnt) {
   extern __shared__ char smem[];
   double* ls = reinterpret_cast<double*>(smem);
-  unsigned int* lc = reinterpret_cast<unsigned int*>(ls + nb);
+  double* lbox = ls + nb;
+  unsigned int* lc = reinterpret_cast<unsigned int*>(lbox + nb);
   for (int i = threadIdx.x; i < nb; i += blockDim.x) {
     ls[i] = 0.0;
+    lbox[i] = boxes[i];
     lc[i] = 0u;
   }
   __syncthreads();
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const double inv_wslice = 1.0 / wslice;
   for (int64_t i0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i0 < n; i0 += 4 * stride) {
     double wv[4];
 #pragma unroll
@@ -138,7 +144,7 @@ __global__ void k_slab_sums(const double* __restrict__ uvw, int64_t n, const dou
     for (int u = 0; u < 4; ++u) {
       if (i0 + u * stride >= n) break;
       const double w = wv[u];
-      Members m = slab_members(w, boxes, nb, wslice);
+      Members m = slab_members(w, lbox, nb, wslice, inv_wslice);
 #pragma unroll
       for (int j = 0; j < 3; ++j) {
         if (j < m.n) {
@@ -198,7 +204,7 @@ struct Enum {
 };
 
 __device__ inline Enum enumerate_row(const DevGeom& g, const double* __restrict__ uvw, const double* __restrict__ freq,
-                                     int64_t row, const double* __restrict__ boxes, double wslice,
+                                     int64_t row, const double* __restrict__ boxes, double wslice, double inv_wslice,
                                      const double* __restrict__ plane_w, uint32_t flags, uint32_t* err_out) {
   Enum e;
   e.valid = false;
@@ -210,7 +216,7 @@ __device__ inline Enum enumerate_row(const DevGeom& g, const double* __restrict_
   const double f = freq ? freq[row] : 0.0;
   uint32_t err = 0;
   if (g.mode == ARLMI_MODE_WSTACK) {
-    Members m = slab_members(w, boxes, g.nplanes, wslice);
+    Members m = slab_members(w, boxes, g.nplanes, wslice, inv_wslice);
     if (m.n == 0) return e;  // in no slab: never mapped, never asserted (imaging_serial.py:93-95)
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
@@ -276,26 +282,50 @@ struct Rec {
 __global__ void k_p1_count(DevGeom g, const double* __restrict__ uvw, const double* __restrict__ freq, int64_t n,
                            const double* __restrict__ boxes, double wslice, const double* __restrict__ plane_w,
                            uint32_t flags, uint32_t* __restrict__ h1, int nb1, uint32_t* misc) {
-  extern __shared__ uint32_t lh[];
+  // slab centres and means staged in LDS (looked up per row), then the counters
+  extern __shared__ double lsh[];
   const int P = g.nplanes;
+  double* lbox = lsh;
+  double* lpw = lsh + P;
+  uint32_t* lh = reinterpret_cast<uint32_t*>(lsh + 2 * P);
+  for (int i = threadIdx.x; i < P; i += blockDim.x) {
+    lbox[i] = boxes[i];
+    lpw[i] = plane_w[i];
+  }
   for (int i = threadIdx.x; i < P + 3; i += blockDim.x) lh[i] = 0u;
   __syncthreads();
   const int64_t r0 = (int64_t)blockIdx.x * K1_ROWS;
   const int64_t r1 = r0 + K1_ROWS < n ? r0 + K1_ROWS : n;
+  const double inv_wslice = 1.0 / wslice;
   uint32_t err = 0;
+  // per-thread tallies of rows in no plane, dropped rows and memberships, summed once per wave
+  // at the end (an LDS add of a varying value to one address per row would be expanded into a
+  // loop over the active lanes)
+  uint32_t n_orph = 0, n_drop = 0, n_mem = 0;
   for (int64_t row = r0 + threadIdx.x; row < r1; row += blockDim.x) {
-    Enum e = enumerate_row(g, uvw, freq, row, boxes, wslice, plane_w, flags, &err);
+    Enum e = enumerate_row(g, uvw, freq, row, lbox, wslice, inv_wslice, lpw, flags, &err);
     if (!e.valid) {
-      atomicAdd(&lh[P], 1u);
-      if (e.dropped) atomicAdd(&lh[P + 1], 1u);
+      ++n_orph;
+      n_drop += e.dropped ? 1u : 0u;
     } else {
 #pragma unroll
       for (int j = 0; j < 3; ++j)
         if (j < e.nm) atomicAdd(&lh[e.plane[j]], 1u);
-      atomicAdd(&lh[P + 2], (uint32_t)e.nm);
+      n_mem += (uint32_t)e.nm;
     }
   }
-  if (err) atomicOr(&misc[0], err);
+  for (int o = 32; o > 0; o >>= 1) {
+    n_orph += __shfl_xor(n_orph, o);
+    n_drop += __shfl_xor(n_drop, o);
+    n_mem += __shfl_xor(n_mem, o);
+    err |= __shfl_xor(err, o);
+  }
+  if (__lane_id() == 0) {
+    if (n_orph) atomicAdd(&lh[P], n_orph);
+    if (n_drop) atomicAdd(&lh[P + 1], n_drop);
+    if (n_mem) atomicAdd(&lh[P + 2], n_mem);
+    if (err) atomicOr(&misc[0], err);
+  }
   __syncthreads();
   for (int i = threadIdx.x; i <= P; i += blockDim.x) h1[(int64_t)i * nb1 + blockIdx.x] = lh[i];
   if (threadIdx.x == 0) {
@@ -307,15 +337,23 @@ __global__ void k_p1_count(DevGeom g, const double* __restrict__ uvw, const doub
 __global__ void k_p1_scatter(DevGeom g, const double* __restrict__ uvw, const double* __restrict__ freq, int64_t n,
                              const double* __restrict__ boxes, double wslice, const double* __restrict__ plane_w,
                              uint32_t flags, const uint32_t* __restrict__ s1, int nb1, Rec* __restrict__ recs) {
-  extern __shared__ uint32_t cur[];
+  extern __shared__ double lsh[];
   const int P = g.nplanes;
+  double* lbox = lsh;
+  double* lpw = lsh + P;
+  uint32_t* cur = reinterpret_cast<uint32_t*>(lsh + 2 * P);
+  for (int i = threadIdx.x; i < P; i += blockDim.x) {
+    lbox[i] = boxes[i];
+    lpw[i] = plane_w[i];
+  }
   for (int i = threadIdx.x; i <= P; i += blockDim.x) cur[i] = s1[(int64_t)i * nb1 + blockIdx.x];
   __syncthreads();
   const int64_t r0 = (int64_t)blockIdx.x * K1_ROWS;
   const int64_t r1 = r0 + K1_ROWS < n ? r0 + K1_ROWS : n;
+  const double inv_wslice = 1.0 / wslice;
   uint32_t dummy = 0;
   for (int64_t row = r0 + threadIdx.x; row < r1; row += blockDim.x) {
-    Enum e = enumerate_row(g, uvw, freq, row, boxes, wslice, plane_w, flags, &dummy);
+    Enum e = enumerate_row(g, uvw, freq, row, lbox, wslice, inv_wslice, lpw, flags, &dummy);
     if (!e.valid) {
       const uint32_t pos = atomicAdd(&cur[P], 1u);
       recs[pos] = Rec{(uint32_t)row, 0u, 0u, 0u};
@@ -613,6 +651,10 @@ int arlmi::checked_geom(const arlmi_geom* in, DevGeom* g) {
     set_error("nchan %d, nz %d, nplanes %d must be positive", in->nchan, in->nz, in->nplanes);
     return ARLMI_E_ARG;
   }
+  if (in->nplanes > 4096) {
+    set_error("nplanes %d: at most 4096 planes (the per-block slab tables live in LDS)", in->nplanes);
+    return ARLMI_E_UNSUPPORTED;
+  }
   if (in->mode != ARLMI_MODE_2D && in->mode != ARLMI_MODE_WSTACK) {
     set_error("unknown mode %d", in->mode);
     return ARLMI_E_ARG;
@@ -755,7 +797,7 @@ static int plan_build(arlmi_plan* p, const double* uvw, const double* freq, int6
     ARLMI_HIP(hipMemcpyAsync(p->d_boxes, h_boxes, sizeof(double) * P, hipMemcpyHostToDevice, s));
     const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((nrows + 1023) / 1024, SLAB_BLOCKS));
     if (nrows > 0) {
-      const size_t sh = (sizeof(double) + sizeof(unsigned int)) * P;
+      const size_t sh = (2 * sizeof(double) + sizeof(unsigned int)) * P;
       ARLMI_LAUNCH("k_slab_sums", k_slab_sums, dim3((unsigned)blocks), dim3(256), sh, s, uvw, nrows, p->d_boxes, P,
                    wslice, p->d_wsum, p->d_wcnt);
       ARLMI_LAUNCH_CHECK();
@@ -775,7 +817,8 @@ static int plan_build(arlmi_plan* p, const double* uvw, const double* freq, int6
   if ((rc = grow(&p->d_h1, &p->cap_h1, n1))) return rc;
   if ((rc = grow(&p->d_s1, &p->cap_s1, n1 + 1))) return rc;
   if ((rc = ensure_scan_tmp(p, n1))) return rc;
-  ARLMI_LAUNCH("k_p1_count", k_p1_count, dim3(nb1), dim3(256), sizeof(uint32_t) * (P + 3), s, g, uvw, freq, nrows,
+  ARLMI_LAUNCH("k_p1_count", k_p1_count, dim3(nb1), dim3(256), sizeof(double) * 2 * P + sizeof(uint32_t) * (P + 3), s,
+               g, uvw, freq, nrows,
                p->d_boxes, wslice, p->d_plane_w, p->flags, p->d_h1, nb1, p->d_misc);
   ARLMI_LAUNCH_CHECK();
   if ((rc = exclusive_scan_u32(p->d_h1, p->d_s1, n1, p->d_scan_tmp, s))) return rc;
@@ -802,7 +845,8 @@ static int plan_build(arlmi_plan* p, const double* uvw, const double* freq, int6
   p->orphan_start = bounds[P];
   p->norphans = bounds[P + 1] - bounds[P];
   if ((rc = grow_bytes(&p->d_recs, &p->cap_recs, (int64_t)sizeof(Rec) * std::max<int64_t>(1, nrec)))) return rc;
-  ARLMI_LAUNCH("k_p1_scatter", k_p1_scatter, dim3(nb1), dim3(256), sizeof(uint32_t) * (P + 1), s, g, uvw, freq,
+  ARLMI_LAUNCH("k_p1_scatter", k_p1_scatter, dim3(nb1), dim3(256), sizeof(double) * 2 * P + sizeof(uint32_t) * (P + 1),
+               s, g, uvw, freq,
                nrows, p->d_boxes, wslice, p->d_plane_w, p->flags, p->d_s1, nb1, (Rec*)p->d_recs);
   ARLMI_LAUNCH_CHECK();
 
