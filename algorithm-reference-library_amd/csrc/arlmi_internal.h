@@ -86,6 +86,7 @@ struct RowMap {
 // Geometry in the form the device code reads it.
 struct DevGeom {
   int n, nchan, support, os, nz, nplanes, mode, tile, ntx;
+  int tshift;  // log2(tile)
   double du, dv, ddu, ddv, crpix_uv, crpix_d, zstep, crpix_z, wstep_grid, crpix_wg, freq0, dfreq;
   double cellsize;
 };

@@ -339,6 +339,7 @@ struct FftArgs {
   const void* zeros;
   int zero_fill;
   int tile, ntx;
+  int tshift;  // log2(tile)
 };
 
 // Masked-out loads read zeros from a small buffer instead of the grid (lane l reads entry
@@ -420,7 +421,7 @@ __global__ __launch_bounds__(Shape<N>::FNT * Shape<N>::FNB) void k_rows_predict(
           return cmk<R>(mv[q][r] * s.x, -mv[q][r] * s.y);
         },
         [&](int, int, int x, cplx<R> v) {
-          if ((cm >> (x / a.tile)) & 1u) dst[x] = v;
+          if ((cm >> (x >> a.tshift)) & 1u) dst[x] = v;
         });
     __syncthreads();
   }
@@ -444,7 +445,7 @@ __global__ __launch_bounds__(Shape<N>::NT * Shape<N>::template col_nb<R>()) void
   const int x0 = (bid % CB) * NB;
   uint32_t tm = ~0u;
   if (a.tilemask) {  // sparse grids: skip a column group whose tile column holds no entries
-    tm = a.tilemask[(size_t)plane * a.ntx + x0 / a.tile];  // NB divides the tile side
+    tm = a.tilemask[(size_t)plane * a.ntx + (x0 >> a.tshift)];  // NB divides the tile side
     if (!tm) return;
     if (!a.zero_fill) tm = ~0u;
   }
@@ -455,7 +456,7 @@ __global__ __launch_bounds__(Shape<N>::NT * Shape<N>::template col_nb<R>()) void
       lds, b, t, tw,
       [&](int, int, int n) {
         // a tile never written (sparse grids) reads as 0
-        const cplx<R>* src = ((tm >> (n / a.tile)) & 1u) ? col + (size_t)n * N : zero_entry<R>(a.zeros);
+        const cplx<R>* src = ((tm >> (n >> a.tshift)) & 1u) ? col + (size_t)n * N : zero_entry<R>(a.zeros);
         cplx<R> v = *src;
         if (a.in_mod && ((n + x) & 1)) v = cmk<R>(-v.x, -v.y);
         return v;
@@ -516,7 +517,7 @@ __global__ __launch_bounds__(Shape<N>::NT * Shape<N>::ROW_NB) void k_rows_invert
         lds, b, t, tw,
         [&](int, int, int n) {
           // sparse grids: a skipped tile column reads as 0
-          const cplx<R>* src = ((cm >> (n / a.tile)) & 1u) ? row + n : zero_entry<R>(a.zeros);
+          const cplx<R>* src = ((cm >> (n >> a.tshift)) & 1u) ? row + n : zero_entry<R>(a.zeros);
           return *src;
         },
         [&](int q, int r, int x, cplx<R> v) {
@@ -879,6 +880,7 @@ int arlmi_grids_to_image(const arlmi_plan* p, void* d_grids, const double* d_sum
     a.zeros = p->d_zeros;
     a.zero_fill = 1;
     a.tile = p->g.tile;
+    a.tshift = __builtin_ctz((unsigned)p->g.tile);
     a.ntx = p->g.ntx;
   }
   // partial images of the plane groups live with the plan (grown on demand)
@@ -924,6 +926,7 @@ int arlmi_image_to_grids(const arlmi_plan* p, const void* d_model, const double*
     a.zeros = p->d_zeros;
     a.zero_fill = 0;
     a.tile = p->g.tile;
+    a.tshift = __builtin_ctz((unsigned)p->g.tile);
     a.ntx = p->g.ntx;
   }
   return precision == 32 ? dispatch_n<float>(Op::TO_GRIDS, p->g.n, a, 0, as_stream(stream))

@@ -393,10 +393,11 @@ __device__ inline void rec_tiles(const DevGeom& g, const Rec& r, int* tx0, int* 
   const int half = g.support / 2;
   *xs = pu - half;
   *ys = pv - half;
-  *tx0 = *xs / g.tile;
-  *ty0 = *ys / g.tile;
-  *ntx_t = (*xs + g.support - 1) / g.tile - *tx0 + 1;
-  *nty_t = (*ys + g.support - 1) / g.tile - *ty0 + 1;
+  // xs, ys >= 0 (the footprint lies on the grid), so shifts divide exactly
+  *tx0 = *xs >> g.tshift;
+  *ty0 = *ys >> g.tshift;
+  *ntx_t = ((*xs + g.support - 1) >> g.tshift) - *tx0 + 1;
+  *nty_t = ((*ys + g.support - 1) >> g.tshift) - *ty0 + 1;
 }
 
 __global__ void k_p2_count(DevGeom g, const Rec* __restrict__ recs, const Blk2* __restrict__ blks,
@@ -507,27 +508,55 @@ __global__ void k_colmask(const uint32_t* tilemask, int npc, int ntx, uint32_t* 
 
 // Work items: grid items {tile, start, count, slot+1 or 0}, degrid items {tile, start, count, 0};
 // tiles split over several grid items are listed with their first partial slot.
-__global__ void k_items(const uint32_t* cnt, const uint32_t* cntd, const uint32_t* off, const uint32_t* choff,
-                        const uint32_t* choffd, const uint32_t* slotoff, int ntiles, int chunk, int4* gitems,
-                        int4* ditems, int4* multi, uint32_t* misc) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= ntiles) return;
-  const uint32_t c = cnt[t];
-  const uint32_t ng = choff[t + 1] - choff[t];
-  for (uint32_t k = 0; k < ng; ++k) {
+// Largest t in [0, n) with off[t] <= i (off non-decreasing, off[0] = 0 <= i).
+__device__ inline int owner(const uint32_t* __restrict__ off, int n, uint32_t i) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= i) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// One thread per work item (a tile with many entries has many items, so a thread per tile
+// would serialise them): gridding item i belongs to the tile t with choff[t] <= i < choff[t+1],
+// likewise for the degridding items.
+__global__ void k_items(const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ cntd,
+                        const uint32_t* __restrict__ off, const uint32_t* __restrict__ choff,
+                        const uint32_t* __restrict__ choffd, const uint32_t* __restrict__ slotoff, int ntiles,
+                        int chunk, uint32_t ngi, uint32_t ndi, int4* gitems, int4* ditems, int4* multi,
+                        uint32_t* misc) {
+  __shared__ uint32_t lcount, lbase;
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (threadIdx.x == 0) lcount = 0u;
+  __syncthreads();
+  int t = 0;
+  uint32_t k = 0, ng = 0;
+  if (i < ngi) {
+    t = owner(choff, ntiles, i);
+    k = i - choff[t];
+    ng = choff[t + 1] - choff[t];
+  }
+  // split tiles take consecutive slots of `multi`: an LDS count per block and one device add
+  // per block
+  const bool first_split = i < ngi && k == 0 && ng > 1;
+  const uint32_t li = first_split ? atomicAdd(&lcount, 1u) : 0u;
+  __syncthreads();
+  if (threadIdx.x == 0) lbase = lcount ? atomicAdd(&misc[4], lcount) : 0u;
+  __syncthreads();
+  if (i < ngi) {
+    const uint32_t c = cnt[t];
     const uint32_t s = k * chunk;
     const uint32_t len = c > s ? min((uint32_t)chunk, c - s) : 0u;
-    gitems[choff[t] + k] = make_int4(t, (int)(off[t] + s), (int)len, ng > 1 ? (int)(slotoff[t] + k + 1) : 0);
+    gitems[i] = make_int4(t, (int)(off[t] + s), (int)len, ng > 1 ? (int)(slotoff[t] + k + 1) : 0);
+    if (first_split) multi[lbase + li] = make_int4(t, (int)slotoff[t], (int)ng, 0);
   }
-  if (ng > 1) {
-    const uint32_t i = atomicAdd(&misc[4], 1u);  // one per split tile
-    multi[i] = make_int4(t, (int)slotoff[t], (int)ng, 0);
-  }
-  const uint32_t cd = cntd[t];
-  const uint32_t nd = choffd[t + 1] - choffd[t];
-  for (uint32_t k = 0; k < nd; ++k) {
-    const uint32_t s = k * chunk;
-    ditems[choffd[t] + k] = make_int4(t, (int)(off[t] + s), (int)min((uint32_t)chunk, cd - s), 0);
+  if (i < ndi) {
+    const int td = owner(choffd, ntiles, i);
+    const uint32_t kd = i - choffd[td];
+    const uint32_t s = kd * chunk;
+    ditems[i] = make_int4(td, (int)(off[td] + s), (int)min((uint32_t)chunk, cntd[td] - s), 0);
   }
 }
 
@@ -688,6 +717,7 @@ int arlmi::checked_geom(const arlmi_geom* in, DevGeom* g) {
   g->nplanes = in->nplanes;
   g->mode = in->mode;
   g->tile = tile;
+  g->tshift = __builtin_ctz((unsigned)tile);
   g->ntx = n / tile;
   g->du = in->du;
   g->dv = in->dv;
@@ -933,9 +963,10 @@ static int plan_build(arlmi_plan* p, const double* uvw, const double* freq, int6
   p->nslots = tot3[2];
   if ((rc = grow(&p->d_gitems, &p->cap_gitems, std::max<int64_t>(1, p->ngitems)))) return rc;
   if ((rc = grow(&p->d_ditems, &p->cap_ditems, std::max<int64_t>(1, p->nditems)))) return rc;
-  ARLMI_LAUNCH("k_items", k_items, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, p->d_cnt, p->d_cntd,
-               p->d_off, p->d_choff, p->d_choffd, p->d_slotoff, (int)nt, p->chunk, p->d_gitems, p->d_ditems,
-               p->d_multi, p->d_misc);
+  const int64_t nitems = std::max<int64_t>(1, std::max(p->ngitems, p->nditems));
+  ARLMI_LAUNCH("k_items", k_items, dim3((unsigned)((nitems + 255) / 256)), dim3(256), 0, s, p->d_cnt, p->d_cntd,
+               p->d_off, p->d_choff, p->d_choffd, p->d_slotoff, (int)nt, p->chunk, (uint32_t)p->ngitems,
+               (uint32_t)p->nditems, p->d_gitems, p->d_ditems, p->d_multi, p->d_misc);
   ARLMI_LAUNCH_CHECK();
   ARLMI_HIP(hipMemcpyAsync(misc, p->d_misc, sizeof(misc), hipMemcpyDeviceToHost, s));
   ARLMI_HIP(hipStreamSynchronize(s));
