@@ -564,32 +564,59 @@ __global__ void k_items(const uint32_t* __restrict__ cnt, const uint32_t* __rest
 // exclusive scan of uint32 (three-phase: block scans, scan of block sums, add)
 // ---------------------------------------------------------------------------
 
-constexpr int SCAN_BLOCK = 1024;
+constexpr int SCAN_T = 1024;                 // threads per scan block
+constexpr int SCAN_V = 4;                    // consecutive values per thread
+constexpr int SCAN_BLOCK = SCAN_T * SCAN_V;  // values per scan block
 
-__global__ void k_scan_block(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* bsum) {
-  __shared__ uint32_t s[SCAN_BLOCK];
-  const int64_t i = blockIdx.x * (int64_t)SCAN_BLOCK + threadIdx.x;
-  const uint32_t x = i < n ? in[i] : 0u;
-  s[threadIdx.x] = x;
-  __syncthreads();
-  for (int o = 1; o < SCAN_BLOCK; o <<= 1) {
-    uint32_t y = threadIdx.x >= o ? s[threadIdx.x - o] : 0u;
-    __syncthreads();
-    s[threadIdx.x] += y;
-    __syncthreads();
+// Exclusive scan of one block of SCAN_BLOCK values: per-thread sums, a shuffle scan inside each
+// wavefront, then a scan of the wavefront totals; bsum[block] = the block's total.
+__global__ __launch_bounds__(SCAN_T) void k_scan_block(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* bsum) {
+  __shared__ uint32_t wsum[SCAN_T / 64];
+  const int lane = __lane_id(), wave = threadIdx.x >> 6;
+  constexpr int NW = SCAN_T / 64;
+  const int64_t base = blockIdx.x * (int64_t)SCAN_BLOCK + (int64_t)threadIdx.x * SCAN_V;
+  uint32_t v[SCAN_V];
+  uint32_t t = 0;
+#pragma unroll
+  for (int j = 0; j < SCAN_V; ++j) {
+    v[j] = base + j < n ? in[base + j] : 0u;
+    t += v[j];
   }
-  if (i < n) out[i] = s[threadIdx.x] - x;
-  if (threadIdx.x == SCAN_BLOCK - 1) bsum[blockIdx.x] = s[threadIdx.x];
+  uint32_t x = t;  // inclusive scan of the thread sums within the wavefront
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wsum[wave] = x;
+  __syncthreads();
+  if (wave == 0) {
+    uint32_t ws = lane < NW ? wsum[lane] : 0u;
+#pragma unroll
+    for (int o = 1; o < NW; o <<= 1) {
+      const uint32_t y = __shfl_up(ws, o);
+      if (lane >= o) ws += y;
+    }
+    if (lane < NW) wsum[lane] = ws;
+  }
+  __syncthreads();
+  uint32_t off = (wave ? wsum[wave - 1] : 0u) + x - t;
+#pragma unroll
+  for (int j = 0; j < SCAN_V; ++j) {
+    if (base + j < n) out[base + j] = off;
+    off += v[j];
+  }
+  if (threadIdx.x == 0) bsum[blockIdx.x] = wsum[NW - 1];
 }
 
-__global__ void k_scan_add(uint32_t* out, int64_t n, const uint32_t* boff) {
-  const int64_t i = blockIdx.x * (int64_t)SCAN_BLOCK + threadIdx.x;
-  if (i < n) out[i] += boff[blockIdx.x];
-}
-
-__global__ void k_scan_total(const uint32_t* in, uint32_t* out, int64_t n) {
-  // out[n] = out[n-1] + in[n-1]
-  out[n] = n ? out[n - 1] + in[n - 1] : 0u;
+// Adds each block's offset; boff[nb] is the grand total, stored at out[n].
+__global__ __launch_bounds__(SCAN_T) void k_scan_add(uint32_t* out, int64_t n, const uint32_t* boff, int64_t nb) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = boff[nb];
+  const uint32_t add = boff[blockIdx.x];
+  const int64_t base = blockIdx.x * (int64_t)SCAN_BLOCK + (int64_t)threadIdx.x * SCAN_V;
+#pragma unroll
+  for (int j = 0; j < SCAN_V; ++j)
+    if (base + j < n) out[base + j] += add;
 }
 
 size_t scan_tmp_words(int64_t n) {
@@ -608,17 +635,18 @@ int exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* t
     return ARLMI_OK;
   }
   const int64_t nb = (n + SCAN_BLOCK - 1) / SCAN_BLOCK;
+  if (nb == 1) {  // one block: its total is out[n]
+    ARLMI_LAUNCH("k_scan_block", k_scan_block, dim3(1), dim3(SCAN_T), 0, s, in, out, n, out + n);
+    ARLMI_LAUNCH_CHECK();
+    return ARLMI_OK;
+  }
   uint32_t* bsum = tmp;
   uint32_t* boff = tmp + nb;
-  ARLMI_LAUNCH("k_scan_block", k_scan_block, dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, s, in, out, n, bsum);
+  ARLMI_LAUNCH("k_scan_block", k_scan_block, dim3((unsigned)nb), dim3(SCAN_T), 0, s, in, out, n, bsum);
   ARLMI_LAUNCH_CHECK();
-  if (nb > 1) {
-    int rc = exclusive_scan_u32(bsum, boff, nb, tmp + 2 * nb + 1, s);
-    if (rc) return rc;
-    ARLMI_LAUNCH("k_scan_add", k_scan_add, dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, s, out, n, boff);
-    ARLMI_LAUNCH_CHECK();
-  }
-  ARLMI_LAUNCH("k_scan_total", k_scan_total, dim3(1), dim3(1), 0, s, in, out, n);
+  int rc = exclusive_scan_u32(bsum, boff, nb, tmp + 2 * nb + 1, s);
+  if (rc) return rc;
+  ARLMI_LAUNCH("k_scan_add", k_scan_add, dim3((unsigned)nb), dim3(SCAN_T), 0, s, out, n, boff, nb);
   ARLMI_LAUNCH_CHECK();
   return ARLMI_OK;
 }
