@@ -119,11 +119,15 @@ __device__ inline void lds_add_f64(double* p, double v) {
 // Grid: (split tiles, T*T/256 pixel blocks); each thread sums one pixel over the tile's slots
 // in slot order (four loads in flight).
 template <typename R>
-__global__ void k_reduce_tiles(const int4* __restrict__ multi, const cplx<R>* __restrict__ partial,
-                               cplx<R>* __restrict__ grids, int T, int ntx, int N) {
-  const int4 m = multi[blockIdx.x];
+__global__ void k_reduce_tiles(const int4* __restrict__ multi, const uint32_t* __restrict__ nmulti,
+                               const cplx<R>* __restrict__ partial, cplx<R>* __restrict__ grids, int T, int ntx,
+                               int N) {
   const int i = blockIdx.y * blockDim.x + threadIdx.x;
   if (i >= T * T) return;
+  // the number of split tiles is read here, so the host never waits for it
+  const uint32_t count = *nmulti;
+  for (uint32_t mi = blockIdx.x; mi < count; mi += gridDim.x) {
+  const int4 m = multi[mi];
   const TilePos tp = tile_pos(m.x, ntx);
   const size_t tt = (size_t)T * T;
   const cplx<R>* src = partial + (size_t)m.y * tt + i;
@@ -147,6 +151,7 @@ __global__ void k_reduce_tiles(const int4* __restrict__ multi, const cplx<R>* __
     im += a.y;
   }
   grids[(size_t)tp.plane * N * N + (size_t)(tp.ty * T + i / T) * N + (size_t)tp.tx * T + (i % T)] = mk<R>(re, im);
+  }
 }
 
 // Per-entry values a wavefront broadcasts while it walks its 64 entries:
@@ -641,9 +646,10 @@ static int grid_impl(const arlmi_plan* p, const void* vis, const void* wt, int d
   if (rc) return rc;
   rc = grid_launch<R>(p, vis, wt, dopsf, k, grids, partial, sumwt, s);
   if (rc) return rc;
-  if (p->nmulti > 0) {
-    ARLMI_LAUNCH("k_reduce_tiles", k_reduce_tiles<R>, dim3((unsigned)p->nmulti, (g.tile * g.tile + 255) / 256),
-                 dim3(256), 0, s, p->d_multi, partial, (cplx<R>*)grids, g.tile, g.ntx, g.n);
+  if (p->nslots > 0) {  // every split tile has at least two slots
+    const int64_t gx = std::min<int64_t>(std::max<int64_t>(1, p->nslots / 2), 4096);
+    ARLMI_LAUNCH("k_reduce_tiles", k_reduce_tiles<R>, dim3((unsigned)gx, (g.tile * g.tile + 255) / 256),
+                 dim3(256), 0, s, p->d_multi, p->d_misc + 4, partial, (cplx<R>*)grids, g.tile, g.ntx, g.n);
     ARLMI_LAUNCH_CHECK();
   }
   return ARLMI_OK;

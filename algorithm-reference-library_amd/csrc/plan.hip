@@ -952,12 +952,13 @@ static int plan_build(arlmi_plan* p, const double* uvw, const double* freq, int6
   ARLMI_LAUNCH("k_p2_keys", k_p2_keys, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, p->d_s2, p->d_hbase,
                p->d_nbp, P, tpp, p->d_cnt, p->d_cntd, p->d_off);
   ARLMI_LAUNCH_CHECK();
-  uint32_t total = 0;
-  ARLMI_HIP(hipMemcpyAsync(&total, p->d_s2 + n2, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  ARLMI_HIP(hipStreamSynchronize(s));
-  p->nentries = total;
-  if ((rc = grow(&p->d_entries, &p->cap_entries, std::max<int64_t>(1, total)))) return rc;
-  if (g.nz > 1 && (rc = grow(&p->d_entry_z, &p->cap_z, std::max<int64_t>(1, total)))) return rc;
+  // entry buffers sized by an upper bound (each membership's footprint touches at most
+  // ((S-1)/T + 2)^2 tiles), so the host need not wait for the exact count here; it is read with
+  // the work-item totals below
+  const int64_t per_axis = (g.support - 1) / g.tile + 2;
+  const int64_t ebound = std::max<int64_t>(1, p->nmember * per_axis * per_axis);
+  if ((rc = grow(&p->d_entries, &p->cap_entries, ebound))) return rc;
+  if (g.nz > 1 && (rc = grow(&p->d_entry_z, &p->cap_z, ebound))) return rc;
   ARLMI_LAUNCH("k_p2_scatter", k_p2_scatter, dim3((unsigned)blks.size()), dim3(512), sh2, s, g,
                (const Rec*)p->d_recs, (const Blk2*)p->d_blk2, p->d_s2, p->d_entries,
                g.nz > 1 ? p->d_entry_z : nullptr);
@@ -982,10 +983,17 @@ static int plan_build(arlmi_plan* p, const double* uvw, const double* freq, int6
   if ((rc = exclusive_scan_u32(p->d_nchd, p->d_choffd, nt, p->d_scan_tmp, s))) return rc;
   if ((rc = exclusive_scan_u32(p->d_nslot, p->d_slotoff, nt, p->d_scan_tmp, s))) return rc;
   uint32_t tot3[3] = {0, 0, 0};
+  uint32_t total = 0;
+  ARLMI_HIP(hipMemcpyAsync(&total, p->d_s2 + n2, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   ARLMI_HIP(hipMemcpyAsync(&tot3[0], p->d_choff + nt, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   ARLMI_HIP(hipMemcpyAsync(&tot3[1], p->d_choffd + nt, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   ARLMI_HIP(hipMemcpyAsync(&tot3[2], p->d_slotoff + nt, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   ARLMI_HIP(hipStreamSynchronize(s));
+  p->nentries = total;
+  if (total > ebound) {
+    set_error("internal: %u entries exceed the bound %lld", total, (long long)ebound);
+    return ARLMI_E_UNSUPPORTED;
+  }
   p->ngitems = tot3[0];
   p->nditems = tot3[1];
   p->nslots = tot3[2];
@@ -996,9 +1004,7 @@ static int plan_build(arlmi_plan* p, const double* uvw, const double* freq, int6
                p->d_off, p->d_choff, p->d_choffd, p->d_slotoff, (int)nt, p->chunk, (uint32_t)p->ngitems,
                (uint32_t)p->nditems, p->d_gitems, p->d_ditems, p->d_multi, p->d_misc);
   ARLMI_LAUNCH_CHECK();
-  ARLMI_HIP(hipMemcpyAsync(misc, p->d_misc, sizeof(misc), hipMemcpyDeviceToHost, s));
-  ARLMI_HIP(hipStreamSynchronize(s));
-  p->nmulti = misc[4];
+  p->nmulti = -1;  // left on the device (d_misc[4]); the reduction kernel reads it
   return ARLMI_OK;
 }
 
