@@ -32,7 +32,7 @@ KERNEL_TABLE = 1
 # Names exported by the C ABI (include/arlmi.h).
 EXPORTS = (
     "arlmi_version", "arlmi_last_error", "arlmi_wmax", "arlmi_plan_create", "arlmi_plan_update",
-    "arlmi_plan_set_chunk", "arlmi_plan_stats_get", "arlmi_plan_plane_w", "arlmi_plan_destroy", "arlmi_convolution_mapping",
+    "arlmi_plan_set_chunk", "arlmi_plan_stats_get", "arlmi_plan_plane_w", "arlmi_plan_slab_boxes", "arlmi_plan_destroy", "arlmi_convolution_mapping",
     "arlmi_grid", "arlmi_degrid",
     "arlmi_grids_to_image", "arlmi_image_to_grids", "arlmi_fft2d", "arlmi_w_screen", "arlmi_dft",
     "arlmi_grid_weights", "arlmi_reweight", "arlmi_block_to_rows", "arlmi_apply_gaintable", "arlmi_apply_gains_rows", "arlmi_apply_gains_rows_adjoint",
@@ -123,6 +123,7 @@ def lib():
         "arlmi_plan_set_chunk": (ctypes.c_int, [vp, i32]),
         "arlmi_plan_stats_get": (ctypes.c_int, [vp, ctypes.POINTER(PlanStats)]),
         "arlmi_plan_plane_w": (ctypes.c_int, [vp, ctypes.POINTER(dbl), i32]),
+        "arlmi_plan_slab_boxes": (ctypes.c_int, [vp, ctypes.POINTER(dbl), i32]),
         "arlmi_plan_destroy": (None, [vp]),
         "arlmi_convolution_mapping": (ctypes.c_int, [ctypes.POINTER(Geom), vp, vp, i64, vp, vp]),
         "arlmi_grid": (ctypes.c_int, [vp, vp, vp, ctypes.c_int, ctypes.POINTER(Kernel), vp, vp, ctypes.c_int, vp]),

@@ -120,16 +120,13 @@ class Plan:
             raise TypeError("uvw must be float64")
         nrows = uvw.shape[0]
         boxes_arr, width = None, 0.0
-        if self.mode == _lib.MODE_WSTACK:
-            if slab_boxes is None:
-                wmax = ctypes.c_double()
-                _lib.check(self.L.arlmi_wmax(ptr(uvw), nrows, stream_handle(), ctypes.byref(wmax)))
-                boxes, width = wslice_boxes(wmax.value, self.nplanes)
-            else:
-                boxes, width = slab_boxes
-                boxes = numpy.ascontiguousarray(boxes, dtype=numpy.float64)
+        if self.mode == _lib.MODE_WSTACK and slab_boxes is not None:
+            boxes, width = slab_boxes
+            boxes = numpy.ascontiguousarray(boxes, dtype=numpy.float64)
             self.boxes, self.width = boxes, width
             boxes_arr = boxes.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        # with no centres given, the library computes ARL's default ones (numpy.linspace over
+        # +-max|w|) on the device
         fptr = self._freq_ptr(vis)
         if create:
             _lib.check(self.L.arlmi_plan_create(ctypes.byref(self.geom), ptr(uvw), fptr, nrows, boxes_arr, width,
@@ -153,6 +150,13 @@ class Plan:
         _lib.check(self.L.arlmi_plan_plane_w(self.handle, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                                              self.nplanes))
         return out
+
+    def slab_boxes(self):
+        """(centres [P], width) of a w-stack plan, as the library holds them."""
+        out = numpy.zeros(self.nplanes + 1)
+        _lib.check(self.L.arlmi_plan_slab_boxes(self.handle, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                                self.nplanes + 1))
+        return out[:-1], float(out[-1])
 
     def close(self):
         if self.handle:

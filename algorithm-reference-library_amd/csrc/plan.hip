@@ -83,6 +83,44 @@ __global__ __launch_bounds__(256) void k_wmax(const double* __restrict__ uvw, in
   if (threadIdx.x == 0) part[blockIdx.x] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
 }
 
+// ARL's default slab centres from the k_wmax partials, on the device so that the host need not
+// wait for max |w|: boxes = numpy.linspace(-wmax, wmax, P) with numpy's operations (step =
+// delta / (P - 1), y_k = k * step + start, the last value = stop; for a zero step y_k =
+// (k / (P - 1)) * delta + start; for P = 1, y_0 = 0 * delta + start), and after them the slab
+// width boxes[1] - boxes[0], or 2 * wmax for one slab (iterators.py:99-106).  One block.
+__global__ __launch_bounds__(256) void k_boxes(const double* __restrict__ part, int nparts, int P, double* boxes) {
+  __shared__ double red[4];
+  __shared__ double wmax_s;
+  double m = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x) m = fmax(m, part[i]);
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+  if (__lane_id() == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) wmax_s = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+  __syncthreads();
+  const double wmax = wmax_s;
+  const double start = -wmax, stop = wmax;
+  const double delta = stop - start;
+  const int div = P - 1;
+  if (div > 0) {
+    const double step = delta / (double)div;
+    for (int k = threadIdx.x; k < P; k += blockDim.x) {
+      double y = (double)k;
+      y = step == 0.0 ? (y / (double)div) * delta : y * step;
+      y += start;
+      boxes[k] = k == P - 1 ? stop : y;
+    }
+    if (threadIdx.x == 0) {
+      const double y0 = (step == 0.0 ? (0.0 / (double)div) * delta : 0.0 * step) + start;
+      const double y1 = P == 2 ? stop : (step == 0.0 ? (1.0 / (double)div) * delta : 1.0 * step) + start;
+      boxes[P] = y1 - y0;
+    }
+  } else if (threadIdx.x == 0) {
+    boxes[0] = 0.0 * delta + start;
+    boxes[1] = 2.0 * wmax;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // w-slab membership: |w - boxes[k]| < 0.5 * wslice (iterators.py:107-109)
 // ---------------------------------------------------------------------------
@@ -120,7 +158,8 @@ __device__ inline Members slab_members(double w, const double* boxes, int nb, do
 // blocks writes its partial sums; k_slab_means adds them.
 constexpr int SLAB_BLOCKS = 2048;
 __global__ void k_slab_sums(const double* __restrict__ uvw, int64_t n, const double* __restrict__ boxes, int nb,
-                            double wslice, double* wsum, uint32_t* wcnt) {
+                            double* wsum, uint32_t* wcnt) {
+  const double wslice = boxes[nb];  // the slab width follows the centres
   extern __shared__ char smem[];
   double* ls = reinterpret_cast<double*>(smem);
   double* lbox = ls + nb;
@@ -280,20 +319,20 @@ struct Rec {
 };
 
 __global__ void k_p1_count(DevGeom g, const double* __restrict__ uvw, const double* __restrict__ freq, int64_t n,
-                           const double* __restrict__ boxes, double wslice, const double* __restrict__ plane_w,
+                           const double* __restrict__ boxes, const double* __restrict__ plane_w,
                            uint32_t flags, uint32_t* __restrict__ h1, int nb1, uint32_t* misc) {
-  // slab centres and means staged in LDS (looked up per row), then the counters
+  // slab centres (and, after them, the slab width) and slab means staged in LDS (looked up per
+  // row), then the counters
   extern __shared__ double lsh[];
   const int P = g.nplanes;
   double* lbox = lsh;
-  double* lpw = lsh + P;
-  uint32_t* lh = reinterpret_cast<uint32_t*>(lsh + 2 * P);
-  for (int i = threadIdx.x; i < P; i += blockDim.x) {
-    lbox[i] = boxes[i];
-    lpw[i] = plane_w[i];
-  }
+  double* lpw = lsh + P + 1;
+  uint32_t* lh = reinterpret_cast<uint32_t*>(lsh + 2 * P + 1);
+  for (int i = threadIdx.x; i <= P; i += blockDim.x) lbox[i] = boxes[i];
+  for (int i = threadIdx.x; i < P; i += blockDim.x) lpw[i] = plane_w[i];
   for (int i = threadIdx.x; i < P + 3; i += blockDim.x) lh[i] = 0u;
   __syncthreads();
+  const double wslice = lbox[P];
   const int64_t r0 = (int64_t)blockIdx.x * K1_ROWS;
   const int64_t r1 = r0 + K1_ROWS < n ? r0 + K1_ROWS : n;
   const double inv_wslice = 1.0 / wslice;
@@ -335,19 +374,18 @@ __global__ void k_p1_count(DevGeom g, const double* __restrict__ uvw, const doub
 }
 
 __global__ void k_p1_scatter(DevGeom g, const double* __restrict__ uvw, const double* __restrict__ freq, int64_t n,
-                             const double* __restrict__ boxes, double wslice, const double* __restrict__ plane_w,
+                             const double* __restrict__ boxes, const double* __restrict__ plane_w,
                              uint32_t flags, const uint32_t* __restrict__ s1, int nb1, Rec* __restrict__ recs) {
   extern __shared__ double lsh[];
   const int P = g.nplanes;
   double* lbox = lsh;
-  double* lpw = lsh + P;
-  uint32_t* cur = reinterpret_cast<uint32_t*>(lsh + 2 * P);
-  for (int i = threadIdx.x; i < P; i += blockDim.x) {
-    lbox[i] = boxes[i];
-    lpw[i] = plane_w[i];
-  }
+  double* lpw = lsh + P + 1;
+  uint32_t* cur = reinterpret_cast<uint32_t*>(lsh + 2 * P + 1);
+  for (int i = threadIdx.x; i <= P; i += blockDim.x) lbox[i] = boxes[i];
+  for (int i = threadIdx.x; i < P; i += blockDim.x) lpw[i] = plane_w[i];
   for (int i = threadIdx.x; i <= P; i += blockDim.x) cur[i] = s1[(int64_t)i * nb1 + blockIdx.x];
   __syncthreads();
+  const double wslice = lbox[P];
   const int64_t r0 = (int64_t)blockIdx.x * K1_ROWS;
   const int64_t r1 = r0 + K1_ROWS < n ? r0 + K1_ROWS : n;
   const double inv_wslice = 1.0 / wslice;
@@ -798,7 +836,7 @@ static void free_plan(arlmi_plan* p) {
 static int plan_alloc_fixed(arlmi_plan* p) {
   const int P = p->g.nplanes;
   const int64_t nt = p->ntiles;
-  ARLMI_HIP(hipMalloc(&p->d_boxes, sizeof(double) * P));
+  ARLMI_HIP(hipMalloc(&p->d_boxes, sizeof(double) * (P + 1)));  // centres, then the slab width
   ARLMI_HIP(hipMalloc(&p->d_plane_w, sizeof(double) * P));
   ARLMI_HIP(hipMalloc(&p->d_wsum, sizeof(double) * P * SLAB_BLOCKS));
   ARLMI_HIP(hipMalloc(&p->d_wcnt, sizeof(uint32_t) * P * SLAB_BLOCKS));
@@ -847,17 +885,29 @@ static int plan_build(arlmi_plan* p, const double* uvw, const double* freq, int6
   p->nrows = nrows;
   int rc;
   ARLMI_HIP(hipMemsetAsync(p->d_plane_w, 0, sizeof(double) * P, s));
+  std::vector<double> hboxes;  // host staging of the given centres; outlives the copy (the
+                               // stream is synchronised before this function returns)
   if (g.mode == ARLMI_MODE_WSTACK) {
-    if (!h_boxes) {
-      set_error("w-stack mode needs the slab centres");
-      return ARLMI_E_ARG;
+    if (h_boxes) {
+      hboxes.assign(h_boxes, h_boxes + P);
+      hboxes.push_back(wslice);
+      ARLMI_HIP(hipMemcpyAsync(p->d_boxes, hboxes.data(), sizeof(double) * (P + 1), hipMemcpyHostToDevice, s));
+    } else {
+      // ARL's default centres from max |w|, computed on the device (d_wsum holds the partials
+      // until the slab sums overwrite it)
+      const int wb = (int)std::max<int64_t>(1, std::min<int64_t>((nrows + 1023) / 1024, 1024));
+      if (nrows > 0) {
+        ARLMI_LAUNCH("k_wmax", k_wmax, dim3((unsigned)wb), dim3(256), 0, s, uvw, nrows, p->d_wsum);
+        ARLMI_LAUNCH_CHECK();
+      }
+      ARLMI_LAUNCH("k_boxes", k_boxes, dim3(1), dim3(256), 0, s, p->d_wsum, nrows > 0 ? wb : 0, P, p->d_boxes);
+      ARLMI_LAUNCH_CHECK();
     }
-    ARLMI_HIP(hipMemcpyAsync(p->d_boxes, h_boxes, sizeof(double) * P, hipMemcpyHostToDevice, s));
     const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((nrows + 1023) / 1024, SLAB_BLOCKS));
     if (nrows > 0) {
       const size_t sh = (2 * sizeof(double) + sizeof(unsigned int)) * P;
       ARLMI_LAUNCH("k_slab_sums", k_slab_sums, dim3((unsigned)blocks), dim3(256), sh, s, uvw, nrows, p->d_boxes, P,
-                   wslice, p->d_wsum, p->d_wcnt);
+                   p->d_wsum, p->d_wcnt);
       ARLMI_LAUNCH_CHECK();
     } else {
       ARLMI_HIP(hipMemsetAsync(p->d_wsum, 0, sizeof(double) * P, s));
@@ -875,9 +925,8 @@ static int plan_build(arlmi_plan* p, const double* uvw, const double* freq, int6
   if ((rc = grow(&p->d_h1, &p->cap_h1, n1))) return rc;
   if ((rc = grow(&p->d_s1, &p->cap_s1, n1 + 1))) return rc;
   if ((rc = ensure_scan_tmp(p, n1))) return rc;
-  ARLMI_LAUNCH("k_p1_count", k_p1_count, dim3(nb1), dim3(256), sizeof(double) * 2 * P + sizeof(uint32_t) * (P + 3), s,
-               g, uvw, freq, nrows,
-               p->d_boxes, wslice, p->d_plane_w, p->flags, p->d_h1, nb1, p->d_misc);
+  ARLMI_LAUNCH("k_p1_count", k_p1_count, dim3(nb1), dim3(256), sizeof(double) * (2 * P + 1) + sizeof(uint32_t) * (P + 3),
+               s, g, uvw, freq, nrows, p->d_boxes, p->d_plane_w, p->flags, p->d_h1, nb1, p->d_misc);
   ARLMI_LAUNCH_CHECK();
   if ((rc = exclusive_scan_u32(p->d_h1, p->d_s1, n1, p->d_scan_tmp, s))) return rc;
   ARLMI_LAUNCH("k_p1_bounds", k_p1_bounds, dim3((P + 2 + 255) / 256), dim3(256), 0, s, p->d_s1, nb1, P,
@@ -903,9 +952,9 @@ static int plan_build(arlmi_plan* p, const double* uvw, const double* freq, int6
   p->orphan_start = bounds[P];
   p->norphans = bounds[P + 1] - bounds[P];
   if ((rc = grow_bytes(&p->d_recs, &p->cap_recs, (int64_t)sizeof(Rec) * std::max<int64_t>(1, nrec)))) return rc;
-  ARLMI_LAUNCH("k_p1_scatter", k_p1_scatter, dim3(nb1), dim3(256), sizeof(double) * 2 * P + sizeof(uint32_t) * (P + 1),
-               s, g, uvw, freq,
-               nrows, p->d_boxes, wslice, p->d_plane_w, p->flags, p->d_s1, nb1, (Rec*)p->d_recs);
+  ARLMI_LAUNCH("k_p1_scatter", k_p1_scatter, dim3(nb1), dim3(256),
+               sizeof(double) * (2 * P + 1) + sizeof(uint32_t) * (P + 1), s, g, uvw, freq, nrows, p->d_boxes,
+               p->d_plane_w, p->flags, p->d_s1, nb1, (Rec*)p->d_recs);
   ARLMI_LAUNCH_CHECK();
 
   // pass 2: plane records -> tile entries
@@ -1108,6 +1157,15 @@ int arlmi_plan_plane_w(const arlmi_plan* p, double* h_out, int32_t n) {
     return ARLMI_E_ARG;
   }
   ARLMI_HIP(hipMemcpy(h_out, p->d_plane_w, sizeof(double) * p->g.nplanes, hipMemcpyDeviceToHost));
+  return ARLMI_OK;
+}
+
+int arlmi_plan_slab_boxes(const arlmi_plan* p, double* h_out, int32_t n) {
+  if (!p || !h_out || p->g.mode != ARLMI_MODE_WSTACK || n < p->g.nplanes + 1) {
+    set_error("arlmi_plan_slab_boxes: bad argument (w-stack plans only, n >= nplanes + 1)");
+    return ARLMI_E_ARG;
+  }
+  ARLMI_HIP(hipMemcpy(h_out, p->d_boxes, sizeof(double) * (p->g.nplanes + 1), hipMemcpyDeviceToHost));
   return ARLMI_OK;
 }
 

@@ -115,8 +115,10 @@ const char* arlmi_last_error(void);
 int arlmi_wmax(const double* d_uvw, int64_t nrows, void* stream, double* wmax_out);
 
 /* Build the gridding plan: per-row convolution mapping (gridding.py:24-96), w-slab
- * membership (iterators.py:91-109, boxes = numpy.linspace(-wmax, wmax, P) given by the
- * host in h_boxes, wslice = boxes[1]-boxes[0]), slab mean w (wstack_single.py:43, :83),
+ * membership (iterators.py:91-109: slab centres h_boxes[P] and width wslice from the host, or,
+ * with h_boxes NULL, ARL's default boxes = numpy.linspace(-max|w|, max|w|, P) and wslice =
+ * boxes[1] - boxes[0] (2 max|w| for P = 1) computed on the device with numpy's operations),
+ * slab mean w (wstack_single.py:43, :83),
  * and the tile-sorted entry lists the grid/degrid kernels walk.  d_freq may be NULL
  * (all rows in image channel 0).  Synchronises `stream`.  Returns an ARLMI_E_* code if
  * a row violates one of ARL's asserts and ARLMI_FLAG_DROP_OFF_GRID does not cover it. */
@@ -130,6 +132,9 @@ int arlmi_plan_update(arlmi_plan* plan, const double* d_uvw, const double* d_fre
  * entries of one tile (default 4096).  Takes effect at the next arlmi_plan_update. */
 int arlmi_plan_set_chunk(arlmi_plan* plan, int32_t chunk);
 int arlmi_plan_stats_get(const arlmi_plan* plan, arlmi_plan_stats* out);
+/* Host copy of a w-stack plan's slab centres and, after them, the slab width (nplanes + 1
+ * values; iterators.py:99-106). */
+int arlmi_plan_slab_boxes(const arlmi_plan* plan, double* h_out, int32_t n);
 /* Host copy of the per-plane mean w (numpy.average(vis.w) per slab; 0 in MODE_2D). */
 int arlmi_plan_plane_w(const arlmi_plan* plan, double* h_out, int32_t n);
 void arlmi_plan_destroy(arlmi_plan* plan);

@@ -311,3 +311,24 @@ def test_off_grid_raises(gpu):
         arlmi.invert_2d(vis, im, gcfcf=(gcf, cf))
     imager = arlmi.Imager(vis, im, gcfcf=(gcf, cf), drop_off_grid=True)
     assert imager.plan.stats()["ndropped"] == 1
+
+
+@pytest.mark.parametrize("P", [1, 2, 3, 9, 64, 128])
+def test_device_slab_boxes_match_numpy(gpu, P):
+    """With no centres given, the plan's slab centres and width equal numpy.linspace(-max|w|,
+    max|w|, P) and ARL's width (iterators.py:99-106) bit for bit, including all-zero w."""
+    import torch
+    import arlmi
+    from arlmi.plan import Plan, wslice_boxes
+    rng = numpy.random.default_rng(P)
+    n, cell = 256, 1e-3
+    im, gcf, cf = _pswf(n, cell, 8, 8, 64)
+    for wmax in (987.654321, 3.0e-3, 0.0):
+        d = random_rows(rng, 5000, n, cell, 8, wmax=wmax)
+        vis = to_device_vis(d, 64)
+        plan = Plan(vis, im, cf, arlmi._lib.MODE_WSTACK, P)
+        boxes, width = plan.slab_boxes()
+        rb, rw = wslice_boxes(float(numpy.max(numpy.abs(d["uvw"][:, 2]))), P)
+        assert numpy.array_equal(boxes, rb), (P, wmax)
+        assert width == rw, (P, wmax)
+        plan.close()
