@@ -210,9 +210,9 @@ struct arlmi_plan {
   uint32_t* d_choffd = nullptr;          // [ntiles+1]
   uint32_t* d_slotoff = nullptr;         // [ntiles+1]
   uint32_t* d_misc = nullptr;            // [8]: err bits, -, dropped, nmember, nmulti
-  uint32_t* d_bounds = nullptr;          // [P+2] plane segment starts in the record array
-  uint32_t* d_hbase = nullptr;           // [P+1] plane bases in h2
-  uint32_t* d_nbp = nullptr;             // [P] pass-2 blocks per plane
+  uint32_t* d_bounds = nullptr;          // [P+2] plane segment starts in the record array, [8] misc
+                                         // copy, [4] totals (gathered for single copies)
+  uint32_t* d_hbase = nullptr;           // [P+1] plane bases in h2, then [P] pass-2 blocks per plane
   uint32_t *d_h1 = nullptr, *d_s1 = nullptr;
   uint32_t *d_h2 = nullptr, *d_s2 = nullptr;
   uint32_t* d_scan_tmp = nullptr;

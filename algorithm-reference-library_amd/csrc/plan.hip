@@ -412,10 +412,23 @@ __global__ void k_p1_scatter(DevGeom g, const double* __restrict__ uvw, const do
   }
 }
 
-// gathers s1[i * nb1] for i = 0..P+1 (the plane segment starts and the total)
-__global__ void k_p1_bounds(const uint32_t* __restrict__ s1, int nb1, int P, uint32_t* out) {
+// gathers s1[i * nb1] for i = 0..P+1 (the plane segment starts and the total) and, after them,
+// the 8 words of misc, so that one copy brings both to the host
+__global__ void k_p1_bounds(const uint32_t* __restrict__ s1, int nb1, int P, const uint32_t* __restrict__ misc,
+                            uint32_t* out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i <= P + 1) out[i] = s1[(int64_t)i * nb1];
+  else if (i < P + 10) out[i] = misc[i - (P + 2)];
+}
+
+// The four totals the host sizes the work-item lists with, gathered for one copy.
+__global__ void k_totals(const uint32_t* __restrict__ s2, int64_t n2, const uint32_t* __restrict__ choff,
+                         const uint32_t* __restrict__ choffd, const uint32_t* __restrict__ slotoff, int64_t nt,
+                         uint32_t* out) {
+  out[0] = s2[n2];
+  out[1] = choff[nt];
+  out[2] = choffd[nt];
+  out[3] = slotoff[nt];
 }
 
 // Pass-2 block descriptor: plane, block index in the plane, record range, h2 base, blocks in plane.
@@ -826,7 +839,7 @@ static void free_plan(arlmi_plan* p) {
   if (!p) return;
   void* ptrs[] = {p->d_boxes, p->d_plane_w, p->d_wsum, p->d_wcnt, p->d_cnt, p->d_tilemask, p->d_colmask, p->d_zeros, p->d_cntd, p->d_off, p->d_nch,
                   p->d_nchd, p->d_nslot, p->d_choff, p->d_choffd, p->d_slotoff, p->d_misc, p->d_bounds,
-                  p->d_hbase, p->d_nbp, p->d_h1, p->d_s1, p->d_h2, p->d_s2, p->d_scan_tmp, p->d_recs, p->d_blk2,
+                  p->d_hbase, p->d_h1, p->d_s1, p->d_h2, p->d_s2, p->d_scan_tmp, p->d_recs, p->d_blk2,
                   p->d_entries, p->d_entry_z, p->d_gitems, p->d_ditems, p->d_multi, p->d_partial,
                   p->d_img_part};
   for (void* q : ptrs)
@@ -857,9 +870,10 @@ static int plan_alloc_fixed(arlmi_plan* p) {
   ARLMI_HIP(hipMalloc(&p->d_choffd, sizeof(uint32_t) * (nt + 1)));
   ARLMI_HIP(hipMalloc(&p->d_slotoff, sizeof(uint32_t) * (nt + 1)));
   ARLMI_HIP(hipMalloc(&p->d_misc, sizeof(uint32_t) * 8));
-  ARLMI_HIP(hipMalloc(&p->d_bounds, sizeof(uint32_t) * (P + 2)));
-  ARLMI_HIP(hipMalloc(&p->d_hbase, sizeof(uint32_t) * (P + 1)));
-  ARLMI_HIP(hipMalloc(&p->d_nbp, sizeof(uint32_t) * P));
+  // [P+2] segment bounds, [8] misc copy, [4] totals
+  ARLMI_HIP(hipMalloc(&p->d_bounds, sizeof(uint32_t) * (P + 14)));
+  // [P+1] plane bases, then [P] blocks per plane (one copy from the host)
+  ARLMI_HIP(hipMalloc(&p->d_hbase, sizeof(uint32_t) * (2 * P + 1)));
   ARLMI_HIP(hipMalloc(&p->d_multi, sizeof(int4) * nt));
   return ARLMI_OK;
 }
@@ -929,14 +943,13 @@ static int plan_build(arlmi_plan* p, const double* uvw, const double* freq, int6
                s, g, uvw, freq, nrows, p->d_boxes, p->d_plane_w, p->flags, p->d_h1, nb1, p->d_misc);
   ARLMI_LAUNCH_CHECK();
   if ((rc = exclusive_scan_u32(p->d_h1, p->d_s1, n1, p->d_scan_tmp, s))) return rc;
-  ARLMI_LAUNCH("k_p1_bounds", k_p1_bounds, dim3((P + 2 + 255) / 256), dim3(256), 0, s, p->d_s1, nb1, P,
+  ARLMI_LAUNCH("k_p1_bounds", k_p1_bounds, dim3((P + 10 + 255) / 256), dim3(256), 0, s, p->d_s1, nb1, P, p->d_misc,
                p->d_bounds);
   ARLMI_LAUNCH_CHECK();
-  std::vector<uint32_t> bounds(P + 2);
-  uint32_t misc[8];
-  ARLMI_HIP(hipMemcpyAsync(bounds.data(), p->d_bounds, sizeof(uint32_t) * (P + 2), hipMemcpyDeviceToHost, s));
-  ARLMI_HIP(hipMemcpyAsync(misc, p->d_misc, sizeof(misc), hipMemcpyDeviceToHost, s));
+  std::vector<uint32_t> bounds(P + 10);
+  ARLMI_HIP(hipMemcpyAsync(bounds.data(), p->d_bounds, sizeof(uint32_t) * (P + 10), hipMemcpyDeviceToHost, s));
   ARLMI_HIP(hipStreamSynchronize(s));
+  const uint32_t* misc = bounds.data() + P + 2;
   if (misc[0]) {
     rc = first_error_code(misc[0]);
     const char* what = rc == ARLMI_E_UV_OFF_GRID ? "uv (or kernel footprint) off the grid"
@@ -959,7 +972,8 @@ static int plan_build(arlmi_plan* p, const double* uvw, const double* freq, int6
 
   // pass 2: plane records -> tile entries
   std::vector<Blk2> blks;
-  std::vector<uint32_t> hbase(P + 1), nbp(P);
+  std::vector<uint32_t> hbase(2 * P + 1);  // plane bases, then blocks per plane
+  uint32_t* nbp = hbase.data() + P + 1;
   uint64_t hb = 0;
   for (int pl = 0; pl < P; ++pl) {
     const uint32_t rs = bounds[pl], re = bounds[pl + 1];
@@ -988,8 +1002,7 @@ static int plan_build(arlmi_plan* p, const double* uvw, const double* freq, int6
   if ((rc = grow(&p->d_h2, &p->cap_h2, n2 + 1))) return rc;
   if ((rc = grow(&p->d_s2, &p->cap_s2, n2 + 2))) return rc;
   ARLMI_HIP(hipMemcpyAsync(p->d_blk2, blks.data(), sizeof(Blk2) * blks.size(), hipMemcpyHostToDevice, s));
-  ARLMI_HIP(hipMemcpyAsync(p->d_hbase, hbase.data(), sizeof(uint32_t) * (P + 1), hipMemcpyHostToDevice, s));
-  ARLMI_HIP(hipMemcpyAsync(p->d_nbp, nbp.data(), sizeof(uint32_t) * P, hipMemcpyHostToDevice, s));
+  ARLMI_HIP(hipMemcpyAsync(p->d_hbase, hbase.data(), sizeof(uint32_t) * (2 * P + 1), hipMemcpyHostToDevice, s));
   if ((rc = ensure_scan_tmp(p, n2 + 1))) return rc;
   const size_t sh2 = sizeof(uint32_t) * 2 * (size_t)tpp;
   ARLMI_LAUNCH("k_p2_count", k_p2_count, dim3((unsigned)blks.size()), dim3(512), sh2, s, g, (const Rec*)p->d_recs,
@@ -999,7 +1012,7 @@ static int plan_build(arlmi_plan* p, const double* uvw, const double* freq, int6
   ARLMI_HIP(hipMemsetAsync(p->d_h2 + n2, 0, sizeof(uint32_t), s));
   if ((rc = exclusive_scan_u32(p->d_h2, p->d_s2, n2 + 1, p->d_scan_tmp, s))) return rc;
   ARLMI_LAUNCH("k_p2_keys", k_p2_keys, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, p->d_s2, p->d_hbase,
-               p->d_nbp, P, tpp, p->d_cnt, p->d_cntd, p->d_off);
+               p->d_hbase + P + 1, P, tpp, p->d_cnt, p->d_cntd, p->d_off);
   ARLMI_LAUNCH_CHECK();
   // entry buffers sized by an upper bound (each membership's footprint touches at most
   // ((S-1)/T + 2)^2 tiles), so the host need not wait for the exact count here; it is read with
@@ -1031,13 +1044,14 @@ static int plan_build(arlmi_plan* p, const double* uvw, const double* freq, int6
   if ((rc = exclusive_scan_u32(p->d_nch, p->d_choff, nt, p->d_scan_tmp, s))) return rc;
   if ((rc = exclusive_scan_u32(p->d_nchd, p->d_choffd, nt, p->d_scan_tmp, s))) return rc;
   if ((rc = exclusive_scan_u32(p->d_nslot, p->d_slotoff, nt, p->d_scan_tmp, s))) return rc;
-  uint32_t tot3[3] = {0, 0, 0};
-  uint32_t total = 0;
-  ARLMI_HIP(hipMemcpyAsync(&total, p->d_s2 + n2, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  ARLMI_HIP(hipMemcpyAsync(&tot3[0], p->d_choff + nt, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  ARLMI_HIP(hipMemcpyAsync(&tot3[1], p->d_choffd + nt, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  ARLMI_HIP(hipMemcpyAsync(&tot3[2], p->d_slotoff + nt, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  uint32_t tot[4] = {0, 0, 0, 0};
+  ARLMI_LAUNCH("k_totals", k_totals, dim3(1), dim3(1), 0, s, p->d_s2, n2, p->d_choff, p->d_choffd, p->d_slotoff, nt,
+               p->d_bounds + P + 10);
+  ARLMI_LAUNCH_CHECK();
+  ARLMI_HIP(hipMemcpyAsync(tot, p->d_bounds + P + 10, sizeof(tot), hipMemcpyDeviceToHost, s));
   ARLMI_HIP(hipStreamSynchronize(s));
+  const uint32_t total = tot[0];
+  const uint32_t* tot3 = tot + 1;
   p->nentries = total;
   if (total > ebound) {
     set_error("internal: %u entries exceed the bound %lld", total, (long long)ebound);
