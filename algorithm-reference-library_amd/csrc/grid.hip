@@ -724,7 +724,7 @@ int arlmi_grid(const arlmi_plan* p, const void* d_vis, const void* d_weight, int
                void* d_grids, double* d_sumwt, int precision, void* stream) {
   int rc = check_kernel(p, k);
   if (rc) return rc;
-  if (!d_weight || !d_grids || !d_sumwt || (!dopsf && !d_vis && p->nentries > 0)) {
+  if (!d_grids || !d_sumwt || (p->nentries > 0 && (!d_weight || (!dopsf && !d_vis)))) {
     set_error("arlmi_grid: NULL buffer");
     return ARLMI_E_ARG;
   }
