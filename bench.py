@@ -248,6 +248,14 @@ def bench_batch(args):
     torch.cuda.synchronize()
     t_graph = (time.perf_counter() - a) / args.steps
     print("batch: graph %.1f ms (tuning and capture %.1f s)" % (t_graph * 1e3, t_tune), file=sys.stderr, flush=True)
+    # the same whole-step roofline as the headline line, summed over the observations (dense
+    # planes; sparse grids skip the tiles no row touches, so the measured time can approach it)
+    bw_hbm, bw_lds, fl = 8.0e12, 78.6e12, 157.3e12
+    t_roof = 0.0
+    for t in tasks:
+        n, S, P, nv = t["im"].data.shape[-1], t["gcfcf"][1].support, t["vis_slices"], t["vis"].nvis
+        t_roof += max(36.0 * nv / bw_hbm, 16.0 * S * S * nv / bw_lds) + max(32.0 * nv / bw_hbm, 8.0 * S * S * nv / bw_lds)
+        t_roof += max(108.0 * n * n * P / bw_hbm, 10.0 * n * n * numpy.log2(n * n) * P / fl) + 24.0 * nv / bw_hbm
     choices = {}
     for c in batch.choices:
         choices["%d/%d" % c] = choices.get("%d/%d" % c, 0) + 1
@@ -274,6 +282,8 @@ def bench_batch(args):
             "graph_over_eager": round(t_eager / t_graph, 3),
             "autotune_and_capture_s": round(t_tune, 1),
             "tile_chunk_choices": choices,
+            "step_roofline": {"t_roof_ms": round(t_roof * 1e3, 3), "frac": round(t_roof / t_graph, 3),
+                              "note": "dense-plane algorithmic counts, as for the headline line"},
             "per_observation": [[t["im"].data.shape[-1], t["gcfcf"][1].support, t["vis_slices"], t["vis"].nvis]
                                 for t in tasks],
         },
