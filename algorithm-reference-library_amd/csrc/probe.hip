@@ -1,0 +1,90 @@
+// LDS microbenchmark: the cost of ds_add_f64 wavefront instructions on gfx950, for the
+// gridder's access pattern and for a conflict-free one.  Used by the measurement (the peak the
+// gridder's atomic throughput is compared with), not by the imaging path.
+#include "arlmi_internal.h"
+
+namespace arlmi {
+namespace {
+
+constexpr int kProbeThreads = 512;
+constexpr int kProbeTile = 4608;  // 64 rows x 72 doubles, the gridder's accumulator tile
+
+__device__ inline unsigned lcg(unsigned& s) {
+  s = s * 1664525u + 1013904223u;
+  return s >> 8;
+}
+
+// pattern 0: each wavefront adds 1 to an 8 x 8 footprint (row stride 72) at a random origin of
+// the tile, real and imaginary planes, as the gridder does; pattern 1: lane l adds to address
+// 64 * k + l (64 distinct consecutive doubles: no bank conflict).  All addresses < 2 * 4608.
+template <int PATTERN>
+__global__ __launch_bounds__(kProbeThreads) void k_probe_lds_add(double* out, int iters, unsigned seed) {
+  extern __shared__ double t[];
+  for (int i = threadIdx.x; i < 2 * kProbeTile; i += blockDim.x) t[i] = 0.0;
+  __syncthreads();
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int ta = lane >> 3, tb = lane & 7;
+  unsigned s = seed + w * 977u + blockIdx.x * 131u;
+  for (int it = 0; it < iters; ++it) {
+    const unsigned r = __builtin_amdgcn_readfirstlane(lcg(s));
+    int o;
+    if (PATTERN == 0) {
+      const int ox = (int)(r % 57u), oy = (int)((r >> 8) % 57u);
+      o = (oy + ta) * 72 + ox + tb;
+    } else {
+      o = (int)(r % 72u) * 64 + lane;
+    }
+    __hip_atomic_fetch_add(&t[o], 1.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(&t[kProbeTile + o], 2.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) out[blockIdx.x] = t[0] + t[kProbeTile + 100];
+}
+
+}  // namespace
+}  // namespace arlmi
+
+using namespace arlmi;
+
+extern "C" {
+
+int arlmi_probe_lds_add(int32_t pattern, int32_t iters, double* cycles_per_instr, float* ms_out) {
+  if ((pattern != 0 && pattern != 1) || iters <= 0 || !cycles_per_instr) {
+    set_error("arlmi_probe_lds_add: bad argument");
+    return ARLMI_E_ARG;
+  }
+  int dev = 0, ncu = 0, clk_khz = 0;
+  ARLMI_HIP(hipGetDevice(&dev));
+  ARLMI_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  ARLMI_HIP(hipDeviceGetAttribute(&clk_khz, hipDeviceAttributeClockRate, dev));
+  const int blocks = 2 * ncu;  // two work groups of 8 wavefronts per CU
+  double* out = nullptr;
+  ARLMI_HIP(hipMalloc(&out, sizeof(double) * blocks));
+  hipEvent_t a, b;
+  ARLMI_HIP(hipEventCreate(&a));
+  ARLMI_HIP(hipEventCreate(&b));
+  const size_t sh = sizeof(double) * 2 * kProbeTile;
+  float ms = 0.f;
+  for (int rep = 0; rep < 2; ++rep) {  // the second launch is timed
+    ARLMI_HIP(hipEventRecord(a, nullptr));
+    if (pattern == 0)
+      hipLaunchKernelGGL(k_probe_lds_add<0>, dim3(blocks), dim3(kProbeThreads), sh, nullptr, out, iters, 1u);
+    else
+      hipLaunchKernelGGL(k_probe_lds_add<1>, dim3(blocks), dim3(kProbeThreads), sh, nullptr, out, iters, 1u);
+    ARLMI_LAUNCH_CHECK();
+    ARLMI_HIP(hipEventRecord(b, nullptr));
+    ARLMI_HIP(hipEventSynchronize(b));
+    ARLMI_HIP(hipEventElapsedTime(&ms, a, b));
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  (void)hipFree(out);
+  // wavefront instructions per CU: 2 groups x 8 wavefronts x iters x 2 adds
+  const double instr = 2.0 * 8.0 * iters * 2.0;
+  const double cycles = (double)ms * 1e-3 * (double)clk_khz * 1e3;
+  *cycles_per_instr = cycles / instr;
+  if (ms_out) *ms_out = ms;
+  return ARLMI_OK;
+}
+
+}  // extern "C"

@@ -61,6 +61,8 @@ def parse():
                     help="wstack: the headline line (default); batch: 64 uneven observations replayed from one "
                          "captured graph; gradient: value and gradient of ||V - g_p conj(g_q) predict(I)||^2")
     ap.add_argument("--nobs", type=int, default=64)
+    ap.add_argument("--lds-probe", action="store_true",
+                    help="time ds_add_f64 for the gridder's pattern and a conflict-free one, and exit")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
@@ -408,6 +410,19 @@ def main():
     args = parse()
     if args.copy_probe:
         copy_probe()
+        return
+    if args.lds_probe:
+        import ctypes
+        import torch
+        import arlmi
+        torch.cuda.set_device(0)
+        L = arlmi._lib.lib()
+        res = {}
+        for pattern, name in ((0, "gridder_footprints"), (1, "conflict_free")):
+            cyc, ms = ctypes.c_double(), ctypes.c_float()
+            arlmi._lib.check(L.arlmi_probe_lds_add(pattern, 4096, ctypes.byref(cyc), ctypes.byref(ms)))
+            res[name] = {"cycles_per_ds_add_f64_wave_instr_per_cu": round(cyc.value, 3), "ms": round(ms.value, 3)}
+        print(json.dumps({"lds_probe": res}))
         return
     if args.gpus != 1:
         print("bench.py measures one GPU only: every box in this setup has a single MI355X, and the "
