@@ -460,6 +460,20 @@ def main():
         roof = {"bound": "hbm", "kernel": name, "achieved": round(ach, 1), "peak": 8000.0, "unit": "GB/s",
                 "frac": round(ach / 8000.0, 4), "traffic": measured_traffic(name, N, P),
                 "bytes_per_launch": by, "avg_launch_ms": round(per_launch_ms, 4)}
+        if name == "k_grid_sep":
+            # the gridder's real bound: LDS float64 atomics, 2 wavefront instructions per entry,
+            # against the rate a microbenchmark of the same access pattern reaches on this GPU
+            import ctypes
+            import arlmi
+            cyc = ctypes.c_double()
+            arlmi._lib.check(arlmi._lib.lib().arlmi_probe_lds_add(0, 4096, ctypes.byref(cyc), None))
+            props = torch.cuda.get_device_properties(0)
+            peak = props.multi_processor_count * 2.4e9 / cyc.value
+            rate = 2.0 * E / (per_launch_ms * 1e-3)
+            roof["lds_atomics"] = {"achieved": round(rate / 1e9, 2), "peak": round(peak / 1e9, 2),
+                                   "unit": "G ds_add_f64 wavefront-instructions/s", "frac": round(rate / peak, 3),
+                                   "peak_source": "arlmi_probe_lds_add, the gridder's access pattern "
+                                                  "(%.2f cycles per instruction per CU)" % cyc.value}
     # Whole-step roofline: sum over phases of max(HBM, LDS, FLOP) time with the algorithmic
     # counts of the row and plane work (float32): gridding reads 36 B/row and does 16 S^2 B of
     # LDS read-modify-write per row; degridding reads 24 B/row, writes 8 B/row and reads 8 S^2 B
