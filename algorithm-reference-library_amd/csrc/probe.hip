@@ -16,7 +16,8 @@ __device__ inline unsigned lcg(unsigned& s) {
 
 // pattern 0: each wavefront adds 1 to an 8 x 8 footprint (row stride 72) at a random origin of
 // the tile, real and imaginary planes, as the gridder does; pattern 1: lane l adds to address
-// 64 * k + l (64 distinct consecutive doubles: no bank conflict).  All addresses < 2 * 4608.
+// 64 * k + l (64 distinct consecutive doubles: no bank conflict); pattern 2: pattern 0 with
+// 64-bit integer adds.  All addresses < 2 * 4608.
 template <int PATTERN>
 __global__ __launch_bounds__(kProbeThreads) void k_probe_lds_add(double* out, int iters, unsigned seed) {
   extern __shared__ double t[];
@@ -28,14 +29,20 @@ __global__ __launch_bounds__(kProbeThreads) void k_probe_lds_add(double* out, in
   for (int it = 0; it < iters; ++it) {
     const unsigned r = __builtin_amdgcn_readfirstlane(lcg(s));
     int o;
-    if (PATTERN == 0) {
+    if (PATTERN != 1) {
       const int ox = (int)(r % 57u), oy = (int)((r >> 8) % 57u);
       o = (oy + ta) * 72 + ox + tb;
     } else {
       o = (int)(r % 72u) * 64 + lane;
     }
-    __hip_atomic_fetch_add(&t[o], 1.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    __hip_atomic_fetch_add(&t[kProbeTile + o], 2.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (PATTERN == 2) {  // the gridder pattern with 64-bit integer adds
+      unsigned long long* u = reinterpret_cast<unsigned long long*>(t);
+      __hip_atomic_fetch_add(&u[o], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      __hip_atomic_fetch_add(&u[kProbeTile + o], 2ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    } else {
+      __hip_atomic_fetch_add(&t[o], 1.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      __hip_atomic_fetch_add(&t[kProbeTile + o], 2.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
   }
   __syncthreads();
   if (threadIdx.x == 0) out[blockIdx.x] = t[0] + t[kProbeTile + 100];
@@ -49,7 +56,7 @@ using namespace arlmi;
 extern "C" {
 
 int arlmi_probe_lds_add(int32_t pattern, int32_t iters, double* cycles_per_instr, float* ms_out) {
-  if ((pattern != 0 && pattern != 1) || iters <= 0 || !cycles_per_instr) {
+  if (pattern < 0 || pattern > 2 || iters <= 0 || !cycles_per_instr) {
     set_error("arlmi_probe_lds_add: bad argument");
     return ARLMI_E_ARG;
   }
@@ -69,8 +76,10 @@ int arlmi_probe_lds_add(int32_t pattern, int32_t iters, double* cycles_per_instr
     ARLMI_HIP(hipEventRecord(a, nullptr));
     if (pattern == 0)
       hipLaunchKernelGGL(k_probe_lds_add<0>, dim3(blocks), dim3(kProbeThreads), sh, nullptr, out, iters, 1u);
-    else
+    else if (pattern == 1)
       hipLaunchKernelGGL(k_probe_lds_add<1>, dim3(blocks), dim3(kProbeThreads), sh, nullptr, out, iters, 1u);
+    else
+      hipLaunchKernelGGL(k_probe_lds_add<2>, dim3(blocks), dim3(kProbeThreads), sh, nullptr, out, iters, 1u);
     ARLMI_LAUNCH_CHECK();
     ARLMI_HIP(hipEventRecord(b, nullptr));
     ARLMI_HIP(hipEventSynchronize(b));

@@ -290,10 +290,11 @@ int arlmi_apply_gains_rows_adjoint(int64_t nrows, const int32_t* d_gidx1, const 
                                    int64_t ngain, const void* d_in, const void* d_grad_out, void* d_grad_in,
                                    void* d_grad_gain, int precision, void* stream);
 
-/* LDS microbenchmark (no ARL counterpart; measurement only): cycles per ds_add_f64 wavefront
- * instruction per CU for pattern 0 (the gridder's 8x8 footprints at random origins of a 64x72
- * tile) or 1 (64 distinct consecutive addresses), two work groups of 8 wavefronts per CU, the
- * device clock taken from hipDeviceAttributeClockRate.  Synchronises the device. */
+/* LDS microbenchmark (no ARL counterpart; measurement only): cycles per LDS-add wavefront
+ * instruction per CU for pattern 0 (ds_add_f64 on the gridder's 8x8 footprints at random
+ * origins of a 64x72 tile), 1 (ds_add_f64 on 64 distinct consecutive addresses) or 2 (pattern 0
+ * with 64-bit integer adds), two work groups of 8 wavefronts per CU, the device clock taken from
+ * hipDeviceAttributeClockRate.  Synchronises the device. */
 int arlmi_probe_lds_add(int32_t pattern, int32_t iters, double* cycles_per_instr, float* ms_out);
 
 /* Opt-in kernel timing: with profiling on, every library kernel launch is bracketed by two
