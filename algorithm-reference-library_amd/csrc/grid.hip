@@ -488,37 +488,93 @@ __global__ __launch_bounds__(degrid_threads<R>()) void k_degrid_sep(const int4* 
   }
 }
 
-template <typename R>
-__global__ __launch_bounds__(degrid_threads<R>()) void k_degrid_tab(const int4* __restrict__ items, const uint2* __restrict__ entries,
-                                                    const uint16_t* __restrict__ entry_z,
-                                                    const cplx<R>* __restrict__ grids, const cplx<R>* __restrict__ tab,
-                                                    int os, int S, cplx<R>* __restrict__ vis_out, int T, int ntx,
-                                                    int N) {
+// DPP moves (0 where a row is masked off) of a 32-bit or 64-bit value within the wavefront.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_move(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xf, false));
+}
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_move(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, ROW_MASK, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW_MASK, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Sum over each group of L consecutive lanes (L a power of two, 2..64), all lanes active: quad
+// permutations, half-row and row mirrors, then row broadcasts.  Every lane of a group holds the
+// sum for L <= 16; for L = 32 and 64 the group's last lane does.
+template <int L, typename T>
+__device__ __forceinline__ T group_sum(T v) {
+  if (L >= 2) v += dpp_move<0xb1, 0xf>(v);   // quad_perm [1,0,3,2]
+  if (L >= 4) v += dpp_move<0x4e, 0xf>(v);   // quad_perm [2,3,0,1]
+  if (L >= 8) v += dpp_move<0x141, 0xf>(v);  // row_half_mirror
+  if (L >= 16) v += dpp_move<0x140, 0xf>(v); // row_mirror
+  if (L >= 32) v += dpp_move<0x142, 0xa>(v); // row_bcast:15 into rows 1 and 3
+  if (L >= 64) v += dpp_move<0x143, 0xc>(v); // row_bcast:31 into rows 2 and 3
+  return v;
+}
+
+constexpr int DEGRID_TAB_THREADS = 256;
+
+// Degridding with a full complex kernel table (AW / any CF).  A group of L lanes (L the power of
+// two >= S) handles one entry, one footprint column per lane: at step b the group reads row b of
+// the entry's kernel slice and row fy + b of the LDS tile + halo, S consecutive values each (so
+// the table reads coalesce and the LDS reads are consecutive), and at the end it sums its
+// columns with DPP; 64 / L entries per wavefront step.
+template <typename R, int L>
+__global__ __launch_bounds__(DEGRID_TAB_THREADS) void k_degrid_tab(const int4* __restrict__ items,
+                                                                  const uint2* __restrict__ entries,
+                                                                  const uint16_t* __restrict__ entry_z,
+                                                                  const cplx<R>* __restrict__ grids,
+                                                                  const cplx<R>* __restrict__ tab, int os, int S,
+                                                                  cplx<R>* __restrict__ vis_out, int T, int ntx,
+                                                                  int N) {
   extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int E = 64 / L;  // entries per wavefront step
   const int Wd = T + S;
   R* lds = reinterpret_cast<R*>(smem);
   const int4 it = items[blockIdx.x];
   const TilePos tp = tile_pos(it.x, ntx);
   load_halo<R>(grids, lds, Wd, T, N, tp);
   __syncthreads();
-  for (int e = threadIdx.x; e < it.z; e += blockDim.x) {
-    const uint2 en = entries[it.y + e];
-    const uint32_t p = en.y;
-    const int fx = entry_lx(p) - S, fy = entry_ly(p) - S, xo = entry_xoff(p), yo = entry_yoff(p);
-    const int z = entry_z ? (int)entry_z[it.y + e] : 0;
-    const cplx<R>* k = tab + ((size_t)(z * os + yo) * os + xo) * S * S;
-    const R* base = lds + 2 * (fy * Wd + fx);
-    R ar = R(0), ai = R(0);
-    for (int a = 0; a < S; ++a) {
-      const R* row = base + 2 * a * Wd;
-      for (int b = 0; b < S; ++b) {
-        const cplx<R> c = k[a * S + b];
-        const R gr = row[2 * b], gi = row[2 * b + 1];
-        ar += gr * c.x - gi * c.y;
-        ai += gr * c.y + gi * c.x;
-      }
+  const int lane = __lane_id(), wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+  const int slot = lane / L, a = lane % L;
+  const bool col_ok = a < S;
+  const int ac = col_ok ? a : 0;  // lanes past the footprint read column 0 and contribute 0
+  const int per = __builtin_amdgcn_readfirstlane((it.z + nwave - 1) / nwave);
+  const int e0 = min(it.z, wave * per), e1 = min(it.z, e0 + per);
+  for (int base = e0; base < e1; base += E) {
+    const int e = base + slot;
+    const bool ok = e < e1;
+    uint32_t row = 0, p = 0;
+    int z = 0;
+    if (ok) {
+      const uint2 en = entries[it.y + e];
+      row = en.x;
+      p = en.y;
+      if (entry_z) z = (int)entry_z[it.y + e];
     }
-    vis_out[en.x] = mk<R>(ar, ai);
+    // a slot past the end works on the tile origin with sub-sample 0 (in range) and stores nothing
+    const int fx = ok ? entry_lx(p) - S : 0, fy = ok ? entry_ly(p) - S : 0;
+    const int xo = ok ? entry_xoff(p) : 0, yo = ok ? entry_yoff(p) : 0;
+    const cplx<R>* k = tab + ((size_t)(z * os + yo) * os + xo) * S * S + ac;
+    const R* g = lds + 2 * (fy * Wd + fx + ac);
+    R sr = R(0), si = R(0);
+#pragma unroll 4
+    for (int b = 0; b < S; ++b) {
+      const cplx<R> c = k[b * S];
+      const R gr = g[2 * b * Wd], gi = g[2 * b * Wd + 1];
+      sr += gr * c.x - gi * c.y;
+      si += gr * c.y + gi * c.x;
+    }
+    if (!col_ok) {
+      sr = R(0);
+      si = R(0);
+    }
+    sr = group_sum<L>(sr);
+    si = group_sum<L>(si);
+    if (ok && a == L - 1) vis_out[row] = mk<R>(sr, si);
   }
 }
 
@@ -685,10 +741,26 @@ static int degrid_impl(const arlmi_plan* p, const void* grids, const arlmi_kerne
     return ARLMI_E_UNSUPPORTED;
   }
   if (p->nditems == 0) return ARLMI_OK;
-  ARLMI_LAUNCH("k_degrid_tab", k_degrid_tab<R>, dim3((unsigned)p->nditems), dim3(degrid_threads<R>()), sh, s, p->d_ditems,
-               p->d_entries,
-                     g.nz > 1 ? p->d_entry_z : nullptr, (const cplx<R>*)grids, (const cplx<R>*)k->d_table, g.os,
-                     g.support, (cplx<R>*)vis_out, g.tile, g.ntx, g.n);
+  const uint16_t* ez = g.nz > 1 ? p->d_entry_z : nullptr;
+  const cplx<R>* tab = (const cplx<R>*)k->d_table;
+#define ARLMI_DEGRID_TAB(L)                                                                                       \
+  ARLMI_LAUNCH("k_degrid_tab", (k_degrid_tab<R, L>), dim3((unsigned)p->nditems), dim3(DEGRID_TAB_THREADS), sh, s, \
+               p->d_ditems, p->d_entries, ez, (const cplx<R>*)grids, tab, g.os, g.support, (cplx<R>*)vis_out,    \
+               g.tile, g.ntx, g.n)
+  const int S = g.support;
+  if (S <= 2)
+    ARLMI_DEGRID_TAB(2);
+  else if (S <= 4)
+    ARLMI_DEGRID_TAB(4);
+  else if (S <= 8)
+    ARLMI_DEGRID_TAB(8);
+  else if (S <= 16)
+    ARLMI_DEGRID_TAB(16);
+  else if (S <= 32)
+    ARLMI_DEGRID_TAB(32);
+  else
+    ARLMI_DEGRID_TAB(64);
+#undef ARLMI_DEGRID_TAB
   ARLMI_LAUNCH_CHECK();
   return ARLMI_OK;
 }

@@ -57,9 +57,11 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=1000000)
-    ap.add_argument("--workload", choices=("wstack", "batch", "gradient"), default="wstack",
-                    help="wstack: the headline line (default); batch: 64 uneven observations replayed from one "
+    ap.add_argument("--workload", choices=("wstack", "wstack-aw", "batch", "gradient"), default="wstack",
+                    help="wstack: the headline line (default); wstack-aw: the same with an AW w-projection kernel "
+                         "for the residual w of each slab; batch: 64 uneven observations replayed from one "
                          "captured graph; gradient: value and gradient of ||V - g_p conj(g_q) predict(I)||^2")
+    ap.add_argument("--support", type=int, default=16, help="AW kernel support (wstack-aw)")
     ap.add_argument("--nobs", type=int, default=64)
     ap.add_argument("--lds-probe", action="store_true",
                     help="time ds_add_f64 for the gridder's pattern and a conflict-free one, and exit")
@@ -110,9 +112,18 @@ def time_form(precision, args, dev):
     import torch
     import arlmi
     from arlmi import simulation
-    vis, im, comps = simulation.config2(precision=precision, npixel=args.npixel)
+    support = args.support if args.workload == "wstack-aw" else 8
+    vis, im, comps = simulation.config2(precision=precision, npixel=args.npixel, support=support)
     torch.cuda.synchronize()
-    gcf, cf = arlmi.create_pswf_convolutionfunction(im, oversampling=8, support=8)
+    if args.workload == "wstack-aw":
+        # AW kernel for the residual w of each slab: residuals stay within about one slab width of
+        # the slab mean, covered by 11 kernel planes a quarter of a slab width apart
+        wmax = float(vis.data["uvw"][:, 2].abs().max())
+        width = 2.0 * wmax / max(args.slices - 1, 1)
+        gcf, cf = arlmi.create_awterm_convolutionfunction(im, nw=11, wstep=width / 4.0, oversampling=8,
+                                                          support=args.support)
+    else:
+        gcf, cf = arlmi.create_pswf_convolutionfunction(im, oversampling=8, support=8)
     imager = arlmi.Imager(vis, im, gcfcf=(gcf, cf), mode="wstack", vis_slices=args.slices)
     # model image: the sources at their nearest pixels ('Nearest' insertion,
     # skycomponent/operations.py:441-444)
@@ -433,6 +444,10 @@ def main():
         torch.cuda.set_device(0)
         bench_batch(args)
         return
+    if args.workload == "wstack-aw":
+        global METRIC
+        METRIC = ("Mvis/s invert+predict at 2048² w-stack + AW w-projection (S=%d, os=8), FP32 vs FP64 GPU"
+                  % args.support)
     if args.workload == "gradient":
         import torch
         torch.cuda.set_device(0)
@@ -480,11 +495,12 @@ def main():
     # of LDS per row; the plane work (grid write-back, two-pass FFTs, screens, gcf, model) moves
     # 108 N^2 B per plane and runs 2 x 5 N^2 log2(N^2) FFT flops per plane; taps cost 4 S^2 flops
     # per row and direction.
-    S = 8
+    S = 8 if args.workload == "wstack" else args.support
+    cflops = 4.0 if args.workload == "wstack" else 8.0  # real separable vs complex kernel, per tap
     bw_hbm, bw_lds, fl = 8.0e12, 78.6e12, 157.3e12
     nm = r32["stats"]["nmember"]
-    t_grid = max(36.0 * nm / bw_hbm, 16.0 * S * S * nm / bw_lds, 4.0 * S * S * nm / fl)
-    t_degrid = max(32.0 * nm / bw_hbm, 8.0 * S * S * nm / bw_lds, 4.0 * S * S * nm / fl)
+    t_grid = max(36.0 * nm / bw_hbm, 16.0 * S * S * nm / bw_lds, cflops * S * S * nm / fl)
+    t_degrid = max(32.0 * nm / bw_hbm, 8.0 * S * S * nm / bw_lds, cflops * S * S * nm / fl)
     t_planes = max(108.0 * N * N * P / bw_hbm, 2 * 5.0 * N * N * numpy.log2(N * N) * P / fl)
     t_plan = 24.0 * rows / bw_hbm  # the mapping reads uvw once
     t_roof = t_grid + t_degrid + t_planes + t_plan
@@ -538,8 +554,10 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "w-stacked wide field: 256 antennas log-radial to 10 km, 8 channels 100-150 MHz, "
-                        "120 integrations, 200 power-law sources; %d^2 image, cell 1.5e-4 rad, %d w slabs, "
-                        "PSWF S=8 os=8" % (N, P),
+                        "120 integrations, 200 power-law sources; %d^2 image, cell 1.5e-4 rad, %d w slabs, %s"
+                        % (N, P, "PSWF S=8 os=8" if args.workload == "wstack" else
+                           "AW w-projection kernel for the residual w of each slab, 11 planes, S=%d os=8"
+                           % args.support),
             "rows": rows,
             "rows_generated_before_uv_selection": r32["rows_total"],
             "baseline_times": 32640 * 120,

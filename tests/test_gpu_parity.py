@@ -332,3 +332,30 @@ def test_device_slab_boxes_match_numpy(gpu, P):
         assert numpy.array_equal(boxes, rb), (P, wmax)
         assert width == rw, (P, wmax)
         plan.close()
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_wstack_with_wprojection_parity(gpu, precision):
+    """The hybrid of configuration 2: w slabs (screens at the slab mean) with an AW kernel for the
+    residual w within each slab (the kernel's w plane from w - mean, gridding.py:70-76 on the
+    shifted rows of wstack_single.py:83-90), against the oracle's list workflow."""
+    import arlmi
+    rng = numpy.random.default_rng(21)
+    n, cell, S, P = 256, 5e-4, 16, 5
+    d = random_rows(rng, 12000, n, cell, S, wmax=150.0)
+    im = device_image(n, cell, precision)
+    gcf, cf = arlmi.create_awterm_convolutionfunction(im, nw=9, wstep=20.0, oversampling=4, support=S)
+    g = oracle_geom(n, cell)
+    ogcf, ocf, cfg = O.create_awterm_convolutionfunction(g, nw=9, wstep=20.0, oversampling=4, support=S)
+    vis = to_device_vis(d, precision)
+    (img, sw), = arlmi.invert_list_serial_workflow([vis], [im], context="wstack", vis_slices=P, gcfcf=[(gcf, cf)])
+    rimg, rsw = O.invert_list_wstack(d, g, ogcf, ocf, cfg, P)
+    assert_image(img.numpy(), rimg, precision)
+    assert abs(sw[0, 0] - rsw[0, 0]) < 1e-9 * rsw[0, 0]
+    model = numpy.zeros((1, 1, n, n))
+    model[0, 0, 120, 100] = 1.0
+    model[0, 0, 150, 160] = 0.5
+    mim = device_image(n, cell, precision, data=model)
+    pv, = arlmi.predict_list_serial_workflow([vis], [mim], context="wstack", vis_slices=P, gcfcf=[(gcf, cf)])
+    rpv = O.predict_list_wstack(d, model, g, ogcf, ocf, cfg, P)
+    assert rel_err(pv.vis.cpu().numpy(), rpv) < TOL[precision]
