@@ -7,7 +7,7 @@ PyTorch-ROCm tensors holding device memory.
 
 from . import _lib
 from .data_models import (WCS, PolarisationFrame, Image, GridData, ConvolutionFunction, Visibility, BlockVisibility,
-                          GainTable, Skycomponent)
+                          GainTable, Skycomponent, SkyCoord)
 from .kernels import (create_image, create_convolutionfunction_from_image, create_pswf_convolutionfunction,
                       create_box_convolutionfunction, create_awterm_convolutionfunction,
                       apply_bounding_box_convolutionfunction, calculate_bounding_box_convolutionfunction)
@@ -25,6 +25,7 @@ from .calibration import (create_gaintable_from_blockvisibility, apply_gaintable
                           apply_gains_rows_adjoint)
 from .differentiable import PredictFunction, GainsFunction, predict_with_gains, visibility_loss
 from .batch import Autotuner, BatchImager
+from .phaserotate import skycoord_to_lmn, phaserotate_visibility
 from .weighting import (grid_weight_to_griddata, griddata_merge_weights, griddata_reweight, weight_visibility,
                         weight_list_serial_workflow)
 

@@ -23,7 +23,9 @@ FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
     # packed FP32 instructions are kept out of every kernel
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-ffp-contract=fast", "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+    # multiply-add fusion everywhere except inside `#pragma clang fp contract(off)` scopes (the
+    # numpy-order products of the visibility kernels); plain "fast" would ignore the pragmas
+    "-ffp-contract=fast-honor-pragmas", "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
     "-I", INCLUDE, "-I", CSRC,
 ]
 

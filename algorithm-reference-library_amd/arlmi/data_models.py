@@ -350,6 +350,31 @@ class BlockVisibility:
     def uvw(self):
         return self.data["uvw"]
 
+    def copy(self, zero=False):
+        """copy_visibility (visibility/base.py:33-47) for the block layout."""
+        new = BlockVisibility({k: v.clone() for k, v in self.data.items()}, frequency=self.frequency,
+                              channel_bandwidth=self.channel_bandwidth, polarisation_frame=self.polarisation_frame,
+                              phasecentre=self.phasecentre)
+        if zero:
+            new.data["vis"].zero_()
+        return new
+
+
+class SkyCoord:
+    """An ICRS direction (ra, dec) in radians: the part of astropy's SkyCoord that phase centres
+    use here (astropy is not a dependency).  ``ra`` and ``dec`` are floats."""
+
+    def __init__(self, ra, dec):
+        self.ra = float(ra)
+        self.dec = float(dec)
+
+    @classmethod
+    def from_degrees(cls, ra, dec):
+        return cls(numpy.deg2rad(ra), numpy.deg2rad(dec))
+
+    def __repr__(self):
+        return "SkyCoord(ra=%r, dec=%r)" % (self.ra, self.dec)
+
 
 class GainTable:
     """Scalar antenna gains (memory_data_models.py:132-234): gain [ntimes, nants, nchan, 1, 1]

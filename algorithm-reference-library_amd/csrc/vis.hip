@@ -207,6 +207,124 @@ int prec_check(int precision) {
   return ARLMI_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Phase rotation (visibility/base.py:295-381)
+// ---------------------------------------------------------------------------
+
+constexpr double kRotMinusTwoPi = -2.0 * 3.14159265358979323846264338327950288;  // -2 * numpy.pi
+
+struct RotArgs {
+  double l, m, n1;  // direction cosines of the new phase centre, n1 = sqrt(1 - l^2 - m^2) - 1
+  int inverse;      // multiply by the phasor instead of its conjugate
+  int rotate;       // tangent = False: rotate uvw
+  double ch0, sh0, cd0, sd0;  // cos / sin of ha0 = -ra0 and dec0 (old phase centre)
+  double ch1, sh1, cd1, sd1;  // the same for the new phase centre
+};
+
+// exp(-2 pi i (u l + v m + w n1)) as (cos, sin) of (-2 pi) * d: numpy's -2j * pi * dot(uvw, s)
+// has real part -0 * d and imaginary part (-2 pi) * d, and exp of (0, phi) is (cos, sin)(phi).
+__device__ __forceinline__ double2 rot_phasor(const RotArgs& a, double u, double v, double w) {
+#pragma clang fp contract(off)
+  const double d = u * a.l + v * a.m + w * a.n1;
+  double sn, cs;
+  sincos(kRotMinusTwoPi * d, &sn, &cs);
+  return make_double2(cs, sn);
+}
+
+// vis *= conj(phasor) (or phasor), in complex128 as numpy's in-place product with a complex64
+// or complex128 left operand computes it, rounded once to the storage precision.
+template <typename R>
+__device__ __forceinline__ void rot_apply(cplx<R>* __restrict__ vis, int npol, double2 ph, int inverse) {
+  const double2 f = make_double2(ph.x, inverse ? ph.y : -ph.y);
+  for (int p = 0; p < npol; ++p) {
+    const cplx<R> v = vis[p];
+    const double2 r = cmul(make_double2((double)v.x, (double)v.y), f);
+    cplx<R> o;
+    o.x = (R)r.x;
+    o.y = (R)r.y;
+    vis[p] = o;
+  }
+}
+
+// xyz = uvw_to_xyz(uvw, ha0, dec0), uvw = xyz_to_uvw(xyz, ha1, dec1) (coordinate_support.py:59-113),
+// operation for operation.
+__device__ __forceinline__ void rot_uvw(const RotArgs& a, double& u, double& v, double& w) {
+#pragma clang fp contract(off)
+  const double v0 = v * a.sd0 - w * a.cd0;
+  const double z = v * a.cd0 + w * a.sd0;
+  const double x = u * a.ch0 + v0 * a.sh0;
+  const double y = -u * a.sh0 + v0 * a.ch0;
+  const double u1 = x * a.ch1 - y * a.sh1;
+  const double q0 = x * a.sh1 + y * a.ch1;
+  w = z * a.sd1 - q0 * a.cd1;
+  v = z * a.cd1 + q0 * a.sd1;
+  u = u1;
+}
+
+// One thread per row: the phasor from the row's uvw before rotation, then the rotated uvw.
+template <typename R>
+__global__ void __launch_bounds__(kVisThreads)
+    k_rotate_rows(double* __restrict__ uvw, cplx<R>* __restrict__ vis, int npol, RotArgs a, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += stride) {
+    double u = uvw[3 * r], v = uvw[3 * r + 1], w = uvw[3 * r + 2];
+    rot_apply<R>(vis + r * npol, npol, rot_phasor(a, u, v, w), a.inverse);
+    if (a.rotate) {
+      rot_uvw(a, u, v, w);
+      uvw[3 * r] = u;
+      uvw[3 * r + 1] = v;
+      uvw[3 * r + 2] = w;
+    }
+  }
+}
+
+// One thread per (block element, channel): uvw_m * (f / c) per component, then the phasor.
+template <typename R>
+__global__ void __launch_bounds__(kVisThreads)
+    k_rotate_blocks(const double* __restrict__ uvw, cplx<R>* __restrict__ vis, int nchan, int npol,
+                    const double* __restrict__ freq, RotArgs a, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += stride) {
+    const int64_t b = e / nchan;
+    const int ch = (int)(e - b * nchan);
+    const double k = freq[ch] / kLightSpeed;
+    const double u = uvw[3 * b] * k, v = uvw[3 * b + 1] * k, w = uvw[3 * b + 2] * k;
+    rot_apply<R>(vis + e * npol, npol, rot_phasor(a, u, v, w), a.inverse);
+  }
+}
+
+// In-place uvw rotation of the block elements (after every phasor has read the old uvw).
+__global__ void __launch_bounds__(kVisThreads) k_rotate_uvw(double* __restrict__ uvw, RotArgs a, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; b < n; b += stride) {
+    double u = uvw[3 * b], v = uvw[3 * b + 1], w = uvw[3 * b + 2];
+    rot_uvw(a, u, v, w);
+    uvw[3 * b] = u;
+    uvw[3 * b + 1] = v;
+    uvw[3 * b + 2] = w;
+  }
+}
+
+RotArgs rot_args(const double* lmn, const double* trig, int inverse) {
+  RotArgs a{};
+  a.l = lmn[0];
+  a.m = lmn[1];
+  a.n1 = lmn[2];
+  a.inverse = inverse ? 1 : 0;
+  a.rotate = trig ? 1 : 0;
+  if (trig) {
+    a.ch0 = trig[0];
+    a.sh0 = trig[1];
+    a.cd0 = trig[2];
+    a.sd0 = trig[3];
+    a.ch1 = trig[4];
+    a.sh1 = trig[5];
+    a.cd1 = trig[6];
+    a.sd1 = trig[7];
+  }
+  return a;
+}
+
 unsigned grid_for(int64_t n, int64_t cap) {
   const int64_t b = (n + kVisThreads - 1) / kVisThreads;
   return (unsigned)(b < cap ? b : cap);
@@ -328,6 +446,59 @@ int arlmi_apply_gains_rows_adjoint(int64_t nrows, const int32_t* d_gidx1, const 
                  gg);
   }
   ARLMI_LAUNCH_CHECK();
+  return ARLMI_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int arlmi_phaserotate_rows(double* d_uvw, void* d_vis, int64_t nrows, int32_t npol, const double* h_lmn,
+                           const double* h_trig, int inverse, int precision, void* stream) {
+  if (int rc = prec_check(precision)) return rc;
+  if (nrows < 0 || npol < 1 || !h_lmn || (nrows > 0 && (!d_uvw || !d_vis))) {
+    set_error("arlmi_phaserotate_rows: bad argument");
+    return ARLMI_E_ARG;
+  }
+  if (nrows == 0) return ARLMI_OK;
+  hipStream_t s = as_stream(stream);
+  const RotArgs a = rot_args(h_lmn, h_trig, inverse);
+  const unsigned g = grid_for(nrows, 1 << 16);
+  if (precision == 32)
+    ARLMI_LAUNCH("k_rotate_rows", k_rotate_rows<float>, dim3(g), dim3(kVisThreads), 0, s, d_uvw, (float2*)d_vis, npol,
+                 a, nrows);
+  else
+    ARLMI_LAUNCH("k_rotate_rows", k_rotate_rows<double>, dim3(g), dim3(kVisThreads), 0, s, d_uvw, (double2*)d_vis,
+                 npol, a, nrows);
+  ARLMI_LAUNCH_CHECK();
+  return ARLMI_OK;
+}
+
+int arlmi_phaserotate_blocks(double* d_uvw, void* d_vis, int64_t nblock, int32_t nchan, int32_t npol,
+                             const double* d_frequency, const double* h_lmn, const double* h_trig, int inverse,
+                             int precision, void* stream) {
+  if (int rc = prec_check(precision)) return rc;
+  if (nblock < 0 || nchan < 1 || npol < 1 || !h_lmn || (nblock > 0 && (!d_uvw || !d_vis || !d_frequency))) {
+    set_error("arlmi_phaserotate_blocks: bad argument");
+    return ARLMI_E_ARG;
+  }
+  if (nblock == 0) return ARLMI_OK;
+  hipStream_t s = as_stream(stream);
+  const RotArgs a = rot_args(h_lmn, h_trig, inverse);
+  const int64_t n = nblock * nchan;
+  const unsigned g = grid_for(n, 1 << 16);
+  if (precision == 32)
+    ARLMI_LAUNCH("k_rotate_blocks", k_rotate_blocks<float>, dim3(g), dim3(kVisThreads), 0, s, d_uvw, (float2*)d_vis,
+                 nchan, npol, d_frequency, a, n);
+  else
+    ARLMI_LAUNCH("k_rotate_blocks", k_rotate_blocks<double>, dim3(g), dim3(kVisThreads), 0, s, d_uvw,
+                 (double2*)d_vis, nchan, npol, d_frequency, a, n);
+  ARLMI_LAUNCH_CHECK();
+  if (h_trig) {
+    ARLMI_LAUNCH("k_rotate_uvw", k_rotate_uvw, dim3(grid_for(nblock, 1 << 16)), dim3(kVisThreads), 0, s, d_uvw, a,
+                 nblock);
+    ARLMI_LAUNCH_CHECK();
+  }
   return ARLMI_OK;
 }
 

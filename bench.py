@@ -57,10 +57,12 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=1000000)
-    ap.add_argument("--workload", choices=("wstack", "wstack-aw", "batch", "gradient"), default="wstack",
+    ap.add_argument("--workload", choices=("wstack", "wstack-aw", "batch", "gradient", "phaserotate"),
+                    default="wstack",
                     help="wstack: the headline line (default); wstack-aw: the same with an AW w-projection kernel "
                          "for the residual w of each slab; batch: 64 uneven observations replayed from one "
-                         "captured graph; gradient: value and gradient of ||V - g_p conj(g_q) predict(I)||^2")
+                         "captured graph; gradient: value and gradient of ||V - g_p conj(g_q) predict(I)||^2; "
+                         "phaserotate: phase rotation (tangent=False) of the w-stack workload's rows")
     ap.add_argument("--support", type=int, default=16, help="AW kernel support (wstack-aw)")
     ap.add_argument("--nobs", type=int, default=64)
     ap.add_argument("--lds-probe", action="store_true",
@@ -417,6 +419,106 @@ def bench_gradient(args):
     print(json.dumps(out))
 
 
+def bench_phaserotate(args):
+    """phaserotate_visibility with tangent=False (phasor and uvw rotation) over the rows of the
+    w-stacked 2048^2 workload, float32 visibilities: one step = one in-place rotation of every
+    row through arlmi_phaserotate_rows (steps alternate between two phase centres 2 degrees
+    apart, so the data stay bounded).  ARL's copy_visibility before the rotation is not timed."""
+    import ctypes
+    import numpy
+    import torch
+    import arlmi
+    from arlmi import simulation
+    from arlmi.phaserotate import skycoord_to_lmn
+    from arlmi.plan import ptr, stream_handle
+    D = numpy.pi / 180.0
+    vis, im, _ = simulation.config2(precision=32, sky=False)
+    rows = vis.nvis
+    g = torch.Generator(device="cuda").manual_seed(7)
+    vis.data["vis"] = torch.complex(torch.randn(rows, 1, device="cuda", generator=g),
+                                    torch.randn(rows, 1, device="cuda", generator=g)).contiguous()
+    a, b = (0.0, -30.0 * D), (2.0 * D, -31.5 * D)
+
+    def scalars(old, new):
+        l, m, _ = skycoord_to_lmn(arlmi.SkyCoord(*new), arlmi.SkyCoord(*old))
+        lmn = numpy.array([l, m, numpy.sqrt(1 - l ** 2 - m ** 2) - 1.0])
+        trig = numpy.array([numpy.cos(-old[0]), numpy.sin(-old[0]), numpy.cos(old[1]), numpy.sin(old[1]),
+                            numpy.cos(-new[0]), numpy.sin(-new[0]), numpy.cos(new[1]), numpy.sin(new[1])])
+        return lmn, trig
+
+    sc = [scalars(a, b), scalars(b, a)]
+    L = arlmi._lib.lib()
+    uvw, vd = vis.data["uvw"], vis.data["vis"]
+
+    def one(k):
+        lmn, trig = sc[k & 1]
+        arlmi._lib.check(L.arlmi_phaserotate_rows(ptr(uvw), ptr(vd), rows, 1, lmn.ctypes.data_as(ctypes.c_void_p),
+                                                  trig.ctypes.data_as(ctypes.c_void_p), 0, 32, stream_handle()))
+
+    steps = args.steps + (args.steps & 1)  # even: the data end where they started
+    for k in range(args.warmup * 2):
+        one(k)
+    torch.cuda.synchronize()
+    L.arlmi_profile_reset()
+    L.arlmi_profile_enable(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        one(k)
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / steps
+    L.arlmi_profile_enable(0)
+    prof = arlmi._lib.profile_report()
+    kms, kn = prof["k_rotate_rows"]
+    kavg = kms / kn * 1e-3
+    per_row = 24 + 24 + 8 + 8  # uvw read + write (float64), vis read + write (complex64)
+    achieved = rows * per_row / kavg / 1e9
+    # CPU baseline: the oracle (numpy, as ARL computes it) on a bounded sample of the same rows
+    sys.path.insert(0, ROOT)
+    from oracle import arl_oracle as O
+    m = min(rows, 8_000_000)
+    idx = torch.arange(0, m, device="cuda") * (rows // m)
+    sample = {"uvw": uvw[idx].cpu().numpy(), "vis": vd[idx].cpu().numpy().astype(numpy.complex128)}
+    c0 = time.perf_counter()
+    ruvw, rvis = O.phaserotate_rows(sample, a, b, tangent=False)
+    c1 = time.perf_counter()
+    # parity on the sample: the GPU's rotation of the same rows
+    gv = arlmi.Visibility({"uvw": uvw[idx].contiguous(), "vis": vd[idx].contiguous()}, phasecentre=arlmi.SkyCoord(*a))
+    out = arlmi.phaserotate_visibility(gv, arlmi.SkyCoord(*b), tangent=False)
+    uvw_exact = bool(numpy.array_equal(out.data["uvw"].cpu().numpy(), ruvw))
+    verr = float(numpy.max(numpy.abs(out.vis.cpu().numpy() - rvis)) / numpy.max(numpy.abs(rvis)))
+    res = {
+        "metric": "Mvis/s phaserotate_visibility (tangent=False: phasor + uvw rotation), rows of the 2048^2 "
+                  "w-stack workload",
+        "value": round(rows / t / 1e6, 1),
+        "unit": "Mvis/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup * 2,
+        "ms_per_step": round(t * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 phase, c64 vis",
+        "data": "synthetic",
+        "config": {
+            "workload": "256 antennas log-radial to 10 km, 8 channels 100-150 MHz, 120 integrations (rows kept "
+                        "for the 2048^2 grid), random complex64 vis, rotation between (0, -30) and (2, -31.5) deg",
+            "rows": rows,
+            "bytes_per_row": per_row,
+            "sample_uvw_bit_exact_vs_oracle": uvw_exact,
+            "sample_vis_max_err_rel_to_peak": verr,
+        },
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
+                     "frac": round(achieved / 8000.0, 4), "traffic": None,
+                     "kernel": "k_rotate_rows", "kernel_ms": round(kavg * 1e3, 4)},
+        "cpu_baseline": {"value": round(m / (c1 - c0) / 1e6, 3), "unit": "Mvis/s", "cores": 1, "kind": "port",
+                         "sample": "%d rows (every %d-th row), oracle phaserotate_rows in numpy, %.2f s"
+                                   % (m, rows // m, c1 - c0)},
+    }
+    print(json.dumps(res))
+
+
 def main():
     args = parse()
     if args.copy_probe:
@@ -452,6 +554,11 @@ def main():
         import torch
         torch.cuda.set_device(0)
         bench_gradient(args)
+        return
+    if args.workload == "phaserotate":
+        import torch
+        torch.cuda.set_device(0)
+        bench_phaserotate(args)
         return
     import numpy
     import torch

@@ -290,6 +290,32 @@ int arlmi_apply_gains_rows_adjoint(int64_t nrows, const int32_t* d_gidx1, const 
                                    int64_t ngain, const void* d_in, const void* d_grad_out, void* d_grad_in,
                                    void* d_grad_gain, int precision, void* stream);
 
+/* phaserotate_visibility (processing_components/visibility/base.py:295-381), in place on the
+ * caller's copy (ARL rotates a copy_visibility; the caller copies first and handles the
+ * |n| < 1e-15 early return, :312-313).  h_lmn = {l, m, sqrt(1 - l^2 - m^2) - 1} of the new
+ * phase centre relative to the old one (skycoord_to_lmn, coordinate_support.py:150-169), all
+ * computed by the caller.  With the uvw held before any rotation, per element
+ *   d = u l + v m + w n1,   phasor = exp(-2 pi i d) = cos(-2 pi d) + i sin(-2 pi d)
+ * (simulate_point, coordinate_support.py:192-212, evaluated without contraction in that order),
+ *   vis *= conj(phasor)  (inverse = 0)  or  vis *= phasor  (inverse = 1),
+ * the complex product in float64 (numpy's complex128 order) and stored in the vis precision.
+ * h_trig != NULL (tangent = False) then rotates uvw in place (:329-339):
+ *   xyz = uvw_to_xyz(uvw, ha0, dec0),  uvw = xyz_to_uvw(xyz, ha1, dec1)
+ * (coordinate_support.py:59-113, numpy's operation order) with h_trig = {cos ha0, sin ha0,
+ * cos dec0, sin dec0, cos ha1, sin ha1, cos dec1, sin dec1}, ha = -ra of the old (0) and new (1)
+ * phase centres (for inverse = 1 ARL uses the same angles, :330-333).
+ *
+ * Rows (Visibility, :318-340): d_uvw [V][3] wavelengths, d_vis [V][npol]. */
+int arlmi_phaserotate_rows(double* d_uvw, void* d_vis, int64_t nrows, int32_t npol, const double* h_lmn,
+                           const double* h_trig, int inverse, int precision, void* stream);
+
+/* Blocks (BlockVisibility, :342-379): d_uvw [nblock][3] metres (nblock = ntimes * nants * nants,
+ * every (i, j) pair), d_vis [nblock][nchan][npol]; the phasor of channel ch uses
+ * uvw_m * (f[ch] / c), c = 299792458 m/s (:344-350), d_frequency [nchan] Hz on the device. */
+int arlmi_phaserotate_blocks(double* d_uvw, void* d_vis, int64_t nblock, int32_t nchan, int32_t npol,
+                             const double* d_frequency, const double* h_lmn, const double* h_trig, int inverse,
+                             int precision, void* stream);
+
 /* LDS microbenchmark (no ARL counterpart; measurement only): cycles per LDS-add wavefront
  * instruction per CU for pattern 0 (ds_add_f64 on the gridder's 8x8 footprints at random
  * origins of a 64x72 tile), 1 (ds_add_f64 on 64 distinct consecutive addresses) or 2 (pattern 0

@@ -52,6 +52,16 @@ Pinning (what checks each function; see tests/test_oracle_*.py):
                                            to exactly 1 on every occupied pixel); ARL's own test of
                                            it (test_griddata_gridding.py:283-295) needs LOWBD2 data
                                            and is not reproducible here
+  skycoord_to_lmn                          ARL known answers (test_coordinate_support.py:125-141)
+  uvw_to_xyz                               inverse of xyz_to_uvw, rotation invariants
+  phaserotate_rows, phaserotate_blocks     ARL's identities (test_coordinate_support.py:143-170:
+                                           vis x phasor = the rotated prediction, 6 x 6 x 6
+                                           centre/source combinations) and ARL's tests
+                                           (test_visibility_operations.py:140-245): rotation
+                                           equals a prediction made at the new centre with the
+                                           hour angle shifted, rotating there and back (and with
+                                           inverse) is the identity; blocks agree with rows
+                                           through convert_blocks
 """
 
 import numpy
@@ -70,6 +80,7 @@ __all__ = [
     "hadec_to_azel", "create_visibility", "apply_gains_rows", "create_blockvisibility", "convert_blocks",
     "convert_blocks_loop", "decoalesce_visibility", "apply_gaintable", "apply_gains_rows_adjoint",
     "grid_weight_to_griddata", "griddata_merge_weights", "griddata_reweight", "weight_visibility",
+    "uvw_to_xyz", "skycoord_to_lmn", "phaserotate_rows", "phaserotate_blocks",
 ]
 
 D2R = numpy.pi / 180.0
@@ -779,6 +790,80 @@ def xyz_to_uvw(xyz, ha, dec):
     w = z * numpy.sin(dec) - v0 * numpy.cos(dec)
     v = z * numpy.cos(dec) + v0 * numpy.sin(dec)
     return numpy.hstack([u, v, w])
+
+
+def uvw_to_xyz(uvw, ha, dec):
+    """Rotate UVW for (ha, dec) back to earth XYZ (coordinate_support.py:88-113)."""
+    u, v, w = numpy.hsplit(uvw, 3)
+    v0 = v * numpy.sin(dec) - w * numpy.cos(dec)
+    z = v * numpy.cos(dec) + w * numpy.sin(dec)
+    x = u * numpy.cos(ha) + v0 * numpy.sin(ha)
+    y = -u * numpy.sin(ha) + v0 * numpy.cos(ha)
+    return numpy.hstack([x, y, z])
+
+
+def skycoord_to_lmn(ra, dec, ra0, dec0):
+    """(l, m, n) of direction (ra, dec) relative to the phase centre (ra0, dec0), radians
+    (coordinate_support.py:150-169).  ARL takes astropy's SkyOffsetFrame Cartesian (x towards the
+    centre, y east, z north) and returns (y, z, x - 1); written out here as the direction cosines
+    of spherical trigonometry:
+        x = cos d cos d0 cos(ra - ra0) + sin d sin d0,  y = cos d sin(ra - ra0),
+        z = sin d cos d0 - cos d sin d0 cos(ra - ra0)."""
+    dra = ra - ra0
+    x = numpy.cos(dec) * numpy.cos(dec0) * numpy.cos(dra) + numpy.sin(dec) * numpy.sin(dec0)
+    y = numpy.cos(dec) * numpy.sin(dra)
+    z = numpy.sin(dec) * numpy.cos(dec0) - numpy.cos(dec) * numpy.sin(dec0) * numpy.cos(dra)
+    return y, z, x - 1.0
+
+
+def phaserotate_rows(vis, phasecentre, newphasecentre, tangent=True, inverse=False):
+    """phaserotate_visibility for a row-form Visibility (visibility/base.py:295-340).
+
+    vis: dict with 'uvw' [V, 3] wavelengths and 'vis' [V, npol]; centres are (ra, dec) radians.
+    Returns (uvw, vis) of the rotated copy, or the inputs themselves when |n| < 1e-15."""
+    l, m, n = skycoord_to_lmn(newphasecentre[0], newphasecentre[1], phasecentre[0], phasecentre[1])
+    if numpy.abs(n) < 1e-15:
+        return vis['uvw'], vis['vis']
+    uvw = numpy.array(vis['uvw'], dtype=float, copy=True)
+    out = numpy.array(vis['vis'], copy=True)
+    phasor = simulate_point(uvw, l, m)
+    if out.ndim > phasor.ndim:
+        phasor = phasor[:, numpy.newaxis]
+    if inverse:
+        out *= phasor
+    else:
+        out *= numpy.conj(phasor)
+    if not tangent:
+        xyz = uvw_to_xyz(uvw, ha=-phasecentre[0], dec=phasecentre[1])
+        uvw = xyz_to_uvw(xyz, ha=-newphasecentre[0], dec=newphasecentre[1])
+    return uvw, out
+
+
+def phaserotate_blocks(uvw_m, vis, frequency, phasecentre, newphasecentre, tangent=True, inverse=False):
+    """phaserotate_visibility for a BlockVisibility (visibility/base.py:342-379): uvw_m
+    [nt, na, na, 3] metres, vis [nt, na, na, nchan, npol]; the phasor of channel ch is
+    simulate_point(uvw_m * f[ch] / c).  Returns (uvw_m, vis) of the rotated copy."""
+    l, m, n = skycoord_to_lmn(newphasecentre[0], newphasecentre[1], phasecentre[0], phasecentre[1])
+    if numpy.abs(n) < 1e-15:
+        return uvw_m, vis
+    k = numpy.asarray(frequency, dtype=float) / C_LIGHT
+    uvw = uvw_m[..., numpy.newaxis] * k
+    out = numpy.array(vis, copy=True)
+    phasor = numpy.ones_like(out, dtype='complex')
+    nchan = out.shape[3]
+    for chan in range(nchan):
+        phasor[:, :, :, chan, :] = simulate_point(uvw[..., chan], l, m)[..., numpy.newaxis]
+    if inverse:
+        out *= phasor
+    else:
+        out *= numpy.conj(phasor)
+    uvw_out = uvw_m
+    if not tangent:
+        nt, na, _, _ = uvw_m.shape
+        lin = uvw_m.reshape([nt * na * na, 3])
+        xyz = uvw_to_xyz(lin, ha=-phasecentre[0], dec=phasecentre[1])
+        uvw_out = xyz_to_uvw(xyz, ha=-newphasecentre[0], dec=newphasecentre[1]).reshape([nt, na, na, 3])
+    return uvw_out, out
 
 
 def hadec_to_azel(ha, dec, latitude):
