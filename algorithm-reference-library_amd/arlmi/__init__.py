@@ -26,6 +26,7 @@ from .calibration import (create_gaintable_from_blockvisibility, apply_gaintable
 from .differentiable import PredictFunction, GainsFunction, predict_with_gains, visibility_loss
 from .batch import Autotuner, BatchImager
 from .phaserotate import skycoord_to_lmn, phaserotate_visibility
+from .deconvolution import hogbom, deconvolve_cube
 from .weighting import (grid_weight_to_griddata, griddata_merge_weights, griddata_reweight, weight_visibility,
                         weight_list_serial_workflow)
 

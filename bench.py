@@ -57,12 +57,14 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=1000000)
-    ap.add_argument("--workload", choices=("wstack", "wstack-aw", "batch", "gradient", "phaserotate"),
+    ap.add_argument("--workload", choices=("wstack", "wstack-aw", "batch", "gradient", "phaserotate", "clean"),
                     default="wstack",
                     help="wstack: the headline line (default); wstack-aw: the same with an AW w-projection kernel "
                          "for the residual w of each slab; batch: 64 uneven observations replayed from one "
                          "captured graph; gradient: value and gradient of ||V - g_p conj(g_q) predict(I)||^2; "
-                         "phaserotate: phase rotation (tangent=False) of the w-stack workload's rows")
+                         "phaserotate: phase rotation (tangent=False) of the w-stack workload's rows; "
+                         "clean: Hogbom minor cycle on the w-stack workload's dirty image and PSF")
+    ap.add_argument("--niter", type=int, default=1000, help="Hogbom iterations (clean)")
     ap.add_argument("--support", type=int, default=16, help="AW kernel support (wstack-aw)")
     ap.add_argument("--nobs", type=int, default=64)
     ap.add_argument("--lds-probe", action="store_true",
@@ -519,6 +521,94 @@ def bench_phaserotate(args):
     print(json.dumps(res))
 
 
+def bench_clean(args):
+    """Hogbom minor cycle (cleaners.py:12-70) on the w-stacked 2048^2 workload's dirty image and
+    PSF (float32 images promoted to float64, as ARL's arrays are), PSF cut to +-1024 pixels (ARL's
+    default psf_support), gain 0.1, fractional threshold 0.001, --niter iterations at most: one
+    step = one whole minor cycle; value = iterations per second."""
+    import ctypes
+    import numpy
+    import torch
+    import arlmi
+    from arlmi import simulation
+    vis, im, _ = simulation.config2(precision=32)
+    gcf, cf = arlmi.create_pswf_convolutionfunction(im, oversampling=8, support=8)
+    imager = arlmi.Imager(vis, im, gcfcf=(gcf, cf), mode="wstack", vis_slices=args.slices)
+    dirty = imager.invert(vis)[0][0, 0].double().contiguous()
+    psf = imager.invert(vis, dopsf=True)[0][0, 0].double().contiguous()
+    del imager, vis
+    torch.cuda.empty_cache()
+    n = dirty.shape[0]
+    gain, frac, niter = 0.1, 0.001, args.niter
+    L = arlmi._lib.lib()
+    for _ in range(args.warmup):
+        arlmi.hogbom(dirty, psf, None, gain, 0.0, niter, frac)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    steps = max(1, args.steps // 5)
+    for _ in range(steps):
+        comps, res = arlmi.hogbom(dirty, psf, None, gain, 0.0, niter, frac)
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / steps
+    iters = arlmi.hogbom.last_iterations
+    scanned, area = ctypes.c_int64(), ctypes.c_int64()
+    L.arlmi_hogbom_traffic(ctypes.byref(scanned), ctypes.byref(area))
+    # kernel times from one more cycle with per-launch events (kept out of the timed loop)
+    L.arlmi_profile_reset()
+    L.arlmi_profile_enable(1)
+    arlmi.hogbom(dirty, psf, None, gain, 0.0, niter, frac)
+    torch.cuda.synchronize()
+    L.arlmi_profile_enable(0)
+    prof = arlmi._lib.profile_report()
+    scan_s, pick_s = prof["k_clean_scan"][0] * 1e-3, prof["k_clean_pick"][0] * 1e-3
+    kernel_s = scan_s + pick_s
+    algo = 8 * scanned.value + 16 * area.value
+    achieved = algo / scan_s / 1e9
+    # CPU baseline and parity: the oracle (ARL's numpy loop) for the first iterations of the same cycle
+    sys.path.insert(0, ROOT)
+    from oracle import arl_oracle as O
+    k = 40
+    d, p = dirty.cpu().numpy(), psf.cpu().numpy()
+    c0 = time.perf_counter()
+    oc, orr, oit = O.hogbom(d, p, None, gain, 0.0, k, frac)
+    c1 = time.perf_counter()
+    gc, gr = arlmi.hogbom(dirty, psf, None, gain, 0.0, k, frac)
+    exact = bool(numpy.array_equal(gc.cpu().numpy(), oc) and numpy.array_equal(gr.cpu().numpy(), orr))
+    res_out = {
+        "metric": "Hogbom minor-cycle iterations/s (cleaners.py hogbom), 2048^2 dirty image and PSF of the w-stack "
+                  "workload",
+        "value": round(iters / t, 1),
+        "unit": "iterations/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "dirty image and PSF (%d^2, 64 w slabs, 29.7 M rows, 200 sources) from the imager, "
+                        "PSF +-%d, gain %g, fractional threshold %g, niter %d" % (n, n // 2, gain, frac, niter),
+            "iterations": iters,
+            "us_per_iteration": round(t / iters * 1e6, 2),
+            "kernel_us_per_iteration": {"k_clean_scan": round(scan_s / iters * 1e6, 2),
+                                        "k_clean_pick": round(pick_s / iters * 1e6, 2)},
+            "scanned_pixels": scanned.value,
+            "subtracted_pixels": area.value,
+            "first_%d_iterations_bit_exact_vs_oracle" % k: exact,
+        },
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
+                     "frac": round(achieved / 8000.0, 4), "traffic": None, "kernel": "k_clean_scan",
+                     "algorithmic_bytes": "8 x scanned + 16 x subtracted pixels"},
+        "cpu_baseline": {"value": round(oit / (c1 - c0), 2), "unit": "iterations/s", "cores": 1, "kind": "port",
+                         "sample": "the first %d iterations of the same cycle, oracle (ARL's numpy loop), %.2f s"
+                                   % (oit, c1 - c0)},
+    }
+    print(json.dumps(res_out))
+
+
 def main():
     args = parse()
     if args.copy_probe:
@@ -559,6 +649,11 @@ def main():
         import torch
         torch.cuda.set_device(0)
         bench_phaserotate(args)
+        return
+    if args.workload == "clean":
+        import torch
+        torch.cuda.set_device(0)
+        bench_clean(args)
         return
     import numpy
     import torch

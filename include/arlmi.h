@@ -316,6 +316,29 @@ int arlmi_phaserotate_blocks(double* d_uvw, void* d_vis, int64_t nblock, int32_t
                              const double* d_frequency, const double* h_lmn, const double* h_trig, int inverse,
                              int precision, void* stream);
 
+/* hogbom (processing_library/arrays/cleaners.py:12-70), float64, one image plane:
+ *   absolutethresh = max(thresh, fracthresh * max|dirty|), pmax = max(psf) (> 0, else ARLMI_E_ARG),
+ *   repeat up to niter times: (mx, my) = first index of max |res * window| (|res| for a NULL window,
+ *   row-major order, as numpy's argmax), mval = res[mx, my] * gain / pmax, comps[mx, my] += mval,
+ *   res[overlap] -= psf[overlap] * mval over overlapIndices (:134-154): rows
+ *   [max(0, mx - pnx/2), min(nx, mx + pnx/2)) against PSF rows from pnx/2 + (row - mx), likewise for
+ *   columns; stop once |res[mx, my]| < 0.9 * absolutethresh after the subtraction.
+ * d_res [nx][ny]: the dirty image on entry, the residual on return; d_comps [nx][ny]: the
+ * components (zeroed here); d_psf [psf_nx][psf_ny]; d_window [nx][ny] or NULL.  *iterations (may be
+ * NULL) receives ARL's iteration count.  0 < gain < 2 and niter > 0 are required (:33-34).  The
+ * operations are ARL's, uncontracted, so the results equal numpy's bit for bit (a NaN is never
+ * chosen as the peak, where numpy's argmax would choose it).  Synchronises the stream about every 64
+ * iterations to read the end flag. */
+int arlmi_hogbom(double* d_res, double* d_comps, const double* d_psf, int32_t psf_nx, int32_t psf_ny,
+                 const double* d_window, int32_t nx, int32_t ny, double gain, double thresh, int32_t niter,
+                 double fracthresh, int32_t* iterations, void* stream);
+
+/* Measurement only (no ARL counterpart): for the last arlmi_hogbom call on the calling thread,
+ * the pixels whose bands were scanned for the peak (the whole image once, then the bands of
+ * rows each PSF subtraction changed) and the pixels the PSF subtractions covered; the minor
+ * cycle's algorithmic HBM bytes are 8 * scanned (+ 8 * scanned with a window) + 16 * subtracted. */
+int arlmi_hogbom_traffic(int64_t* scanned_pixels, int64_t* subtracted_pixels);
+
 /* LDS microbenchmark (no ARL counterpart; measurement only): cycles per LDS-add wavefront
  * instruction per CU for pattern 0 (ds_add_f64 on the gridder's 8x8 footprints at random
  * origins of a 64x72 tile), 1 (ds_add_f64 on 64 distinct consecutive addresses) or 2 (pattern 0

@@ -62,6 +62,11 @@ Pinning (what checks each function; see tests/test_oracle_*.py):
                                            hour angle shifted, rotating there and back (and with
                                            inverse) is the identity; blocks agree with rows
                                            through convert_blocks
+  overlapIndices                           ARL known answer (test_image_deconvolution.py:59-68)
+  hogbom, deconvolve_cube_hogbom           closed form of a delta PSF (geometric decay of the
+                                           peak, iteration count from the stop rule), flux
+                                           conservation dirty - res = comps convolved with the
+                                           PSF (an FFT convolution), windows, argmax ties
 """
 
 import numpy
@@ -80,7 +85,8 @@ __all__ = [
     "hadec_to_azel", "create_visibility", "apply_gains_rows", "create_blockvisibility", "convert_blocks",
     "convert_blocks_loop", "decoalesce_visibility", "apply_gaintable", "apply_gains_rows_adjoint",
     "grid_weight_to_griddata", "griddata_merge_weights", "griddata_reweight", "weight_visibility",
-    "uvw_to_xyz", "skycoord_to_lmn", "phaserotate_rows", "phaserotate_blocks",
+    "uvw_to_xyz", "skycoord_to_lmn", "phaserotate_rows", "phaserotate_blocks", "overlapIndices", "hogbom",
+    "deconvolve_cube_hogbom",
 ]
 
 D2R = numpy.pi / 180.0
@@ -1119,3 +1125,79 @@ def weight_visibility(vis, geom, cfg):
     """Uniform weighting (imaging/weighting.py:19-40): returns the new imaging_weight."""
     grid, _ = grid_weight_to_griddata(vis, geom, cfg)
     return griddata_reweight(vis, grid, geom, cfg)
+
+
+# ---------------------------------------------------------------------------
+# Hogbom CLEAN: processing_library/arrays/cleaners.py:12-70, 134-154
+# ---------------------------------------------------------------------------
+
+def overlapIndices(res, psf, peakx, peaky):
+    """Limits in res and in psf where the PSF centred on (peakx, peaky) overlaps the residual
+    (cleaners.py:134-154): ((res rows, res cols), (psf rows, psf cols)) as 4-tuples."""
+    nx, ny = res.shape[0], res.shape[1]
+    psfwidthx, psfwidthy = psf.shape[0] // 2, psf.shape[1] // 2
+    psfpeakx, psfpeaky = psf.shape[0] // 2, psf.shape[1] // 2
+    res_lower = (max(0, peakx - psfwidthx), max(0, peaky - psfwidthy))
+    res_upper = (min(nx, peakx + psfwidthx), min(peaky + psfwidthy, ny))
+    psf_lower = (max(0, psfpeakx + (res_lower[0] - peakx)), max(0, psfpeaky + (res_lower[1] - peaky)))
+    psf_upper = (min(psf.shape[0], psfpeakx + (res_upper[0] - peakx)),
+                 min(psfpeaky + (res_upper[1] - peaky), psf.shape[1]))
+    return ((res_lower[0], res_upper[0], res_lower[1], res_upper[1]),
+            (psf_lower[0], psf_upper[0], psf_lower[1], psf_upper[1]))
+
+
+def hogbom(dirty, psf, window, gain, thresh, niter, fracthresh):
+    """Hogbom CLEAN of one plane (cleaners.py:12-70), ARL's loop: returns (comps, res, aiter)."""
+    assert 0.0 < gain < 2.0
+    assert niter > 0
+    absolutethresh = max(thresh, fracthresh * numpy.fabs(dirty).max())
+    comps = numpy.zeros(dirty.shape)
+    res = numpy.array(dirty)
+    pmax = psf.max()
+    assert pmax > 0.0
+    aiter = 0
+    for i in range(niter):
+        aiter = i + 1
+        if window is not None:
+            mx, my = numpy.unravel_index((numpy.fabs(res * window)).argmax(), dirty.shape)
+        else:
+            mx, my = numpy.unravel_index((numpy.fabs(res)).argmax(), dirty.shape)
+        mval = res[mx, my] * gain / pmax
+        comps[mx, my] += mval
+        a1o, a2o = overlapIndices(dirty, psf, mx, my)
+        res[a1o[0]:a1o[1], a1o[2]:a1o[3]] -= psf[a2o[0]:a2o[1], a2o[2]:a2o[3]] * mval
+        if numpy.abs(res[mx, my]) < 0.9 * absolutethresh:
+            break
+    return comps, res, aiter
+
+
+def deconvolve_cube_hogbom(dirty, psf, window_shape=None, window_edge=16, psf_support=None, gain=0.7,
+                           threshold=0.0, niter=100, fractional_threshold=0.1):
+    """deconvolve_cube with algorithm='hogbom' (processing_components/image/deconvolution.py:
+    43-240) on arrays [nchan, npol, ny, nx]: windows, PSF cut, per-plane hogbom; planes whose PSF
+    maximum is 0 are skipped and stay 0 in both outputs.  Returns (comp, residual, psf_cut)."""
+    shape = dirty.shape
+    if window_shape == 'quarter':
+        qx, qy = shape[3] // 4, shape[2] // 4
+        window = numpy.zeros_like(dirty)
+        window[..., (qy + 1):3 * qy, (qx + 1):3 * qx] = 1.0
+    elif window_shape == 'no_edge':
+        nx, ny = shape[3], shape[2]
+        window = numpy.zeros_like(dirty)
+        window[..., (window_edge + 1):(ny - window_edge), (window_edge + 1):(nx - window_edge)] = 1.0
+    else:
+        window = None
+    if psf_support is None:
+        psf_support = max(shape[2] // 2, shape[3] // 2)
+    if psf_support <= psf.shape[2] // 2 and psf_support <= psf.shape[3] // 2:
+        c0, c1 = psf.shape[2] // 2, psf.shape[3] // 2
+        psf = psf[..., (c0 - psf_support):(c0 + psf_support), (c1 - psf_support):(c1 + psf_support)]
+    comp = numpy.zeros(shape)
+    residual = numpy.zeros(shape)
+    for ch in range(shape[0]):
+        for pol in range(shape[1]):
+            if psf[ch, pol].max():
+                w = None if window is None else window[ch, pol]
+                comp[ch, pol], residual[ch, pol], _ = hogbom(dirty[ch, pol], psf[ch, pol], w, gain, threshold,
+                                                              niter, fractional_threshold)
+    return comp, residual, psf
