@@ -198,8 +198,11 @@ def predict_skycomponent_visibility(vis, sc, chan_map=None):
     out = vis.vis.clone()
     uvw = vis.data["uvw"]
     chan = None if chan_map is None else chan_map.to(device=device(), dtype=torch.int32).contiguous()
-    _lib.check(_lib.lib().arlmi_dft(ptr(uvw), ptr(chan), uvw.shape[0], ptr(d_lmf), len(sc), nchan, ptr(out),
-                                    vis.precision, stream_handle()))
+    per_call = max(1, (64 * 1024) // (8 * (3 + nchan)))  # components staged in one block's LDS
+    for k0 in range(0, len(sc), per_call):
+        part = d_lmf[k0:k0 + per_call]
+        _lib.check(_lib.lib().arlmi_dft(ptr(uvw), ptr(chan), uvw.shape[0], ptr(part), part.shape[0], nchan,
+                                        ptr(out), vis.precision, stream_handle()))
     vis.data["vis"] = out
     return vis
 

@@ -627,12 +627,45 @@ __global__ void k_w_screen(int n, double fov, double w, cplx<R>* out) {
   out[i] = cmk<R>((R)c, (R)s);
 }
 
+// exp(-2 pi i d) from the exact reduction of d modulo 1, e = d - rint(d) in [-1/2, 1/2], and
+// r = -2 e in [-1, 1] (exact): cos(pi r) + i sin(pi r).  This is the phasor of d itself (numpy
+// evaluates cos(fl(-2 pi d)), whose rounding of 2 pi d adds ~|2 pi d| * 1e-16 rad).  Single
+// precision visibilities take the reduced angle in float32 (error ~1e-7, the storage precision);
+// double precision keeps float64.
 template <typename R>
-__global__ void k_dft(const double* __restrict__ uvw, const int32_t* __restrict__ chan, int64_t nrows,
-                      const double* __restrict__ lmf, int ncomp, int nchan, cplx<R>* __restrict__ vis) {
+__device__ __forceinline__ void dft_phasor(double d, double& c, double& s) {
+  const double r = -2.0 * (d - rint(d));
+  if (sizeof(R) == 4) {
+    float sf, cf;
+    sincospif((float)r, &sf, &cf);
+    s = sf;
+    c = cf;
+  } else {
+    sincospi(r, &s, &c);
+  }
+}
+
+// One thread per row, every component: component k at (l, m) with n - 1 = sqrt(1 - l^2 - m^2) - 1
+// (simulate_point, coordinate_support.py:192-212) and flux f[chan] (imaging/base.py:207-220).
+// The components' (l, m, n - 1, flux) are staged in LDS once per block.
+template <typename R>
+__global__ void __launch_bounds__(256) k_dft(const double* __restrict__ uvw, const int32_t* __restrict__ chan,
+                                             int64_t nrows, const double* __restrict__ lmf, int ncomp, int nchan,
+                                             cplx<R>* __restrict__ vis) {
   extern __shared__ double comp[];
-  const int stride = 2 + nchan;
-  for (int i = threadIdx.x; i < ncomp * stride; i += blockDim.x) comp[i] = lmf[i];
+  const int in_stride = 2 + nchan, stride = 3 + nchan;
+  for (int k = threadIdx.x; k < ncomp; k += blockDim.x) {
+    const double l = lmf[k * in_stride], m = lmf[k * in_stride + 1];
+    double nm1;
+    {
+#pragma clang fp contract(off)
+      nm1 = sqrt(1.0 - l * l - m * m) - 1.0;
+    }
+    comp[k * stride] = l;
+    comp[k * stride + 1] = m;
+    comp[k * stride + 2] = nm1;
+    for (int c = 0; c < nchan; ++c) comp[k * stride + 3 + c] = lmf[k * in_stride + 2 + c];
+  }
   __syncthreads();
   const int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (row >= nrows) return;
@@ -640,12 +673,11 @@ __global__ void k_dft(const double* __restrict__ uvw, const int32_t* __restrict_
   const int ch = chan ? chan[row] : 0;
   double ar = 0.0, ai = 0.0;
   for (int k = 0; k < ncomp; ++k) {
-    const double l = comp[k * stride], m = comp[k * stride + 1];
-    const double f = comp[k * stride + 2 + ch];
-    const double nm1 = sqrt(1.0 - l * l - m * m) - 1.0;
-    const double d = u * l + v * m + w * nm1;
+    const double* cp = comp + k * stride;
+    const double d = u * cp[0] + v * cp[1] + w * cp[2];
     double s, c;
-    sincos(kMinusTwoPi * d, &s, &c);  // exp(-2 pi i d)
+    dft_phasor<R>(d, c, s);
+    const double f = cp[3 + ch];
     ar += f * c;
     ai += f * s;
   }
@@ -959,7 +991,7 @@ int arlmi_dft(const double* d_uvw, const int32_t* d_chan, int64_t nrows, const d
     set_error("arlmi_dft: bad argument");
     return ARLMI_E_ARG;
   }
-  const size_t sh = sizeof(double) * (size_t)ncomp * (2 + nchan);
+  const size_t sh = sizeof(double) * (size_t)ncomp * (3 + nchan);
   if (sh > 64 * 1024) {
     set_error("too many components for one call (%d)", ncomp);
     return ARLMI_E_UNSUPPORTED;

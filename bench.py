@@ -57,13 +57,14 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=1000000)
-    ap.add_argument("--workload", choices=("wstack", "wstack-aw", "batch", "gradient", "phaserotate", "clean"),
+    ap.add_argument("--workload", choices=("wstack", "wstack-aw", "batch", "gradient", "phaserotate", "clean", "dft"),
                     default="wstack",
                     help="wstack: the headline line (default); wstack-aw: the same with an AW w-projection kernel "
                          "for the residual w of each slab; batch: 64 uneven observations replayed from one "
                          "captured graph; gradient: value and gradient of ||V - g_p conj(g_q) predict(I)||^2; "
                          "phaserotate: phase rotation (tangent=False) of the w-stack workload's rows; "
-                         "clean: Hogbom minor cycle on the w-stack workload's dirty image and PSF")
+                         "clean: Hogbom minor cycle on the w-stack workload's dirty image and PSF; "
+                         "dft: direct DFT of the workload's 200 sources onto its rows")
     ap.add_argument("--niter", type=int, default=1000, help="Hogbom iterations (clean)")
     ap.add_argument("--support", type=int, default=16, help="AW kernel support (wstack-aw)")
     ap.add_argument("--nobs", type=int, default=64)
@@ -609,6 +610,85 @@ def bench_clean(args):
     print(json.dumps(res_out))
 
 
+def bench_dft(args):
+    """predict_skycomponent_visibility (imaging/base.py:192-244): the direct DFT of the w-stack
+    workload's 200 point sources onto its 29.7 M rows, complex64 visibilities (phases in float64):
+    one step = one call over all rows and sources."""
+    import numpy
+    import torch
+    import arlmi
+    from arlmi import simulation
+    vis, im, comps = simulation.config2(precision=32, sky=False)
+    rows, nc = vis.nvis, len(comps)
+    vis.data["vis"].zero_()
+    for _ in range(args.warmup):
+        arlmi.predict_skycomponent_visibility(vis, comps)
+    torch.cuda.synchronize()
+    L = arlmi._lib.lib()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        vis.data["vis"].zero_()
+        arlmi.predict_skycomponent_visibility(vis, comps)
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / args.steps
+    L.arlmi_profile_reset()
+    L.arlmi_profile_enable(1)
+    vis.data["vis"].zero_()
+    arlmi.predict_skycomponent_visibility(vis, comps)
+    torch.cuda.synchronize()
+    L.arlmi_profile_enable(0)
+    kms, kn = arlmi._lib.profile_report()["k_dft"]
+    ks = kms * 1e-3 / kn
+    flops = 9.0 * rows * nc  # per (row, source): d = u l + v m + w (n - 1) and the two accumulations, float64
+    # parity and CPU baseline on a sample of rows: the oracle (numpy, ARL's per-component loop)
+    sys.path.insert(0, ROOT)
+    from oracle import arl_oracle as O
+    m = 200_000
+    idx = torch.arange(0, m, device="cuda") * (rows // m)
+    sample = {"uvw": vis.data["uvw"][idx].cpu().numpy(), "vis": numpy.zeros((m, 1), dtype=complex)}
+    oc = [(c.l, c.m, float(c.flux[0, 0])) for c in comps]
+    c0 = time.perf_counter()
+    ref = O.predict_skycomponent_visibility(sample, oc)
+    c1 = time.perf_counter()
+    got = vis.vis[idx].cpu().numpy().astype(numpy.complex128)
+    err = float(numpy.linalg.norm(got - ref) / numpy.linalg.norm(ref))
+    v64 = arlmi.Visibility({"uvw": vis.data["uvw"][idx].contiguous(),
+                            "vis": torch.zeros((m, 1), dtype=torch.complex128, device="cuda")})
+    arlmi.predict_skycomponent_visibility(v64, comps)
+    err64 = float(numpy.linalg.norm(v64.vis.cpu().numpy() - ref) / numpy.linalg.norm(ref))
+    out = {
+        "metric": "Mvis/s direct DFT of 200 point sources (predict_skycomponent_visibility), rows of the 2048^2 "
+                  "w-stack workload",
+        "value": round(rows / t / 1e6, 1),
+        "unit": "Mvis/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 phase, f32 reduced angle, c64 vis",
+        "data": "synthetic",
+        "config": {
+            "workload": "256 antennas log-radial to 10 km, 8 channels 100-150 MHz, 120 integrations (rows kept for "
+                        "the 2048^2 grid), 200 power-law sources within +-0.15 rad",
+            "rows": rows,
+            "sources": nc,
+            "k_dft_ms": round(ks * 1e3, 3),
+            "sample_rel_l2_err_vs_oracle_fp32": err,
+            "sample_rel_l2_err_vs_oracle_fp64": err64,
+        },
+        "roofline": {"bound": "fp64", "achieved": round(flops / ks / 1e12, 2), "peak": 78.6, "unit": "TFLOP/s",
+                     "frac": round(flops / ks / 1e12 / 78.6, 4), "traffic": None, "kernel": "k_dft",
+                     "algorithmic_flops": "9 float64 flops per (row, source); the phasor itself is not counted"},
+        "cpu_baseline": {"value": round(m / (c1 - c0) / 1e6, 4), "unit": "Mvis/s", "cores": 1, "kind": "port",
+                         "sample": "%d rows x %d sources, oracle predict_skycomponent_visibility (numpy), %.2f s"
+                                   % (m, nc, c1 - c0)},
+    }
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.copy_probe:
@@ -649,6 +729,11 @@ def main():
         import torch
         torch.cuda.set_device(0)
         bench_phaserotate(args)
+        return
+    if args.workload == "dft":
+        import torch
+        torch.cuda.set_device(0)
+        bench_dft(args)
         return
     if args.workload == "clean":
         import torch

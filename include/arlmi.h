@@ -192,7 +192,11 @@ int arlmi_w_screen(int32_t npixel, double cellsize, double w, void* d_out, int p
  * simulate_point, coordinate_support.py:192-212): d_vis[row] += sum_k flux[k][chan(row)] *
  * exp(-2 pi i (u l_k + v m_k + w (sqrt(1 - l_k^2 - m_k^2) - 1))).  d_lmflux: float64 [ncomp][2 + nchan]
  * (l, m, flux per image channel).  d_chan: int32 [nrows] image channel of each row, or NULL (0).
- * Phases are evaluated in float64 in both precisions. */
+ * The phase d = u l + v m + w (n - 1) is float64 in both precisions; exp(-2 pi i d) is evaluated as
+ * cos(pi r) + i sin(pi r) of r = -2 (d - rint(d)) (an exact reduction, so the phasor is that of d
+ * itself; numpy's cos(fl(-2 pi d)) carries an extra ~|2 pi d| * 1e-16 rad), with r in float32 for
+ * precision 32.  Components are summed per row in float64 and added to d_vis once.  At most
+ * 64 KiB / (8 (3 + nchan)) components per call. */
 int arlmi_dft(const double* d_uvw, const int32_t* d_chan, int64_t nrows, const double* d_lmflux, int32_t ncomp,
               int32_t nchan, void* d_vis, int precision, void* stream);
 
