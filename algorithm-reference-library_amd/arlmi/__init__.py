@@ -22,7 +22,7 @@ from .fft_support import fft, ifft
 from .plan import Plan
 from .coalesce import convert_blockvisibility_to_visibility, convert_blocks, decoalesce_visibility
 from .calibration import (create_gaintable_from_blockvisibility, apply_gaintable, gain_row_indices, apply_gains_rows,
-                          apply_gains_rows_adjoint)
+                          apply_gains_rows_adjoint, solve_gaintable)
 from .differentiable import PredictFunction, GainsFunction, predict_with_gains, visibility_loss
 from .batch import Autotuner, BatchImager
 from .phaserotate import skycoord_to_lmn, phaserotate_visibility

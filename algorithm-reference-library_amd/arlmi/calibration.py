@@ -14,7 +14,7 @@ from .data_models import BlockVisibility, GainTable
 from .plan import ptr, stream_handle
 
 __all__ = ["create_gaintable_from_blockvisibility", "apply_gaintable", "gain_slots", "gain_row_indices",
-           "apply_gains_rows", "apply_gains_rows_adjoint"]
+           "apply_gains_rows", "apply_gains_rows_adjoint", "solve_gaintable"]
 
 
 def _np(x):
@@ -125,3 +125,57 @@ def apply_gains_rows_adjoint(vis_rows, grad_out, gidx1, gidx2, gain, need_vis=Tr
     _lib.check(_lib.lib().arlmi_apply_gains_rows_adjoint(v.numel(), ptr(gidx1), ptr(gidx2), ptr(g), g.numel(), ptr(v),
                                                          ptr(go), ptr(gv), ptr(gg), precision, stream_handle()))
     return gv, (gg.reshape(gain.shape) if gg is not None else None)
+
+
+def solve_gaintable(vis, modelvis=None, gt=None, phase_only=True, niter=30, tol=1e-8, crosspol=False,
+                    normalise_gains=True, refant=0, damping=0.5, **kwargs):
+    """Scalar gain solution on the GPU (calibration.py:30-101 with solvers.py:61-140, 302-332).
+
+    vis, modelvis: BlockVisibility (npol = 1); gt: a GainTable to start from, else
+    create_gaintable_from_blockvisibility(vis, **kwargs).  Returns a GainTable with complex128
+    gains, float64 weights (the last substitution's denominators) and ``residual`` [ntimes, nchan,
+    1, 1]; ``iterations`` holds the iterations each row ran.  refant and damping are the solver's
+    defaults, which ARL's solve_gaintable does not forward."""
+    if not isinstance(vis, BlockVisibility):
+        raise AssertionError("vis is not a BlockVisibility: %r" % (vis,))
+    if modelvis is not None:
+        if not isinstance(modelvis, BlockVisibility):
+            raise AssertionError("modelvis is not a BlockVisibility: %r" % (modelvis,))
+        if not float(modelvis.data["vis"].abs().max()) > 0.0:
+            raise AssertionError("Model visibility is zero")
+    if crosspol or vis.data["vis"].shape[4] != 1:
+        raise NotImplementedError("scalar (npol = 1) gain solutions only")
+    if gt is None:
+        gt = create_gaintable_from_blockvisibility(vis, **kwargs)
+    dev = vis.data["vis"].device
+    ntimes, nants, nchan = vis.ntimes, vis.nants, vis.nchan
+    if gt.nants != nants or gt.nchan != nchan:
+        raise ValueError("gain table does not match the visibility (antennas / channels)")
+    slots = gain_slots(vis.time, gt)
+    active = numpy.bincount(slots[slots >= 0], minlength=gt.ntimes)[:gt.ntimes].astype(numpy.int32)
+    v = vis.data["vis"].contiguous()
+    m = None if modelvis is None else modelvis.data["vis"].to(v.dtype).contiguous()
+    w = vis.data.get("weight")
+    w = None if w is None else w.to(dtype=v.real.dtype).contiguous()
+    gain = gt.gain.to(device=dev, dtype=torch.complex128).reshape(gt.ntimes, nants, nchan).contiguous().clone()
+    gwt = torch.zeros((gt.ntimes, nants, nchan), dtype=torch.float64, device=dev)
+    if gt.weight is not None:
+        gwt.copy_(gt.weight.to(device=dev, dtype=torch.float64).reshape(gt.ntimes, nants, nchan))
+    residual = torch.zeros((gt.ntimes, nchan), dtype=torch.float64, device=dev)
+    old_res = getattr(gt, "residual", None)
+    if old_res is not None:
+        residual.copy_(torch.as_tensor(old_res).to(device=dev, dtype=torch.float64).reshape(gt.ntimes, nchan))
+    iters = torch.zeros(gt.ntimes, dtype=torch.int32, device=dev)
+    work = torch.empty(3 * gt.ntimes * nants * nants * nchan, dtype=torch.float64, device=dev)
+    d_slots = torch.as_tensor(slots).to(dev)
+    d_active = torch.as_tensor(active).to(dev)
+    _lib.check(_lib.lib().arlmi_solve_gains(ptr(v), ptr(m), ptr(w), ntimes, nants, nchan, ptr(d_slots), gt.ntimes,
+                                            ptr(d_active), ptr(gain), ptr(gwt), ptr(residual), ptr(iters), ptr(work),
+                                            int(bool(phase_only)), int(niter), float(tol), int(refant),
+                                            float(damping), int(bool(normalise_gains)), vis.precision,
+                                            stream_handle()))
+    out = GainTable(gain.reshape(gt.ntimes, nants, nchan, 1, 1), gt.time, gt.interval,
+                    weight=gwt.reshape(gt.ntimes, nants, nchan, 1, 1), frequency=gt.frequency)
+    out.residual = residual.reshape(gt.ntimes, nchan, 1, 1)
+    out.iterations = iters
+    return out

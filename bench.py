@@ -57,14 +57,15 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=1000000)
-    ap.add_argument("--workload", choices=("wstack", "wstack-aw", "batch", "gradient", "phaserotate", "clean", "dft"),
+    ap.add_argument("--workload", choices=("wstack", "wstack-aw", "batch", "gradient", "phaserotate", "clean", "dft", "solve"),
                     default="wstack",
                     help="wstack: the headline line (default); wstack-aw: the same with an AW w-projection kernel "
                          "for the residual w of each slab; batch: 64 uneven observations replayed from one "
                          "captured graph; gradient: value and gradient of ||V - g_p conj(g_q) predict(I)||^2; "
                          "phaserotate: phase rotation (tangent=False) of the w-stack workload's rows; "
                          "clean: Hogbom minor cycle on the w-stack workload's dirty image and PSF; "
-                         "dft: direct DFT of the workload's 200 sources onto its rows")
+                         "dft: direct DFT of the workload's 200 sources onto its rows; "
+                         "solve: scalar gain solution of the 128-antenna differentiation workload")
     ap.add_argument("--niter", type=int, default=1000, help="Hogbom iterations (clean)")
     ap.add_argument("--support", type=int, default=16, help="AW kernel support (wstack-aw)")
     ap.add_argument("--nobs", type=int, default=64)
@@ -689,6 +690,78 @@ def bench_dft(args):
     print(json.dumps(out))
 
 
+def bench_solve(args):
+    """solve_gaintable (scalar, amplitude and phase, 200 iterations at most, tol 1e-8) of the
+    differentiation workload's BlockVisibility: 128 antennas, 4 channels, 60 integrations with one
+    gain-table row each, a random Hermitian model with the workload's gains applied.  One step =
+    one whole solve (point-source equivalents, iterations, residuals)."""
+    import numpy
+    import torch
+    import arlmi
+    from arlmi.simulation import config4
+    c = config4(precision=64)
+    bv, gt = c["bvis"], c["gaintable"]
+    rng = numpy.random.default_rng(21)
+    shape = tuple(bv.data["vis"].shape)
+    m = (rng.normal(size=shape) + 1j * rng.normal(size=shape)) * 5.0
+    m = 0.5 * (m + numpy.conj(numpy.swapaxes(m, 1, 2)))
+    model = bv.copy()
+    model.data["vis"] = torch.as_tensor(m).to("cuda").contiguous()
+    obs = model.copy()
+    arlmi.apply_gaintable(obs, gt)
+    start = arlmi.create_gaintable_from_blockvisibility(bv)
+    start.gain = start.gain.to(torch.complex128)
+
+    def solve():
+        return arlmi.solve_gaintable(obs, model, gt=start, phase_only=False, niter=200, normalise_gains=False)
+
+    for _ in range(args.warmup):
+        solve()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sol = solve()
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / args.steps
+    iters = sol.iterations.cpu().numpy()
+    sys.path.insert(0, ROOT)
+    from oracle import arl_oracle as O
+    k = 4  # oracle on the first gain rows (ARL's per-antenna numpy loop)
+    sub = {'vis': obs.data["vis"][:k].cpu().numpy(), 'weight': numpy.ones((k,) + shape[1:]),
+           'time': bv.time[:k].cpu().numpy()}
+    c0 = time.perf_counter()
+    want = O.solve_gaintable_scalar(sub, m[:k], numpy.ones((k, 128, 4, 1, 1), dtype=complex), start.time[:k],
+                                    start.interval[:k], phase_only=False, niter=200, normalise_gains=False)
+    c1 = time.perf_counter()
+    err = float(numpy.max(numpy.abs(sol.gain.cpu().numpy()[:k] - want[0])))
+    out = {
+        "metric": "gain-table rows solved per second (solve_gaintable, scalar, amplitude + phase)",
+        "value": round(gt.ntimes / t, 1),
+        "unit": "rows/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "128 antennas, 4 channels 120-140 MHz, 60 integrations (one gain row each), random Hermitian "
+                        "model with gains amplitude 1+-0.1 and phase U(-pi/4, pi/4) applied",
+            "iterations_per_row_mean": float(iters.mean()),
+            "max_residual": float(sol.residual.max()),
+            "max_abs_gain_diff_vs_oracle_first_%d_rows" % k: err,
+        },
+        "roofline": None,
+        "cpu_baseline": {"value": round(k / (c1 - c0), 3), "unit": "rows/s", "cores": 1, "kind": "port",
+                         "sample": "the first %d gain rows, oracle (ARL's per-antenna numpy loop), %.2f s"
+                                   % (k, c1 - c0)},
+    }
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.copy_probe:
@@ -729,6 +802,11 @@ def main():
         import torch
         torch.cuda.set_device(0)
         bench_phaserotate(args)
+        return
+    if args.workload == "solve":
+        import torch
+        torch.cuda.set_device(0)
+        bench_solve(args)
         return
     if args.workload == "dft":
         import torch

@@ -337,6 +337,34 @@ int arlmi_hogbom(double* d_res, double* d_comps, const double* d_psf, int32_t ps
                  const double* d_window, int32_t nx, int32_t ny, double gain, double thresh, int32_t niter,
                  double fracthresh, int32_t* iterations, void* stream);
 
+/* solve_gaintable for scalar gains (processing_components/calibration/calibration.py:30-101,
+ * npol = 1, crosspol = False), float64 arithmetic:
+ *   for every gain-table row s with at least one integration (d_active[s] != 0; d_slot[t] is the
+ *   row of integration t, or -1, as |time - gt.time| < interval / 2 selects it, :61-63):
+ *     point-source equivalents (:64-78; divide_visibility, visibility/operations.py:241-264): with
+ *       a model, x_t = vis / model and w_t = |model|^2 * weight where that is > 0 (else 0); without,
+ *       x_t = vis and w_t = weight; x = sum_t x_t w_t / sum_t w_t (0 where the sum is 0), for the
+ *       lower-triangle element [a2, a1] of every (antenna pair, channel);
+ *     solve_antenna_gains_itsubs_scalar (processing_library/calibration/solvers.py:61-100):
+ *       x[i, j] = conj(x[j, i]) above the diagonal, 0 on it; up to niter iterations of
+ *       gain_substitution_scalar (:103-127: new g_a = sum_k g_k x[k, a] w[k, a] / sum_k |g_k|^2 w[k, a],
+ *       0 for every channel of an antenna whose denominator is 0 in any channel), |g| = 1 if
+ *       phase_only, the phase of antenna refant removed per channel, damping
+ *       g = (1 - damping) g + damping g_last, stop when max |g - g_last| < tol;
+ *     solution_residual_scalar (:302-332): sqrt(sum over channels and pairs of |x - conj(g_i) g_j|^2 w
+ *       / sum of w), one value written for every channel (as ARL's broadcast sum gives);
+ *     normalise and not phase_only: g /= mean |g| (:79-81).
+ * d_vis, d_model (or NULL), d_weight (or NULL = 1): BlockVisibility [ntimes][nants][nants][nchan] in
+ * the given precision; d_gain complex128 [ngain][nants][nchan] (in: start, out: solution);
+ * d_gain_weight float64 (the last denominators), d_residual float64 [ngain][nchan],
+ * d_iterations int32 [ngain]; d_work: 3 * ngain * nants^2 * nchan doubles of scratch.
+ * nants * nchan <= 2048 and nchan <= 64.  Rows without integrations are left unchanged. */
+int arlmi_solve_gains(const void* d_vis, const void* d_model, const void* d_weight, int32_t ntimes, int32_t nants,
+                      int32_t nchan, const int32_t* d_slot, int32_t ngain, const int32_t* d_active, void* d_gain,
+                      double* d_gain_weight, double* d_residual, int32_t* d_iterations, double* d_work,
+                      int phase_only, int32_t niter, double tol, int32_t refant, double damping, int normalise,
+                      int precision, void* stream);
+
 /* Measurement only (no ARL counterpart): for the last arlmi_hogbom call on the calling thread,
  * the pixels whose bands were scanned for the peak (the whole image once, then the bands of
  * rows each PSF subtraction changed) and the pixels the PSF subtractions covered; the minor

@@ -67,6 +67,12 @@ Pinning (what checks each function; see tests/test_oracle_*.py):
                                            peak, iteration count from the stop rule), flux
                                            conservation dirty - res = comps convolved with the
                                            PSF (an FFT convolution), windows, argmax ties
+  solve_gaintable_scalar and its solver    ARL's own assertions (test_calibration_solvers.py:
+                                           55-115: residual < 3e-8 and gains away from 1 for
+                                           phase-only, amplitude + phase with normalisation,
+                                           several integrations per solution, point-source input),
+                                           recovery of the simulated gains up to the reference
+                                           phase, and a dead antenna's zero gain
 """
 
 import numpy
@@ -86,7 +92,8 @@ __all__ = [
     "convert_blocks_loop", "decoalesce_visibility", "apply_gaintable", "apply_gains_rows_adjoint",
     "grid_weight_to_griddata", "griddata_merge_weights", "griddata_reweight", "weight_visibility",
     "uvw_to_xyz", "skycoord_to_lmn", "phaserotate_rows", "phaserotate_blocks", "overlapIndices", "hogbom",
-    "deconvolve_cube_hogbom",
+    "deconvolve_cube_hogbom", "gain_substitution_scalar", "solution_residual_scalar",
+    "solve_antenna_gains_itsubs_scalar", "solve_gaintable_scalar",
 ]
 
 D2R = numpy.pi / 180.0
@@ -1201,3 +1208,117 @@ def deconvolve_cube_hogbom(dirty, psf, window_shape=None, window_edge=16, psf_su
                 comp[ch, pol], residual[ch, pol], _ = hogbom(dirty[ch, pol], psf[ch, pol], w, gain, threshold,
                                                               niter, fractional_threshold)
     return comp, residual, psf
+
+
+# ---------------------------------------------------------------------------
+# Scalar gain solution: processing_components/calibration/calibration.py:30-101,
+# processing_library/calibration/solvers.py:61-140, 302-332,
+# processing_components/visibility/operations.py:241-264
+# ---------------------------------------------------------------------------
+
+def gain_substitution_scalar(gain, x, xwt):
+    """One substitution (solvers.py:103-127); gain [nants, nchan, 1, 1]."""
+    nants, nchan, nrec, _ = gain.shape
+    newgain = numpy.ones_like(gain, dtype='complex')
+    gwt = numpy.zeros_like(gain, dtype='float')
+    x = x.reshape([nants, nants, nchan, nrec, nrec])
+    xwt = xwt.reshape([nants, nants, nchan, nrec, nrec])
+    xxwt = x[:, :, :, 0, 0] * xwt[:, :, :, 0, 0]
+    cgain = numpy.conjugate(gain)
+    gcg = gain[:, :, 0, 0] * cgain[:, :, 0, 0]
+    for ant1 in range(nants):
+        top = numpy.sum(gain[:, :, 0, 0] * xxwt[:, ant1, :], axis=0)
+        bot = numpy.sum((gcg[:, :] * xwt[:, ant1, :, 0, 0]).real, axis=0)
+        if bot.all() > 0.0:
+            newgain[ant1, :, 0, 0] = top / bot
+            gwt[ant1, :, 0, 0] = bot
+        else:
+            newgain[ant1, :, 0, 0] = 0.0
+            gwt[ant1, :, 0, 0] = 0.0
+    return newgain, gwt
+
+
+def solution_residual_scalar(gain, x, xwt):
+    """(solvers.py:302-332): the residual summed over every channel, broadcast to each."""
+    nant, nchan, nrec, _ = gain.shape
+    x = x.reshape(nant, nant, nchan, nrec, nrec)
+    xwt = xwt.reshape(nant, nant, nchan, nrec, nrec)
+    residual = numpy.zeros([nchan, nrec, nrec])
+    sumwt = numpy.zeros([nchan, nrec, nrec])
+    for chan in range(nchan):
+        lgain = gain[:, chan, 0, 0]
+        clgain = numpy.conjugate(lgain)
+        smueller = numpy.outer(clgain, lgain).reshape([nant, nant])
+        error = x[:, :, chan, 0, 0] - smueller
+        for i in range(nant):
+            error[i, i] = 0.0
+        residual += numpy.sum(error * xwt[:, :, chan, 0, 0] * numpy.conjugate(error)).real
+        sumwt += numpy.sum(xwt[:, :, chan, 0, 0])
+    residual[sumwt > 0.0] = numpy.sqrt(residual[sumwt > 0.0] / sumwt[sumwt > 0.0])
+    residual[sumwt <= 0.0] = 0.0
+    return residual
+
+
+def solve_antenna_gains_itsubs_scalar(gain, gwt, x, xwt, niter=30, tol=1e-8, phase_only=True, refant=0,
+                                      damping=0.5):
+    """(solvers.py:61-100): returns (gain, gwt, residual, iterations run)."""
+    nants = x.shape[0]
+    for ant1 in range(nants):
+        x[ant1, ant1, ...] = 0.0
+        xwt[ant1, ant1, ...] = 0.0
+        for ant2 in range(ant1 + 1, nants):
+            x[ant1, ant2, ...] = numpy.conjugate(x[ant2, ant1, ...])
+            xwt[ant1, ant2, ...] = xwt[ant2, ant1, ...]
+    for it in range(niter):
+        gainLast = gain
+        gain, gwt = gain_substitution_scalar(gain, x, xwt)
+        if phase_only:
+            mask = numpy.abs(gain) > 0.0
+            gain[mask] = gain[mask] / numpy.abs(gain[mask])
+        angles = numpy.angle(gain)
+        gain *= numpy.exp(-1j * angles)[refant, ...]
+        gain = (1.0 - damping) * gain + damping * gainLast
+        change = numpy.max(numpy.abs(gain - gainLast))
+        if change < tol:
+            return gain, gwt, solution_residual_scalar(gain, x, xwt), it + 1
+    return gain, gwt, solution_residual_scalar(gain, x, xwt), niter
+
+
+def solve_gaintable_scalar(bvis, model=None, gain=None, gtime=None, interval=None, phase_only=True, niter=30,
+                           tol=1e-8, normalise_gains=True):
+    """solve_gaintable for npol = 1 (calibration.py:30-101) on a BlockVisibility dict ('vis',
+    'weight' [nt, na, na, nchan, 1], 'time'), an optional model vis array of the same shape and a
+    gain table (gain [ngain, na, nchan, 1, 1], gtime, interval).  Returns (gain, weight,
+    residual [ngain, nchan, 1, 1], iterations per row)."""
+    vis, wt = bvis['vis'], bvis['weight']
+    nt, na, _, nchan, _ = vis.shape
+    gain = numpy.array(gain, dtype=complex, copy=True)
+    ngain = gain.shape[0]
+    gwt_all = numpy.ones(gain.shape)
+    res_all = numpy.zeros([ngain, nchan, 1, 1])
+    iters = numpy.zeros(ngain, dtype=int)
+    for row in range(ngain):
+        rows = numpy.abs(bvis['time'] - gtime[row]) < interval[row] / 2.0
+        if numpy.sum(rows) == 0:
+            continue
+        sv, sw = vis[rows], wt[rows]
+        if model is not None:
+            sm = model[rows]
+            xwt_p = numpy.abs(sm) ** 2 * sw
+            x_p = numpy.zeros_like(sv)
+            mask = xwt_p > 0.0
+            x_p[mask] = sv[mask] / sm[mask]
+            x = numpy.sum(x_p * xwt_p, axis=0)
+            xwt = numpy.sum(xwt_p, axis=0)
+        else:
+            x = numpy.sum(sv * sw, axis=0)
+            xwt = numpy.sum(sw, axis=0)
+        mask = numpy.abs(xwt) > 0.0
+        x[mask] = x[mask] / xwt[mask]
+        x[~mask] = 0.0
+        g, gw, r, k = solve_antenna_gains_itsubs_scalar(gain[row], gwt_all[row], x, xwt, niter=niter, tol=tol,
+                                                        phase_only=phase_only)
+        gain[row], gwt_all[row], res_all[row], iters[row] = g, gw, r, k
+        if normalise_gains and not phase_only:
+            gain[row] /= numpy.average(numpy.abs(gain[row]))
+    return gain, gwt_all, res_all, iters
