@@ -37,6 +37,7 @@ EXPORTS = (
     "arlmi_grids_to_image", "arlmi_image_to_grids", "arlmi_fft2d", "arlmi_w_screen", "arlmi_dft",
     "arlmi_grid_weights", "arlmi_reweight", "arlmi_block_to_rows", "arlmi_apply_gaintable", "arlmi_apply_gains_rows", "arlmi_apply_gains_rows_adjoint",
     "arlmi_phaserotate_rows", "arlmi_phaserotate_blocks", "arlmi_hogbom", "arlmi_hogbom_traffic", "arlmi_solve_gains",
+    "arlmi_bda_uvdist", "arlmi_bda_average",
     "arlmi_probe_lds_add", "arlmi_profile_enable", "arlmi_profile_collect", "arlmi_profile_reset",
     "arlmi_profile_report",
 )
@@ -148,6 +149,9 @@ def lib():
                                         ctypes.POINTER(i32), vp]),
         "arlmi_solve_gains": (ctypes.c_int, [vp, vp, vp, i32, i32, i32, vp, i32, vp, vp, vp, vp, vp, vp,
                                              ctypes.c_int, i32, dbl, i32, dbl, ctypes.c_int, ctypes.c_int, vp]),
+        "arlmi_bda_uvdist": (ctypes.c_int, [vp, vp, i32, i32, i32, vp, vp, ctypes.c_int, vp]),
+        "arlmi_bda_average": (ctypes.c_int, [ctypes.POINTER(Blocks), vp, vp, vp, i64, ctypes.POINTER(Rows), vp,
+                                             ctypes.c_int, vp]),
         "arlmi_hogbom_traffic": (ctypes.c_int, [ctypes.POINTER(i64), ctypes.POINTER(i64)]),
         "arlmi_probe_lds_add": (ctypes.c_int, [i32, i32, ctypes.POINTER(dbl), ctypes.POINTER(ctypes.c_float)]),
         "arlmi_profile_enable": (ctypes.c_int, [ctypes.c_int]),

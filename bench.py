@@ -57,7 +57,8 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=1000000)
-    ap.add_argument("--workload", choices=("wstack", "wstack-aw", "batch", "gradient", "phaserotate", "clean", "dft", "solve"),
+    ap.add_argument("--workload", choices=("wstack", "wstack-aw", "batch", "gradient", "phaserotate", "clean", "dft", "solve",
+                             "bda"),
                     default="wstack",
                     help="wstack: the headline line (default); wstack-aw: the same with an AW w-projection kernel "
                          "for the residual w of each slab; batch: 64 uneven observations replayed from one "
@@ -65,7 +66,8 @@ def parse():
                          "phaserotate: phase rotation (tangent=False) of the w-stack workload's rows; "
                          "clean: Hogbom minor cycle on the w-stack workload's dirty image and PSF; "
                          "dft: direct DFT of the workload's 200 sources onto its rows; "
-                         "solve: scalar gain solution of the 128-antenna differentiation workload")
+                         "solve: scalar gain solution of the 128-antenna differentiation workload; "
+                         "bda: baseline-dependent averaging of the w-stack observation as a BlockVisibility")
     ap.add_argument("--niter", type=int, default=1000, help="Hogbom iterations (clean)")
     ap.add_argument("--support", type=int, default=16, help="AW kernel support (wstack-aw)")
     ap.add_argument("--nobs", type=int, default=64)
@@ -762,6 +764,98 @@ def bench_solve(args):
     print(json.dumps(out))
 
 
+def bench_bda(args):
+    """coalesce_visibility / average_in_blocks (time_coal = frequency_coal = 1, caps 100) of the
+    w-stack observation held as a BlockVisibility: 256 antennas log-radial to 10 km, 120
+    integrations, 8 channels (63 M block elements, both triangles and the diagonal, complex64),
+    random visibilities and weights.  One step = one averaging (factors, row offsets, rows, cindex)."""
+    import numpy
+    import torch
+    import arlmi
+    from arlmi import simulation
+    from arlmi.data_models import BlockVisibility
+    rng = numpy.random.default_rng(2)
+    enu = simulation.log_radial(rng, 256, 10000.0)
+    ha = numpy.linspace(-numpy.pi / 4, numpy.pi / 4, 120)
+    freq = numpy.linspace(100e6, 150e6, 8)
+    uvw = simulation.block_uvw(simulation.site_xyz(enu), ha, -30.0 * numpy.pi / 180.0, dev="cuda")
+    shape = (120, 256, 256, 8, 1)
+    g = torch.Generator(device="cuda").manual_seed(3)
+    data = {
+        "uvw": uvw,
+        "vis": torch.complex(torch.randn(shape, device="cuda", generator=g),
+                             torch.randn(shape, device="cuda", generator=g)).contiguous(),
+        "weight": (torch.rand(shape, device="cuda", generator=g) + 0.5).contiguous(),
+        "imaging_weight": torch.ones(shape, device="cuda"),
+        "time": torch.as_tensor(ha * 43200.0 / numpy.pi).to("cuda"),
+        "integration_time": torch.full((120,), 180.0, dtype=torch.float64, device="cuda"),
+    }
+    bv = BlockVisibility(data, frequency=freq, channel_bandwidth=numpy.full(8, 6.25e6))
+    nel = int(numpy.prod(shape))
+    for _ in range(args.warmup):
+        out = arlmi.average_in_blocks(bv, 1.0, 100, 1.0, 100)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = arlmi.average_in_blocks(bv, 1.0, 100, 1.0, 100)
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / args.steps
+    L = arlmi._lib.lib()
+    L.arlmi_profile_reset()
+    L.arlmi_profile_enable(1)
+    out = arlmi.average_in_blocks(bv, 1.0, 100, 1.0, 100)
+    torch.cuda.synchronize()
+    L.arlmi_profile_enable(0)
+    prof = arlmi._lib.profile_report()
+    rows = out["vis"].shape[0]
+    krows = prof["k_bda_rows"][0] * 1e-3
+    # algorithmic bytes of k_bda_rows: every element's vis, weight, imaging weight once (16 B), every
+    # (integration, baseline) uvw once (24 B), each output row's columns once (8 + 4 + 4 + 24 + 4 x 8 + 2 x 8)
+    algo = nel * 16 + 120 * 256 * 256 * 24 + rows * (8 + 4 + 4 + 24 + 32 + 16)
+    sys.path.insert(0, ROOT)
+    from oracle import arl_oracle as O
+    ta, fa = out["time_average"], out["frequency_average"]
+    v = bv.data["vis"][:, :, :8].cpu().numpy()
+    w = bv.data["weight"][:, :, :8].cpu().numpy()
+    c0 = time.perf_counter()
+    nb = 0
+    for a2 in range(256):
+        for a1 in range(8):
+            O.average_chunks2(v[:, a2, a1, :, 0], w[:, a2, a1, :, 0], (ta[a2, a1], fa[a2, a1]))
+            O.average_chunks2(w[:, a2, a1, :, 0], w[:, a2, a1, :, 0], (ta[a2, a1], fa[a2, a1]))
+            nb += 1
+    c1 = time.perf_counter()
+    res = {
+        "metric": "M block elements/s, baseline-dependent averaging (coalesce_visibility, time_coal = "
+                  "frequency_coal = 1) of the w-stack observation as a BlockVisibility",
+        "value": round(nel / t / 1e6, 1),
+        "unit": "Melements/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64 vis, f64 coordinates",
+        "data": "synthetic",
+        "config": {
+            "workload": "256 antennas log-radial to 10 km, 120 integrations, 8 channels 100-150 MHz, every "
+                        "(a2, a1) pair, random vis and weights",
+            "elements": nel,
+            "rows_out": rows,
+            "kernel_ms": {k: round(v_[0], 3) for k, v_ in prof.items()},
+        },
+        "roofline": {"bound": "hbm", "achieved": round(algo / krows / 1e9, 1), "peak": 8000.0, "unit": "GB/s",
+                     "frac": round(algo / krows / 1e9 / 8000.0, 4), "traffic": None, "kernel": "k_bda_rows"},
+        "cpu_baseline": {"value": round(nb * 120 * 8 / (c1 - c0) / 1e6, 4), "unit": "Melements/s", "cores": 1,
+                         "kind": "port",
+                         "sample": "%d baselines: the oracle's average_chunks2 of vis and of the weights with the same "
+                                   "factors (ARL's per-baseline loop does ~10 such calls), %.2f s" % (nb, c1 - c0)},
+    }
+    print(json.dumps(res))
+
+
 def main():
     args = parse()
     if args.copy_probe:
@@ -802,6 +896,11 @@ def main():
         import torch
         torch.cuda.set_device(0)
         bench_phaserotate(args)
+        return
+    if args.workload == "bda":
+        import torch
+        torch.cuda.set_device(0)
+        bench_bda(args)
         return
     if args.workload == "solve":
         import torch

@@ -259,6 +259,31 @@ typedef struct arlmi_rows {
 int arlmi_block_to_rows(const arlmi_blocks* blocks, const arlmi_rows* rows, int64_t* d_cindex, int precision,
                         void* stream);
 
+/* Baseline-dependent averaging, coalesce_visibility with time_coal or frequency_coal != 0
+ * (processing_components/visibility/coalesce.py:36-82, average_in_blocks :146-328), Stokes I.
+ * Step 1: for every baseline b = a2 * nants + a1 (both triangles and the diagonal) the max over
+ * time of u*u + v*v (uvw in metres, :198-207) into d_uvdist2[b], and its summed weight over
+ * (time, channel) into d_wsum[b] (d_weight NULL = 1).  The caller derives the factors as ARL does
+ * (:208-226): uvmax = sqrt(max d_uvdist2); factor = round(coal * uvmax / sqrt(d_uvdist2[b]))
+ * (max_coal where that distance is 0), 1 for baselines without weight, clipped to [1, max_coal];
+ * a baseline then has ceil(ntimes / tf) x ceil(nchan / ff) cells (factors <= 1 keep every sample). */
+int arlmi_bda_uvdist(const double* d_uvw, const void* d_weight, int32_t ntimes, int32_t nants, int32_t nchan,
+                     double* d_uvdist2, double* d_wsum, int precision, void* stream);
+/* Step 2 (:247-328): d_tav, d_fav int32 [nants^2] the factors, d_rowoff int64 [nants^2 + 1] the
+ * exclusive sum of the cells per baseline (nrows = d_rowoff[nants^2]).  Row d_rowoff[b] + T * nf + F
+ * of baseline b = (a2, a1) holds the [time chunk T, frequency chunk F] cell, with average_chunks2's
+ * two-stage weighted averages (array_functions.py:85-118: each integration's channel chunk first,
+ * then the integrations with those partial weights; a factor <= 1 passes values through
+ * unchanged): vis (weights: weight), weight = the summed weights, imaging_weight (weights: weight),
+ * time, frequency, uvw in wavelengths (uvw_m * (f / c) per sample), integration time and
+ * channel bandwidth as averages times the baseline's number of cells (ARL's sum_from_grid),
+ * antenna1 = a1, antenna2 = a2.  The blocks need time, integration time, frequency and channel
+ * bandwidth; weight and imaging weight may be NULL (1).  d_cindex (int64 [ntimes][nants][nants]
+ * [nchan], or NULL) receives d_rowoff[b] + ((t * nchan + ch) mod cells of b) for element
+ * (t, a2, a1, ch), as ARL's flat assignment of range(cells) gives it (:278). */
+int arlmi_bda_average(const arlmi_blocks* blocks, const int32_t* d_tav, const int32_t* d_fav, const int64_t* d_rowoff,
+                      int64_t nrows, const arlmi_rows* rows, int64_t* d_cindex, int precision, void* stream);
+
 /* apply_gaintable with scalar gains (processing_components/calibration/operations.py:79-164,
  * scalar branch :127-140), in place on a BlockVisibility's vis [ntimes][nants][nants][nchan]:
  *   vis[t][i][j][ch] *= lg[s][i][ch] * conj(lg[s][j][ch]),   s = d_gain_slot[t],

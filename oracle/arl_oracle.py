@@ -73,6 +73,12 @@ Pinning (what checks each function; see tests/test_oracle_*.py):
                                            several integrations per solution, point-source input),
                                            recovery of the simulated gains up to the reference
                                            phase, and a dead antenna's zero gain
+  average_chunks(2), bda_factors, average_in_blocks
+                                           ARL's assertions (test_visibility_coalesce.py:39-110:
+                                           row counts, the minimum frequency kept, decoalesce
+                                           sizes), closed forms (factor 1 is the plain conversion;
+                                           constant data average to themselves; weighted sums
+                                           are conserved), brute-force chunk sums
 """
 
 import numpy
@@ -93,7 +99,8 @@ __all__ = [
     "grid_weight_to_griddata", "griddata_merge_weights", "griddata_reweight", "weight_visibility",
     "uvw_to_xyz", "skycoord_to_lmn", "phaserotate_rows", "phaserotate_blocks", "overlapIndices", "hogbom",
     "deconvolve_cube_hogbom", "gain_substitution_scalar", "solution_residual_scalar",
-    "solve_antenna_gains_itsubs_scalar", "solve_gaintable_scalar",
+    "solve_antenna_gains_itsubs_scalar", "solve_gaintable_scalar", "average_chunks", "average_chunks2",
+    "bda_factors", "average_in_blocks",
 ]
 
 D2R = numpy.pi / 180.0
@@ -1322,3 +1329,114 @@ def solve_gaintable_scalar(bvis, model=None, gain=None, gtime=None, interval=Non
         if normalise_gains and not phase_only:
             gain[row] /= numpy.average(numpy.abs(gain[row]))
     return gain, gwt_all, res_all, iters
+
+
+# ---------------------------------------------------------------------------
+# Baseline-dependent averaging: processing_components/visibility/coalesce.py:36-82, 146-328,
+# processing_library/util/array_functions.py:49-118
+# ---------------------------------------------------------------------------
+
+def average_chunks(arr, wts, chunksize):
+    """(array_functions.py:49-82): chunk sums of wts * arr and of wts, the former divided by the
+    latter where it is > 0; chunksize <= 1 returns the inputs themselves."""
+    if chunksize <= 1:
+        return arr, wts
+    n = len(arr)
+    starts = range(0, n, chunksize)
+    chunks = numpy.array([numpy.sum((wts * arr)[i:i + chunksize]) for i in starts])
+    weights = numpy.array([numpy.sum(wts[i:i + chunksize]) for i in starts])
+    out = chunks.copy()
+    pos = weights > 0.0
+    out[pos] = chunks[pos] / weights[pos]
+    return out, weights
+
+
+def average_chunks2(arr, wts, chunksize):
+    """(array_functions.py:85-118): average_chunks along axis 1 for every row, then along axis 0
+    for every column of the result, with the partial weights."""
+    wts = wts.reshape(arr.shape)
+    l0 = len(average_chunks(arr[:, 0], wts[:, 0], chunksize[0])[0])
+    l1 = len(average_chunks(arr[0, :], wts[0, :], chunksize[1])[0])
+    tempchunks = numpy.zeros([arr.shape[0], l1], dtype=arr.dtype)
+    tempwt = numpy.zeros([arr.shape[0], l1])
+    for i in range(arr.shape[0]):
+        r = average_chunks(arr[i, :], wts[i, :], chunksize[1])
+        tempchunks[i, :], tempwt[i, :] = r[0].flatten(), r[1].flatten()
+    chunks = numpy.zeros([l0, l1], dtype=arr.dtype)
+    weights = numpy.zeros([l0, l1])
+    for i in range(l1):
+        r = average_chunks(tempchunks[:, i], tempwt[:, i], chunksize[0])
+        chunks[:, i], weights[:, i] = r[0].flatten(), r[1].flatten()
+    return chunks, weights
+
+
+def bda_factors(uvw, wts, time_coal, max_time_coal, frequency_coal, max_frequency_coal):
+    """Per-baseline averaging factors of average_in_blocks (coalesce.py:198-233): uvmax over every
+    element, uvdist_max over time per (a2, a1), round(coal * uvmax / uvdist_max) clipped to
+    [1, max] (max where uvdist_max is 0; baselines without weight also end at 1)."""
+    nant = uvw.shape[1]
+    uvd = uvw[..., 0] * uvw[..., 0] + uvw[..., 1] * uvw[..., 1]
+    uvmax = numpy.sqrt(numpy.max(uvd))
+    uvdist_max = numpy.sqrt(numpy.max(uvd, axis=0))
+    out = []
+    for coal, mx in ((time_coal, max_time_coal), (frequency_coal, max_frequency_coal)):
+        f = numpy.ones([nant, nant], dtype=numpy.int64)
+        pos = uvdist_max > 0.0
+        f[pos] = numpy.round(coal * uvmax / uvdist_max[pos])
+        f[~pos] = mx
+        f[numpy.sum(wts, axis=(0, 3, 4)) == 0] = 0  # putmask(..., allpwtsgrid_bool == 0, 0)
+        f[f < 1] = 1
+        f[f > mx] = mx
+        out.append(f)
+    return out[0], out[1]
+
+
+def average_in_blocks(vis, uvw, wts, imaging_wts, times, integration_time, frequency, channel_bandwidth,
+                      time_coal=1.0, max_time_coal=100, frequency_coal=1.0, max_frequency_coal=100):
+    """(coalesce.py:146-328): rows of every (a2, a1) baseline in a2, a1 order (both triangles and the
+    diagonal), each the [time chunk, frequency chunk] grid of its averages.  Returns a dict of the
+    columns and cindex, whose element (t, a2, a1, ch) takes row start + ((t * nchan + ch) mod rows)
+    of its baseline (ARL's flat assignment of range(rows) cycles over the ntimes x nchan indices)."""
+    ntimes, nant, _, nchan, npol = vis.shape
+    allp = numpy.sum(wts, axis=4)
+    ta, fa = bda_factors(uvw, wts, time_coal, max_time_coal, frequency_coal, max_frequency_coal)
+    frequency_grid, time_grid = numpy.meshgrid(frequency, times)
+    bw_grid, it_grid = numpy.meshgrid(channel_bandwidth, integration_time)
+    cols = {k: [] for k in ('vis', 'weight', 'imaging_weight', 'time', 'frequency', 'uvw', 'integration_time',
+                            'channel_bandwidth', 'antenna1', 'antenna2')}
+    cindex = numpy.zeros(ntimes * nant * nant * nchan, dtype=int)
+    start = 0
+    for a2 in range(nant):
+        for a1 in range(nant):
+            w2 = allp[:, a2, a1, :]
+            ch = (ta[a2, a1], fa[a2, a1])
+
+            def avg(arr):
+                return average_chunks2(arr, w2, ch)[0]
+
+            t_avg = avg(time_grid)
+            rows = t_avg.size
+            cols['time'].append(t_avg.flatten())
+            cols['frequency'].append(avg(frequency_grid).flatten())
+            cols['uvw'].append(numpy.stack([avg(numpy.outer(uvw[:, a2, a1, k], frequency / C_LIGHT)).flatten()
+                                            for k in range(3)], axis=1))
+            cols['integration_time'].append((avg(it_grid) * rows).flatten())
+            cols['channel_bandwidth'].append((avg(bw_grid) * rows).flatten())
+            pv, pw, pi = [], [], []
+            for pol in range(npol):
+                r = average_chunks2(vis[:, a2, a1, :, pol], wts[:, a2, a1, :, pol], ch)
+                pv.append(r[0].flatten())
+                pw.append(r[1].flatten())
+                pi.append(average_chunks2(imaging_wts[:, a2, a1, :, pol], wts[:, a2, a1, :, pol], ch)[0].flatten())
+            cols['vis'].append(numpy.stack(pv, axis=1))
+            cols['weight'].append(numpy.stack(pw, axis=1))
+            cols['imaging_weight'].append(numpy.stack(pi, axis=1))
+            cols['antenna1'].append(numpy.full(rows, a1))
+            cols['antenna2'].append(numpy.full(rows, a2))
+            tt, cc = numpy.meshgrid(numpy.arange(ntimes), numpy.arange(nchan), indexing='ij')
+            elem = ((tt * nant + a2) * nant + a1) * nchan + cc
+            cindex[elem.ravel()] = start + (numpy.arange(ntimes * nchan) % rows)
+            start += rows
+    out = {k: numpy.concatenate(v) for k, v in cols.items()}
+    out['cindex'] = cindex
+    return out
