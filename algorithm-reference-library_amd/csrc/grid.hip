@@ -428,11 +428,37 @@ __global__ __launch_bounds__(GRID_THREADS) void k_grid_tab(const int4* __restric
 // degridding
 // ---------------------------------------------------------------------------
 
-template <typename R>
+// Stage the tile and its halo (Wd x Wd grid values from the tile origin, 0 past the grid edge)
+// in LDS.  When every thread has at most MAXL values to move, all of its loads are issued before
+// any LDS store, so a block waits for one HBM latency, not one per value.
+template <typename R, int MAXL = 16>
 __device__ inline void load_halo(const cplx<R>* __restrict__ grids, R* lds, int Wd, int T, int N, const TilePos& tp) {
   const cplx<R>* g = grids + (size_t)tp.plane * N * N;
   const int y0 = tp.ty * T, x0 = tp.tx * T;
-  for (int i = threadIdx.x; i < Wd * Wd; i += blockDim.x) {
+  const int n = Wd * Wd;
+  if (n <= MAXL * (int)blockDim.x) {
+    cplx<R> v[MAXL];
+#pragma unroll
+    for (int j = 0; j < MAXL; ++j) {
+      const int i = threadIdx.x + j * (int)blockDim.x;
+      v[j] = mk<R>(R(0), R(0));
+      if (i < n) {
+        const int y = i / Wd, x = i - (i / Wd) * Wd;
+        const int gy = y0 + y, gx = x0 + x;
+        if (gy < N && gx < N) v[j] = g[(size_t)gy * N + gx];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < MAXL; ++j) {
+      const int i = threadIdx.x + j * (int)blockDim.x;
+      if (i < n) {
+        lds[2 * i] = v[j].x;
+        lds[2 * i + 1] = v[j].y;
+      }
+    }
+    return;
+  }
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
     const int y = i / Wd, x = i - (i / Wd) * Wd;
     const int gy = y0 + y, gx = x0 + x;
     cplx<R> v = mk<R>(R(0), R(0));
@@ -536,7 +562,7 @@ __global__ __launch_bounds__(DEGRID_TAB_THREADS) void k_degrid_tab(const int4* _
   R* lds = reinterpret_cast<R*>(smem);
   const int4 it = items[blockIdx.x];
   const TilePos tp = tile_pos(it.x, ntx);
-  load_halo<R>(grids, lds, Wd, T, N, tp);
+  load_halo<R, 32>(grids, lds, Wd, T, N, tp);
   __syncthreads();
   const int lane = __lane_id(), wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
   const int slot = lane / L, a = lane % L;
