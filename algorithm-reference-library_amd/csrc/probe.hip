@@ -17,7 +17,8 @@ __device__ inline unsigned lcg(unsigned& s) {
 // pattern 0: each wavefront adds 1 to an 8 x 8 footprint (row stride 72) at a random origin of
 // the tile, real and imaginary planes, as the gridder does; pattern 1: lane l adds to address
 // 64 * k + l (64 distinct consecutive doubles: no bank conflict); pattern 2: pattern 0 with
-// 64-bit integer adds.  All addresses < 2 * 4608.
+// 64-bit integer adds; pattern 3: pattern 0 with float32 adds (ds_add_f32) on a float tile of the
+// same shape; pattern 4: pattern 1 with float32 adds.  All addresses < 2 * 4608.
 template <int PATTERN>
 __global__ __launch_bounds__(kProbeThreads) void k_probe_lds_add(double* out, int iters, unsigned seed) {
   extern __shared__ double t[];
@@ -29,13 +30,17 @@ __global__ __launch_bounds__(kProbeThreads) void k_probe_lds_add(double* out, in
   for (int it = 0; it < iters; ++it) {
     const unsigned r = __builtin_amdgcn_readfirstlane(lcg(s));
     int o;
-    if (PATTERN != 1) {
+    if (PATTERN != 1 && PATTERN != 4) {
       const int ox = (int)(r % 57u), oy = (int)((r >> 8) % 57u);
       o = (oy + ta) * 72 + ox + tb;
     } else {
       o = (int)(r % 72u) * 64 + lane;
     }
-    if (PATTERN == 2) {  // the gridder pattern with 64-bit integer adds
+    if (PATTERN == 3 || PATTERN == 4) {  // float32 adds
+      float* f = reinterpret_cast<float*>(t);
+      __hip_atomic_fetch_add(&f[o], 1.0f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      __hip_atomic_fetch_add(&f[kProbeTile + o], 2.0f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    } else if (PATTERN == 2) {  // the gridder pattern with 64-bit integer adds
       unsigned long long* u = reinterpret_cast<unsigned long long*>(t);
       __hip_atomic_fetch_add(&u[o], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       __hip_atomic_fetch_add(&u[kProbeTile + o], 2ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -56,7 +61,7 @@ using namespace arlmi;
 extern "C" {
 
 int arlmi_probe_lds_add(int32_t pattern, int32_t iters, double* cycles_per_instr, float* ms_out) {
-  if (pattern < 0 || pattern > 2 || iters <= 0 || !cycles_per_instr) {
+  if (pattern < 0 || pattern > 4 || iters <= 0 || !cycles_per_instr) {
     set_error("arlmi_probe_lds_add: bad argument");
     return ARLMI_E_ARG;
   }
@@ -78,8 +83,12 @@ int arlmi_probe_lds_add(int32_t pattern, int32_t iters, double* cycles_per_instr
       hipLaunchKernelGGL(k_probe_lds_add<0>, dim3(blocks), dim3(kProbeThreads), sh, nullptr, out, iters, 1u);
     else if (pattern == 1)
       hipLaunchKernelGGL(k_probe_lds_add<1>, dim3(blocks), dim3(kProbeThreads), sh, nullptr, out, iters, 1u);
-    else
+    else if (pattern == 2)
       hipLaunchKernelGGL(k_probe_lds_add<2>, dim3(blocks), dim3(kProbeThreads), sh, nullptr, out, iters, 1u);
+    else if (pattern == 3)
+      hipLaunchKernelGGL(k_probe_lds_add<3>, dim3(blocks), dim3(kProbeThreads), sh, nullptr, out, iters, 1u);
+    else
+      hipLaunchKernelGGL(k_probe_lds_add<4>, dim3(blocks), dim3(kProbeThreads), sh, nullptr, out, iters, 1u);
     ARLMI_LAUNCH_CHECK();
     ARLMI_HIP(hipEventRecord(b, nullptr));
     ARLMI_HIP(hipEventSynchronize(b));

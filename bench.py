@@ -868,7 +868,8 @@ def main():
         torch.cuda.set_device(0)
         L = arlmi._lib.lib()
         res = {}
-        for pattern, name in ((0, "gridder_footprints"), (1, "conflict_free"), (2, "gridder_footprints_u64")):
+        for pattern, name in ((0, "gridder_footprints"), (1, "conflict_free"), (2, "gridder_footprints_u64"),
+                              (3, "gridder_footprints_f32"), (4, "conflict_free_f32")):
             cyc, ms = ctypes.c_double(), ctypes.c_float()
             arlmi._lib.check(L.arlmi_probe_lds_add(pattern, 4096, ctypes.byref(cyc), ctypes.byref(ms)))
             res[name] = {"cycles_per_lds_add_wave_instr_per_cu": round(cyc.value, 3), "ms": round(ms.value, 3)}
