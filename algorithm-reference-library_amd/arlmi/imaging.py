@@ -122,14 +122,22 @@ class Imager:
         """[nchan, 1] float64 (host) sum of weights of the last invert."""
         return self.sumwt_planes.sum(dim=0).cpu().numpy()[:, None]
 
+    def separate_predict_grids(self, on=True):
+        """Give predict grids of its own, so that a predict and an invert on the same plan may run
+        at the same time on two streams (they share nothing else the calls write).  Costs one more
+        grid buffer."""
+        self._pgrids = grids_buffer(self.plan, self.precision, self.dev) if on else None
+
     def predict(self, vis, model_data, out=None):
         """Model visibilities: gcf * conj(screen) * model, shifted FFT, degridding."""
         rdt = _real_dtype(self.precision)
         model = model_data.to(device=self.dev, dtype=rdt).contiguous()
-        _lib.check(_lib.lib().arlmi_image_to_grids(self.plan.handle, ptr(model), ptr(self.gcf_dev), ptr(self.grids),
+        grids = getattr(self, "_pgrids", None)
+        grids = self.grids if grids is None else grids
+        _lib.check(_lib.lib().arlmi_image_to_grids(self.plan.handle, ptr(model), ptr(self.gcf_dev), ptr(grids),
                                                    self.precision, stream_handle()))
         vis_out = out if out is not None else torch.empty_like(vis.vis)
-        degrid_planes(self.plan, self.grids, self.cf, vis_out, self.precision)
+        degrid_planes(self.plan, grids, self.cf, vis_out, self.precision)
         return vis_out
 
 

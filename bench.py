@@ -9,7 +9,9 @@ kernel footprint leaves the grid are not generated (the kept count is what is pr
 One step = one pass of the hot path over all rows with the inputs resident in HBM:
   plan rebuild from uvw (convolution mapping, w-slab membership and means, tile binning)
   + invert (gridding, shifted inverse FFTs, screens, gcf, sum-of-weights normalisation)
-  + predict (model * gcf * conj(screen), shifted FFTs, degridding).
+  + predict (model * gcf * conj(screen), shifted FFTs, degridding);
+after the plan, the predict runs on a second stream (with grids of its own) concurrently with the
+invert (--no-overlap runs them one after the other).
 The same step is timed in the single precision form (complex64 visibilities and grids, float64
 coordinates / phases / correction) and the double precision form; the reported value is the
 single precision form's whole-job Mvis/s (one row = one channel-visibility through one
@@ -71,6 +73,9 @@ def parse():
     ap.add_argument("--niter", type=int, default=1000, help="Hogbom iterations (clean)")
     ap.add_argument("--support", type=int, default=16, help="AW kernel support (wstack-aw)")
     ap.add_argument("--nobs", type=int, default=64)
+    ap.add_argument("--no-overlap", action="store_true",
+                    help="run each step's predict after its invert on one stream (default: predict on a "
+                         "second stream, concurrently with the invert)")
     ap.add_argument("--lds-probe", action="store_true",
                     help="time ds_add_f64 for the gridder's pattern and a conflict-free one, and exit")
     ap.add_argument("--copy-probe", action="store_true",
@@ -110,10 +115,22 @@ def measured_traffic(kernel, N, P):
     return None if k is None else k.get("hbm_bytes_per_launch")
 
 
-def step(imager, vis, model, out_img, out_vis):
+def step(imager, vis, model, out_img, out_vis, side=None):
+    """Plan rebuild, invert and predict.  With a side stream, predict (its own grids) runs on it
+    while invert runs on the current stream: both only read the plan, so the LDS-bound gridding
+    of the invert can overlap the HBM-bound transforms and degridding of the predict."""
+    import torch
     imager.update(vis)
+    if side is None:
+        imager.invert(vis, out=out_img)
+        imager.predict(vis, model, out=out_vis)
+        return
+    main = torch.cuda.current_stream()
+    side.wait_stream(main)  # the plan is complete
+    with torch.cuda.stream(side):
+        imager.predict(vis, model, out=out_vis)
     imager.invert(vis, out=out_img)
-    imager.predict(vis, model, out=out_vis)
+    main.wait_stream(side)
 
 
 def time_form(precision, args, dev):
@@ -145,8 +162,12 @@ def time_form(precision, args, dev):
             model[0, 0, y, x] += float(c.flux[0, 0])
     out_img = torch.empty_like(im.data)
     out_vis = torch.empty_like(vis.vis)
+    side = None
+    if not args.no_overlap:
+        imager.separate_predict_grids()
+        side = torch.cuda.Stream()
     for _ in range(args.warmup):
-        step(imager, vis, model, out_img, out_vis)
+        step(imager, vis, model, out_img, out_vis, side)
     torch.cuda.synchronize()
     L = arlmi._lib.lib()
     L.arlmi_profile_reset()
@@ -154,7 +175,7 @@ def time_form(precision, args, dev):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        step(imager, vis, model, out_img, out_vis)
+        step(imager, vis, model, out_img, out_vis, side)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     L.arlmi_profile_enable(0)
