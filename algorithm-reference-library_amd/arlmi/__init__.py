@@ -15,7 +15,8 @@ from .gridding import (convolution_mapping, grid_visibility_to_griddata, grid_vi
                        degrid_visibility_from_griddata, fft_griddata_to_image, fft_image_to_griddata,
                        create_griddata_from_image)
 from .imaging import (invert_2d, predict_2d, invert_wstack_single, predict_wstack_single, normalize_sumwt,
-                      shift_vis_to_image, predict_skycomponent_visibility, create_image_from_visibility, Imager)
+                      shift_vis_to_image, image_phasecentre, predict_skycomponent_visibility,
+                      create_image_from_visibility, Imager)
 from .workflows import (imaging_contexts, imaging_context, invert_list_serial_workflow,
                         predict_list_serial_workflow, sum_invert_results)
 from .fft_support import fft, ifft
@@ -26,7 +27,7 @@ from .calibration import (create_gaintable_from_blockvisibility, apply_gaintable
                           apply_gains_rows_adjoint, solve_gaintable)
 from .differentiable import PredictFunction, GainsFunction, predict_with_gains, visibility_loss
 from .batch import Autotuner, BatchImager
-from .phaserotate import skycoord_to_lmn, phaserotate_visibility
+from .phaserotate import skycoord_to_lmn, separation, phaserotate_visibility
 from .deconvolution import hogbom, deconvolve_cube
 from .weighting import (grid_weight_to_griddata, griddata_merge_weights, griddata_reweight, weight_visibility,
                         weight_list_serial_workflow)

@@ -40,6 +40,8 @@ class WCS:
     def __init__(self, naxis=4):
         self.naxis = naxis
         self.wcs = _WcsPrm(naxis)
+        # True once crval of the RA---SIN / DEC--SIN axes holds a sky position
+        self.celestial = False
 
     def deepcopy(self):
         return copy.deepcopy(self)
@@ -90,6 +92,14 @@ class Image:
     @property
     def npixel(self):
         return self.data.shape[3]
+
+    @property
+    def phasecentre(self):
+        """SkyCoord of crval's (ra, dec) (memory_data_models.py:422-424), or None for an image
+        made without a sky position."""
+        if self.wcs is None or not getattr(self.wcs, "celestial", False):
+            return None
+        return SkyCoord.from_degrees(self.wcs.wcs.crval[0], self.wcs.wcs.crval[1])
 
     @property
     def cellsize(self):
@@ -374,6 +384,17 @@ class SkyCoord:
 
     def __repr__(self):
         return "SkyCoord(ra=%r, dec=%r)" % (self.ra, self.dec)
+
+
+def _radec(pos):
+    """(ra, dec) in radians of a SkyCoord here, an (ra, dec) pair in radians, or an object with
+    astropy's .ra.rad / .dec.rad."""
+    if isinstance(pos, SkyCoord):
+        return pos.ra, pos.dec
+    if hasattr(pos, "ra") and hasattr(pos.ra, "rad"):
+        return float(pos.ra.rad), float(pos.dec.rad)
+    ra, dec = pos
+    return float(ra), float(dec)
 
 
 class GainTable:

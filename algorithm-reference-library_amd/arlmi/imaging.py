@@ -8,12 +8,13 @@ import numpy
 import torch
 
 from . import _lib
-from .data_models import Image, Visibility, Skycomponent, device
+from .data_models import BlockVisibility, D2R, Image, SkyCoord, Skycomponent, Visibility, device
 from .kernels import create_pswf_convolutionfunction, create_image
 from .plan import Plan, ptr, stream_handle, grids_buffer
 from .gridding import grid_planes, degrid_planes
+from .phaserotate import phaserotate_visibility, rotation_lmn, separation
 
-__all__ = ["get_parameter", "normalize_sumwt", "shift_vis_to_image", "invert_2d", "predict_2d",
+__all__ = ["get_parameter", "normalize_sumwt", "shift_vis_to_image", "image_phasecentre", "invert_2d", "predict_2d",
            "invert_wstack_single", "predict_wstack_single", "predict_skycomponent_visibility",
            "create_image_from_visibility", "Imager"]
 
@@ -36,9 +37,73 @@ def normalize_sumwt(im, sumwt):
     return im
 
 
+def image_phasecentre(im):
+    """Sky direction of the image's FFT centre, the 1-relative pixel (N//2 + 1, N//2 + 1), as
+    imaging/base.py:70 computes it with pixel_to_skycoord (origin 1); None for an image made
+    without a sky position.  The RA---SIN / DEC--SIN intermediate coordinates
+    (l, m) = cdelt * (pixel - crpix) in radians are direction cosines about crval, inverted with
+    n = sqrt(1 - l^2 - m^2): dec = asin(m cos dec0 + n sin dec0),
+    ra = ra0 + atan2(l, n cos dec0 - m sin dec0).  At crpix it is crval exactly."""
+    if im.wcs is None or not getattr(im.wcs, "celestial", False):
+        return None
+    w = im.wcs.wcs
+    ny, nx = im.data.shape[2], im.data.shape[3]
+    l = w.cdelt[0] * D2R * ((nx // 2 + 1) - w.crpix[0])
+    m = w.cdelt[1] * D2R * ((ny // 2 + 1) - w.crpix[1])
+    if l == 0.0 and m == 0.0:
+        return im.phasecentre
+    ra0, dec0 = w.crval[0] * D2R, w.crval[1] * D2R
+    n = numpy.sqrt(1.0 - l * l - m * m)
+    dec = numpy.arcsin(m * numpy.cos(dec0) + n * numpy.sin(dec0))
+    ra = ra0 + numpy.arctan2(l, n * numpy.cos(dec0) - m * numpy.sin(dec0))
+    return SkyCoord(ra, dec)
+
+
+def _shift_lmn(vis, im):
+    """The rotation shift_vis_to_image makes for (vis, im) as [l, m, n - 1] float64, or None: no
+    sky position on either side, centres within 1e-15 rad (imaging/base.py:71), or |n| < 1e-15."""
+    centre = image_phasecentre(im)
+    if centre is None or vis.phasecentre is None:
+        return None
+    if separation(vis.phasecentre, centre) <= 1e-15:
+        return None
+    return rotation_lmn(centre, vis.phasecentre)
+
+
+def _rotate_rows(uvw, data, lmn, inverse, precision):
+    """Multiply row visibilities ``data`` [V, npol] in place by the conjugate phasor of the
+    direction lmn (by the phasor itself for ``inverse``) evaluated on ``uvw`` [V, 3], which is only
+    read: the tangent-plane rotation of phaserotate_visibility (visibility/base.py:318-340)."""
+    npol = data.shape[1] if data.dim() > 1 else 1
+    _lib.check(_lib.lib().arlmi_phaserotate_rows(ptr(uvw), ptr(data), data.shape[0], npol,
+                                                 lmn.ctypes.data_as(ctypes.c_void_p), None, int(bool(inverse)),
+                                                 precision, stream_handle()))
+
+
+def _labelled(out, vis, im):
+    """Predicted visibilities carry the image's phase centre when a shift was made, as
+    shift_vis_to_image(inverse=True) leaves them (imaging/base.py:71-80)."""
+    centre = image_phasecentre(im)
+    if centre is not None and vis.phasecentre is not None and separation(vis.phasecentre, centre) > 1e-15:
+        out.phasecentre = im.phasecentre
+    return out
+
+
 def shift_vis_to_image(vis, im, tangent=True, inverse=False):
-    """imaging/base.py:50-81.  Images here are always centred on the visibility phase centre,
-    so the phase rotation ARL would apply is the identity."""
+    """Shift visibilities to the image's FFT-centre direction (imaging/base.py:50-81).
+
+    When that direction is more than 1e-15 rad from the visibility phase centre, the result is
+    phaserotate_visibility(vis, centre, tangent, inverse) with its phase centre set to the
+    image's (for ``inverse`` too, as ARL labels it); otherwise ``vis`` itself.  A visibility or
+    image without a sky position is returned unchanged."""
+    if not isinstance(vis, (Visibility, BlockVisibility)):
+        raise ValueError("vis is not a Visibility or BlockVisibility: %r" % (vis,))
+    centre = image_phasecentre(im)
+    if centre is None or vis.phasecentre is None:
+        return vis
+    if separation(vis.phasecentre, centre) > 1e-15:
+        vis = phaserotate_visibility(vis, centre, tangent=tangent, inverse=inverse)
+        vis.phasecentre = im.phasecentre
     return vis
 
 
@@ -91,6 +156,11 @@ class Imager:
         """Dirty image (or PSF): grid, shifted inverse FFT, screens, gcf and sum-of-weights."""
         if vis.precision != self.precision:
             raise TypeError("visibility precision changed")
+        lmn = _shift_lmn(vis, self.im)
+        if lmn is not None:  # shift_vis_to_image on a copy (imaging/base.py:160-165)
+            rot = torch.ones_like(vis.vis) if dopsf else vis.vis.clone()
+            _rotate_rows(self._shift_uvw(vis), rot, lmn, False, self.precision)
+            vis, dopsf = vis.with_vis(rot), False
         grid_planes(self.plan, vis, self.cf, self.grids, self.sumwt_planes, dopsf=dopsf)
         n, nchan = self.plan.npixel, self.plan.nchan
         rdt = _real_dtype(self.precision)
@@ -138,7 +208,28 @@ class Imager:
                                                    self.precision, stream_handle()))
         vis_out = out if out is not None else torch.empty_like(vis.vis)
         degrid_planes(self.plan, grids, self.cf, vis_out, self.precision)
+        lmn = _shift_lmn(vis, self.im)
+        if lmn is not None:  # back from the image centre (imaging/base.py:135-136)
+            _rotate_rows(self._shift_uvw(vis), vis_out, lmn, True, self.precision)
         return vis_out
+
+    def _shift_uvw(self, vis):
+        """The uvw the image-centre shift is evaluated on: the rows' own in 2-D mode; in w-stack
+        mode w less the mean w of the row's slab (rows placed by |w - box| < width/2), since
+        invert_wstack_single / predict_wstack_single subtract it before invert_2d / predict_2d
+        shift (wstack_single.py:43-47, 83-86)."""
+        uvw = vis.data["uvw"]
+        if self.mode != "wstack":
+            return uvw
+        boxes, width = self.plan.slab_boxes()
+        wbar = self.plan.plane_w()
+        w = uvw[:, 2]
+        off = torch.zeros_like(w)
+        for box, mean in zip(boxes.tolist(), wbar.tolist()):
+            off = torch.where(torch.abs(w - box) < 0.5 * width, torch.full_like(w, mean), off)
+        out = uvw.clone()
+        out[:, 2] = w - off
+        return out
 
 
 def invert_2d(vis, im, dopsf=False, normalize=True, gcfcf=None, **kwargs):
@@ -162,7 +253,7 @@ def predict_2d(vis, model, gcfcf=None, **kwargs):
     if model is None:
         return vis
     imager = Imager(vis, model, gcfcf=gcfcf, mode="2d", **kwargs)
-    return vis.with_vis(imager.predict(vis, model.data))
+    return _labelled(vis.with_vis(imager.predict(vis, model.data)), vis, model)
 
 
 def invert_wstack_single(vis, im, dopsf, normalize=True, remove=True, gcfcf=None, **kwargs):
@@ -181,7 +272,7 @@ def predict_wstack_single(vis, model, remove=True, gcfcf=None, **kwargs):
         raise NotImplementedError("remove=False is not supported")
     imager = Imager(vis, model, gcfcf=gcfcf, mode="wstack", vis_slices=1,
                     slab_boxes=(numpy.zeros(1), float("inf")), **kwargs)
-    return vis.with_vis(imager.predict(vis, model.data))
+    return _labelled(vis.with_vis(imager.predict(vis, model.data)), vis, model)
 
 
 def predict_skycomponent_visibility(vis, sc, chan_map=None):
@@ -217,7 +308,8 @@ def predict_skycomponent_visibility(vis, sc, chan_map=None):
 
 def create_image_from_visibility(vis, **kwargs):
     """Empty image for a visibility set (imaging/base.py:247-339): default cell is half the
-    critical cell 1/(2 max|u|), capped at the critical cell; MFS unless nchan is given."""
+    critical cell 1/(2 max|u|), capped at the critical cell; MFS unless nchan is given; centred
+    on the ``phasecentre`` keyword, by default the visibility phase centre (base.py:269, 333)."""
     npixel = get_parameter(kwargs, "npixel", 512)
     uvmax = float(torch.max(torch.abs(vis.data["uvw"][:, 0:1])).item())
     critical = 1.0 / (uvmax * 2.0)
@@ -231,5 +323,7 @@ def create_image_from_visibility(vis, **kwargs):
     bw = vis.data["channel_bandwidth"][0].item() if "channel_bandwidth" in vis.data else 1e6
     channel_bandwidth = get_parameter(kwargs, "channel_bandwidth", 0.99999999999 * bw)
     dtype = torch.float32 if vis.precision == 32 else torch.float64
+    phasecentre = get_parameter(kwargs, "phasecentre", vis.phasecentre)
     return create_image(npixel=npixel, cellsize=cellsize, frequency=numpy.atleast_1d(frequency),
-                        channel_bandwidth=numpy.atleast_1d(channel_bandwidth), nchan=nchan, dtype=dtype)
+                        channel_bandwidth=numpy.atleast_1d(channel_bandwidth), nchan=nchan, dtype=dtype,
+                        phasecentre=phasecentre)

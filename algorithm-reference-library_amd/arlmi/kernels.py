@@ -13,7 +13,7 @@ import numpy
 import torch
 
 from . import _lib
-from .data_models import WCS, ConvolutionFunction, Image, D2R, device
+from .data_models import WCS, ConvolutionFunction, Image, D2R, device, _radec
 from .plan import ptr, stream_handle
 
 __all__ = ["grdsf", "create_image", "create_convolutionfunction_from_image", "create_pswf_convolutionfunction",
@@ -52,10 +52,13 @@ def _coordinates(n):
 
 
 def create_image(npixel=512, cellsize=0.000015, frequency=None, channel_bandwidth=None, nchan=None,
-                 dtype=torch.float32, dev=None):
+                 dtype=torch.float32, dev=None, phasecentre=None):
     """Empty image with ARL's WCS numbers (processing_library/image/operations.py:33-69).
 
-    cdelt = [-cell*180/pi, +cell*180/pi, 1, bandwidth], crpix = [N//2+1, N//2+1, 1, 1].
+    cdelt = [-cell*180/pi, +cell*180/pi, 1, bandwidth], crpix = [N//2+1, N//2+1, 1, 1],
+    crval = [ra, dec (degrees) of ``phasecentre``, 1, frequency].  Without a phase centre the
+    image has no sky position (crval 0, 0 and ``wcs.celestial`` False): invert and predict then
+    take it to be centred on the visibility phase centre.
     """
     frequency = numpy.atleast_1d(frequency if frequency is not None else numpy.array([1e8]))
     channel_bandwidth = numpy.atleast_1d(channel_bandwidth if channel_bandwidth is not None else numpy.array([1e6]))
@@ -67,6 +70,10 @@ def create_image(npixel=512, cellsize=0.000015, frequency=None, channel_bandwidt
     w.wcs.crpix = numpy.array([npixel // 2 + 1, npixel // 2 + 1, 1.0, 1.0], dtype=float)
     w.wcs.ctype = ["RA---SIN", "DEC--SIN", "STOKES", "FREQ"]
     w.wcs.crval = numpy.array([0.0, 0.0, 1.0, float(frequency[0])])
+    if phasecentre is not None:
+        ra, dec = _radec(phasecentre)
+        w.wcs.crval[0], w.wcs.crval[1] = numpy.rad2deg(ra), numpy.rad2deg(dec)
+        w.celestial = True
     data = torch.zeros((nchan, 1, npixel, npixel), dtype=dtype, device=dev or device())
     return Image(data, w)
 

@@ -14,21 +14,10 @@ import numpy
 import torch
 
 from . import _lib
-from .data_models import BlockVisibility, SkyCoord, Visibility
+from .data_models import BlockVisibility, SkyCoord, Visibility, _radec
 from .plan import ptr, stream_handle
 
-__all__ = ["skycoord_to_lmn", "phaserotate_visibility", "SkyCoord"]
-
-
-def _radec(pos):
-    """(ra, dec) in radians of a SkyCoord here, an (ra, dec) pair in radians, or an object with
-    astropy's .ra.rad / .dec.rad."""
-    if isinstance(pos, SkyCoord):
-        return pos.ra, pos.dec
-    if hasattr(pos, "ra") and hasattr(pos.ra, "rad"):
-        return float(pos.ra.rad), float(pos.dec.rad)
-    ra, dec = pos
-    return float(ra), float(dec)
+__all__ = ["skycoord_to_lmn", "separation", "phaserotate_visibility", "SkyCoord"]
 
 
 def _rotation(angle, axis):
@@ -54,6 +43,28 @@ def skycoord_to_lmn(pos, phasecentre):
     return numpy.float64(y), numpy.float64(z), numpy.float64(x - 1.0)
 
 
+def separation(a, b):
+    """Angular distance in radians between two directions, as astropy's SkyCoord.separation
+    computes it (the Vincenty formula of angle_utilities.angular_separation)."""
+    ra1, dec1 = _radec(a)
+    ra2, dec2 = _radec(b)
+    dl = ra2 - ra1
+    n1 = numpy.cos(dec2) * numpy.sin(dl)
+    n2 = numpy.cos(dec1) * numpy.sin(dec2) - numpy.sin(dec1) * numpy.cos(dec2) * numpy.cos(dl)
+    d = numpy.sin(dec1) * numpy.sin(dec2) + numpy.cos(dec1) * numpy.cos(dec2) * numpy.cos(dl)
+    return float(numpy.arctan2(numpy.hypot(n1, n2), d))
+
+
+def rotation_lmn(newphasecentre, phasecentre):
+    """[l, m, sqrt(1 - l^2 - m^2) - 1] (float64) of the new centre seen from the old one, the
+    direction whose phasor the rotation applies, or None when |n| < 1e-15 (no rotation is made,
+    visibility/base.py:311)."""
+    l, m, n = skycoord_to_lmn(newphasecentre, phasecentre)
+    if numpy.abs(n) < 1e-15:
+        return None
+    return numpy.array([l, m, numpy.sqrt(1 - l ** 2 - m ** 2) - 1.0], dtype=numpy.float64)
+
+
 def phaserotate_visibility(vis, newphasecentre, tangent=True, inverse=False):
     """Phase rotate from the current phase centre to ``newphasecentre`` (base.py:295-381).
 
@@ -66,11 +77,10 @@ def phaserotate_visibility(vis, newphasecentre, tangent=True, inverse=False):
         raise ValueError("vis argument neither Visibility or BlockVisibility")
     if vis.phasecentre is None:
         raise ValueError("phaserotate_visibility: the visibility has no phase centre")
-    l, m, n = skycoord_to_lmn(newphasecentre, vis.phasecentre)
-    if numpy.abs(n) < 1e-15:
+    lmn = rotation_lmn(newphasecentre, vis.phasecentre)
+    if lmn is None:
         return vis
     newvis = vis.copy()
-    lmn = numpy.array([l, m, numpy.sqrt(1 - l ** 2 - m ** 2) - 1.0], dtype=numpy.float64)
     trig = None
     if not tangent:
         ra0, dec0 = _radec(vis.phasecentre)

@@ -357,7 +357,9 @@ int arlmi_phaserotate_blocks(double* d_uvw, void* d_vis, int64_t nblock, int32_t
  * NULL) receives ARL's iteration count.  0 < gain < 2 and niter > 0 are required (:33-34).  The
  * operations are ARL's, uncontracted, so the results equal numpy's bit for bit (a NaN is never
  * chosen as the peak, where numpy's argmax would choose it).  Synchronises the stream about every 64
- * iterations to read the end flag. */
+ * iterations to read the end flag.  The control block and band maxima live in a per-device buffer
+ * of the library (grown on demand, kept until exit), so calls that overlap on one device race:
+ * run them one after another (on one stream, or synchronised). */
 int arlmi_hogbom(double* d_res, double* d_comps, const double* d_psf, int32_t psf_nx, int32_t psf_ny,
                  const double* d_window, int32_t nx, int32_t ny, double gain, double thresh, int32_t niter,
                  double fracthresh, int32_t* iterations, void* stream);

@@ -62,7 +62,14 @@ Pinning (what checks each function; see tests/test_oracle_*.py):
                                            hour angle shifted, rotating there and back (and with
                                            inverse) is the identity; blocks agree with rows
                                            through convert_blocks
-  overlapIndices                           ARL known answer (test_image_deconvolution.py:59-68)
+  pixel_to_skycoord_sin, separation, shift_vis_to_image
+                                           closed forms (the reference pixel is crval; on the
+                                           equator an x offset gives ra = ra0 + asin(l); known
+                                           separations), skycoord_to_lmn of the deprojected
+                                           direction returns the pixel's (l, m), and a point
+                                           source imaged through the shift peaks at the pixel of
+                                           its (l, m) about the image centre
+  overlapIndices                          ARL known answer (test_image_deconvolution.py:59-68)
   hogbom, deconvolve_cube_hogbom           closed form of a delta PSF (geometric decay of the
                                            peak, iteration count from the stop rule), flux
                                            conservation dirty - res = comps convolved with the
@@ -97,7 +104,8 @@ __all__ = [
     "hadec_to_azel", "create_visibility", "apply_gains_rows", "create_blockvisibility", "convert_blocks",
     "convert_blocks_loop", "decoalesce_visibility", "apply_gaintable", "apply_gains_rows_adjoint",
     "grid_weight_to_griddata", "griddata_merge_weights", "griddata_reweight", "weight_visibility",
-    "uvw_to_xyz", "skycoord_to_lmn", "phaserotate_rows", "phaserotate_blocks", "overlapIndices", "hogbom",
+    "uvw_to_xyz", "skycoord_to_lmn", "phaserotate_rows", "phaserotate_blocks", "pixel_to_skycoord_sin",
+    "separation", "shift_vis_to_image", "overlapIndices", "hogbom",
     "deconvolve_cube_hogbom", "gain_substitution_scalar", "solution_residual_scalar",
     "solve_antenna_gains_itsubs_scalar", "solve_gaintable_scalar", "average_chunks", "average_chunks2",
     "bda_factors", "average_in_blocks",
@@ -652,15 +660,19 @@ def _copy_vis(vis):
     return {k: numpy.array(v, copy=True) for k, v in vis.items()}
 
 
-def invert_2d(vis, geom, gcf, cf, cfg, dopsf=False, normalize=True, imaginary=False, loop=False):
+def invert_2d(vis, geom, gcf, cf, cfg, dopsf=False, normalize=True, imaginary=False, loop=False, centres=None):
     """Dirty image by convolutional gridding (imaging/base.py:141-189).
 
-    The image centre equals the visibility phase centre for every workload built
-    here, so shift_vis_to_image (base.py:50-81) is the identity and is omitted.
+    centres: None when the image is centred on the visibility phase centre (shift_vis_to_image,
+    base.py:50-81, is then the identity), else (visibility phase centre, image centre) as
+    (ra, dec) radians, and the rows are shifted to the image centre after dopsf's unit
+    visibilities are set (base.py:160-165).
     """
     svis = _copy_vis(vis)
     if dopsf:
         svis['vis'][...] = 1.0 + 0.0j
+    if centres is not None:
+        svis['uvw'], svis['vis'], _ = shift_vis_to_image(svis, centres[0], centres[1])
     gridder = grid_visibility_to_griddata_loop if loop else grid_visibility_to_griddata
     grid, sumwt = gridder(svis, geom, cf, cfg)
     if imaginary:
@@ -675,11 +687,16 @@ def invert_2d(vis, geom, gcf, cf, cfg, dopsf=False, normalize=True, imaginary=Fa
     return re, sumwt
 
 
-def predict_2d(vis, model, geom, gcf, cf, cfg, loop=False):
-    """Model visibilities by convolutional degridding (imaging/base.py:104-138). Returns vis array."""
+def predict_2d(vis, model, geom, gcf, cf, cfg, loop=False, centres=None):
+    """Model visibilities by convolutional degridding (imaging/base.py:104-138). Returns vis array.
+    centres as for invert_2d: the degridded rows are shifted back from the image centre
+    (shift_vis_to_image with inverse, base.py:136)."""
     grid = fft_image_to_griddata(model, gcf)
     degridder = degrid_visibility_from_griddata_loop if loop else degrid_visibility_from_griddata
-    return degridder(vis, grid, geom, cf, cfg)
+    out = degridder(vis, grid, geom, cf, cfg)
+    if centres is not None:
+        _, out, _ = shift_vis_to_image({'uvw': vis['uvw'], 'vis': out}, centres[0], centres[1], inverse=True)
+    return out
 
 
 # ---------------------------------------------------------------------------
@@ -704,29 +721,31 @@ def _rows(vis, rows):
     return {k: numpy.array(v[rows], copy=True) for k, v in vis.items()}
 
 
-def invert_wstack_single(vis, geom, gcf, cf, cfg, dopsf=False, normalize=True, loop=False):
-    """One w slab: shift w by the slab mean, invert, rotate by the screen (wstack_single.py:67-96)."""
+def invert_wstack_single(vis, geom, gcf, cf, cfg, dopsf=False, normalize=True, loop=False, centres=None):
+    """One w slab: shift w by the slab mean, invert, rotate by the screen (wstack_single.py:67-96).
+    An image-centre shift (centres) is evaluated on the w-shifted rows, inside invert_2d."""
     vis = _copy_vis(vis)
     w_average = numpy.average(vis['uvw'][:, 2])
     vis['uvw'][..., 2] -= w_average
     re, sumwt, im = invert_2d(vis, geom, gcf, cf, cfg, dopsf=dopsf, normalize=normalize, imaginary=True,
-                              loop=loop)
+                              loop=loop, centres=centres)
     b = create_w_term_like(geom, w_average)
     return b.real * re - b.imag * im, sumwt
 
 
-def predict_wstack_single(vis, model, geom, gcf, cf, cfg, loop=False):
-    """One w slab: P(Re b * I) - 1j P(Im b * I) with w shifted by the slab mean (wstack_single.py:26-64)."""
+def predict_wstack_single(vis, model, geom, gcf, cf, cfg, loop=False, centres=None):
+    """One w slab: P(Re b * I) - 1j P(Im b * I) with w shifted by the slab mean (wstack_single.py:26-64).
+    An image-centre shift (centres) is evaluated on the w-shifted rows, inside predict_2d."""
     vis = _copy_vis(vis)
     w_average = numpy.average(vis['uvw'][:, 2])
     vis['uvw'][..., 2] -= w_average
     b = create_w_term_like(geom, w_average)
-    v_re = predict_2d(vis, b.real * model, geom, gcf, cf, cfg, loop=loop)
-    v_im = predict_2d(vis, b.imag * model, geom, gcf, cf, cfg, loop=loop)
+    v_re = predict_2d(vis, b.real * model, geom, gcf, cf, cfg, loop=loop, centres=centres)
+    v_im = predict_2d(vis, b.imag * model, geom, gcf, cf, cfg, loop=loop, centres=centres)
     return v_re - 1j * v_im
 
 
-def invert_list_wstack(vis, geom, gcf, cf, cfg, vis_slices, dopsf=False, normalize=True, loop=False):
+def invert_list_wstack(vis, geom, gcf, cf, cfg, vis_slices, dopsf=False, normalize=True, loop=False, centres=None):
     """Serial w-stack invert of one vis list (imaging_serial.py:181-197).
 
     Each slab's normalised image is re-weighted by its sumwt, summed, and the
@@ -738,13 +757,13 @@ def invert_list_wstack(vis, geom, gcf, cf, cfg, vis_slices, dopsf=False, normali
         if numpy.sum(rows) == 0:
             continue
         img, sumwt = invert_wstack_single(_rows(vis, rows), geom, gcf, cf, cfg, dopsf=dopsf, normalize=normalize,
-                                          loop=loop)
+                                          loop=loop, centres=centres)
         result += sumwt[:, :, numpy.newaxis, numpy.newaxis] * img
         result_sumwt += sumwt
     return normalize_sumwt(result, result_sumwt), result_sumwt
 
 
-def predict_list_wstack(vis, model, geom, gcf, cf, cfg, vis_slices, loop=False):
+def predict_list_wstack(vis, model, geom, gcf, cf, cfg, vis_slices, loop=False, centres=None):
     """Serial w-stack predict of one vis list (imaging_serial.py:86-98).
 
     Rows in no slab stay zero; a row in two slabs keeps the later slab's value.
@@ -753,7 +772,8 @@ def predict_list_wstack(vis, model, geom, gcf, cf, cfg, vis_slices, loop=False):
     for rows in vis_wslice_iter(vis['uvw'][:, 2], vis_slices):
         if numpy.sum(rows) == 0:
             continue
-        out[rows, ...] = predict_wstack_single(_rows(vis, rows), model, geom, gcf, cf, cfg, loop=loop)
+        out[rows, ...] = predict_wstack_single(_rows(vis, rows), model, geom, gcf, cf, cfg, loop=loop,
+                                               centres=centres)
     return out
 
 
@@ -884,6 +904,45 @@ def phaserotate_blocks(uvw_m, vis, frequency, phasecentre, newphasecentre, tange
         xyz = uvw_to_xyz(lin, ha=-phasecentre[0], dec=phasecentre[1])
         uvw_out = xyz_to_uvw(xyz, ha=-newphasecentre[0], dec=newphasecentre[1]).reshape([nt, na, na, 3])
     return uvw_out, out
+
+
+def pixel_to_skycoord_sin(px, py, crpix, cdelt_deg, crval_deg):
+    """(ra, dec) radians of the 1-relative pixel (px, py) of a RA---SIN / DEC--SIN image without
+    a rotation matrix: astropy's pixel_to_skycoord(px, py, wcs, origin=1), which
+    imaging/base.py:72 applies to the FFT centre (N//2 + 1, N//2 + 1).  The intermediate world
+    coordinates (l, m) = cdelt * (p - crpix), in radians, are the direction cosines of the slant
+    orthographic projection about (ra0, dec0) = crval (FITS SIN with LONPOLE 180), inverted as
+        n = sqrt(1 - l^2 - m^2),  dec = asin(m cos dec0 + n sin dec0),
+        ra = ra0 + atan2(l, n cos dec0 - m sin dec0)."""
+    ra0, dec0 = crval_deg[0] * D2R, crval_deg[1] * D2R
+    l = cdelt_deg[0] * D2R * (px - crpix[0])
+    m = cdelt_deg[1] * D2R * (py - crpix[1])
+    if l == 0.0 and m == 0.0:
+        return ra0, dec0
+    n = numpy.sqrt(1.0 - l * l - m * m)
+    dec = numpy.arcsin(m * numpy.cos(dec0) + n * numpy.sin(dec0))
+    ra = ra0 + numpy.arctan2(l, n * numpy.cos(dec0) - m * numpy.sin(dec0))
+    return ra, dec
+
+
+def separation(a, b):
+    """Angular distance in radians between directions (ra, dec): astropy's SkyCoord.separation,
+    the Vincenty form atan2(sqrt(n1^2 + n2^2), d) of angle_utilities.angular_separation."""
+    dl = b[0] - a[0]
+    n1 = numpy.cos(b[1]) * numpy.sin(dl)
+    n2 = numpy.cos(a[1]) * numpy.sin(b[1]) - numpy.sin(a[1]) * numpy.cos(b[1]) * numpy.cos(dl)
+    d = numpy.sin(a[1]) * numpy.sin(b[1]) + numpy.cos(a[1]) * numpy.cos(b[1]) * numpy.cos(dl)
+    return numpy.arctan2(numpy.hypot(n1, n2), d)
+
+
+def shift_vis_to_image(vis, vis_phasecentre, image_centre, tangent=True, inverse=False):
+    """imaging/base.py:50-81 for row visibilities: rotate to the image's FFT-centre direction when
+    it is more than 1e-15 rad from the visibility phase centre.  Returns (uvw, vis) (the inputs
+    themselves when no rotation is made) and the phase centre the result is labelled with."""
+    if separation(vis_phasecentre, image_centre) > 1e-15:
+        uvw, v = phaserotate_rows(vis, vis_phasecentre, image_centre, tangent=tangent, inverse=inverse)
+        return uvw, v, image_centre
+    return vis['uvw'], vis['vis'], vis_phasecentre
 
 
 def hadec_to_azel(ha, dec, latitude):
