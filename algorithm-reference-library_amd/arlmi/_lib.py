@@ -39,7 +39,7 @@ EXPORTS = (
     "arlmi_phaserotate_rows", "arlmi_phaserotate_blocks", "arlmi_hogbom", "arlmi_hogbom_traffic", "arlmi_solve_gains",
     "arlmi_bda_uvdist", "arlmi_bda_average",
     "arlmi_probe_lds_add", "arlmi_profile_enable", "arlmi_profile_collect", "arlmi_profile_reset",
-    "arlmi_profile_report",
+    "arlmi_profile_report", "arlmi_profile_only",
 )
 
 
@@ -158,6 +158,7 @@ def lib():
         "arlmi_profile_collect": (ctypes.c_int, []),
         "arlmi_profile_reset": (ctypes.c_int, []),
         "arlmi_profile_report": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int]),
+        "arlmi_profile_only": (ctypes.c_int, [ctypes.c_char_p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -7,6 +7,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -26,6 +27,7 @@ bool g_on = false;
 std::vector<Pending> g_pending;
 std::vector<hipEvent_t> g_pool;
 std::map<std::string, Totals> g_tot;
+std::set<std::string> g_only;  // kernels to time; empty = all
 
 hipEvent_t take() {
   if (!g_pool.empty()) {
@@ -44,6 +46,7 @@ bool profile_on() { return g_on; }
 void* profile_begin(const char* name, hipStream_t s) {
   if (!g_on) return nullptr;
   std::lock_guard<std::mutex> lk(g_mu);
+  if (!g_only.empty() && g_only.find(name) == g_only.end()) return nullptr;
   Pending* p = new Pending{name, take(), take()};
   if (p->a) (void)hipEventRecord(p->a, s);
   return p;
@@ -67,6 +70,26 @@ extern "C" {
 int arlmi_profile_enable(int on) {
   std::lock_guard<std::mutex> lk(g_mu);
   g_on = on != 0;
+  return ARLMI_OK;
+}
+
+// Restrict the timing to the kernels named in a comma-separated list (NULL or "" = all).  Each
+// timed launch puts two event markers on its stream, which cost a few microseconds of stream
+// time each; timing only the long kernels keeps that out of short launch sequences.
+int arlmi_profile_only(const char* names) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_only.clear();
+  if (!names) return ARLMI_OK;
+  std::string cur;
+  for (const char* c = names;; ++c) {
+    if (*c == ',' || *c == 0) {
+      if (!cur.empty()) g_only.insert(cur);
+      cur.clear();
+      if (*c == 0) break;
+    } else if (*c != ' ') {
+      cur += *c;
+    }
+  }
   return ARLMI_OK;
 }
 

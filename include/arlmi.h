@@ -406,10 +406,14 @@ int arlmi_hogbom_traffic(int64_t* scanned_pixels, int64_t* subtracted_pixels);
  * hipDeviceAttributeClockRate.  Synchronises the device. */
 int arlmi_probe_lds_add(int32_t pattern, int32_t iters, double* cycles_per_instr, float* ms_out);
 
-/* Opt-in kernel timing: with profiling on, every library kernel launch is bracketed by two
- * HIP events on its stream.  collect() waits for them and adds the elapsed times per kernel;
- * report() writes "kernel total_ms launches" lines into buf and returns the size needed. */
+/* Opt-in kernel timing (no ARL counterpart; measurement only): with profiling on, every library
+ * kernel launch is bracketed by two HIP events on its stream.  collect() waits for them and adds
+ * the elapsed times per kernel; report() writes "kernel total_ms launches" lines into buf and
+ * returns the size needed.  only() restricts the timing to a comma-separated list of kernel
+ * names (NULL or "" times every kernel): each timed launch adds two event markers to its
+ * stream, a few microseconds of stream time each. */
 int arlmi_profile_enable(int on);
+int arlmi_profile_only(const char* names);
 int arlmi_profile_collect(void);
 int arlmi_profile_reset(void);
 int arlmi_profile_report(char* buf, int cap);
