@@ -8,7 +8,7 @@ import numpy
 import torch
 
 from . import _lib
-from .data_models import BlockVisibility, D2R, Image, SkyCoord, Skycomponent, Visibility, device
+from .data_models import BlockVisibility, C_LIGHT, D2R, Image, SkyCoord, Skycomponent, Visibility, device
 from .kernels import create_pswf_convolutionfunction, create_image
 from .plan import Plan, ptr, stream_handle, grids_buffer
 from .gridding import grid_planes, degrid_planes
@@ -307,23 +307,45 @@ def predict_skycomponent_visibility(vis, sc, chan_map=None):
 
 
 def create_image_from_visibility(vis, **kwargs):
-    """Empty image for a visibility set (imaging/base.py:247-339): default cell is half the
-    critical cell 1/(2 max|u|), capped at the critical cell; MFS unless nchan is given; centred
-    on the ``phasecentre`` keyword, by default the visibility phase centre (base.py:269, 333)."""
+    """Empty image for a visibility set (imaging/base.py:247-339), on the visibilities' device.
+
+    Reference frequency: the first of ``frequency`` (default the visibilities' frequencies, per
+    row for a Visibility, per channel for a BlockVisibility); nchan defaults to the number of
+    distinct frequencies, and an image of several channels needs several in the data (else
+    ValueError, as ARL's unknown spectral mode).  Cell: half the critical cell 1/(2 max|u|) by
+    default, capped at the critical cell unless override_cellsize=False, with u in wavelengths
+    (metres times max(frequency)/c for a BlockVisibility).  Centred on the ``phasecentre``
+    keyword, by default the visibility phase centre (base.py:269, 333)."""
+    if not isinstance(vis, (Visibility, BlockVisibility)):
+        raise AssertionError("vis is not a Visibility or a BlockVisibility: %r" % (vis,))
+    uvw = vis.data["uvw"]
+    if isinstance(vis, BlockVisibility):
+        vfreq = numpy.asarray(vis.frequency, dtype=numpy.float64)
+        vbw = numpy.atleast_1d(numpy.asarray(vis.channel_bandwidth, dtype=numpy.float64))
+    else:
+        vfreq = vis.data["frequency"].cpu().numpy()
+        vbw = vis.data["channel_bandwidth"][0:1].cpu().numpy() if "channel_bandwidth" in vis.data else \
+            numpy.array([1e6])
+    vnchan = len(numpy.unique(vfreq))
+    frequency = numpy.atleast_1d(numpy.asarray(get_parameter(kwargs, "frequency", vfreq), dtype=numpy.float64))
+    nchan = get_parameter(kwargs, "nchan", vnchan)
+    channel_bandwidth = get_parameter(kwargs, "channel_bandwidth", 0.99999999999 * float(vbw[0]))
+    # spectral modes (base.py:280-300): one image channel per data channel, or MFS / regridded
+    # channels, which need a non-zero channel width
+    if not (nchan == vnchan and vnchan > 1):
+        if not ((nchan == 1 and vnchan >= 1) or (nchan > 1 and vnchan > 1)):
+            raise ValueError("create_image_from_visibility: unknown spectral mode ")
+        assert numpy.abs(channel_bandwidth) > 0.0, "Channel width must be non-zero for mfs mode"
     npixel = get_parameter(kwargs, "npixel", 512)
-    uvmax = float(torch.max(torch.abs(vis.data["uvw"][:, 0:1])).item())
+    uvmax = float(torch.max(torch.abs(uvw[..., 0:1])).item())
+    if isinstance(vis, BlockVisibility):
+        uvmax *= numpy.max(frequency) / C_LIGHT
     critical = 1.0 / (uvmax * 2.0)
     cellsize = get_parameter(kwargs, "cellsize", 0.5 * critical)
     if get_parameter(kwargs, "override_cellsize", True) and cellsize > critical:
         cellsize = critical
-    freq = vis.data["frequency"]
-    ufreq = numpy.unique(freq.cpu().numpy())
-    frequency = get_parameter(kwargs, "frequency", ufreq)
-    nchan = get_parameter(kwargs, "nchan", len(ufreq))
-    bw = vis.data["channel_bandwidth"][0].item() if "channel_bandwidth" in vis.data else 1e6
-    channel_bandwidth = get_parameter(kwargs, "channel_bandwidth", 0.99999999999 * bw)
     dtype = torch.float32 if vis.precision == 32 else torch.float64
     phasecentre = get_parameter(kwargs, "phasecentre", vis.phasecentre)
-    return create_image(npixel=npixel, cellsize=cellsize, frequency=numpy.atleast_1d(frequency),
+    return create_image(npixel=npixel, cellsize=cellsize, frequency=frequency[:1],
                         channel_bandwidth=numpy.atleast_1d(channel_bandwidth), nchan=nchan, dtype=dtype,
-                        phasecentre=phasecentre)
+                        phasecentre=phasecentre, dev=uvw.device)
