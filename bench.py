@@ -67,7 +67,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=1000000)
     ap.add_argument("--workload", choices=("wstack", "wstack-aw", "batch", "gradient", "phaserotate", "clean", "dft", "solve",
-                             "bda"),
+                             "bda", "weighting"),
                     default="wstack",
                     help="wstack: the headline line (default); wstack-aw: the same with an AW w-projection kernel "
                          "for the residual w of each slab; batch: 64 uneven observations replayed from one "
@@ -76,7 +76,8 @@ def parse():
                          "clean: Hogbom minor cycle on the w-stack workload's dirty image and PSF; "
                          "dft: direct DFT of the workload's 200 sources onto its rows; "
                          "solve: scalar gain solution of the 128-antenna differentiation workload; "
-                         "bda: baseline-dependent averaging of the w-stack observation as a BlockVisibility")
+                         "bda: baseline-dependent averaging of the w-stack observation as a BlockVisibility; "
+                         "weighting: uniform weighting of the w-stack workload's rows on its 2048^2 grid")
     ap.add_argument("--niter", type=int, default=1000, help="Hogbom iterations (clean)")
     ap.add_argument("--support", type=int, default=16, help="AW kernel support (wstack-aw)")
     ap.add_argument("--nobs", type=int, default=64)
@@ -740,6 +741,91 @@ def bench_dft(args):
     print(json.dumps(out))
 
 
+def bench_weighting(args):
+    """weight_visibility (imaging/weighting.py:19-40), uniform: the w-stack workload's 29.7 M rows
+    added at their nearest pixel of a float64 density grid on its 2048^2 image, then every row's
+    imaging weight divided by the density at its pixel.  One step = one call (a new density grid,
+    both passes); the weights keep shrinking from step to step, the work does not change."""
+    import numpy
+    import torch
+    import arlmi
+    from arlmi import simulation
+    vis, im, _ = simulation.config2(precision=32, sky=False)
+    gcf, cf = arlmi.create_pswf_convolutionfunction(im, oversampling=8, support=8)
+    rows = vis.nvis
+    vis.data["imaging_weight"].fill_(1.0)
+    for _ in range(args.warmup):
+        arlmi.weight_visibility(vis, im, gcfcf=(gcf, cf))
+    torch.cuda.synchronize()
+    L = arlmi._lib.lib()
+    L.arlmi_profile_reset()
+    L.arlmi_profile_enable(1)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        arlmi.weight_visibility(vis, im, gcfcf=(gcf, cf))
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / args.steps
+    L.arlmi_profile_enable(0)
+    prof = arlmi._lib.profile_report()
+    g_ms, g_n = prof["k_weight_grid"]
+    a_ms, a_n = prof["k_weight_apply"]
+    # one density pass and two row passes per call: the mapping check ARL's asserts need before any
+    # weight changes (uvw and frequency, 32 B/row), then the division (48 B/row with the density
+    # gather and the weight read and write)
+    g_s, a_s = g_ms * 1e-3 / g_n, a_ms * 1e-3 / args.steps
+    # parity and CPU baseline on a 1 M-row subset (its own density grid): the oracle's row loops
+    sys.path.insert(0, ROOT)
+    from oracle import arl_oracle as O
+    m = 1_000_000
+    idx = torch.arange(0, m, device="cuda") * (rows // m)
+    sub = arlmi.Visibility({k: v[idx].contiguous() for k, v in vis.data.items()})
+    sub.data["imaging_weight"] = torch.ones((m, 1), dtype=torch.float32, device="cuda")
+    arlmi.weight_visibility(sub, im, gcfcf=(gcf, cf))
+    got = sub.data["imaging_weight"].cpu().numpy()
+    n, cell = im.npixel, im.cellsize
+    geom = O.ImageGeom(n, cell, frequency=float(im.wcs.wcs.crval[3]), channel_bandwidth=float(im.wcs.wcs.cdelt[3]))
+    cfg = O.CFGeom(geom, 8, 8)
+    d = {"uvw": sub.data["uvw"].cpu().numpy(), "frequency": sub.data["frequency"].cpu().numpy(),
+         "imaging_weight": numpy.ones((m, 1))}
+    c0 = time.perf_counter()
+    ref = O.weight_visibility(d, geom, cfg)
+    c1 = time.perf_counter()
+    exact = bool(numpy.array_equal(got, ref.astype(numpy.float32)))
+    gb, ab = 36.0 * rows, 80.0 * rows
+    out = {
+        "metric": "Mvis/s uniform weighting (weight_visibility), rows of the 2048^2 w-stack workload",
+        "value": round(rows / t / 1e6, 1),
+        "unit": "Mvis/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 density, f32 weights",
+        "data": "synthetic",
+        "config": {
+            "workload": "256 antennas log-radial to 10 km, 8 channels 100-150 MHz, 120 integrations (rows kept for "
+                        "the 2048^2 grid); uniform weighting on the 2048^2 grid, one image channel",
+            "rows": rows,
+            "k_weight_grid_ms": round(g_s * 1e3, 3),
+            "k_weight_apply_ms_per_step": round(a_s * 1e3, 3),
+            "k_weight_apply_launches_per_step": a_n / args.steps,
+            "k_weight_apply_hbm_frac": round(ab / a_s / 8e12, 4),
+            "subset_bit_exact_vs_oracle": exact,
+        },
+        "roofline": {"bound": "hbm", "achieved": round(gb / g_s / 1e9, 1), "peak": 8000.0, "unit": "GB/s",
+                     "frac": round(gb / g_s / 8e12, 4), "traffic": None, "kernel": "k_weight_grid",
+                     "algorithmic_bytes": "36 B per row (uvw, frequency, weight); the float64 density adds are "
+                                          "device atomics, not counted"},
+        "cpu_baseline": {"value": round(m / (c1 - c0) / 1e6, 4), "unit": "Mvis/s", "cores": 1, "kind": "port",
+                         "sample": "%d rows (every %d-th), oracle weight_visibility (ARL's row loops), %.2f s"
+                                   % (m, rows // m, c1 - c0)},
+    }
+    print(json.dumps(out))
+
+
 def bench_solve(args):
     """solve_gaintable (scalar, amplitude and phase, 200 iterations at most, tol 1e-8) of the
     differentiation workload's BlockVisibility: 128 antennas, 4 channels, 60 integrations with one
@@ -965,6 +1051,11 @@ def main():
         import torch
         torch.cuda.set_device(0)
         bench_clean(args)
+        return
+    if args.workload == "weighting":
+        import torch
+        torch.cuda.set_device(0)
+        bench_weighting(args)
         return
     import numpy
     import torch
