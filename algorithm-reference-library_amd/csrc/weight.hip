@@ -42,11 +42,15 @@ __device__ inline int64_t weight_pixel(const DevGeom& g, const double* uvw, cons
 
 constexpr int kThreads = 256;
 
+// Work group b adds into copy b % ncopy of the density (copies nd doubles apart): work groups
+// are dispatched to the eight XCDs in turn, so with eight copies the adds of one XCD meet only
+// its own copy.
 template <typename R>
 __global__ void __launch_bounds__(kThreads)
     k_weight_grid(DevGeom g, const double* __restrict__ uvw, const double* __restrict__ freq, int64_t n,
-                  const R* __restrict__ wt, double* __restrict__ density, double* __restrict__ sumwt,
-                  uint32_t* __restrict__ err) {
+                  const R* __restrict__ wt, double* __restrict__ density_copies, int ncopy, int64_t nd,
+                  double* __restrict__ sumwt, uint32_t* __restrict__ err) {
+  double* __restrict__ density = density_copies + (int64_t)(blockIdx.x % (unsigned)ncopy) * nd;
   __shared__ double lsum[256];
   __shared__ uint32_t lerr;
   for (int i = threadIdx.x; i < g.nchan; i += blockDim.x) lsum[i] = 0.0;
@@ -91,6 +95,38 @@ __global__ void k_fill_f64(double* __restrict__ p, int64_t n) {
 }
 
 __global__ void k_fill_u32(uint32_t* p) { *p = 0u; }
+
+// density[i] = sum of the ncopy copies at i, in copy order.
+__global__ void __launch_bounds__(kThreads)
+    k_sum_copies(const double* __restrict__ copies, int ncopy, int64_t nd, double* __restrict__ density) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nd; i += stride) {
+    double t = 0.0;
+    for (int c = 0; c < ncopy; ++c) t += copies[(int64_t)c * nd + i];
+    density[i] = t;
+  }
+}
+
+// Per-device density copies (grow only, kept until exit).
+constexpr int kCopies = 8;
+constexpr size_t kCopiesMaxBytes = (size_t)2 << 30;
+std::mutex g_copies_mutex;
+std::map<int, std::pair<void*, size_t>> g_copies;
+
+double* density_copies(size_t bytes) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  std::lock_guard<std::mutex> lock(g_copies_mutex);
+  auto& c = g_copies[dev];
+  if (c.second < bytes) {
+    if (c.first) (void)hipFree(c.first);
+    c.first = nullptr;
+    c.second = 0;
+    if (hipMalloc(&c.first, bytes) != hipSuccess) return nullptr;
+    c.second = bytes;
+  }
+  return static_cast<double*>(c.first);
+}
 
 // One error word per device, allocated on first use.
 std::mutex g_err_mutex;
@@ -148,16 +184,27 @@ int arlmi_grid_weights(const arlmi_geom* gin, const double* d_uvw, const double*
     return ARLMI_E_HIP;
   }
   const int64_t nd = (int64_t)g.nchan * g.nplanes * g.n * g.n;
-  ARLMI_LAUNCH("k_fill_f64", k_fill_f64, dim3(blocks_for(nd)), dim3(kThreads), 0, s, d_density, nd);
+  // one density copy per XCD when they fit, else the caller's grid alone
+  double* copies = nullptr;
+  int ncopy = 1;
+  if (nrows > 0 && (size_t)nd * sizeof(double) * kCopies <= kCopiesMaxBytes) {
+    copies = density_copies((size_t)nd * sizeof(double) * kCopies);
+    if (copies) ncopy = kCopies;
+  }
+  double* target = copies ? copies : d_density;
+  ARLMI_LAUNCH("k_fill_f64", k_fill_f64, dim3(blocks_for(nd * ncopy)), dim3(kThreads), 0, s, target, nd * ncopy);
   ARLMI_LAUNCH("k_fill_f64", k_fill_f64, dim3(1), dim3(kThreads), 0, s, d_sumwt, (int64_t)g.nchan);
   ARLMI_LAUNCH("k_fill_u32", k_fill_u32, dim3(1), dim3(1), 0, s, d_err);
   if (nrows > 0) {
     if (precision == 32)
       ARLMI_LAUNCH("k_weight_grid", k_weight_grid<float>, dim3(blocks_for(nrows)), dim3(kThreads), 0, s, g, d_uvw,
-                   d_freq, nrows, (const float*)d_weight, d_density, d_sumwt, d_err);
+                   d_freq, nrows, (const float*)d_weight, target, ncopy, nd, d_sumwt, d_err);
     else
       ARLMI_LAUNCH("k_weight_grid", k_weight_grid<double>, dim3(blocks_for(nrows)), dim3(kThreads), 0, s, g, d_uvw,
-                   d_freq, nrows, (const double*)d_weight, d_density, d_sumwt, d_err);
+                   d_freq, nrows, (const double*)d_weight, target, ncopy, nd, d_sumwt, d_err);
+    if (copies)
+      ARLMI_LAUNCH("k_sum_copies", k_sum_copies, dim3(blocks_for(nd)), dim3(kThreads), 0, s, copies, ncopy, nd,
+                   d_density);
   }
   ARLMI_LAUNCH_CHECK();
   return finish(d_err, s);
