@@ -151,6 +151,7 @@ class Imager:
 
     def update(self, vis, slab_boxes=None):
         self.plan.update(vis, slab_boxes)
+        self._shift_cache = None
 
     def invert(self, vis, dopsf=False, normalize=True, imaginary=False, out=None):
         """Dirty image (or PSF): grid, shifted inverse FFT, screens, gcf and sum-of-weights."""
@@ -221,6 +222,11 @@ class Imager:
         uvw = vis.data["uvw"]
         if self.mode != "wstack":
             return uvw
+        # kept until the plan is rebuilt: the slab centres and means are read back to the host,
+        # which a call under stream capture (after an eager call) must not do
+        cache = getattr(self, "_shift_cache", None)
+        if cache is not None and cache[0] is uvw:
+            return cache[1]
         boxes, width = self.plan.slab_boxes()
         wbar = self.plan.plane_w()
         w = uvw[:, 2]
@@ -229,6 +235,7 @@ class Imager:
             off = torch.where(torch.abs(w - box) < 0.5 * width, torch.full_like(w, mean), off)
         out = uvw.clone()
         out[:, 2] = w - off
+        self._shift_cache = (uvw, out)
         return out
 
 

@@ -170,3 +170,36 @@ def test_point_source_peak_and_adjoint(gpu):
     lhs = float(torch.sum(ax.real * y.real + ax.imag * y.imag))
     rhs = float(torch.sum(x * aty))
     assert abs(lhs - rhs) < 1e-10 * abs(lhs)
+
+
+def test_batch_graph_with_shift(gpu):
+    """A shifted w-stack observation in a captured batch: three replays, with eager work between
+    them, each equal to the eager batch (image to float32 accumulation order, visibilities to
+    rounding); the eager batch's image equals the single imager's."""
+    import torch
+    import arlmi
+    rng = numpy.random.default_rng(15)
+    n, cell, S = 256, 5e-4, 8
+    d = random_rows(rng, 6000, n, cell, S, wmax=50.0)
+    vis = to_device_vis(d, 32)
+    vis.phasecentre = arlmi.SkyCoord(*C0)
+    im = _image(n, cell, 32, _centre(n, cell, 3, -7))
+    gcf, cf = arlmi.create_pswf_convolutionfunction(im, oversampling=8, support=S)
+    model = torch.zeros((1, 1, n, n), dtype=torch.float32, device="cuda")
+    model[0, 0, 100, 140] = 1.0
+    task = {"vis": vis, "im": im, "gcfcf": (gcf, cf), "mode": "wstack", "vis_slices": 4, "model": model}
+    eager = arlmi.BatchImager([task], graph=False)
+    eager.run()
+    torch.cuda.synchronize()
+    ref_img, ref_pv = eager.images[0].cpu().numpy(), eager.predicted[0].cpu().numpy()
+    (single, _), = arlmi.invert_list_serial_workflow([vis], [im], context="wstack", vis_slices=4, gcfcf=[(gcf, cf)])
+    _assert_image(ref_img, single.numpy(), 32)
+    batch = arlmi.BatchImager([task], graph=True)
+    for _ in range(3):
+        batch.images[0].zero_()
+        batch.predicted[0].zero_()
+        torch.cuda.synchronize()
+        batch.run()
+        torch.cuda.synchronize()
+        _assert_image(batch.images[0].cpu().numpy(), ref_img, 32)
+        assert rel_err(batch.predicted[0].cpu().numpy(), ref_pv) < 1e-6
