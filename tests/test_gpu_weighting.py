@@ -91,3 +91,25 @@ def test_weights_off_grid_raise_and_leave_weights(gpu):
     with pytest.raises(AssertionError):
         arlmi.griddata_reweight(vis, gd, cf)
     assert numpy.array_equal(vis.imaging_weight.cpu().numpy(), before)
+
+
+def test_full_size_uniform_identity(gpu):
+    """The w-stack workload's 29.7 M rows (unit float32 weights) on its 2048^2 grid: the density
+    counts every row exactly; after uniform weighting the new weights grid to 1 on every occupied
+    pixel (within the float32 rounding of 1/count) and to 0 elsewhere."""
+    import torch
+    import arlmi
+    from arlmi.simulation import config2
+    vis, im, _ = config2(precision=32, sky=False)
+    vis.data["imaging_weight"].fill_(1.0)
+    _, cf = arlmi.create_pswf_convolutionfunction(im, oversampling=8, support=8)
+    gd0, sw0 = arlmi.grid_weight_to_griddata(vis, arlmi.create_griddata_from_image(im), cf)
+    count = gd0.data.real
+    occ = count != 0
+    assert float(count.sum()) == float(vis.nvis) and float(numpy.sum(sw0)) == float(vis.nvis)
+    arlmi.weight_visibility(vis, im, gcfcf=(None, cf))
+    gd, _ = arlmi.grid_weight_to_griddata(vis, arlmi.create_griddata_from_image(im), cf)
+    dens = gd.data.real
+    assert torch.equal(dens != 0, occ)
+    assert float(torch.max(torch.abs(dens[occ] - 1.0))) < 1e-6
+    print("full-size weighting: %d rows on %d occupied pixels" % (vis.nvis, int(occ.sum())))
