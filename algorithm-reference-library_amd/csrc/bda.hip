@@ -89,23 +89,31 @@ __global__ void __launch_bounds__(kBdaThreads)
   const int nf = (nchan + fstep - 1) / fstep, F = fl / fstep;
   const int f1 = min(nchan, fl + fstep);
   const int64_t cells = off[b + 1] - off[b];
-  const cplx<R>* vis = reinterpret_cast<const cplx<R>*>(blk.d_vis);
-  const R* wt = reinterpret_cast<const R*>(blk.d_weight);
-  const R* iwt = reinterpret_cast<const R*>(blk.d_imaging_weight);
+  // the inputs are never written here: restrict lets the next integration's loads issue before
+  // this one's stores
+  const cplx<R>* __restrict__ vis = reinterpret_cast<const cplx<R>*>(blk.d_vis);
+  const R* __restrict__ wt = reinterpret_cast<const R*>(blk.d_weight);
+  const R* __restrict__ iwt = reinterpret_cast<const R*>(blk.d_imaging_weight);
+  const double* __restrict__ in_uvw = blk.d_uvw;
+  const double* __restrict__ in_time = blk.d_time;
+  const double* __restrict__ in_itime = blk.d_integration_time;
+  const double* __restrict__ in_freq = blk.d_frequency;
+  const double* __restrict__ in_bw = blk.d_channel_bandwidth;
   Acc s2{};
+#pragma unroll 4
   for (int t = 0; t < ntimes; ++t) {
 #pragma clang fp contract(off)
     const int64_t e = (int64_t)t * nb + b;
-    const double uvw0 = blk.d_uvw[3 * e], uvw1 = blk.d_uvw[3 * e + 1], uvw2 = blk.d_uvw[3 * e + 2];
-    const double tt = blk.d_time[t], it = blk.d_integration_time[t];
+    const double uvw0 = in_uvw[3 * e], uvw1 = in_uvw[3 * e + 1], uvw2 = in_uvw[3 * e + 2];
+    const double tt = in_time[t], it = in_itime[t];
     Acc s1{};
     for (int f = fl; f < f1; ++f) {
       const int64_t k = e * nchan + f;
       const double w = wt ? (double)wt[k] : 1.0;
-      const double kf = blk.d_frequency[f] / kC;  // frequency / constants.c
+      const double kf = in_freq[f] / kC;  // frequency / constants.c
       const cplx<R> vv = vis[k];
-      const double vals[10] = {tt, blk.d_frequency[f], uvw0 * kf, uvw1 * kf, uvw2 * kf, it,
-                               blk.d_channel_bandwidth[f], (double)vv.x, (double)vv.y, iwt ? (double)iwt[k] : 1.0};
+      const double vals[10] = {tt, in_freq[f], uvw0 * kf, uvw1 * kf, uvw2 * kf, it,
+                               in_bw[f], (double)vv.x, (double)vv.y, iwt ? (double)iwt[k] : 1.0};
       if (fa <= 1) {  // average_chunks returns the values themselves
         s1 = Acc{w, vals[0], vals[1], vals[2], vals[3], vals[4], vals[5], vals[6], vals[7], vals[8], vals[9]};
       } else {

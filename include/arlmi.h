@@ -280,7 +280,8 @@ int arlmi_bda_uvdist(const double* d_uvw, const void* d_weight, int32_t ntimes, 
  * antenna1 = a1, antenna2 = a2.  The blocks need time, integration time, frequency and channel
  * bandwidth; weight and imaging weight may be NULL (1).  d_cindex (int64 [ntimes][nants][nants]
  * [nchan], or NULL) receives d_rowoff[b] + ((t * nchan + ch) mod cells of b) for element
- * (t, a2, a1, ch), as ARL's flat assignment of range(cells) gives it (:278). */
+ * (t, a2, a1, ch), as ARL's flat assignment of range(cells) gives it (:278).  The output
+ * columns must not overlap the block arrays. */
 int arlmi_bda_average(const arlmi_blocks* blocks, const int32_t* d_tav, const int32_t* d_fav, const int64_t* d_rowoff,
                       int64_t nrows, const arlmi_rows* rows, int64_t* d_cindex, int precision, void* stream);
 
