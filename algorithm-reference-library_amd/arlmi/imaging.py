@@ -153,8 +153,13 @@ class Imager:
         self.plan.update(vis, slab_boxes)
         self._shift_cache = None
 
-    def invert(self, vis, dopsf=False, normalize=True, imaginary=False, out=None):
-        """Dirty image (or PSF): grid, shifted inverse FFT, screens, gcf and sum-of-weights."""
+    def invert(self, vis, dopsf=False, normalize=True, imaginary=False, out=None, weigh_slabs=False):
+        """Dirty image (or PSF): grid, shifted inverse FFT, screens, gcf and sum-of-weights.
+
+        normalize=False returns the plain sum over planes (invert_2d / invert_wstack_single with
+        normalize=False); with weigh_slabs it returns what the list workflows return for
+        normalize=False instead: each slab's unnormalised image weighted by the slab's sum of
+        weights, the sum divided by the total (imaging_serial.py:166-197)."""
         if vis.precision != self.precision:
             raise TypeError("visibility precision changed")
         lmn = _shift_lmn(vis, self.im)
@@ -168,7 +173,8 @@ class Imager:
         re = out if out is not None else torch.empty((nchan, 1, n, n), dtype=rdt, device=self.dev)
         imo = torch.empty_like(re) if imaginary else None
         _lib.check(_lib.lib().arlmi_grids_to_image(self.plan.handle, ptr(self.grids), ptr(self.sumwt_planes),
-                                                   ptr(self.gcf_dev), int(bool(normalize)), ptr(re), ptr(imo),
+                                                   ptr(self.gcf_dev), 1 if normalize else (2 if weigh_slabs else 0),
+                                                   ptr(re), ptr(imo),
                                                    self.precision, stream_handle()))
         return re, imo
 

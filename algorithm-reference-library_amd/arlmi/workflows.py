@@ -90,7 +90,9 @@ def invert_list_serial_workflow(vis_list, template_model_imagelist, dopsf=False,
             from .kernels import create_pswf_convolutionfunction
             g = create_pswf_convolutionfunction(template_model_imagelist[0])
         imager = Imager(vis, model, gcfcf=g, mode=mode, vis_slices=vis_slices if mode == "wstack" else 1, **kwargs)
-        re, _ = imager.invert(vis, dopsf=dopsf, normalize=True)
+        # ARL passes normalize to each slab's invert, weights each slab's image by its sumwt and
+        # normalises the sum (imaging_serial.py:166-197)
+        re, _ = imager.invert(vis, dopsf=dopsf, normalize=normalize, weigh_slabs=True)
         results.append((Image(re, model.wcs.deepcopy(), model.polarisation_frame), imager.sumwt()))
     return results
 

@@ -507,9 +507,13 @@ __global__ __launch_bounds__(Shape<N>::NT * Shape<N>::ROW_NB) void k_rows_invert
         if (o) outside |= 1u << (q * RADL + r);
       }
     }
+  // normalize 2 (the list workflows' normalize=False): each w slab's unnormalised image weighted
+  // by its sum of weights (imaging_serial.py:193-194)
+  const bool weigh = a.wstack && a.normalize == 2;
   for (int p = p0; p < p1; ++p) {
     const double sw = a.sumwt[(size_t)p * a.nchan + c];
-    if (a.wstack && a.normalize && !(sw > 0.0)) continue;  // normalize_sumwt zeroes this slab's image
+    if (a.wstack && a.normalize == 1 && !(sw > 0.0)) continue;  // normalize_sumwt zeroes this slab's image
+    const R wsc = weigh ? (R)sw : R(1);
     const cplx<R>* row = grids + ((size_t)p * a.nchan + c) * N * N + (size_t)y * N;
     const uint32_t cm = a.colmask ? a.colmask[(size_t)p * a.nchan + c] : ~0u;
     const double wp = SCREENS ? a.plane_w[p] : 0.0;
@@ -525,7 +529,7 @@ __global__ __launch_bounds__(Shape<N>::NT * Shape<N>::ROW_NB) void k_rows_invert
           if constexpr (SCREENS) {
             if (!((outside >> (q * RADL + r)) & 1u)) {
               const cplx<R> s = expi_turns<R>(screen_turns(wp, qs[q][r]));
-              acc[q][r] += s.x * v.x - s.y * v.y;
+              acc[q][r] += wsc * (s.x * v.x - s.y * v.y);
             }
           } else {
             acc[q][r] += v.x;
@@ -553,6 +557,7 @@ __global__ __launch_bounds__(Shape<N>::NT * Shape<N>::ROW_NB) void k_rows_invert
   R* img = reinterpret_cast<R*>(a.img_re);
   R* imi = reinterpret_cast<R*>(a.img_im);
   const bool zero = a.normalize && !(tot > 0.0);
+  const bool divide = a.normalize == 1 || weigh;
 #pragma unroll
   for (int q = 0; q < PER; ++q)
 #pragma unroll
@@ -562,10 +567,11 @@ __global__ __launch_bounds__(Shape<N>::NT * Shape<N>::ROW_NB) void k_rows_invert
       double re = (double)acc[q][r] * g;
       double im = 0.0;
       if constexpr (!SCREENS) im = (double)acci[q][r] * g;
-      if (a.normalize) {
-        re = zero ? 0.0 : re / tot;
-        im = zero ? 0.0 : im / tot;
+      if (divide) {
+        re /= tot;
+        im /= tot;
       }
+      if (zero) re = im = 0.0;
       img[gi] = (R)re;
       if (imi) imi[gi] = (R)im;
     }
@@ -598,11 +604,11 @@ __global__ __launch_bounds__(256) void k_invert_finish(FftArgs a, int n, int G) 
   const double gc = a.gcf[gi];
   re *= gc;
   im *= gc;
-  if (a.normalize) {
-    const bool zero = !(tot > 0.0);
-    re = zero ? 0.0 : re / tot;
-    im = zero ? 0.0 : im / tot;
+  if (a.normalize == 1 || (a.wstack && a.normalize == 2)) {
+    re /= tot;
+    im /= tot;
   }
+  if (a.normalize && !(tot > 0.0)) re = im = 0.0;
   reinterpret_cast<R*>(a.img_re)[gi] = (R)re;
   if (a.img_im) reinterpret_cast<R*>(a.img_im)[gi] = (R)im;
 }
@@ -887,6 +893,10 @@ int arlmi_grids_to_image(const arlmi_plan* p, void* d_grids, const double* d_sum
   if (int rc = prec_ok(precision)) return rc;
   if (!p || !d_grids || !d_sumwt || !d_gcf || !d_img_re) {
     set_error("arlmi_grids_to_image: NULL argument");
+    return ARLMI_E_ARG;
+  }
+  if (normalize < 0 || normalize > 2) {
+    set_error("arlmi_grids_to_image: normalize must be 0, 1 or 2");
     return ARLMI_E_ARG;
   }
   if (d_img_im && p->g.mode == ARLMI_MODE_WSTACK) {

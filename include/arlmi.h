@@ -166,7 +166,11 @@ int arlmi_degrid(const arlmi_plan* plan, const void* d_grids, const arlmi_kernel
  *   img[c] = gcf[c] / S_c * sum_planes Re(b_p * N^2 ifft(G[p][c]))   (b_p = 1 in MODE_2D)
  *   S_c = sum over all planes of sumwt[p][c]; img = 0 if S_c <= 0; a plane with
  *   sumwt[p][c] <= 0 contributes nothing (normalize_sumwt zeroes its slab image).
- * normalize = 0 skips the division.  d_img_im (may be NULL, MODE_2D only) receives the
+ * normalize = 0 skips the division (the plain plane sum: invert_wstack_single / invert_2d with
+ * normalize=False, and the adjoint of predict).  normalize = 2 is the list workflows'
+ * normalize=False (imaging_serial.py:166-197): every slab's unnormalised image weighted by its
+ * sumwt[p][c], summed and divided by S_c (MODE_WSTACK); in MODE_2D the plain sum, 0 if S_c <= 0.
+ * d_img_im (may be NULL, MODE_2D only) receives the
  * imaginary part (invert_2d imaginary=True).  d_grids is used as scratch (destroyed).
  * d_gcf: float64 [nchan][N][N]. */
 int arlmi_grids_to_image(const arlmi_plan* plan, void* d_grids, const double* d_sumwt, const double* d_gcf,

@@ -31,7 +31,15 @@ Pinning (what checks each function; see tests/test_oracle_*.py):
                                            point-source agreement with the direct DFT
   vis_wslice_iter, *_wstack_single, w-stack list workflows
                                            closed form of the slab partition, adjoint
-                                           identity, direct DFT with w != 0
+                                           identity, and ARL's point-source recipe with w != 0
+                                           (tests/workflows/test_imaging_serial.py:153-187,
+                                           232-243) against the direct DFT: the 101-slab residual
+                                           falls to the w = 0 kernel floor, the w-stack +
+                                           w-projection hybrid stays under ARL's 1.0 on 100 Jy,
+                                           and a conjugated screen or a w shift of the wrong
+                                           sign fails (tests/test_oracle_wstack_dft.py); the
+                                           normalize semantics of the list workflow by weight
+                                           scaling
   simulate_point / predict_skycomponent_visibility
                                            ARL known answers
       (tests/processing_library/test_coordinate_support.py:104-123)
