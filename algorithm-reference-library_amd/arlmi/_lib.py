@@ -36,7 +36,7 @@ EXPORTS = (
     "arlmi_grid", "arlmi_degrid",
     "arlmi_grids_to_image", "arlmi_image_to_grids", "arlmi_fft2d", "arlmi_w_screen", "arlmi_dft",
     "arlmi_grid_weights", "arlmi_reweight", "arlmi_block_to_rows", "arlmi_apply_gaintable", "arlmi_apply_gains_rows", "arlmi_apply_gains_rows_adjoint",
-    "arlmi_phaserotate_rows", "arlmi_phaserotate_blocks", "arlmi_hogbom", "arlmi_hogbom_traffic", "arlmi_solve_gains",
+    "arlmi_phaserotate_rows", "arlmi_phaserotate_blocks", "arlmi_hogbom", "arlmi_hogbom_noalloc", "arlmi_hogbom_traffic", "arlmi_solve_gains",
     "arlmi_bda_uvdist", "arlmi_bda_average",
     "arlmi_probe_lds_add", "arlmi_profile_enable", "arlmi_profile_collect", "arlmi_profile_reset",
     "arlmi_profile_report", "arlmi_profile_only",
@@ -147,6 +147,8 @@ def lib():
                                                     vp]),
         "arlmi_hogbom": (ctypes.c_int, [vp, vp, vp, i32, i32, vp, i32, i32, dbl, dbl, i32, dbl,
                                         ctypes.POINTER(i32), vp]),
+        "arlmi_hogbom_noalloc": (ctypes.c_int, [vp, vp, vp, i32, i32, vp, i32, i32, dbl, dbl, i32, dbl,
+                                                ctypes.POINTER(i32), vp]),
         "arlmi_solve_gains": (ctypes.c_int, [vp, vp, vp, i32, i32, i32, vp, i32, vp, vp, vp, vp, vp, vp,
                                              ctypes.c_int, i32, dbl, i32, dbl, ctypes.c_int, ctypes.c_int, vp]),
         "arlmi_bda_uvdist": (ctypes.c_int, [vp, vp, i32, i32, i32, vp, vp, ctypes.c_int, vp]),

@@ -262,45 +262,53 @@ void* clean_scratch(size_t bytes) {
   return s.mem;
 }
 
-}  // namespace
-}  // namespace arlmi
+// Scratch layout of a minor cycle on an nx-row image: the control word, the partial maxima of
+// k_clean_maxes and the (value, index) maximum of every band.  About 2048 bands of whole rows:
+// enough blocks to fill the GPU, few maxima to reduce, so the layout is at most 49,408 bytes.
+constexpr int kMaxBlocks = 1024;  // blocks of the max pass
+struct CleanLayout {
+  int rows, nb;
+  size_t off_part, off_pval, off_pidx, bytes;
+};
+CleanLayout clean_layout(int nx) {
+  CleanLayout l;
+  l.rows = std::max(1, (nx + 2047) / 2048);
+  l.nb = (nx + l.rows - 1) / l.rows;
+  l.off_part = 256;
+  l.off_pval = l.off_part + sizeof(double) * 2 * kMaxBlocks;
+  l.off_pidx = l.off_pval + sizeof(double) * l.nb;
+  l.bytes = l.off_pidx + sizeof(int64_t) * l.nb;
+  return l;
+}
 
-using namespace arlmi;
-
-extern "C" {
-
-int arlmi_hogbom(double* d_res, double* d_comps, const double* d_psf, int32_t psf_nx, int32_t psf_ny,
-                 const double* d_window, int32_t nx, int32_t ny, double gain, double thresh, int32_t niter,
-                 double fracthresh, int32_t* iterations, void* stream) {
+int check_hogbom_args(const char* who, const double* d_res, const double* d_comps, const double* d_psf, int32_t psf_nx,
+                      int32_t psf_ny, int32_t nx, int32_t ny, double gain, int32_t niter) {
   if (!d_res || !d_comps || !d_psf || nx < 1 || ny < 1 || psf_nx < 1 || psf_ny < 1) {
-    set_error("arlmi_hogbom: bad argument");
+    set_error("%s: bad argument", who);
     return ARLMI_E_ARG;
   }
   if (!(gain > 0.0 && gain < 2.0) || niter < 1) {  // cleaners.py:33-34
-    set_error("arlmi_hogbom: need 0 < gain < 2 and niter > 0 (gain %g, niter %d)", gain, niter);
+    set_error("%s: need 0 < gain < 2 and niter > 0 (gain %g, niter %d)", who, gain, niter);
     return ARLMI_E_ARG;
   }
-  hipStream_t s = as_stream(stream);
+  return ARLMI_OK;
+}
+
+// The minor cycle on `mem` (device memory of at least clean_layout(nx).bytes); allocates nothing.
+int hogbom_run(unsigned char* mem, double* d_res, double* d_comps, const double* d_psf, int32_t psf_nx, int32_t psf_ny,
+               const double* d_window, int32_t nx, int32_t ny, double gain, double thresh, int32_t niter,
+               double fracthresh, int32_t* iterations, hipStream_t s) {
   const int64_t npix = (int64_t)nx * ny, npsf = (int64_t)psf_nx * psf_ny;
-  // about 2048 bands of whole rows: enough blocks to fill the GPU, few maxima to reduce
-  const int rows = std::max(1, (nx + 2047) / 2048);
-  const int nb = (nx + rows - 1) / rows;
-  const int nmax = 1024;  // blocks of the max pass
-  const size_t off_part = 256, off_pval = off_part + sizeof(double) * 2 * nmax;
-  const size_t off_pidx = off_pval + sizeof(double) * nb, bytes = off_pidx + sizeof(int64_t) * nb;
-  unsigned char* scratch = static_cast<unsigned char*>(clean_scratch(bytes));
-  if (!scratch) {
-    set_error("arlmi_hogbom: scratch allocation of %zu bytes failed", bytes);
-    return ARLMI_E_HIP;
-  }
-  CleanCtl* ctl = reinterpret_cast<CleanCtl*>(scratch);
-  double* part = reinterpret_cast<double*>(scratch + off_part);
+  const CleanLayout lay = clean_layout(nx);
+  const int rows = lay.rows, nb = lay.nb, nmax = kMaxBlocks;
+  CleanCtl* ctl = reinterpret_cast<CleanCtl*>(mem);
+  double* part = reinterpret_cast<double*>(mem + lay.off_part);
   ARLMI_LAUNCH("k_clean_init", k_clean_init, dim3(1024), dim3(kCleanThreads), 0, s, d_comps, npix, ctl);
   ARLMI_LAUNCH("k_clean_maxes", k_clean_maxes, dim3(nmax), dim3(kCleanThreads), 0, s, (const double*)d_res, npix,
                d_psf, npsf, part);
   ARLMI_LAUNCH_CHECK();
-  std::vector<double> hp(2 * nmax);
-  if (hipMemcpyAsync(hp.data(), part, sizeof(double) * 2 * nmax, hipMemcpyDeviceToHost, s) != hipSuccess ||
+  double hp[2 * kMaxBlocks];
+  if (hipMemcpyAsync(hp, part, sizeof(double) * 2 * nmax, hipMemcpyDeviceToHost, s) != hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess) {
     set_error("arlmi_hogbom: %s", hipGetErrorString(hipGetLastError()));
     return ARLMI_E_HIP;
@@ -330,8 +338,8 @@ int arlmi_hogbom(double* d_res, double* d_comps, const double* d_psf, int32_t ps
   a.niter = niter;
   a.rows = rows;
   a.ctl = ctl;
-  a.pval = reinterpret_cast<double*>(scratch + off_pval);
-  a.pidx = reinterpret_cast<int64_t*>(scratch + off_pidx);
+  a.pval = reinterpret_cast<double*>(mem + lay.off_pval);
+  a.pidx = reinterpret_cast<int64_t*>(mem + lay.off_pidx);
   // niter peaks take niter + 1 launches (the last one only subtracts); launches after the end
   // return at once
   CleanCtl hc{};
@@ -358,6 +366,78 @@ int arlmi_hogbom(double* d_res, double* d_comps, const double* d_psf, int32_t ps
   g_clean_scanned = hc.scanned;
   g_clean_area = hc.area;
   return ARLMI_OK;
+}
+
+// Scratch of arlmi_hogbom_noalloc: a file-scope device array, one copy per GPU, held until the
+// process ends.  Host code never passes `scratch` itself (a host-side stand-in): it passes the
+// device address hipGetSymbolAddress gives, looked up once per device.
+constexpr size_t kNoAllocScratchBytes = 64 * 1024;
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mutex;
+void* g_noalloc_ptr[kMaxDevices] = {};
+
+}  // namespace
+}  // namespace arlmi
+
+static __device__ __align__(256) unsigned char scratch[arlmi::kNoAllocScratchBytes];
+
+namespace arlmi {
+namespace {
+
+unsigned char* noalloc_scratch() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    set_error("arlmi_hogbom_noalloc: no usable current device (%d)", dev);
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+  if (!g_noalloc_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error("arlmi_hogbom_noalloc: hipGetSymbolAddress failed: %s", hipGetErrorString(e));
+      return nullptr;
+    }
+    g_noalloc_ptr[dev] = p;
+  }
+  return static_cast<unsigned char*>(g_noalloc_ptr[dev]);
+}
+
+}  // namespace
+}  // namespace arlmi
+
+using namespace arlmi;
+
+extern "C" {
+
+int arlmi_hogbom(double* d_res, double* d_comps, const double* d_psf, int32_t psf_nx, int32_t psf_ny,
+                 const double* d_window, int32_t nx, int32_t ny, double gain, double thresh, int32_t niter,
+                 double fracthresh, int32_t* iterations, void* stream) {
+  if (int rc = check_hogbom_args("arlmi_hogbom", d_res, d_comps, d_psf, psf_nx, psf_ny, nx, ny, gain, niter)) return rc;
+  const size_t bytes = clean_layout(nx).bytes;
+  unsigned char* mem = static_cast<unsigned char*>(clean_scratch(bytes));
+  if (!mem) {
+    set_error("arlmi_hogbom: scratch allocation of %zu bytes failed", bytes);
+    return ARLMI_E_HIP;
+  }
+  return hogbom_run(mem, d_res, d_comps, d_psf, psf_nx, psf_ny, d_window, nx, ny, gain, thresh, niter, fracthresh,
+                    iterations, as_stream(stream));
+}
+
+int arlmi_hogbom_noalloc(double* d_res, double* d_comps, const double* d_psf, int32_t psf_nx, int32_t psf_ny,
+                         const double* d_window, int32_t nx, int32_t ny, double gain, double thresh, int32_t niter,
+                         double fracthresh, int32_t* iterations, void* stream) {
+  if (int rc = check_hogbom_args("arlmi_hogbom_noalloc", d_res, d_comps, d_psf, psf_nx, psf_ny, nx, ny, gain, niter))
+    return rc;
+  const size_t bytes = clean_layout(nx).bytes;
+  if (bytes > kNoAllocScratchBytes) {
+    set_error("arlmi_hogbom_noalloc: %zu bytes of scratch needed, %zu available", bytes, kNoAllocScratchBytes);
+    return ARLMI_E_UNSUPPORTED;
+  }
+  unsigned char* mem = noalloc_scratch();
+  if (!mem) return ARLMI_E_HIP;
+  return hogbom_run(mem, d_res, d_comps, d_psf, psf_nx, psf_ny, d_window, nx, ny, gain, thresh, niter, fracthresh,
+                    iterations, as_stream(stream));
 }
 
 int arlmi_hogbom_traffic(int64_t* scanned_pixels, int64_t* subtracted_pixels) {

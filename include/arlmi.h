@@ -369,6 +369,18 @@ int arlmi_hogbom(double* d_res, double* d_comps, const double* d_psf, int32_t ps
                  const double* d_window, int32_t nx, int32_t ny, double gain, double thresh, int32_t niter,
                  double fracthresh, int32_t* iterations, void* stream);
 
+/* arlmi_hogbom with no allocation (no counterpart in ARL): for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, results and synchronisation as arlmi_hogbom, but the control block
+ * and band maxima live in a fixed 64 KiB device array of the library (one per GPU, held until the
+ * process ends) instead of a buffer grown with hipMalloc; it calls no hipMalloc / hipFree.  Every
+ * image size fits (the scratch layout needs at most 49,408 bytes); a larger need would return
+ * ARLMI_E_UNSUPPORTED.  All calls on one GPU share that array and the control block is cleared at
+ * the start of every call, so calls that overlap on one GPU race: run them one after another. */
+int arlmi_hogbom_noalloc(double* d_res, double* d_comps, const double* d_psf, int32_t psf_nx, int32_t psf_ny,
+                         const double* d_window, int32_t nx, int32_t ny, double gain, double thresh, int32_t niter,
+                         double fracthresh, int32_t* iterations, void* stream);
+
 /* solve_gaintable for scalar gains (processing_components/calibration/calibration.py:30-101,
  * npol = 1, crosspol = False), float64 arithmetic:
  *   for every gain-table row s with at least one integration (d_active[s] != 0; d_slot[t] is the
