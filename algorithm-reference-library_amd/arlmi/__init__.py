@@ -9,6 +9,7 @@ from . import _lib
 from .data_models import (WCS, PolarisationFrame, Image, GridData, ConvolutionFunction, Visibility, BlockVisibility,
                           GainTable, Skycomponent, SkyCoord)
 from .kernels import (create_image, create_convolutionfunction_from_image, create_pswf_convolutionfunction,
+                      convolutionfunction_from_arrays,
                       create_box_convolutionfunction, create_awterm_convolutionfunction,
                       apply_bounding_box_convolutionfunction, calculate_bounding_box_convolutionfunction)
 from .gridding import (convolution_mapping, grid_visibility_to_griddata, grid_visibility_to_griddata_fast,

@@ -292,7 +292,11 @@ def predict_skycomponent_visibility(vis, sc, chan_map=None):
     """Add the direct-DFT visibilities of point components (imaging/base.py:192-220), on the GPU.
 
     sc: Skycomponent or list of them, directions given as (l, m); flux [nchan_img, npol].
-    chan_map: optional int32 device tensor, the image channel of each row.
+    chan_map: optional int32 device tensor, the image channel of each row.  By default, for fluxes
+    of several channels, ARL's map (get_frequency_map(vis, None),
+    processing_library/imaging/imaging_params.py:23-34): the index of each row's frequency among
+    the sorted distinct frequencies.  A single-channel flux applies to every row.  A map outside the
+    flux's channels raises IndexError, as numpy's comp.flux[map] would.
     """
     if sc is None:
         return vis
@@ -309,7 +313,18 @@ def predict_skycomponent_visibility(vis, sc, chan_map=None):
     d_lmf = torch.as_tensor(lmf).to(device())
     out = vis.vis.clone()
     uvw = vis.data["uvw"]
-    chan = None if chan_map is None else chan_map.to(device=device(), dtype=torch.int32).contiguous()
+    if chan_map is not None:
+        chan = chan_map.to(device=device(), dtype=torch.int32).contiguous()
+    elif nchan > 1:
+        _, inv = torch.unique(vis.data["frequency"], sorted=True, return_inverse=True)
+        chan = inv.to(torch.int32).contiguous()
+    else:
+        chan = None
+    if chan is not None and chan.numel():
+        lo, hi = int(chan.min().item()), int(chan.max().item())
+        if lo < 0 or hi >= nchan:
+            raise IndexError("frequency map reaches image channels %d..%d; the component fluxes have %d"
+                             % (lo, hi, nchan))
     per_call = max(1, (64 * 1024) // (8 * (3 + nchan)))  # components staged in one block's LDS
     for k0 in range(0, len(sc), per_call):
         part = d_lmf[k0:k0 + per_call]

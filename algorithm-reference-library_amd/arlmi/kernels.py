@@ -17,6 +17,7 @@ from .data_models import WCS, ConvolutionFunction, Image, D2R, device, _radec
 from .plan import ptr, stream_handle
 
 __all__ = ["grdsf", "create_image", "create_convolutionfunction_from_image", "create_pswf_convolutionfunction",
+           "convolutionfunction_from_arrays",
            "create_box_convolutionfunction", "create_awterm_convolutionfunction",
            "apply_bounding_box_convolutionfunction", "calculate_bounding_box_convolutionfunction"]
 
@@ -236,3 +237,40 @@ def calculate_bounding_box_convolutionfunction(cf, fractional_level=1e-4):
         hi = coords.max(dim=0).values
         out.append((z, (int(lo[1]), int(hi[1])), (int(lo[0]), int(hi[0]))))
     return out
+
+
+def convolutionfunction_from_arrays(im, gcf_data, cf_data, cf_crpix=None, cf_cdelt=None):
+    """(gcf Image, ConvolutionFunction) on the device from another library's arrays, e.g. the
+    gcfcf pair ARL's create_pswf_convolutionfunction / create_awterm_convolutionfunction return
+    (kernels.py:52-200): gcf_data [nchan, npol, N, N] real, cf_data [nchan, npol, nz, os, os, S, S]
+    complex, cf_crpix / cf_cdelt the 7 values of the kernel's grid WCS (ARL's
+    cf.grid_wcs.wcs.crpix / .cdelt; by default those create_convolutionfunction_from_image gives a
+    kernel of this shape on ``im``, with zstep 1e15).  A kernel whose taps are real and, with one w
+    plane, factor exactly as CF[yo, xo, a, b] = k[yo, a] k[xo, b] (every PSWF kernel) is marked
+    separable, so the gridders read 2 S numbers a row instead of S^2; any other kernel is used as a
+    full table."""
+    cf_np = numpy.asarray(cf_data)
+    if cf_np.ndim != 7:
+        raise ValueError("cf_data must be [nchan, npol, nz, os, os, S, S]")
+    nz, os_, _, S = cf_np.shape[2], cf_np.shape[3], cf_np.shape[4], cf_np.shape[5]
+    zstep = 1e15 if cf_cdelt is None else float(cf_cdelt[4])
+    cf = create_convolutionfunction_from_image(im, nz=nz, zstep=zstep, oversampling=os_, support=S)
+    if cf_crpix is not None:
+        cf.grid_wcs.wcs.crpix = numpy.asarray(cf_crpix, dtype=numpy.float64).copy()
+    if cf_cdelt is not None:
+        cf.grid_wcs.wcs.cdelt = numpy.asarray(cf_cdelt, dtype=numpy.float64).copy()
+    cf.data = cf_np.astype(numpy.complex128)
+    if nz == 1 and not numpy.any(numpy.imag(cf_np)):
+        # rank-1 test of M[(yo, a), (xo, b)] = CF[0, 0, 0, yo, xo, a, b]
+        m = numpy.real(cf_np[0, 0, 0]).transpose(0, 2, 1, 3).reshape(os_ * S, os_ * S)
+        j = int(numpy.argmax(numpy.diag(m)))
+        if m[j, j] > 0.0:
+            k = m[:, j] / numpy.sqrt(m[j, j])
+            if numpy.max(numpy.abs(m - numpy.outer(k, k))) <= 1e-12 * numpy.max(numpy.abs(m)):
+                cf.separable = k.reshape(os_, S)
+    g = numpy.asarray(gcf_data, dtype=numpy.float64)
+    t = torch.as_tensor(g).to(im.data.device)
+    nchan, npol = im.data.shape[0], im.data.shape[1]
+    t = t.reshape(-1, g.shape[-3] if g.ndim > 3 else 1, g.shape[-2], g.shape[-1])[:nchan, :npol]
+    gcf = Image(t.expand(nchan, npol, *g.shape[-2:]).contiguous(), im.wcs.deepcopy(), im.polarisation_frame)
+    return gcf, cf

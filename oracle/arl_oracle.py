@@ -800,12 +800,17 @@ def predict_skycomponent_visibility(vis, components, chan_map=None):
     """Add the direct-DFT visibilities of point components (imaging/base.py:207-220).
 
     components: list of (l, m, flux) with flux either scalar or [nchan_img, npol].
-    chan_map: image channel of each row (get_frequency_map); default all 0.
+    chan_map: image channel of each row; default ARL's get_frequency_map(vis, None) of the
+    Visibility branch (processing_library/imaging/imaging_params.py:23-34): the index of each row's
+    frequency among numpy.unique(frequency).  A scalar or single-channel flux applies to every row.
     """
     out = numpy.array(vis['vis'], copy=True)
     nvis, npol = out.shape
     if chan_map is None:
-        chan_map = numpy.zeros(nvis, dtype=int)
+        if 'frequency' in vis:
+            chan_map = numpy.searchsorted(numpy.unique(vis['frequency']), vis['frequency'])
+        else:
+            chan_map = numpy.zeros(nvis, dtype=int)
     for (l, m, flux) in components:
         phasor = simulate_point(vis['uvw'], l, m)
         f = numpy.atleast_2d(numpy.asarray(flux, dtype=float))

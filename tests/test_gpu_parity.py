@@ -260,6 +260,33 @@ def test_dft_parity(gpu):
         assert rel_err(vis.vis.cpu().numpy(), ref) < (1e-12 if precision == 64 else 1e-6)
 
 
+def test_dft_multichannel_frequency_map(gpu):
+    """Fluxes of several channels follow ARL's map (the index of each row's frequency among the
+    distinct frequencies, imaging_params.py:23-34); a map past the flux's channels raises."""
+    import arlmi
+    rng = numpy.random.default_rng(18)
+    d = random_rows(rng, 4000, 256, 5e-4, 8, wmax=100.0)
+    d["frequency"] = rng.choice([1.2e8, 1.0e8], 4000)
+    comps = [(rng.uniform(-0.1, 0.1), rng.uniform(-0.1, 0.1), rng.uniform(0.1, 1.0, (2, 1))) for _ in range(5)]
+    dz = dict(d)
+    dz["vis"] = numpy.zeros_like(d["vis"])
+    ref = O.predict_skycomponent_visibility(dz, comps)
+    # the oracle's map: rows at 1.0e8 take flux channel 0, rows at 1.2e8 channel 1
+    lo = d["frequency"] == 1.0e8
+    ph = sum(f[0, 0] * O.simulate_point(d["uvw"][lo], l, m) for l, m, f in comps)
+    assert numpy.max(numpy.abs(ref[lo, 0] - ph)) < 1e-12
+    for precision in (64, 32):
+        vis = to_device_vis(d, precision)
+        vis.data["vis"].zero_()
+        arlmi.predict_skycomponent_visibility(vis, [arlmi.Skycomponent(l, m, f) for l, m, f in comps])
+        assert rel_err(vis.vis.cpu().numpy(), ref) < (1e-12 if precision == 64 else 1e-6)
+    d3 = dict(d)
+    d3["frequency"] = rng.choice([1.2e8, 1.0e8, 1.4e8], 4000)
+    vis = to_device_vis(d3, 64)
+    with pytest.raises(IndexError):
+        arlmi.predict_skycomponent_visibility(vis, [arlmi.Skycomponent(l, m, f) for l, m, f in comps])
+
+
 def test_awterm_kernel_parity(gpu):
     """GPU-built w-projection kernel against the oracle's (kernels.py:100-200)."""
     import arlmi
