@@ -512,7 +512,7 @@ def _check_footprints(pu, pv, n, support):
         raise ValueError("could not broadcast: kernel footprint crosses the grid edge")
 
 
-def grid_visibility_to_griddata_loop(vis, geom, cf, cfg, nw_grid=1):
+def grid_visibility_to_griddata_loop(vis, geom, cf, cfg, nw_grid=1, wstep_grid=1e15):
     """ARL's per-visibility gridding loop, unchanged in structure (gridding.py:99-133).
 
     vis: dict with 'uvw' [V,3] (wavelengths), 'vis' [V,npol], 'imaging_weight' [V,npol],
@@ -521,7 +521,7 @@ def grid_visibility_to_griddata_loop(vis, geom, cf, cfg, nw_grid=1):
     nchan, npol, nz, oversampling, _, support, _ = cf.shape
     sumwt = numpy.zeros([geom.nchan, npol])
     pu, pu_off, pv, pv_off, pwg, pwc, pfreq = convolution_mapping(vis['uvw'], vis['frequency'], geom, cfg,
-                                                                  nw_grid=nw_grid)
+                                                                  nw_grid=nw_grid, wstep_grid=wstep_grid)
     _check_footprints(pu, pv, geom.npixel, support)
     gv, gu = cf.shape[-2:]
     grid = numpy.zeros([geom.nchan, npol, nw_grid, geom.npixel, geom.npixel], dtype='complex')
@@ -536,7 +536,7 @@ def grid_visibility_to_griddata_loop(vis, geom, cf, cfg, nw_grid=1):
     return grid, sumwt
 
 
-def grid_visibility_to_griddata(vis, geom, cf, cfg, nw_grid=1, chunk=1 << 16):
+def grid_visibility_to_griddata(vis, geom, cf, cfg, nw_grid=1, chunk=1 << 16, wstep_grid=1e15):
     """Vectorised form of grid_visibility_to_griddata_loop (gridding.py:99-133).
 
     numpy.add.at adds the contributions to each pixel unbuffered in input order,
@@ -546,7 +546,7 @@ def grid_visibility_to_griddata(vis, geom, cf, cfg, nw_grid=1, chunk=1 << 16):
     nchan, npol, nz, oversampling, _, support, _ = cf.shape
     n = geom.npixel
     pu, pu_off, pv, pv_off, pwg, pwc, pfreq = convolution_mapping(vis['uvw'], vis['frequency'], geom, cfg,
-                                                                  nw_grid=nw_grid)
+                                                                  nw_grid=nw_grid, wstep_grid=wstep_grid)
     _check_footprints(pu, pv, n, support)
     cfc = numpy.conjugate(cf)
     half = support // 2
@@ -587,12 +587,12 @@ def grid_visibility_to_griddata_fast(vis, geom, cfg, nw_grid=1):
     return grid, sumwt
 
 
-def degrid_visibility_from_griddata_loop(vis, grid, geom, cf, cfg):
+def degrid_visibility_from_griddata_loop(vis, grid, geom, cf, cfg, wstep_grid=1e15):
     """ARL's per-visibility degridding loop (gridding.py:233-267); no conjugation."""
     nchan, npol, nz, oversampling, _, support, _ = cf.shape
     nw_grid = grid.shape[2]
     pu, pu_off, pv, pv_off, pwg, pwc, pfreq = convolution_mapping(vis['uvw'], vis['frequency'], geom, cfg,
-                                                                  nw_grid=nw_grid)
+                                                                  nw_grid=nw_grid, wstep_grid=wstep_grid)
     _check_footprints(pu, pv, geom.npixel, support)
     gv, gu = cf.shape[-2:]
     du, dv = gu // 2, gv // 2
@@ -605,13 +605,13 @@ def degrid_visibility_from_griddata_loop(vis, grid, geom, cf, cfg):
     return out
 
 
-def degrid_visibility_from_griddata(vis, grid, geom, cf, cfg, chunk=1 << 15):
+def degrid_visibility_from_griddata(vis, grid, geom, cf, cfg, chunk=1 << 15, wstep_grid=1e15):
     """Vectorised form of degrid_visibility_from_griddata_loop (gridding.py:233-267)."""
     nchan, npol, nz, oversampling, _, support, _ = cf.shape
     n = geom.npixel
     nw_grid = grid.shape[2]
     pu, pu_off, pv, pv_off, pwg, pwc, pfreq = convolution_mapping(vis['uvw'], vis['frequency'], geom, cfg,
-                                                                  nw_grid=nw_grid)
+                                                                  nw_grid=nw_grid, wstep_grid=wstep_grid)
     _check_footprints(pu, pv, n, support)
     half = support // 2
     a = numpy.arange(support)

@@ -269,3 +269,47 @@ def test_gain_convention():
     v = numpy.ones((1, 1), complex)
     out = O.apply_gains_rows(v, numpy.array([0]), numpy.array([2]), g)
     assert abs(out[0, 0] - 0.5j * numpy.conjugate(1.0 + 1j)) < 1e-15
+
+
+def test_grid_w_planes_partition_rows():
+    """A grid with nw WW planes (create_griddata_from_image(nw, wstep), gridding.py:58-65) puts each
+    row on plane round(w/wstep) + nw//2 and nowhere else: every plane equals the single-plane grid
+    of exactly those rows, and the planes sum to the single-plane grid of all rows."""
+    rng = numpy.random.default_rng(41)
+    n, cell, nw, wstep = 64, 0.01, 5, 20.0
+    g = O.ImageGeom(n, cell)
+    gcf, cf, cfg = O.create_awterm_convolutionfunction(g, nw=nw, wstep=wstep, oversampling=4, support=8)
+    d = random_rows(rng, 600, n, cell, 8, wmax=45.0)
+    grid, sw = O.grid_visibility_to_griddata(d, g, cf, cfg, nw_grid=nw, wstep_grid=wstep)
+    one, sw1 = O.grid_visibility_to_griddata(d, g, cf, cfg)
+    assert numpy.max(numpy.abs(grid.sum(axis=2) - one[:, :, 0])) < 1e-12 * numpy.max(numpy.abs(one))
+    plane = numpy.round(d["uvw"][:, 2] / wstep).astype(int) + nw // 2
+    assert set(numpy.unique(plane)) == set(range(nw))
+    for z in range(nw):
+        sel = plane == z
+        part, _ = O.grid_visibility_to_griddata(O._rows(d, sel), g, cf, cfg)
+        assert numpy.max(numpy.abs(grid[:, :, z] - part[:, :, 0])) < 1e-12 * numpy.max(numpy.abs(one))
+    assert abs(sw[0, 0] - numpy.sum(d["imaging_weight"])) < 1e-9
+
+
+def test_spectral_cube_channels_partition_rows():
+    """Image channel c of a cube (pfreq = round((f - f0)/df), gridding.py:80-85) grids exactly the
+    rows whose frequency maps to c, and its sum of weights is theirs."""
+    rng = numpy.random.default_rng(42)
+    n, cell = 64, 0.01
+    g3 = O.ImageGeom(n, cell, nchan=3, frequency=1.0e8, channel_bandwidth=1.0e7)
+    g1 = O.ImageGeom(n, cell)
+    gcf, cf, cfg = O.create_pswf_convolutionfunction(g3, 4, 6)
+    d = random_rows(rng, 500, n, cell, 6)
+    d["frequency"] = rng.choice([1.0e8, 1.1e8, 1.2e8], 500)
+    grid, sw = O.grid_visibility_to_griddata(d, g3, cf, cfg)
+    _, cf1, cfg1 = O.create_pswf_convolutionfunction(g1, 4, 6)
+    for c, f in enumerate((1.0e8, 1.1e8, 1.2e8)):
+        sel = d["frequency"] == f
+        part, psw = O.grid_visibility_to_griddata(dict(O._rows(d, sel), frequency=numpy.full(sel.sum(), 1e8)),
+                                                  g1, cf1, cfg1)
+        assert numpy.array_equal(grid[c], part[0])
+        assert abs(sw[c, 0] - psw[0, 0]) < 1e-12
+    with pytest.raises(AssertionError):
+        bad = dict(d, frequency=numpy.full(500, 1.3e8))
+        O.grid_visibility_to_griddata(bad, g3, cf, cfg)
