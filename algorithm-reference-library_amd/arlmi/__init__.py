@@ -22,6 +22,7 @@ from .workflows import (imaging_contexts, imaging_context, invert_list_serial_wo
                         predict_list_serial_workflow, sum_invert_results)
 from .fft_support import fft, ifft
 from .plan import Plan
+from .streaming import StreamingImager
 from .coalesce import (convert_blockvisibility_to_visibility, convert_blocks, decoalesce_visibility,
                        coalesce_visibility, average_in_blocks)
 from .calibration import (create_gaintable_from_blockvisibility, apply_gaintable, gain_row_indices, apply_gains_rows,

@@ -1,0 +1,181 @@
+"""Invert and predict of one visibility set whose rows are processed in chunks (a set too large for
+one plan, e.g. ~600 M rows at 8192^2), with the result of processing all rows at once.
+
+ARL's w-stack list workflow treats the set as one Visibility: the slab centres come from max |w|
+over every row (processing_components/visibility/iterators.py:99-106), each slab's w shift is the
+mean w of all its rows (processing_components/imaging/wstack_single.py:43, :83), and each slab's
+image sums all its rows (workflows/serial/imaging/imaging_serial.py:166-197).  Here a first pass
+over the chunks gathers max |w| and the per-slab sums and counts of w (arlmi_wmax,
+arlmi_wslab_sums); then one plan, rebuilt per chunk with those centres and means fixed
+(arlmi_plan_fix_plane_w), grids every chunk into resident grids (arlmi_grid_add) and the planes are
+transformed, screened and summed once.  Predict transforms the model once and degrids each chunk
+from the resident grids.  Grids are dense (each chunk touches other tiles).
+"""
+
+import ctypes
+
+import numpy
+import torch
+
+from . import _lib
+from .data_models import device
+from .imaging import _gcfcf, _real_dtype
+from .plan import Plan, ptr, stream_handle, grids_buffer, wslice_boxes, kernel_struct
+
+__all__ = ["StreamingImager"]
+
+
+def _chunks(source):
+    return source() if callable(source) else iter(source)
+
+
+class StreamingImager:
+    """One image geometry and kernel over a visibility set given as chunks.
+
+    chunks: a list of Visibility objects, or a callable returning an iterator over them (called
+    once per pass, so chunks may be produced on the fly).  mode "wstack" (vis_slices slabs, ARL's
+    slab rule over the whole set) or "2d".  The image is centred on the visibilities' phase
+    centre (no image-centre shift).
+    """
+
+    def __init__(self, im, gcfcf=None, mode="wstack", vis_slices=1, precision=32, tile=0, chunk=0, **kwargs):
+        if mode not in ("2d", "wstack"):
+            raise ValueError("mode must be '2d' or 'wstack'")
+        self.gcf, self.cf = _gcfcf(im, gcfcf, kwargs)
+        self.im = im
+        self.mode = mode
+        self.vis_slices = vis_slices if mode == "wstack" else 1
+        self.precision = precision
+        self.tile = tile
+        self.chunk = chunk
+        self.dev = device()
+        self.plan = None
+        self._planned = None
+        self.grids = None
+        self.boxes = None
+        self.plane_w = None
+        self.sumwt_planes = None
+        self.gcf_dev = self.gcf.data.to(torch.float64).contiguous()
+
+    # -- pass 1 ---------------------------------------------------------------------------
+    def scan(self, chunks):
+        """max |w| over all rows -> slab centres and width; per-slab sums and counts of w ->
+        slab means (0 for an empty slab).  Returns (boxes, width, means, counts)."""
+        L = _lib.lib()
+        if self.mode != "wstack":
+            self.boxes = None
+            return None
+        wmax = 0.0
+        for v in _chunks(chunks):
+            w = ctypes.c_double()
+            uvw = v.data["uvw"]
+            _lib.check(L.arlmi_wmax(ptr(uvw), uvw.shape[0], stream_handle(), ctypes.byref(w)))
+            wmax = max(wmax, w.value)
+        boxes, width = wslice_boxes(wmax, self.vis_slices)
+        P = self.vis_slices
+        sums = numpy.zeros(P)
+        counts = numpy.zeros(P, dtype=numpy.int64)
+        bptr = boxes.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        for v in _chunks(chunks):
+            uvw = v.data["uvw"]
+            _lib.check(L.arlmi_wslab_sums(ptr(uvw), uvw.shape[0], bptr, width, P,
+                                          sums.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                          counts.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), stream_handle()))
+        means = numpy.where(counts > 0, sums / numpy.maximum(counts, 1), 0.0)
+        self.boxes = (boxes, width)
+        self.plane_w = numpy.ascontiguousarray(means, dtype=numpy.float64)
+        return boxes, width, self.plane_w, counts
+
+    def _plan_for(self, vis):
+        """The plan rebuilt for one chunk with the whole set's slab centres and means."""
+        if self.plan is not None and self._planned is vis:
+            return self.plan
+        self._planned = vis
+        if self.plan is None:
+            mode = _lib.MODE_WSTACK if self.mode == "wstack" else _lib.MODE_2D
+            self.plan = Plan(vis, self.im, self.cf, mode, self.vis_slices, tile=self.tile, slab_boxes=self.boxes,
+                             sparse=False, chunk=self.chunk)
+            if self.mode == "wstack":
+                _lib.check(_lib.lib().arlmi_plan_fix_plane_w(
+                    self.plan.handle, self.plane_w.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                    len(self.plane_w)))
+                self.plan.update(vis, self.boxes)
+        else:
+            self.plan.update(vis, self.boxes)
+        return self.plan
+
+    def _ensure_grids(self, vis):
+        if self.grids is None:
+            self._plan_for(vis)
+            self.grids = grids_buffer(self.plan, self.precision, self.dev)
+            self.sumwt_planes = torch.empty((self.plan.nplanes, self.plan.nchan), dtype=torch.float64,
+                                            device=self.dev)
+
+    def release(self):
+        """Free the resident grids (the plan keeps its own buffers until it is closed)."""
+        self.grids = None
+        self._planned = None
+        if self.plan is not None:
+            self.plan.close()
+            self.plan = None
+
+    # -- pass 2 ---------------------------------------------------------------------------
+    def invert(self, chunks, dopsf=False, normalize=True):
+        """Dirty image (or PSF) of all rows: [nchan, 1, N, N] in the precision, and sumwt
+        [nchan, 1] (host).  normalize=False gives the list workflows' normalize=False form."""
+        if self.mode == "wstack" and self.boxes is None:
+            self.scan(chunks)
+        L = _lib.lib()
+        first = True
+        for v in _chunks(chunks):
+            if v.precision != self.precision:
+                raise TypeError("chunk precision differs from the imager's")
+            if first:
+                self._ensure_grids(v)
+                self.grids.zero_()
+                self.sumwt_planes.zero_()
+                first = False
+            plan = self._plan_for(v)
+            k, keep = kernel_struct(self.cf, self.precision)
+            _lib.check(L.arlmi_grid_add(plan.handle, ptr(v.data["vis"]), ptr(v.data["imaging_weight"]),
+                                        int(bool(dopsf)), ctypes.byref(k), ptr(self.grids), ptr(self.sumwt_planes),
+                                        self.precision, stream_handle()))
+            del keep
+        if first:
+            raise ValueError("no chunks")
+        n, nchan = self.plan.npixel, self.plan.nchan
+        re = torch.empty((nchan, 1, n, n), dtype=_real_dtype(self.precision), device=self.dev)
+        _lib.check(L.arlmi_grids_to_image(self.plan.handle, ptr(self.grids), ptr(self.sumwt_planes),
+                                          ptr(self.gcf_dev), 1 if normalize else 2, ptr(re), None, self.precision,
+                                          stream_handle()))
+        return re, self.sumwt_planes.sum(dim=0).cpu().numpy()[:, None]
+
+    def predict(self, model_data, chunks, sink=None):
+        """Model visibilities of every chunk: gcf * conj(screen) * model, shifted FFTs once, then
+        degridding per chunk.  Returns the list of [V, 1] complex tensors, or, with ``sink``, calls
+        sink(index, vis_chunk, predicted) per chunk and returns None."""
+        if self.mode == "wstack" and self.boxes is None:
+            self.scan(chunks)
+        L = _lib.lib()
+        rdt = _real_dtype(self.precision)
+        model = model_data.to(device=self.dev, dtype=rdt).contiguous()
+        out = [] if sink is None else None
+        first = True
+        for i, v in enumerate(_chunks(chunks)):
+            if first:
+                self._ensure_grids(v)
+                plan = self.plan
+                _lib.check(L.arlmi_image_to_grids(plan.handle, ptr(model), ptr(self.gcf_dev), ptr(self.grids),
+                                                  self.precision, stream_handle()))
+                first = False
+            plan = self._plan_for(v)
+            pred = torch.empty_like(v.vis)
+            k, keep = kernel_struct(self.cf, self.precision)
+            _lib.check(L.arlmi_degrid(plan.handle, ptr(self.grids), ctypes.byref(k), ptr(pred), self.precision,
+                                      stream_handle()))
+            del keep
+            if sink is None:
+                out.append(pred)
+            else:
+                sink(i, v, pred)
+        return out

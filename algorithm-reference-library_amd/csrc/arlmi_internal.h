@@ -2,6 +2,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <vector>
 
 #include <cstdint>
 #include <cstdio>
@@ -232,4 +233,6 @@ struct arlmi_plan {
   int64_t nentries = 0, ngitems = 0, nditems = 0, nmulti = 0, nslots = 0, norphans = 0, ndropped = 0, nmember = 0;
   int64_t orphan_start = 0;  // orphan rows are records [orphan_start, orphan_start + norphans)
   int chunk = 4096;
+  // slab means given by the caller (arlmi_plan_fix_plane_w; empty: the rows' own means)
+  std::vector<double> fixed_w;
 };

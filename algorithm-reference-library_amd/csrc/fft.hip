@@ -228,8 +228,10 @@ __device__ __forceinline__ void fft_pass(cplx<R>* lds, int b, int t, const cplx<
 // The twiddle table lives in LDS after the transform data (LDS loads cannot be hoisted
 // across the stage barriers, global ones can, which would pin every stage's twiddles in
 // registers at once).  Callers size dynamic LDS as data + N complex values.
-template <typename R, int N>
-__device__ __forceinline__ cplx<R>* stage_twiddles_at(unsigned char* smem, size_t data_bytes, const cplx<R>* twg) {
+template <typename R, int N, bool TWG = false>
+__device__ __forceinline__ const cplx<R>* stage_twiddles_at(unsigned char* smem, size_t data_bytes,
+                                                            const cplx<R>* twg) {
+  if constexpr (TWG) return twg;  // the table does not fit beside the data: read it from L2
   cplx<R>* tw = reinterpret_cast<cplx<R>*>(smem + data_bytes);
   for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = twg[i];
   return tw;
@@ -259,11 +261,27 @@ struct Shape {
   // plain passes
   static constexpr int FNT = N >= 8 ? N / 8 : 1;
   static constexpr int FNB = FNT >= 512 ? 1 : 512 / FNT;
+  // A padded row and the twiddle table together exceed LDS (complex128 at N = 8192): the row
+  // passes then read their twiddles from the global table (L2-resident) instead.
+  template <typename R>
+  static constexpr bool tw_global() {
+    return sizeof(cplx<R>) * (lds_elems<LAY_ROWPAD, N>(FNB > ROW_NB ? FNB : ROW_NB) + N) > 163840;
+  }
+  template <typename R>
+  static constexpr size_t tw_lds() {
+    return tw_global<R>() ? 0 : (size_t)N;
+  }
+  // Two adjacent columns and the twiddles exceed LDS: a column pass would read 16 bytes per grid
+  // row, so the columns are transformed as rows between two in-place transposes instead.
+  template <typename R>
+  static constexpr bool cols_by_transpose() {
+    return (size_t)N * sizeof(cplx<R>) * (size_t)(col_nb<R>() + 1) > 163840;
+  }
   template <typename R>
   static constexpr bool fits() {
-    return sizeof(cplx<R>) * (lds_elems<LAY_ROWPAD, N>(ROW_NB) + N) <= 163840 &&
-           (size_t)N * sizeof(cplx<R>) * (size_t)(col_nb<R>() + 1) <= 163840 &&
-           sizeof(cplx<R>) * (lds_elems<LAY_ROWPAD, N>(FNB) + N) <= 163840;
+    return sizeof(cplx<R>) * (lds_elems<LAY_ROWPAD, N>(ROW_NB) + tw_lds<R>()) <= 163840 &&
+           sizeof(cplx<R>) * (lds_elems<LAY_ROWPAD, N>(FNB) + tw_lds<R>()) <= 163840 &&
+           (cols_by_transpose<R>() || (size_t)N * sizeof(cplx<R>) * (size_t)(col_nb<R>() + 1) <= 163840);
   }
 };
 
@@ -359,7 +377,8 @@ __global__ __launch_bounds__(Shape<N>::NT * Shape<N>::ROW_NB) void k_rows_plain(
   constexpr int NT = Shape<N>::NT, NB = Shape<N>::ROW_NB;
   extern __shared__ __align__(16) unsigned char smem[];
   cplx<R>* lds = reinterpret_cast<cplx<R>*>(smem);
-  cplx<R>* tw = stage_twiddles_at<R, N>(smem, sizeof(cplx<R>) * lds_elems<LAY_ROWPAD, N>(NB), twg);
+  const cplx<R>* tw =
+      stage_twiddles_at<R, N, Shape<N>::template tw_global<R>()>(smem, sizeof(cplx<R>) * lds_elems<LAY_ROWPAD, N>(NB), twg);
   const int b = threadIdx.x / NT, t = threadIdx.x % NT;
   const int64_t row = (int64_t)blockIdx.x * NB + b;
   cplx<R>* data = reinterpret_cast<cplx<R>*>(a.data) + row * N;
@@ -387,7 +406,8 @@ __global__ __launch_bounds__(Shape<N>::FNT * Shape<N>::FNB) void k_rows_predict(
   constexpr int PER = (N / 8) / NT;
   extern __shared__ __align__(16) unsigned char smem[];
   cplx<R>* lds = reinterpret_cast<cplx<R>*>(smem);
-  cplx<R>* tw = stage_twiddles_at<R, N>(smem, sizeof(cplx<R>) * lds_elems<LAY_ROWPAD, N>(NB), twg);
+  const cplx<R>* tw =
+      stage_twiddles_at<R, N, Shape<N>::template tw_global<R>()>(smem, sizeof(cplx<R>) * lds_elems<LAY_ROWPAD, N>(NB), twg);
   cplx<R>* grids = reinterpret_cast<cplx<R>*>(a.data);
   const R* model = reinterpret_cast<const R*>(a.model);
   constexpr int RB = N / NB;  // row blocks per channel
@@ -436,7 +456,7 @@ __global__ __launch_bounds__(Shape<N>::NT * Shape<N>::template col_nb<R>()) void
   constexpr int NT = Shape<N>::NT, NB = Shape<N>::template col_nb<R>();
   extern __shared__ __align__(16) unsigned char smem[];
   cplx<R>* lds = reinterpret_cast<cplx<R>*>(smem);
-  cplx<R>* tw = stage_twiddles_at<R, N>(smem, sizeof(cplx<R>) * (size_t)N * NB, twg);
+  const cplx<R>* tw = stage_twiddles_at<R, N>(smem, sizeof(cplx<R>) * (size_t)N * NB, twg);
   constexpr int CB = N / NB;
   // column groups sharing a 128-byte line run on one XCD
   constexpr int LG = (NB * (int)sizeof(cplx<R>)) >= 128 ? 1 : 128 / (NB * (int)sizeof(cplx<R>));
@@ -481,7 +501,8 @@ __global__ __launch_bounds__(Shape<N>::NT * Shape<N>::ROW_NB) void k_rows_invert
   constexpr int NSL = last_ns<N>(), RADL = N / NSL, PER = NSL / NT;
   extern __shared__ __align__(16) unsigned char smem[];
   cplx<R>* lds = reinterpret_cast<cplx<R>*>(smem);
-  cplx<R>* tw = stage_twiddles_at<R, N>(smem, sizeof(cplx<R>) * lds_elems<LAY_ROW, N>(NB), twg);
+  const cplx<R>* tw =
+      stage_twiddles_at<R, N, Shape<N>::template tw_global<R>()>(smem, sizeof(cplx<R>) * lds_elems<LAY_ROW, N>(NB), twg);
   const cplx<R>* grids = reinterpret_cast<const cplx<R>*>(a.data);
   constexpr int RB = N / NB;  // row blocks per channel
   const int c = blockIdx.x / RB;
@@ -728,6 +749,62 @@ const void* twiddles(int n, int precision) {
   return d;
 }
 
+// In-place transpose of every N x N plane of a batch (N a multiple of 32): block (pair, plane)
+// exchanges tile (i, j) with tile (j, i) through LDS; a diagonal tile is transposed onto itself.
+// Tile rows are 32 contiguous values, so loads and stores stay coalesced.
+template <typename R>
+__global__ __launch_bounds__(256) void k_transpose(cplx<R>* __restrict__ data, int N) {
+  __shared__ cplx<R> ta[32][33], tb[32][33];
+  const int64_t p = blockIdx.x;  // pair index over the lower triangle i >= j
+  int i = (int)((sqrt(8.0 * (double)p + 1.0) - 1.0) * 0.5);
+  while ((int64_t)(i + 1) * (i + 2) / 2 <= p) ++i;
+  while ((int64_t)i * (i + 1) / 2 > p) --i;
+  const int j = (int)(p - (int64_t)i * (i + 1) / 2);
+  cplx<R>* base = data + (size_t)blockIdx.y * N * N;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int r = ty; r < 32; r += 8) {
+    ta[r][tx] = base[(size_t)(i * 32 + r) * N + j * 32 + tx];
+    tb[r][tx] = base[(size_t)(j * 32 + r) * N + i * 32 + tx];
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    base[(size_t)(j * 32 + r) * N + i * 32 + tx] = ta[tx][r];
+    base[(size_t)(i * 32 + r) * N + j * 32 + tx] = tb[tx][r];
+  }
+}
+
+// The column transforms of c.nbatch planes in place (modulation and scale as c says): NB adjacent
+// columns per block, or, where two columns and the twiddles exceed LDS, transpose + row pass +
+// transpose ((-1)^(x+y) is symmetric, so the row pass applies the same modulation).
+template <typename R, int N, bool INV>
+static int run_cols(FftArgs c, const cplx<R>* tw, hipStream_t s) {
+  constexpr int NT = Shape<N>::NT;
+  if constexpr (Shape<N>::template cols_by_transpose<R>()) {
+    if (c.tilemask) {
+      set_error("sparse grids are not supported at N = %d in this precision", N);
+      return ARLMI_E_UNSUPPORTED;
+    }
+    constexpr int RNB = Shape<N>::ROW_NB;
+    const size_t rsh = sizeof(cplx<R>) * (lds_elems<LAY_ROWPAD, N>(RNB) + Shape<N>::template tw_lds<R>());
+    const int64_t nt = N / 32;
+    const dim3 tg((unsigned)(nt * (nt + 1) / 2), (unsigned)c.nbatch);
+    ARLMI_LAUNCH("k_transpose", k_transpose<R>, tg, dim3(256), 0, s, (cplx<R>*)c.data, N);
+    ARLMI_LAUNCH_CHECK();
+    ARLMI_LAUNCH("k_rows_plain", (k_rows_plain<R, N, INV>), dim3((unsigned)((int64_t)c.nbatch * N / RNB)),
+                 dim3(NT * RNB), rsh, s, c, tw);
+    ARLMI_LAUNCH_CHECK();
+    ARLMI_LAUNCH("k_transpose", k_transpose<R>, tg, dim3(256), 0, s, (cplx<R>*)c.data, N);
+    ARLMI_LAUNCH_CHECK();
+  } else {
+    constexpr int CNB = Shape<N>::template col_nb<R>();
+    const size_t csh = sizeof(cplx<R>) * (size_t)N * (CNB + 1);
+    const unsigned cblocks = (unsigned)((int64_t)c.nbatch * N / CNB);
+    ARLMI_LAUNCH("k_cols_plain", (k_cols_plain<R, N, INV>), dim3(cblocks), dim3(NT * CNB), csh, s, c, tw);
+    ARLMI_LAUNCH_CHECK();
+  }
+  return ARLMI_OK;
+}
+
 template <typename R, int N>
 static int run_fft2d(FftArgs a, int direction, hipStream_t s) {
   const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(twiddles(N, sizeof(R) == 4 ? 32 : 64));
@@ -735,9 +812,9 @@ static int run_fft2d(FftArgs a, int direction, hipStream_t s) {
     set_error("twiddle table allocation failed");
     return ARLMI_E_HIP;
   }
-  constexpr int NT = Shape<N>::NT, RNB = Shape<N>::ROW_NB, CNB = Shape<N>::template col_nb<R>();
-  const size_t rsh = sizeof(cplx<R>) * (lds_elems<LAY_ROWPAD, N>(RNB) + N), csh = sizeof(cplx<R>) * (size_t)N * (CNB + 1);
-  const unsigned rblocks = (unsigned)((int64_t)a.nbatch * N / RNB), cblocks = (unsigned)((int64_t)a.nbatch * N / CNB);
+  constexpr int NT = Shape<N>::NT, RNB = Shape<N>::ROW_NB;
+  const size_t rsh = sizeof(cplx<R>) * (lds_elems<LAY_ROWPAD, N>(RNB) + Shape<N>::template tw_lds<R>());
+  const unsigned rblocks = (unsigned)((int64_t)a.nbatch * N / RNB);
   FftArgs r = a, c = a;
   r.in_mod = 1;
   r.out_mod = 0;
@@ -748,15 +825,11 @@ static int run_fft2d(FftArgs a, int direction, hipStream_t s) {
   if (direction < 0) {
     ARLMI_LAUNCH("k_rows_plain", (k_rows_plain<R, N, true>), dim3(rblocks), dim3(NT * RNB), rsh, s, r, tw);
     ARLMI_LAUNCH_CHECK();
-    ARLMI_LAUNCH("k_cols_plain", (k_cols_plain<R, N, true>), dim3(cblocks), dim3(NT * CNB), csh, s, c, tw);
-    ARLMI_LAUNCH_CHECK();
-  } else {
-    ARLMI_LAUNCH("k_rows_plain", (k_rows_plain<R, N, false>), dim3(rblocks), dim3(NT * RNB), rsh, s, r, tw);
-    ARLMI_LAUNCH_CHECK();
-    ARLMI_LAUNCH("k_cols_plain", (k_cols_plain<R, N, false>), dim3(cblocks), dim3(NT * CNB), csh, s, c, tw);
-    ARLMI_LAUNCH_CHECK();
+    return run_cols<R, N, true>(c, tw, s);
   }
-  return ARLMI_OK;
+  ARLMI_LAUNCH("k_rows_plain", (k_rows_plain<R, N, false>), dim3(rblocks), dim3(NT * RNB), rsh, s, r, tw);
+  ARLMI_LAUNCH_CHECK();
+  return run_cols<R, N, false>(c, tw, s);
 }
 
 // Plane groups for the fused row passes: enough blocks to fill the GPU several times over.
@@ -773,17 +846,14 @@ static int run_grids_to_image(FftArgs a, hipStream_t s) {
     set_error("twiddle table allocation failed");
     return ARLMI_E_HIP;
   }
-  constexpr int NT = Shape<N>::NT, RNB = Shape<N>::ROW_NB, CNB = Shape<N>::template col_nb<R>();
-  const size_t rsh = sizeof(cplx<R>) * (lds_elems<LAY_ROWPAD, N>(RNB) + N), csh = sizeof(cplx<R>) * (size_t)N * (CNB + 1);
-  const size_t ish = sizeof(cplx<R>) * (lds_elems<LAY_ROW, N>(RNB) + N);
+  constexpr int NT = Shape<N>::NT, RNB = Shape<N>::ROW_NB;
+  const size_t ish = sizeof(cplx<R>) * (lds_elems<LAY_ROW, N>(RNB) + Shape<N>::template tw_lds<R>());
   FftArgs c = a;
   c.in_mod = 1;
   c.out_mod = 0;
   c.scale = 1.0;
   c.nbatch = a.nplanes * a.nchan;
-  const unsigned cblocks = (unsigned)((int64_t)c.nbatch * N / CNB);
-  ARLMI_LAUNCH("k_cols_plain", (k_cols_plain<R, N, true>), dim3(cblocks), dim3(NT * CNB), csh, s, c, tw);
-  ARLMI_LAUNCH_CHECK();
+  if (int rc = run_cols<R, N, true>(c, tw, s)) return rc;
   const int rowblocks = a.nchan * (N / RNB);
   int G = 1;
   plane_groups(rowblocks, a.nplanes, &a.ppg, &G);
@@ -810,10 +880,8 @@ static int run_image_to_grids(FftArgs a, hipStream_t s) {
     set_error("twiddle table allocation failed");
     return ARLMI_E_HIP;
   }
-  constexpr int NT = Shape<N>::NT, RNB = Shape<N>::ROW_NB, CNB = Shape<N>::template col_nb<R>();
-  const size_t rsh = sizeof(cplx<R>) * (lds_elems<LAY_ROWPAD, N>(RNB) + N), csh = sizeof(cplx<R>) * (size_t)N * (CNB + 1);
   constexpr int FNT = Shape<N>::FNT, FNB = Shape<N>::FNB;
-  const size_t fsh = sizeof(cplx<R>) * (lds_elems<LAY_ROWPAD, N>(FNB) + N);
+  const size_t fsh = sizeof(cplx<R>) * (lds_elems<LAY_ROWPAD, N>(FNB) + Shape<N>::template tw_lds<R>());
   const int rowblocks = a.nchan * (N / FNB);
   int G = 1;
   plane_groups(rowblocks, a.nplanes, &a.ppg, &G);
@@ -825,10 +893,7 @@ static int run_image_to_grids(FftArgs a, hipStream_t s) {
   c.out_mod = 1;
   c.scale = 1.0;
   c.nbatch = a.nplanes * a.nchan;
-  const unsigned cblocks = (unsigned)((int64_t)c.nbatch * N / CNB);
-  ARLMI_LAUNCH("k_cols_plain", (k_cols_plain<R, N, false>), dim3(cblocks), dim3(NT * CNB), csh, s, c, tw);
-  ARLMI_LAUNCH_CHECK();
-  return ARLMI_OK;
+  return run_cols<R, N, false>(c, tw, s);
 }
 
 enum class Op { FFT, TO_IMAGE, TO_GRIDS };
@@ -856,7 +921,7 @@ static int dispatch_n(Op op, int n, FftArgs a, int direction, hipStream_t s) {
       break;
   }
 #undef ARLMI_CASE
-  set_error("FFT size %d not supported at this precision (powers of two 64..8192 single, 64..4096 double)", n);
+  set_error("FFT size %d not supported (powers of two 64..8192)", n);
   return ARLMI_E_UNSUPPORTED;
 }
 

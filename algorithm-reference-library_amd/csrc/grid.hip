@@ -95,19 +95,26 @@ __device__ __forceinline__ uint32_t bcast(uint32_t x, int j) {
 // Writes the T x T accumulator tile (planar float64: real parts from acc[0], imaginary parts
 // from acc[im_off], row stride WS): to the grid, or (slot > 0) to partial tile slot-1 of a
 // split tile.
+// accum: add the tile to what the grid holds (streamed chunks) instead of overwriting it.
 template <typename R>
 __device__ inline void store_tile(const double* acc, int WS, int T, int N, const TilePos& tp, int slot,
-                                  cplx<R>* __restrict__ grids, cplx<R>* __restrict__ partial, int im_off) {
+                                  cplx<R>* __restrict__ grids, cplx<R>* __restrict__ partial, int im_off, int accum) {
   cplx<R>* g = grids + (size_t)tp.plane * N * N + (size_t)(tp.ty * T) * N + (size_t)tp.tx * T;
   cplx<R>* d = partial + (size_t)(slot > 0 ? slot - 1 : 0) * T * T;
   for (int i = threadIdx.x; i < T * T; i += blockDim.x) {
     const int y = i / T, x = i - (i / T) * T;
     const int o = y * WS + x;
-    const cplx<R> c = mk<R>((R)acc[o], (R)acc[im_off + o]);
-    if (slot == 0)
-      g[(size_t)y * N + x] = c;
-    else
-      d[i] = c;
+    if (slot == 0) {
+      double re = acc[o], im = acc[im_off + o];
+      if (accum) {
+        const cplx<R> old = g[(size_t)y * N + x];
+        re += (double)old.x;
+        im += (double)old.y;
+      }
+      g[(size_t)y * N + x] = mk<R>((R)re, (R)im);
+    } else {
+      d[i] = mk<R>((R)acc[o], (R)acc[im_off + o]);
+    }
   }
 }
 
@@ -121,7 +128,7 @@ __device__ inline void lds_add_f64(double* p, double v) {
 template <typename R>
 __global__ void k_reduce_tiles(const int4* __restrict__ multi, const uint32_t* __restrict__ nmulti,
                                const cplx<R>* __restrict__ partial, cplx<R>* __restrict__ grids, int T, int ntx,
-                               int N) {
+                               int N, int accum) {
   const int i = blockIdx.y * blockDim.x + threadIdx.x;
   if (i >= T * T) return;
   // the number of split tiles is read here, so the host never waits for it
@@ -150,7 +157,13 @@ __global__ void k_reduce_tiles(const int4* __restrict__ multi, const uint32_t* _
     re += a.x;
     im += a.y;
   }
-  grids[(size_t)tp.plane * N * N + (size_t)(tp.ty * T + i / T) * N + (size_t)tp.tx * T + (i % T)] = mk<R>(re, im);
+  cplx<R>* dst = grids + (size_t)tp.plane * N * N + (size_t)(tp.ty * T + i / T) * N + (size_t)tp.tx * T + (i % T);
+  if (accum) {
+    const cplx<R> old = *dst;
+    re += old.x;
+    im += old.y;
+  }
+  *dst = mk<R>(re, im);
   }
 }
 
@@ -213,7 +226,8 @@ __global__ __launch_bounds__(GRID_THREADS) void k_grid_sep(const int4* __restric
                                                            const cplx<R>* __restrict__ vis, const R* __restrict__ wt,
                                                            int dopsf, const R* __restrict__ ktab, int os,
                                                            cplx<R>* __restrict__ grids, cplx<R>* __restrict__ partial,
-                                                           double* __restrict__ sumwt, int T, int ntx, int N) {
+                                                           double* __restrict__ sumwt, int T, int ntx, int N,
+                                                           int accum) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double red[GRID_THREADS / 64];
   constexpr int TAPS = S * S;
@@ -223,6 +237,7 @@ __global__ __launch_bounds__(GRID_THREADS) void k_grid_sep(const int4* __restric
   double* acc = reinterpret_cast<double*>(smem);  // [2][TW + 64]: real, imaginary (tile + trash row)
   R* lk = reinterpret_cast<R*>(acc + 2 * (TW + 64));
   const int4 it = items[blockIdx.x];
+  if (accum && it.z == 0) return;  // an empty tile adds nothing
   for (int i = threadIdx.x; i < 2 * (TW + 64); i += blockDim.x) acc[i] = 0.0;
   for (int i = threadIdx.x; i < os * S; i += blockDim.x) lk[i] = ktab[i];
   __syncthreads();
@@ -288,7 +303,7 @@ __global__ __launch_bounds__(GRID_THREADS) void k_grid_sep(const int4* __restric
   }
   __syncthreads();
   const TilePos tp = tile_pos(it.x, ntx);
-  store_tile<R>(acc, WS, T, N, tp, it.w, grids, partial, TW + 64);
+  store_tile<R>(acc, WS, T, N, tp, it.w, grids, partial, TW + 64, accum);
   block_add_double(swt, &sumwt[tp.plane], red);
 }
 
@@ -338,13 +353,15 @@ __global__ __launch_bounds__(GRID_THREADS) void k_grid_tab(const int4* __restric
                                                            const cplx<R>* __restrict__ vis, const R* __restrict__ wt,
                                                            int dopsf, const cplx<R>* __restrict__ tab, int os, int S,
                                                            cplx<R>* __restrict__ grids, cplx<R>* __restrict__ partial,
-                                                           double* __restrict__ sumwt, int T, int ntx, int N) {
+                                                           double* __restrict__ sumwt, int T, int ntx, int N,
+                                                           int accum) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double red[GRID_THREADS / 64];
   const int WS = lds_stride(T);
   const int TW = T * WS;
   double* acc = reinterpret_cast<double*>(smem);  // [2][TW + 64]: real, imaginary (tile + trash row)
   const int4 it = items[blockIdx.x];
+  if (accum && it.z == 0) return;  // an empty tile adds nothing
   for (int i = threadIdx.x; i < 2 * (TW + 64); i += blockDim.x) acc[i] = 0.0;
   __syncthreads();
   const int lane = __lane_id(), wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
@@ -420,7 +437,7 @@ __global__ __launch_bounds__(GRID_THREADS) void k_grid_tab(const int4* __restric
   }
   __syncthreads();
   const TilePos tp = tile_pos(it.x, ntx);
-  store_tile<R>(acc, WS, T, N, tp, it.w, grids, partial, TW + 64);
+  store_tile<R>(acc, WS, T, N, tp, it.w, grids, partial, TW + 64, accum);
   block_add_double(swt, &sumwt[tp.plane], red);
 }
 
@@ -640,7 +657,7 @@ static int partial_buffer(const arlmi_plan* cp, cplx<R>** out) {
 
 template <typename R, int S>
 static int launch_grid_sep(const arlmi_plan* p, const void* vis, const void* wt, int dopsf, const void* ktab,
-                           void* grids, cplx<R>* partial, double* sumwt, hipStream_t s) {
+                           void* grids, cplx<R>* partial, double* sumwt, int accum, hipStream_t s) {
   const DevGeom& g = p->g;
   const size_t sh = sizeof(double) * 2 * ((size_t)g.tile * lds_stride(g.tile) + 64) + sizeof(R) * (size_t)g.os * S;
   if (sh > lds_limit()) {
@@ -650,7 +667,7 @@ static int launch_grid_sep(const arlmi_plan* p, const void* vis, const void* wt,
   if (p->ngitems == 0) return ARLMI_OK;
   ARLMI_LAUNCH("k_grid_sep", (k_grid_sep<R, S>), dim3((unsigned)p->ngitems), dim3(GRID_THREADS), sh, s, p->d_gitems,
                p->d_entries, (const cplx<R>*)vis, (const R*)wt, dopsf, (const R*)ktab, g.os, (cplx<R>*)grids, partial,
-               sumwt, g.tile, g.ntx, g.n);
+               sumwt, g.tile, g.ntx, g.n, accum);
   ARLMI_LAUNCH_CHECK();
   return ARLMI_OK;
 }
@@ -674,17 +691,17 @@ static int launch_degrid_sep(const arlmi_plan* p, const void* grids, const void*
 
 template <typename R>
 static int grid_launch(const arlmi_plan* p, const void* vis, const void* wt, int dopsf, const arlmi_kernel* k,
-                       void* grids, cplx<R>* partial, double* sumwt, hipStream_t s) {
+                       void* grids, cplx<R>* partial, double* sumwt, int accum, hipStream_t s) {
   const DevGeom& g = p->g;
   if (k->kind == ARLMI_KERNEL_SEPARABLE) {
     switch (g.support) {
-      case 2: return launch_grid_sep<R, 2>(p, vis, wt, dopsf, k->d_table, grids, partial, sumwt, s);
-      case 4: return launch_grid_sep<R, 4>(p, vis, wt, dopsf, k->d_table, grids, partial, sumwt, s);
-      case 6: return launch_grid_sep<R, 6>(p, vis, wt, dopsf, k->d_table, grids, partial, sumwt, s);
-      case 8: return launch_grid_sep<R, 8>(p, vis, wt, dopsf, k->d_table, grids, partial, sumwt, s);
-      case 10: return launch_grid_sep<R, 10>(p, vis, wt, dopsf, k->d_table, grids, partial, sumwt, s);
-      case 12: return launch_grid_sep<R, 12>(p, vis, wt, dopsf, k->d_table, grids, partial, sumwt, s);
-      case 16: return launch_grid_sep<R, 16>(p, vis, wt, dopsf, k->d_table, grids, partial, sumwt, s);
+      case 2: return launch_grid_sep<R, 2>(p, vis, wt, dopsf, k->d_table, grids, partial, sumwt, accum, s);
+      case 4: return launch_grid_sep<R, 4>(p, vis, wt, dopsf, k->d_table, grids, partial, sumwt, accum, s);
+      case 6: return launch_grid_sep<R, 6>(p, vis, wt, dopsf, k->d_table, grids, partial, sumwt, accum, s);
+      case 8: return launch_grid_sep<R, 8>(p, vis, wt, dopsf, k->d_table, grids, partial, sumwt, accum, s);
+      case 10: return launch_grid_sep<R, 10>(p, vis, wt, dopsf, k->d_table, grids, partial, sumwt, accum, s);
+      case 12: return launch_grid_sep<R, 12>(p, vis, wt, dopsf, k->d_table, grids, partial, sumwt, accum, s);
+      case 16: return launch_grid_sep<R, 16>(p, vis, wt, dopsf, k->d_table, grids, partial, sumwt, accum, s);
       default: break;
     }
     set_error("separable kernel support %d not instantiated; pass the full table", g.support);
@@ -702,7 +719,7 @@ static int grid_launch(const arlmi_plan* p, const void* vis, const void* wt, int
 #define ARLMI_GRID_TAB(MT)                                                                                        \
   ARLMI_LAUNCH("k_grid_tab", (k_grid_tab<R, MT>), dim3((unsigned)p->ngitems), dim3(GRID_THREADS), sh, s,        \
                p->d_gitems, p->d_entries, ez, (const cplx<R>*)vis, (const R*)wt, dopsf, tab, g.os, g.support,     \
-               (cplx<R>*)grids, partial, sumwt, g.tile, g.ntx, g.n)
+               (cplx<R>*)grids, partial, sumwt, g.tile, g.ntx, g.n, accum)
   // (a variant holding 8 taps per lane gave wrong sums in the single precision build and is
   // not used: supports above 16 take the loop without prefetch)
   if (taps <= 64)
@@ -718,20 +735,22 @@ static int grid_launch(const arlmi_plan* p, const void* vis, const void* wt, int
 
 template <typename R>
 static int grid_impl(const arlmi_plan* p, const void* vis, const void* wt, int dopsf, const arlmi_kernel* k,
-                     void* grids, double* sumwt, hipStream_t s) {
+                     void* grids, double* sumwt, int accum, hipStream_t s) {
   const DevGeom& g = p->g;
   const int64_t nsw = (int64_t)g.nplanes * g.nchan;
-  ARLMI_LAUNCH("k_zero_f64", k_zero_f64, dim3((unsigned)((nsw + 255) / 256)), dim3(256), 0, s, sumwt, nsw);
-  ARLMI_LAUNCH_CHECK();
+  if (!accum) {
+    ARLMI_LAUNCH("k_zero_f64", k_zero_f64, dim3((unsigned)((nsw + 255) / 256)), dim3(256), 0, s, sumwt, nsw);
+    ARLMI_LAUNCH_CHECK();
+  }
   cplx<R>* partial = nullptr;
   int rc = partial_buffer<R>(p, &partial);
   if (rc) return rc;
-  rc = grid_launch<R>(p, vis, wt, dopsf, k, grids, partial, sumwt, s);
+  rc = grid_launch<R>(p, vis, wt, dopsf, k, grids, partial, sumwt, accum, s);
   if (rc) return rc;
   if (p->nslots > 0) {  // every split tile has at least two slots
     const int64_t gx = std::min<int64_t>(std::max<int64_t>(1, p->nslots / 2), 4096);
     ARLMI_LAUNCH("k_reduce_tiles", k_reduce_tiles<R>, dim3((unsigned)gx, (g.tile * g.tile + 255) / 256),
-                 dim3(256), 0, s, p->d_multi, p->d_misc + 4, partial, (cplx<R>*)grids, g.tile, g.ntx, g.n);
+                 dim3(256), 0, s, p->d_multi, p->d_misc + 4, partial, (cplx<R>*)grids, g.tile, g.ntx, g.n, accum);
     ARLMI_LAUNCH_CHECK();
   }
   return ARLMI_OK;
@@ -827,8 +846,28 @@ int arlmi_grid(const arlmi_plan* p, const void* d_vis, const void* d_weight, int
     return ARLMI_E_ARG;
   }
   hipStream_t s = as_stream(stream);
-  if (precision == 32) return grid_impl<float>(p, d_vis, d_weight, dopsf, k, d_grids, d_sumwt, s);
-  if (precision == 64) return grid_impl<double>(p, d_vis, d_weight, dopsf, k, d_grids, d_sumwt, s);
+  if (precision == 32) return grid_impl<float>(p, d_vis, d_weight, dopsf, k, d_grids, d_sumwt, 0, s);
+  if (precision == 64) return grid_impl<double>(p, d_vis, d_weight, dopsf, k, d_grids, d_sumwt, 0, s);
+  set_error("precision must be 32 or 64");
+  return ARLMI_E_ARG;
+}
+
+int arlmi_grid_add(const arlmi_plan* p, const void* d_vis, const void* d_weight, int dopsf, const arlmi_kernel* k,
+                   void* d_grids, double* d_sumwt, int precision, void* stream) {
+  int rc = check_kernel(p, k);
+  if (rc) return rc;
+  if (!d_grids || !d_sumwt || (p->nentries > 0 && (!d_weight || (!dopsf && !d_vis)))) {
+    set_error("arlmi_grid_add: NULL buffer");
+    return ARLMI_E_ARG;
+  }
+  if (p->flags & ARLMI_FLAG_SPARSE_GRIDS) {
+    set_error("arlmi_grid_add: the plan has sparse grids, whose untouched tiles are never written; "
+              "accumulating chunks needs dense grids");
+    return ARLMI_E_ARG;
+  }
+  hipStream_t s = as_stream(stream);
+  if (precision == 32) return grid_impl<float>(p, d_vis, d_weight, dopsf, k, d_grids, d_sumwt, 1, s);
+  if (precision == 64) return grid_impl<double>(p, d_vis, d_weight, dopsf, k, d_grids, d_sumwt, 1, s);
   set_error("precision must be 32 or 64");
   return ARLMI_E_ARG;
 }

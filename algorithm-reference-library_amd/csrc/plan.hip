@@ -201,9 +201,10 @@ __global__ void k_slab_sums(const double* __restrict__ uvw, int64_t n, const dou
 }
 
 // Slab k (blockIdx.x): sums the nblk per-block partials in a fixed order (deterministic),
-// mean = sum / count.
+// mean = sum / count (and, when tsum is given, the totals themselves).
 __global__ __launch_bounds__(256) void k_slab_means(const double* wsum, const uint32_t* wcnt, int nblk, double* wmean,
-                                                    int nb) {
+                                                    int nb, double* tsum = nullptr,
+                                                    unsigned long long* tcnt = nullptr) {
   __shared__ double rs[4];
   __shared__ unsigned long long rc[4];
   const int k = blockIdx.x;
@@ -225,7 +226,11 @@ __global__ __launch_bounds__(256) void k_slab_means(const double* wsum, const ui
   if (threadIdx.x == 0) {
     const double t = (rs[0] + rs[1]) + (rs[2] + rs[3]);
     const unsigned long long c = rc[0] + rc[1] + rc[2] + rc[3];
-    wmean[k] = c ? t / (double)c : 0.0;
+    if (wmean) wmean[k] = c ? t / (double)c : 0.0;
+    if (tsum) {
+      tsum[k] = t;
+      tcnt[k] = c;
+    }
   }
 }
 
@@ -917,6 +922,9 @@ static int plan_build(arlmi_plan* p, const double* uvw, const double* freq, int6
       ARLMI_LAUNCH("k_boxes", k_boxes, dim3(1), dim3(256), 0, s, p->d_wsum, nrows > 0 ? wb : 0, P, p->d_boxes);
       ARLMI_LAUNCH_CHECK();
     }
+    if (!p->fixed_w.empty()) {  // slab means over all chunks of a stream, given by the caller
+      ARLMI_HIP(hipMemcpyAsync(p->d_plane_w, p->fixed_w.data(), sizeof(double) * P, hipMemcpyHostToDevice, s));
+    } else {
     const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((nrows + 1023) / 1024, SLAB_BLOCKS));
     if (nrows > 0) {
       const size_t sh = (2 * sizeof(double) + sizeof(unsigned int)) * P;
@@ -928,8 +936,9 @@ static int plan_build(arlmi_plan* p, const double* uvw, const double* freq, int6
       ARLMI_HIP(hipMemsetAsync(p->d_wcnt, 0, sizeof(uint32_t) * P, s));
     }
     ARLMI_LAUNCH("k_slab_means", k_slab_means, dim3((unsigned)P), dim3(256), 0, s, p->d_wsum, p->d_wcnt, blocks,
-                 p->d_plane_w, P);
+                 p->d_plane_w, P, nullptr, nullptr);
     ARLMI_LAUNCH_CHECK();
+    }
   }
   ARLMI_HIP(hipMemsetAsync(p->d_misc, 0, sizeof(uint32_t) * 8, s));
 
@@ -1127,6 +1136,76 @@ int arlmi_plan_create(const arlmi_geom* gin, const double* d_uvw, const double* 
     return rc;
   }
   *out = p;
+  return ARLMI_OK;
+}
+
+int arlmi_plan_fix_plane_w(arlmi_plan* p, const double* h_plane_w, int32_t n) {
+  if (!p) {
+    set_error("plan is NULL");
+    return ARLMI_E_ARG;
+  }
+  if (n == 0 || !h_plane_w) {
+    p->fixed_w.clear();
+    return ARLMI_OK;
+  }
+  if (p->g.mode != ARLMI_MODE_WSTACK || n != p->g.nplanes) {
+    set_error("arlmi_plan_fix_plane_w: a w-stack plan of %d slabs needs %d means (%d given)", p->g.nplanes,
+              p->g.nplanes, n);
+    return ARLMI_E_ARG;
+  }
+  p->fixed_w.assign(h_plane_w, h_plane_w + n);
+  return ARLMI_OK;
+}
+
+int arlmi_wslab_sums(const double* d_uvw, int64_t nrows, const double* h_boxes, double wslice, int32_t nslabs,
+                     double* h_sum, int64_t* h_count, void* stream) {
+  if (!h_boxes || !h_sum || !h_count || nslabs < 1 || (nrows > 0 && !d_uvw) || nrows < 0) {
+    set_error("arlmi_wslab_sums: bad argument");
+    return ARLMI_E_ARG;
+  }
+  if (nrows == 0) return ARLMI_OK;
+  hipStream_t s = as_stream(stream);
+  const int P = nslabs;
+  const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((nrows + 1023) / 1024, SLAB_BLOCKS));
+  const size_t bytes = sizeof(double) * (P + 1) + (sizeof(double) + sizeof(uint32_t)) * (size_t)blocks * P +
+                       (sizeof(double) + sizeof(unsigned long long)) * P + 64;
+  unsigned char* mem = nullptr;
+  ARLMI_HIP(hipMallocAsync(reinterpret_cast<void**>(&mem), bytes, s));
+  double* d_boxes = reinterpret_cast<double*>(mem);
+  double* d_part = d_boxes + (P + 1);
+  double* d_tsum = d_part + (size_t)blocks * P;
+  unsigned long long* d_tcnt = reinterpret_cast<unsigned long long*>(d_tsum + P);
+  uint32_t* d_pcnt = reinterpret_cast<uint32_t*>(d_tcnt + P);
+  std::vector<double> hb(h_boxes, h_boxes + P);
+  hb.push_back(wslice);
+  std::vector<double> tsum(P);
+  std::vector<unsigned long long> tcnt(P);
+  int rc = ARLMI_OK;
+  do {
+    if (hipMemcpyAsync(d_boxes, hb.data(), sizeof(double) * (P + 1), hipMemcpyHostToDevice, s) != hipSuccess) {
+      rc = ARLMI_E_HIP;
+      break;
+    }
+    const size_t sh = (2 * sizeof(double) + sizeof(unsigned int)) * P;
+    ARLMI_LAUNCH("k_slab_sums", k_slab_sums, dim3((unsigned)blocks), dim3(256), sh, s, d_uvw, nrows, d_boxes, P, d_part,
+                 d_pcnt);
+    ARLMI_LAUNCH("k_slab_means", k_slab_means, dim3((unsigned)P), dim3(256), 0, s, d_part, d_pcnt, blocks, nullptr, P,
+                 d_tsum, d_tcnt);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(tsum.data(), d_tsum, sizeof(double) * P, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipMemcpyAsync(tcnt.data(), d_tcnt, sizeof(unsigned long long) * P, hipMemcpyDeviceToHost, s) != hipSuccess)
+      rc = ARLMI_E_HIP;
+  } while (0);
+  (void)hipFreeAsync(mem, s);
+  if (hipStreamSynchronize(s) != hipSuccess) rc = ARLMI_E_HIP;
+  if (rc) {
+    set_error("arlmi_wslab_sums: %s", hipGetErrorString(hipGetLastError()));
+    return rc;
+  }
+  for (int k = 0; k < P; ++k) {
+    h_sum[k] += tsum[k];
+    h_count[k] += (int64_t)tcnt[k];
+  }
   return ARLMI_OK;
 }
 

@@ -132,6 +132,20 @@ int arlmi_plan_update(arlmi_plan* plan, const double* d_uvw, const double* d_fre
  * entries of one tile (default 4096).  Takes effect at the next arlmi_plan_update. */
 int arlmi_plan_set_chunk(arlmi_plan* plan, int32_t chunk);
 int arlmi_plan_stats_get(const arlmi_plan* plan, arlmi_plan_stats* out);
+/* Streaming (one vis set in chunks of rows, as ARL's list workflows see one Visibility whose rows
+ * arrive in pieces; imaging_serial.py:166-197 sums the per-slab images of all its rows).  The slab
+ * rule and means are those of the whole set (iterators.py:99-106 takes max|w| over every row,
+ * wstack_single.py:43, :83 the mean over every row of a slab), so a first pass gathers them:
+ * arlmi_wmax per chunk (the host keeps the largest), then arlmi_wslab_sums per chunk with the slab
+ * centres: h_sum[k] += sum of w and h_count[k] += count of the chunk's rows in slab k
+ * (|w - h_boxes[k]| < wslice / 2), summed per chunk in a fixed order.  Synchronises `stream`. */
+int arlmi_wslab_sums(const double* d_uvw, int64_t nrows, const double* h_boxes, double wslice, int32_t nslabs,
+                     double* h_sum, int64_t* h_count, void* stream);
+/* Fix the slab means (h_plane_w[nplanes], e.g. sum / count over all chunks) that the following
+ * arlmi_plan_update calls of a w-stack plan use for the slab-mean w shift and the screens, instead
+ * of the means of the rows each update is given.  n = 0 (or NULL) returns to the rows' own means. */
+int arlmi_plan_fix_plane_w(arlmi_plan* plan, const double* h_plane_w, int32_t n);
+
 /* Host copy of a w-stack plan's slab centres and, after them, the slab width (nplanes + 1
  * values; iterators.py:99-106). */
 int arlmi_plan_slab_boxes(const arlmi_plan* plan, double* h_out, int32_t n);
@@ -152,6 +166,14 @@ int arlmi_convolution_mapping(const arlmi_geom* g, const double* d_uvw, const do
  * (imaging/base.py:162-163).  Both outputs are overwritten. */
 int arlmi_grid(const arlmi_plan* plan, const void* d_vis, const void* d_weight, int dopsf,
                const arlmi_kernel* kernel, void* d_grids, double* d_sumwt, int precision, void* stream);
+
+/* arlmi_grid for the rows of one chunk of a streamed vis set: adds the chunk's gridded
+ * visibilities into d_grids and its sums of weights into d_sumwt instead of overwriting them
+ * (the caller zeroes both before the first chunk), so that the grids of all chunks equal those of
+ * one call over their union up to the order of the additions.  Needs a plan with dense grids (a
+ * plan with ARLMI_FLAG_SPARSE_GRIDS returns ARLMI_E_ARG: each chunk would touch other tiles). */
+int arlmi_grid_add(const arlmi_plan* plan, const void* d_vis, const void* d_weight, int dopsf,
+                   const arlmi_kernel* kernel, void* d_grids, double* d_sumwt, int precision, void* stream);
 
 /* degrid_visibility_from_griddata (gridding.py:233-267): d_vis_out[row] = sum CF * G
  * (no conjugation), reading the plane each row belongs to.  Rows in no plane get 0
