@@ -26,6 +26,7 @@ MODE_2D = 0
 MODE_WSTACK = 1
 FLAG_DROP_OFF_GRID = 1
 FLAG_SPARSE_GRIDS = 2
+FLAG_OCCUPIED_TILES = 4
 KERNEL_SEPARABLE = 0
 KERNEL_TABLE = 1
 

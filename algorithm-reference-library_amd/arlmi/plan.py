@@ -93,12 +93,13 @@ class Plan:
     """
 
     def __init__(self, vis, im, cf, mode=_lib.MODE_2D, nplanes=1, nw_grid=1, wstep_grid=1e15, tile=0,
-                 drop_off_grid=False, slab_boxes=None, use_freq=None, sparse=False, chunk=0):
+                 drop_off_grid=False, slab_boxes=None, use_freq=None, sparse=False, chunk=0, occupied_only=False):
         self.L = _lib.lib()
         self.mode = mode
         self.nplanes = nplanes
         self.geom = make_geom(im, cf, mode, nplanes, nw_grid=nw_grid, wstep_grid=wstep_grid, tile=tile)
-        self.flags = (_lib.FLAG_DROP_OFF_GRID if drop_off_grid else 0) | (_lib.FLAG_SPARSE_GRIDS if sparse else 0)
+        self.flags = ((_lib.FLAG_DROP_OFF_GRID if drop_off_grid else 0) | (_lib.FLAG_SPARSE_GRIDS if sparse else 0) |
+                      (_lib.FLAG_OCCUPIED_TILES if occupied_only else 0))
         self.handle = ctypes.c_void_p()
         self.npixel = self.geom.npixel
         self.nchan = self.geom.nchan

@@ -38,7 +38,8 @@ class StreamingImager:
     centre (no image-centre shift).
     """
 
-    def __init__(self, im, gcfcf=None, mode="wstack", vis_slices=1, precision=32, tile=0, chunk=0, **kwargs):
+    def __init__(self, im, gcfcf=None, mode="wstack", vis_slices=1, precision=32, tile=0, chunk=0, keep_plans=False,
+                 **kwargs):
         if mode not in ("2d", "wstack"):
             raise ValueError("mode must be '2d' or 'wstack'")
         self.gcf, self.cf = _gcfcf(im, gcfcf, kwargs)
@@ -51,6 +52,10 @@ class StreamingImager:
         self.dev = device()
         self.plan = None
         self._planned = None
+        # keep_plans: one plan per chunk, kept between calls (invert rebuilds them for the chunks'
+        # uvw, predict reuses them), at the cost of every chunk's plan buffers
+        self.keep_plans = keep_plans
+        self.plans = {}
         self.grids = None
         self.boxes = None
         self.plane_w = None
@@ -86,56 +91,71 @@ class StreamingImager:
         self.plane_w = numpy.ascontiguousarray(means, dtype=numpy.float64)
         return boxes, width, self.plane_w, counts
 
-    def _plan_for(self, vis):
-        """The plan rebuilt for one chunk with the whole set's slab centres and means."""
+    def _new_plan(self, vis):
+        mode = _lib.MODE_WSTACK if self.mode == "wstack" else _lib.MODE_2D
+        # dense grids; gridding work items only for the tiles a chunk touches (the grids are zeroed
+        # once per pass, and every chunk adds into them)
+        plan = Plan(vis, self.im, self.cf, mode, self.vis_slices, tile=self.tile, slab_boxes=self.boxes,
+                    sparse=False, chunk=self.chunk, occupied_only=True)
+        if self.mode == "wstack":
+            _lib.check(_lib.lib().arlmi_plan_fix_plane_w(
+                plan.handle, self.plane_w.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(self.plane_w)))
+            plan.update(vis, self.boxes)
+        return plan
+
+    def _plan_for(self, vis, index=0, rebuild=True):
+        """The plan of one chunk with the whole set's slab centres and means."""
+        if self.keep_plans:
+            plan = self.plans.get(index)
+            if plan is None:
+                plan = self.plans[index] = self._new_plan(vis)
+            elif rebuild:
+                plan.update(vis, self.boxes)
+            self.plan = plan
+            return plan
         if self.plan is not None and self._planned is vis:
             return self.plan
         self._planned = vis
         if self.plan is None:
-            mode = _lib.MODE_WSTACK if self.mode == "wstack" else _lib.MODE_2D
-            self.plan = Plan(vis, self.im, self.cf, mode, self.vis_slices, tile=self.tile, slab_boxes=self.boxes,
-                             sparse=False, chunk=self.chunk)
-            if self.mode == "wstack":
-                _lib.check(_lib.lib().arlmi_plan_fix_plane_w(
-                    self.plan.handle, self.plane_w.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
-                    len(self.plane_w)))
-                self.plan.update(vis, self.boxes)
+            self.plan = self._new_plan(vis)
         else:
             self.plan.update(vis, self.boxes)
         return self.plan
 
     def _ensure_grids(self, vis):
         if self.grids is None:
-            self._plan_for(vis)
             self.grids = grids_buffer(self.plan, self.precision, self.dev)
             self.sumwt_planes = torch.empty((self.plan.nplanes, self.plan.nchan), dtype=torch.float64,
                                             device=self.dev)
 
     def release(self):
-        """Free the resident grids (the plan keeps its own buffers until it is closed)."""
+        """Free the resident grids and the plans' buffers."""
         self.grids = None
         self._planned = None
+        for p in self.plans.values():
+            p.close()
+        self.plans = {}
         if self.plan is not None:
             self.plan.close()
             self.plan = None
 
     # -- pass 2 ---------------------------------------------------------------------------
-    def invert(self, chunks, dopsf=False, normalize=True):
+    def invert(self, chunks, dopsf=False, normalize=True, rebuild=True):
         """Dirty image (or PSF) of all rows: [nchan, 1, N, N] in the precision, and sumwt
         [nchan, 1] (host).  normalize=False gives the list workflows' normalize=False form."""
         if self.mode == "wstack" and self.boxes is None:
             self.scan(chunks)
         L = _lib.lib()
         first = True
-        for v in _chunks(chunks):
+        for i, v in enumerate(_chunks(chunks)):
             if v.precision != self.precision:
                 raise TypeError("chunk precision differs from the imager's")
+            plan = self._plan_for(v, i, rebuild)
             if first:
                 self._ensure_grids(v)
                 self.grids.zero_()
                 self.sumwt_planes.zero_()
                 first = False
-            plan = self._plan_for(v)
             k, keep = kernel_struct(self.cf, self.precision)
             _lib.check(L.arlmi_grid_add(plan.handle, ptr(v.data["vis"]), ptr(v.data["imaging_weight"]),
                                         int(bool(dopsf)), ctypes.byref(k), ptr(self.grids), ptr(self.sumwt_planes),
@@ -150,32 +170,34 @@ class StreamingImager:
                                           stream_handle()))
         return re, self.sumwt_planes.sum(dim=0).cpu().numpy()[:, None]
 
-    def predict(self, model_data, chunks, sink=None):
+    def predict(self, model_data, chunks, sink=None, out=None, rebuild=None):
         """Model visibilities of every chunk: gcf * conj(screen) * model, shifted FFTs once, then
-        degridding per chunk.  Returns the list of [V, 1] complex tensors, or, with ``sink``, calls
-        sink(index, vis_chunk, predicted) per chunk and returns None."""
+        degridding per chunk.  Returns the list of [V, 1] complex tensors (written into the tensors
+        of ``out`` when given, one per chunk), or, with ``sink``, calls sink(index, vis_chunk,
+        predicted) per chunk and returns None."""
         if self.mode == "wstack" and self.boxes is None:
             self.scan(chunks)
         L = _lib.lib()
         rdt = _real_dtype(self.precision)
         model = model_data.to(device=self.dev, dtype=rdt).contiguous()
-        out = [] if sink is None else None
+        if rebuild is None:  # kept plans were built for these chunks by the last invert
+            rebuild = not self.keep_plans
+        result = [] if sink is None else None
         first = True
         for i, v in enumerate(_chunks(chunks)):
+            plan = self._plan_for(v, i, rebuild)
             if first:
                 self._ensure_grids(v)
-                plan = self.plan
                 _lib.check(L.arlmi_image_to_grids(plan.handle, ptr(model), ptr(self.gcf_dev), ptr(self.grids),
                                                   self.precision, stream_handle()))
                 first = False
-            plan = self._plan_for(v)
-            pred = torch.empty_like(v.vis)
+            pred = out[i] if out is not None else torch.empty_like(v.vis)
             k, keep = kernel_struct(self.cf, self.precision)
             _lib.check(L.arlmi_degrid(plan.handle, ptr(self.grids), ctypes.byref(k), ptr(pred), self.precision,
                                       stream_handle()))
             del keep
             if sink is None:
-                out.append(pred)
+                result.append(pred)
             else:
                 sink(i, v, pred)
-        return out
+        return result

@@ -271,11 +271,13 @@ struct Shape {
   static constexpr size_t tw_lds() {
     return tw_global<R>() ? 0 : (size_t)N;
   }
-  // Two adjacent columns and the twiddles exceed LDS: a column pass would read 16 bytes per grid
-  // row, so the columns are transformed as rows between two in-place transposes instead.
+  // Only one column fits LDS beside the twiddles (complex64 at 8192, complex128 from 4096): a
+  // column pass would read one 8- or 16-byte value per grid row, so the columns are transformed
+  // as rows between two in-place transposes instead (three coalesced passes instead of one
+  // scattered one).
   template <typename R>
   static constexpr bool cols_by_transpose() {
-    return (size_t)N * sizeof(cplx<R>) * (size_t)(col_nb<R>() + 1) > 163840;
+    return col_nb<R>() < 2;
   }
   template <typename R>
   static constexpr bool fits() {

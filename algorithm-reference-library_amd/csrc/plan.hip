@@ -1037,7 +1037,8 @@ static int plan_build(arlmi_plan* p, const double* uvw, const double* freq, int6
 
   // work items
   ARLMI_LAUNCH("k_chunks", k_chunks, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, p->d_cnt, p->d_cntd,
-               (int)nt, p->chunk, (p->flags & ARLMI_FLAG_SPARSE_GRIDS) ? 1 : 0, p->d_nch, p->d_nchd, p->d_nslot);
+               (int)nt, p->chunk, (p->flags & (ARLMI_FLAG_SPARSE_GRIDS | ARLMI_FLAG_OCCUPIED_TILES)) ? 1 : 0,
+               p->d_nch, p->d_nchd, p->d_nslot);
   ARLMI_LAUNCH_CHECK();
   if (p->flags & ARLMI_FLAG_SPARSE_GRIDS) {
     const int npc = (int)(nt / ((int64_t)g.ntx * g.ntx));

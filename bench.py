@@ -67,7 +67,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=1000000)
     ap.add_argument("--workload", choices=("wstack", "wstack-aw", "batch", "gradient", "phaserotate", "clean", "dft", "solve",
-                             "bda", "weighting"),
+                             "bda", "weighting", "stream"),
                     default="wstack",
                     help="wstack: the headline line (default); wstack-aw: the same with an AW w-projection kernel "
                          "for the residual w of each slab; batch: 64 uneven observations replayed from one "
@@ -77,7 +77,8 @@ def parse():
                          "dft: direct DFT of the workload's 200 sources onto its rows; "
                          "solve: scalar gain solution of the 128-antenna differentiation workload; "
                          "bda: baseline-dependent averaging of the w-stack observation as a BlockVisibility; "
-                         "weighting: uniform weighting of the w-stack workload's rows on its 2048^2 grid")
+                         "weighting: uniform weighting of the w-stack workload's rows on its 2048^2 grid; "
+                         "stream: ~593 M rows streamed in chunks into resident 8192^2 grids (128 w slabs)")
     ap.add_argument("--niter", type=int, default=1000, help="Hogbom iterations (clean)")
     ap.add_argument("--support", type=int, default=16, help="AW kernel support (wstack-aw)")
     ap.add_argument("--nobs", type=int, default=64)
@@ -1003,6 +1004,145 @@ def bench_bda(args):
     print(json.dumps(res))
 
 
+def _stream_chunks(precision, chunk_ints):
+    """The streamed wide-band observation as resident row chunks (uvw, vis, weights, frequency)."""
+    import arlmi
+    from arlmi.simulation import config5
+    chunks, total, im = [], 0, None
+    for first in range(0, 480, chunk_ints):
+        v, im = config5(precision=precision, first=first, integrations=min(chunk_ints, 480 - first))
+        total += v.total_rows
+        d = {k: v.data[k] for k in ("uvw", "vis", "imaging_weight", "frequency")}
+        d["weight"] = d["imaging_weight"]
+        chunks.append(arlmi.Visibility(d))
+        del v
+    return chunks, total, im
+
+
+def bench_stream(args):
+    """Precision at scale: ~593 M rows streamed in ~16 M-row chunks into resident 8192^2 grids
+    (128 w slabs, PSWF S=8), in float32 and float64; errors against the float64 run and against
+    the direct DFT of 100 sources."""
+    import numpy
+    import torch
+    import arlmi
+    from arlmi.data_models import Skycomponent
+    n, cell, P, S = 8192, 3e-6, 128, 8
+    rng = numpy.random.default_rng(55)
+    # 100 sources on pixel centres within the inner quarter of each axis, power-law fluxes
+    px = rng.integers(n // 2 - n // 8, n // 2 + n // 8, (100, 2))
+    a = -1.5 + 1.0
+    flux = (1e-3 ** a + rng.uniform(0, 1, 100) * (1.0 ** a - 1e-3 ** a)) ** (1.0 / a)
+    comps = [Skycomponent(-(x - n // 2) * cell, (y - n // 2) * cell, [[f]]) for (x, y), f in zip(px, flux)]
+    model = torch.zeros((1, 1, n, n), dtype=torch.float64)
+    for (x, y), f in zip(px, flux):
+        model[0, 0, y, x] += f
+    res = {}
+    sample = 4  # chunks whose visibilities are kept for the error figures
+    for precision in (32, 64):
+        if precision == 64 and args.no_fp64:
+            break
+        t0 = time.perf_counter()
+        chunks, total, im = _stream_chunks(precision, 13)
+        for v in chunks:
+            v.data["vis"].zero_()
+            arlmi.predict_skycomponent_visibility(v, comps)
+        rows = sum(v.nvis for v in chunks)
+        torch.cuda.synchronize()
+        print("stream: %d-bit, %d chunks, %d rows (%d generated) in %.1f s"
+              % (precision, len(chunks), rows, total, time.perf_counter() - t0), file=sys.stderr, flush=True)
+        gcf, cf = arlmi.create_pswf_convolutionfunction(im, oversampling=8, support=S)
+        st = arlmi.StreamingImager(im, gcfcf=(gcf, cf), mode="wstack", vis_slices=P, precision=precision,
+                                   keep_plans=True)
+        st.scan(chunks)
+        mdl = model.to(device="cuda", dtype=torch.float32 if precision == 32 else torch.float64)
+        outs = [torch.empty_like(v.vis) for v in chunks]
+
+        def one_step():
+            img, sw = st.invert(chunks)
+            st.predict(mdl, chunks, out=outs)
+            return img
+
+        steps = args.steps if precision == 32 else max(1, args.steps // 3)
+        for _ in range(max(1, args.warmup if precision == 32 else 1)):
+            img = one_step()
+        torch.cuda.synchronize()
+        L = arlmi._lib.lib()
+        L.arlmi_profile_reset()
+        L.arlmi_profile_only(("k_grid_sep,k_degrid_sep,k_cols_plain,k_rows_invert,k_rows_predict,k_rows_plain,"
+                              "k_transpose,k_p1_scatter,k_p2_scatter").encode())
+        L.arlmi_profile_enable(1)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            img = one_step()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        L.arlmi_profile_enable(0)
+        L.arlmi_profile_only(None)
+        prof = arlmi._lib.profile_report()
+        dft = [v.vis.to(torch.complex128).cpu().numpy() for v in chunks[:sample]]
+        pv = [o.to(torch.complex128).cpu().numpy() for o in outs[:sample]]
+        res[precision] = {"ms": (t2 - t1) * 1e3 / steps, "rows": rows, "total": total, "img": img.double().cpu(),
+                          "pv": pv, "dft": dft, "prof": prof, "steps": steps}
+        print("stream: %d-bit step %.1f ms (%d steps)" % (precision, res[precision]["ms"], steps), file=sys.stderr,
+              flush=True)
+        st.release()
+        del st, chunks, outs, mdl, img
+        torch.cuda.empty_cache()
+    r32 = res[32]
+    errs = {}
+    if 64 in res:
+        a, b = r32["img"].numpy(), res[64]["img"].numpy()
+        q = n // 4
+        errs["rel_image_err_fp32_vs_fp64"] = {
+            "l2": float(numpy.linalg.norm(a - b) / numpy.linalg.norm(b)),
+            "max_inner_half": float(numpy.abs(a - b)[..., q:n - q, q:n - q].max() / numpy.abs(b[..., q:n - q, q:n - q]).max()),
+            "max_full": float(numpy.abs(a - b).max() / numpy.abs(b).max())}
+        pa, pb = numpy.concatenate(r32["pv"]), numpy.concatenate(res[64]["pv"])
+        errs["rel_vis_err_fp32_vs_fp64"] = {"l2": float(numpy.linalg.norm(pa - pb) / numpy.linalg.norm(pb)),
+                                           "max": float(numpy.abs(pa - pb).max() / numpy.abs(pb).max())}
+    for prec in res:
+        pa, d = numpy.concatenate(res[prec]["pv"]), numpy.concatenate(res[prec]["dft"])
+        errs["rel_vis_err_vs_dft_fp%d" % prec] = {"l2": float(numpy.linalg.norm(pa - d) / numpy.linalg.norm(d)),
+                                                  "max": float(numpy.abs(pa - d).max() / numpy.abs(d).max())}
+    kernels = {k: {"ms_per_step": round(v[0] / r32["steps"], 3), "launches_per_step": round(v[1] / r32["steps"], 1)}
+               for k, v in sorted(r32["prof"].items(), key=lambda kv: -kv[1][0])}
+    bw = 8.0e12
+    planes = 108.0 * n * n * P
+    t_roof = (planes + 68.0 * r32["rows"]) / bw
+    out = {
+        "metric": "Mvis/s invert+predict, ~593 M rows streamed into resident 8192² grids (128 w slabs)",
+        "value": round(r32["rows"] / (r32["ms"] * 1e-3) / 1e6, 1),
+        "unit": "Mvis/s",
+        "n_gpus": 1,
+        "steps": r32["steps"],
+        "warmup": args.warmup,
+        "ms_per_step": round(r32["ms"], 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "197 dishes (core + 3 spiral arms to 80 km), 64 channels 0.95-1.76 GHz, 480 integrations, "
+                        "chunks of 13 integrations; 8192^2 at 3e-6 rad, 128 w slabs, PSWF S=8 os=8; 100 power-law "
+                        "sources on pixel centres (inner quarter), visibilities from the direct DFT",
+            "rows": r32["rows"], "rows_generated_before_uv_selection": r32["total"],
+            "chunks": len(range(0, 480, 13)),
+            "fp64_ms_per_step": round(res[64]["ms"], 1) if 64 in res else None,
+            "fp64_mvis_s": round(r32["rows"] / (res[64]["ms"] * 1e-3) / 1e6, 1) if 64 in res else None,
+            "fp32_over_fp64": round(res[64]["ms"] / r32["ms"], 2) if 64 in res else None,
+            "errors": errs,
+            "error_sample": "images over the full 8192^2 field; visibilities of the first %d chunks" % sample,
+            "step_roofline": {"t_roof_ms": round(t_roof * 1e3, 1), "frac": round(t_roof * 1e3 / r32["ms"], 3),
+                              "note": "dense planes 108 N^2 B per plane + 68 B per row at 8 TB/s"},
+            "kernels_fp32": kernels,
+        },
+    }
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.copy_probe:
@@ -1030,6 +1170,11 @@ def main():
         import torch
         torch.cuda.set_device(0)
         bench_batch(args)
+        return
+    if args.workload == "stream":
+        import torch
+        torch.cuda.set_device(0)
+        bench_stream(args)
         return
     if args.workload == "wstack-aw":
         global METRIC

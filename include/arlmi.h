@@ -56,6 +56,10 @@ extern "C" {
  * of such a plan are meant to pass between these calls, not to be read by the caller.  Plans
  * with more than 32 tiles per side ignore the flag (dense grids). */
 #define ARLMI_FLAG_SPARSE_GRIDS 2u
+/* Dense grids whose untouched tiles arlmi_grid / arlmi_grid_add leave as they are (no counterpart
+ * in ARL): the gridding work items cover only tiles that rows touch.  For chunks added into grids
+ * the caller zeroed once (arlmi_grid_add); the FFT-side calls read and write the grids densely. */
+#define ARLMI_FLAG_OCCUPIED_TILES 4u
 
 #define ARLMI_KERNEL_SEPARABLE 0 /* real separable table k[os][S]: CF[yo][xo][a][b] = k[yo][a]*k[xo][b] */
 #define ARLMI_KERNEL_TABLE 1     /* full complex table CF[nz][os][os][S][S] (ARL layout, chan 0, pol 0) */

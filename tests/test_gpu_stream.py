@@ -38,7 +38,8 @@ def test_stream_equals_one_plan(gpu, precision):
     whole = arlmi.Imager(vis, im, gcfcf=(gcf, cf), mode="wstack", vis_slices=P, sparse=False)
     ref_img = whole.invert(vis)[0].cpu().numpy()
     ref_sw = whole.sumwt()
-    st = arlmi.StreamingImager(im, gcfcf=(gcf, cf), mode="wstack", vis_slices=P, precision=precision)
+    st = arlmi.StreamingImager(im, gcfcf=(gcf, cf), mode="wstack", vis_slices=P, precision=precision,
+                               keep_plans=(precision == 32))  # both plan-holding modes
     boxes, width, means, counts = st.scan(chunks)
     wb, ww = whole.plan.slab_boxes()
     assert numpy.array_equal(boxes, wb) and width == ww
