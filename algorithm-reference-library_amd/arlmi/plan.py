@@ -82,6 +82,19 @@ def wslice_boxes(wmaxabs, vis_slices):
     return numpy.ascontiguousarray(boxes, dtype=numpy.float64), float(wstack)
 
 
+def table_tile(npixel, support, precision):
+    """Tile side for a full kernel table: the library default (0, i.e. 64) unless the table
+    degridder's tile plus an S-wide halo (2 (T + S)^2 real values) would exceed 160 KiB of LDS, as
+    for support 64 in double precision; then the largest tile that fits."""
+    rb = 4 if precision == 32 else 8
+    t = 64 if npixel >= 64 else npixel
+    if 2 * rb * (t + support) ** 2 <= 160 * 1024:
+        return 0
+    while t > 16 and 2 * rb * (t + support) ** 2 > 160 * 1024:
+        t //= 2
+    return t
+
+
 class Plan:
     """Owns an arlmi_plan handle.
 
@@ -97,6 +110,8 @@ class Plan:
         self.L = _lib.lib()
         self.mode = mode
         self.nplanes = nplanes
+        if not tile and cf is not None and getattr(cf, "separable", None) is None and "vis" in vis.data:
+            tile = table_tile(im.data.shape[-1], cf.support, 32 if vis.data["vis"].dtype == torch.complex64 else 64)
         self.geom = make_geom(im, cf, mode, nplanes, nw_grid=nw_grid, wstep_grid=wstep_grid, tile=tile)
         self.flags = ((_lib.FLAG_DROP_OFF_GRID if drop_off_grid else 0) | (_lib.FLAG_SPARSE_GRIDS if sparse else 0) |
                       (_lib.FLAG_OCCUPIED_TILES if occupied_only else 0))

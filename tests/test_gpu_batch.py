@@ -9,11 +9,11 @@ from helpers import rel_err, image_errors
 pytestmark = pytest.mark.gpu
 
 
-def _tasks(nobs=3, seed=31):
+def _tasks(nobs=3, seed=31, max_rows=150_000):
     import torch
     import arlmi
     from arlmi.simulation import config3
-    obs = config3(nobs=nobs, seed=seed, max_rows=150_000, min_rows=10_000)
+    obs = config3(nobs=nobs, seed=seed, max_rows=max_rows, min_rows=10_000)
     rng = numpy.random.default_rng(seed)
     tasks = []
     for o in obs:
@@ -79,3 +79,42 @@ def test_autotuned_batch_matches_eager(gpu, tmp_path):
     # the table persists and is reused
     again = arlmi.Autotuner(path=str(tmp_path / "tune.json"))
     assert again.table == tuner.table
+
+
+def test_config3_full_batch(gpu):
+    """The whole batching workload (64 observations of 10k-4M rows, N from 256 to 2048, S in
+    {6, 8, 10}, 8-64 slabs, 102.8 M rows): every observation of the captured graph against its
+    eager run, and the two cheapest observations against the oracle."""
+    import torch
+    import arlmi
+    from oracle import arl_oracle as O
+    tasks = _tasks(nobs=64, seed=3, max_rows=4_000_000)
+    assert sum(t["vis"].nvis for t in tasks) > 90_000_000
+    batch = arlmi.BatchImager(tasks, graph=True)
+    batch.run()
+    torch.cuda.synchronize()
+    for i, t in enumerate(tasks):
+        imager = arlmi.Imager(t["vis"], t["im"], gcfcf=t["gcfcf"], mode="wstack", vis_slices=t["vis_slices"])
+        re, _ = imager.invert(t["vis"])
+        pv = imager.predict(t["vis"], t["model"])
+        l2, inner, full = image_errors(batch.images[i].cpu().numpy(), re.cpu().numpy())
+        assert inner < 1e-5 and full < 5e-3, (i, l2, inner, full)
+        assert rel_err(batch.predicted[i].cpu().numpy(), pv.cpu().numpy()) < 1e-5, i
+        del imager, re, pv
+    cost = [t["im"].data.shape[-1] ** 2 * t["vis_slices"] for t in tasks]
+    for i in numpy.argsort(cost)[:2]:
+        t = tasks[i]
+        n = t["im"].data.shape[-1]
+        w = t["im"].wcs.wcs
+        d = {"uvw": t["vis"].data["uvw"].cpu().numpy(),
+             "vis": t["vis"].vis.cpu().numpy().astype(complex),
+             "imaging_weight": t["vis"].data["imaging_weight"].cpu().numpy().astype(float),
+             "frequency": t["vis"].data["frequency"].cpu().numpy()}
+        g = O.ImageGeom(n, float(t["im"].cellsize), frequency=float(w.crval[3]), channel_bandwidth=float(w.cdelt[3]))
+        ogcf, ocf, cfg = O.create_pswf_convolutionfunction(g, 8, t["gcfcf"][1].support)
+        rimg, _ = O.invert_list_wstack(d, g, ogcf, ocf, cfg, t["vis_slices"])
+        l2, inner, full = image_errors(batch.images[i].cpu().numpy(), rimg)
+        assert inner < 1e-5 and full < 5e-3, (i, l2, inner, full)
+        model = t["model"].double().cpu().numpy()
+        rpv = O.predict_list_wstack(d, model, g, ogcf, ocf, cfg, t["vis_slices"])
+        assert rel_err(batch.predicted[i].cpu().numpy(), rpv) < 1e-5, i

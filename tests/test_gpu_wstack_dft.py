@@ -113,3 +113,53 @@ def test_list_invert_normalize_false(gpu, precision, context, P):
         assert inner < (1e-10 if precision == 64 else 1e-5) and l2 < (1e-10 if precision == 64 else 1e-3), \
             (normalize, l2, inner, full)
         assert abs(sw[0, 0] - rsw[0, 0]) < 1e-9 * rsw[0, 0]
+
+
+def _wprojection_residual(d, precision, gcfcf):
+    """2-D (w-projection) predict subtracted from the DFT rows, then inverted
+    (test_imaging_serial.py:153-170 with context '2d' and an AW gcfcf)."""
+    import arlmi
+    vis = to_device_vis(d, precision)
+    mim = device_image(N, CELL, precision, data=model_image())
+    pv = arlmi.predict_2d(vis, mim, gcfcf=gcfcf)
+    res = dict(d)
+    res["vis"] = d["vis"] - pv.vis.cpu().numpy()
+    im = device_image(N, CELL, precision)
+    dirty, _ = arlmi.invert_2d(to_device_vis(res, precision), im, gcfcf=gcfcf)
+    return dirty.numpy(), pv.vis.cpu().numpy()
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_wprojection_arl_sizes(gpu, obs, precision):
+    """ARL's own w-projection settings (test_imaging_serial.py:121-131, 222-230): nw=61 kernel
+    planes 16 wavelengths apart, support 64, oversampling 8, and the same kernel clipped at 1e-3
+    by apply_bounding_box_convolutionfunction (convolution_functions.py:159-184); ARL's
+    threshold 1.0 on 100 Jy for both.  A 2,000-row subset against the oracle with the same
+    kernels."""
+    import arlmi
+    im = device_image(N, CELL, precision)
+    gcf, cf = arlmi.create_awterm_convolutionfunction(im, nw=61, wstep=16.0, oversampling=8, support=64,
+                                                      use_aaf=True)
+    clipped = arlmi.apply_bounding_box_convolutionfunction(cf, fractional_level=1e-3)
+    assert clipped.support < 64
+    g = O.ImageGeom(N, CELL)
+    ogcf, ocf, cfg = O.create_awterm_convolutionfunction(g, nw=61, wstep=16.0, oversampling=8, support=64,
+                                                         use_aaf=True)
+    ocl = O.apply_bounding_box_convolutionfunction(ocf, 1e-3)
+    sub = {k: v[:2000] for k, v in obs.items()}
+    model = numpy.zeros((1, 1, N, N))
+    model[0, 0, PY, PX] = FLUX
+    for cfx, ocfx in ((cf, ocf), (clipped, ocl)):
+        dirty, _ = _wprojection_residual(obs, precision, (gcf, cfx))
+        assert numpy.max(numpy.abs(dirty)) < 1.0, (cfx.support, numpy.max(numpy.abs(dirty)))
+        S = ocfx.shape[-1]
+        ocfg = O.CFGeom(g, 8, S, nz=61, zstep=16.0)
+        vis = to_device_vis(sub, precision)
+        mim = device_image(N, CELL, precision, data=model)
+        pv = arlmi.predict_2d(vis, mim, gcfcf=(gcf, cfx)).vis.cpu().numpy()
+        rpv = O.predict_2d(sub, model, g, ogcf, ocfx, ocfg)
+        assert rel_err(pv, rpv) < (1e-10 if precision == 64 else 1e-5), S
+        img, _ = arlmi.invert_2d(vis, im, gcfcf=(gcf, cfx))
+        rimg, _ = O.invert_2d(sub, g, ogcf, ocfx, ocfg)
+        l2, inner, full = image_errors(img.numpy(), rimg)
+        assert inner < (1e-10 if precision == 64 else 1e-5), (S, l2, inner, full)
