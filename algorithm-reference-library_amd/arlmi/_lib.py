@@ -34,7 +34,8 @@ KERNEL_TABLE = 1
 EXPORTS = (
     "arlmi_version", "arlmi_last_error", "arlmi_wmax", "arlmi_plan_create", "arlmi_plan_update",
     "arlmi_plan_set_chunk", "arlmi_plan_stats_get", "arlmi_plan_plane_w", "arlmi_plan_slab_boxes", "arlmi_plan_destroy", "arlmi_convolution_mapping",
-    "arlmi_grid", "arlmi_grid_add", "arlmi_degrid", "arlmi_wslab_sums", "arlmi_plan_fix_plane_w",
+    "arlmi_grid", "arlmi_grid_add", "arlmi_degrid", "arlmi_batch_create", "arlmi_batch_grid", "arlmi_batch_degrid",
+    "arlmi_batch_destroy", "arlmi_wslab_sums", "arlmi_plan_fix_plane_w",
     "arlmi_grids_to_image", "arlmi_image_to_grids", "arlmi_fft2d", "arlmi_w_screen", "arlmi_dft",
     "arlmi_grid_weights", "arlmi_reweight", "arlmi_block_to_rows", "arlmi_apply_gaintable", "arlmi_apply_gains_rows", "arlmi_apply_gains_rows_adjoint",
     "arlmi_phaserotate_rows", "arlmi_phaserotate_blocks", "arlmi_hogbom", "arlmi_hogbom_noalloc", "arlmi_hogbom_traffic", "arlmi_solve_gains",
@@ -81,6 +82,13 @@ class Kernel(ctypes.Structure):
     _fields_ = [
         ("kind", ctypes.c_int32), ("nz", ctypes.c_int32), ("oversampling", ctypes.c_int32),
         ("support", ctypes.c_int32), ("d_table", ctypes.c_void_p),
+    ]
+
+
+class BatchItem(ctypes.Structure):
+    _fields_ = [
+        ("plan", ctypes.c_void_p), ("d_vis", ctypes.c_void_p), ("d_weight", ctypes.c_void_p), ("kernel", Kernel),
+        ("d_grids", ctypes.c_void_p), ("d_sumwt", ctypes.c_void_p), ("d_vis_out", ctypes.c_void_p),
     ]
 
 
@@ -131,6 +139,11 @@ def lib():
         "arlmi_plan_destroy": (None, [vp]),
         "arlmi_convolution_mapping": (ctypes.c_int, [ctypes.POINTER(Geom), vp, vp, i64, vp, vp]),
         "arlmi_grid": (ctypes.c_int, [vp, vp, vp, ctypes.c_int, ctypes.POINTER(Kernel), vp, vp, ctypes.c_int, vp]),
+        "arlmi_batch_create": (ctypes.c_int, [ctypes.POINTER(BatchItem), i32, ctypes.c_int,
+                                              ctypes.POINTER(ctypes.c_void_p)]),
+        "arlmi_batch_grid": (ctypes.c_int, [vp, ctypes.c_int, vp]),
+        "arlmi_batch_degrid": (ctypes.c_int, [vp, vp]),
+        "arlmi_batch_destroy": (None, [vp]),
         "arlmi_grid_add": (ctypes.c_int, [vp, vp, vp, ctypes.c_int, ctypes.POINTER(Kernel), vp, vp, ctypes.c_int,
                                           vp]),
         "arlmi_wslab_sums": (ctypes.c_int, [vp, i64, ctypes.POINTER(dbl), dbl, i32, ctypes.POINTER(dbl),

@@ -10,6 +10,7 @@ calls on a plan allocate nothing, record no memset and never synchronise after t
 call, which is what makes the capture valid; the first, eager call is the warm-up below.
 """
 
+import ctypes
 import json
 import math
 import os
@@ -17,8 +18,10 @@ import time
 
 import torch
 
-from .imaging import Imager
+from . import _lib
+from .imaging import Imager, _shift_lmn
 from .data_models import device
+from .plan import kernel_struct, stream_handle
 
 __all__ = ["Autotuner", "BatchImager"]
 
@@ -96,7 +99,7 @@ class BatchImager:
     its sum of weights.
     """
 
-    def __init__(self, tasks, autotuner=None, graph=True):
+    def __init__(self, tasks, autotuner=None, graph=True, persistent=False):
         self.tasks = tasks
         self.imagers = []
         self.images = []
@@ -116,8 +119,13 @@ class BatchImager:
             self.images.append(torch.empty((imager.plan.nchan, 1, n, n), dtype=t["model"].dtype,
                                            device=t["model"].device))
             self.predicted.append(torch.empty_like(t["vis"].vis))
+        self.batch = None
         self._eager()  # warm-up: sizes every buffer and caches every kernel table
         torch.cuda.synchronize()
+        # persistent: the gridding (and degridding) of every observation of one kernel support in
+        # one launch (arlmi_batch_grid / arlmi_batch_degrid); the transforms stay per observation
+        if persistent:
+            self._make_batch()
         self.graph = None
         if graph:
             self.graph = torch.cuda.CUDAGraph()
@@ -125,10 +133,53 @@ class BatchImager:
                 self._eager()
             torch.cuda.synchronize()
 
+    def _make_batch(self):
+        for t in self.tasks:
+            if t.get("dopsf", False) != self.tasks[0].get("dopsf", False):
+                raise ValueError("a persistent batch grids every observation with the same dopsf")
+        items = (_lib.BatchItem * len(self.tasks))()
+        self._keep = []
+        precision = None
+        for i, (t, imager, pv) in enumerate(zip(self.tasks, self.imagers, self.predicted)):
+            if _shift_lmn(t["vis"], imager.im) is not None:
+                raise ValueError("a persistent batch does not shift visibilities to an image centre")
+            precision = imager.precision
+            k, keep = kernel_struct(imager.cf, imager.precision)
+            self._keep.append(keep)
+            it = items[i]
+            it.plan = imager.plan.handle.value
+            it.d_vis = t["vis"].vis.data_ptr()
+            it.d_weight = t["vis"].data["imaging_weight"].data_ptr()
+            it.kernel = k
+            it.d_grids = imager.grids.data_ptr()
+            it.d_sumwt = imager.sumwt_planes.data_ptr()
+            it.d_vis_out = pv.data_ptr()
+        handle = ctypes.c_void_p()
+        _lib.check(_lib.lib().arlmi_batch_create(items, len(self.tasks), precision, ctypes.byref(handle)))
+        self.batch = handle
+
     def _eager(self):
+        if self.batch is not None:
+            L = _lib.lib()
+            dopsf = int(bool(self.tasks[0].get("dopsf", False)))
+            _lib.check(L.arlmi_batch_grid(self.batch, dopsf, stream_handle()))
+            for t, imager, img in zip(self.tasks, self.imagers, self.images):
+                imager.grids_to_image(normalize=t.get("normalize", True), out=img)
+            for t, imager in zip(self.tasks, self.imagers):
+                imager.image_to_grids(t["model"])
+            _lib.check(L.arlmi_batch_degrid(self.batch, stream_handle()))
+            return
         for t, imager, img, pv in zip(self.tasks, self.imagers, self.images, self.predicted):
             imager.invert(t["vis"], dopsf=t.get("dopsf", False), normalize=t.get("normalize", True), out=img)
             imager.predict(t["vis"], t["model"], out=pv)
+
+    def __del__(self):
+        b = getattr(self, "batch", None)
+        if b:
+            try:
+                _lib.lib().arlmi_batch_destroy(b)
+            except Exception:
+                pass
 
     def run(self):
         """Invert and predict every observation (one graph launch, or the eager sequence)."""

@@ -168,6 +168,12 @@ class Imager:
             _rotate_rows(self._shift_uvw(vis), rot, lmn, False, self.precision)
             vis, dopsf = vis.with_vis(rot), False
         grid_planes(self.plan, vis, self.cf, self.grids, self.sumwt_planes, dopsf=dopsf)
+        return self.grids_to_image(normalize=normalize, imaginary=imaginary, out=out, weigh_slabs=weigh_slabs)
+
+    def grids_to_image(self, normalize=True, imaginary=False, out=None, weigh_slabs=False):
+        """The image of the gridded planes (the second half of invert): shifted inverse FFTs,
+        screens, plane sum, gcf and the sum-of-weights normalisation.  The grids are used as
+        scratch."""
         n, nchan = self.plan.npixel, self.plan.nchan
         rdt = _real_dtype(self.precision)
         re = out if out is not None else torch.empty((nchan, 1, n, n), dtype=rdt, device=self.dev)
@@ -177,6 +183,17 @@ class Imager:
                                                    ptr(re), ptr(imo),
                                                    self.precision, stream_handle()))
         return re, imo
+
+    def image_to_grids(self, model_data):
+        """The planes predict degrids (the first half of predict): gcf * conj(screen) * model,
+        shifted FFTs.  Returns the grids written."""
+        rdt = _real_dtype(self.precision)
+        model = model_data.to(device=self.dev, dtype=rdt).contiguous()
+        grids = getattr(self, "_pgrids", None)
+        grids = self.grids if grids is None else grids
+        _lib.check(_lib.lib().arlmi_image_to_grids(self.plan.handle, ptr(model), ptr(self.gcf_dev), ptr(grids),
+                                                   self.precision, stream_handle()))
+        return grids
 
     def adjoint(self, vis, vis_data, out=None):
         """Adjoint of predict for a real image: Re(sum over planes of screen * gcf * F^H grid(v)),
@@ -207,12 +224,7 @@ class Imager:
 
     def predict(self, vis, model_data, out=None):
         """Model visibilities: gcf * conj(screen) * model, shifted FFT, degridding."""
-        rdt = _real_dtype(self.precision)
-        model = model_data.to(device=self.dev, dtype=rdt).contiguous()
-        grids = getattr(self, "_pgrids", None)
-        grids = self.grids if grids is None else grids
-        _lib.check(_lib.lib().arlmi_image_to_grids(self.plan.handle, ptr(model), ptr(self.gcf_dev), ptr(grids),
-                                                   self.precision, stream_handle()))
+        grids = self.image_to_grids(model_data)
         vis_out = out if out is not None else torch.empty_like(vis.vis)
         degrid_planes(self.plan, grids, self.cf, vis_out, self.precision)
         lmn = _shift_lmn(vis, self.im)

@@ -220,23 +220,19 @@ __device__ __forceinline__ void sep_add(double* acc, int T, int TW, int lane, ui
   }
 }
 
+// One work item of the separable gridder (the body of k_grid_sep and of its batched form).
 template <typename R, int S>
-__global__ __launch_bounds__(GRID_THREADS) void k_grid_sep(const int4* __restrict__ items,
-                                                           const uint2* __restrict__ entries,
-                                                           const cplx<R>* __restrict__ vis, const R* __restrict__ wt,
-                                                           int dopsf, const R* __restrict__ ktab, int os,
-                                                           cplx<R>* __restrict__ grids, cplx<R>* __restrict__ partial,
-                                                           double* __restrict__ sumwt, int T, int ntx, int N,
-                                                           int accum) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ double red[GRID_THREADS / 64];
+__device__ __forceinline__ void grid_sep_item(const int4 it, unsigned char* smem, double* red,
+                                              const uint2* __restrict__ entries, const cplx<R>* __restrict__ vis,
+                                              const R* __restrict__ wt, int dopsf, const R* __restrict__ ktab, int os,
+                                              cplx<R>* __restrict__ grids, cplx<R>* __restrict__ partial,
+                                              double* __restrict__ sumwt, int T, int ntx, int N, int accum) {
   constexpr int TAPS = S * S;
   constexpr int TPL = (TAPS + 63) / 64;  // taps per lane
   const int WS = lds_stride(T);
   const int TW = T * WS;
   double* acc = reinterpret_cast<double*>(smem);  // [2][TW + 64]: real, imaginary (tile + trash row)
   R* lk = reinterpret_cast<R*>(acc + 2 * (TW + 64));
-  const int4 it = items[blockIdx.x];
   if (accum && it.z == 0) return;  // an empty tile adds nothing
   for (int i = threadIdx.x; i < 2 * (TW + 64); i += blockDim.x) acc[i] = 0.0;
   for (int i = threadIdx.x; i < os * S; i += blockDim.x) lk[i] = ktab[i];
@@ -305,6 +301,66 @@ __global__ __launch_bounds__(GRID_THREADS) void k_grid_sep(const int4* __restric
   const TilePos tp = tile_pos(it.x, ntx);
   store_tile<R>(acc, WS, T, N, tp, it.w, grids, partial, TW + 64, accum);
   block_add_double(swt, &sumwt[tp.plane], red);
+}
+
+template <typename R, int S>
+__global__ __launch_bounds__(GRID_THREADS) void k_grid_sep(const int4* __restrict__ items,
+                                                           const uint2* __restrict__ entries,
+                                                           const cplx<R>* __restrict__ vis, const R* __restrict__ wt,
+                                                           int dopsf, const R* __restrict__ ktab, int os,
+                                                           cplx<R>* __restrict__ grids, cplx<R>* __restrict__ partial,
+                                                           double* __restrict__ sumwt, int T, int ntx, int N,
+                                                           int accum) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double red[GRID_THREADS / 64];
+  grid_sep_item<R, S>(items[blockIdx.x], smem, red, entries, vis, wt, dopsf, ktab, os, grids, partial, sumwt, T, ntx,
+                      N, accum);
+}
+
+// Per-plan arguments of the batched separable kernels: the plan's work items are blocks
+// [base, base + nitems) of the batched launch.
+struct BatchDesc {
+  const int4* gitems;
+  const int4* ditems;
+  const uint2* entries;
+  const void* vis;
+  const void* wt;
+  const void* ktab;
+  void* grids;
+  void* partial;
+  double* sumwt;
+  void* vis_out;
+  const int4* multi;       // split tiles
+  const uint32_t* nmulti;  // their number (device)
+  const uint32_t* orphans; // records of rows in no plane (4 words each), or null
+  int64_t norphans;
+  int os, T, ntx, N, nsw;
+  int gbase, ngitems, dbase, nditems;
+};
+
+// The descriptor whose [base, base + count) block range holds global block b (descriptors in
+// block order; a scalar search over at most a few hundred plans).
+__device__ __forceinline__ int find_desc(const BatchDesc* __restrict__ d, int nd, int b, bool degrid) {
+  int lo = 0, hi = nd - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if ((degrid ? d[mid].dbase : d[mid].gbase) <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Every plan's separable gridding in one launch: block b runs work item b - gbase of the plan
+// that holds it.
+template <typename R, int S>
+__global__ __launch_bounds__(GRID_THREADS) void k_grid_sep_batch(const BatchDesc* __restrict__ descs, int nd,
+                                                                 int dopsf) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double red[GRID_THREADS / 64];
+  const int b = blockIdx.x;
+  const BatchDesc& d = descs[find_desc(descs, nd, b, false)];
+  grid_sep_item<R, S>(d.gitems[b - d.gbase], smem, red, d.entries, (const cplx<R>*)d.vis, (const R*)d.wt, dopsf,
+                      (const R*)d.ktab, d.os, (cplx<R>*)d.grids, (cplx<R>*)d.partial, d.sumwt, d.T, d.ntx, d.N, 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -492,15 +548,14 @@ constexpr int degrid_threads() {
   return sizeof(R) == 4 ? 512 : 1024;
 }
 
+// One work item of the separable degridder (the body of k_degrid_sep and of its batched form).
 template <typename R, int S>
-__global__ __launch_bounds__(degrid_threads<R>()) void k_degrid_sep(const int4* __restrict__ items, const uint2* __restrict__ entries,
-                                                    const cplx<R>* __restrict__ grids, const R* __restrict__ ktab,
-                                                    int os, cplx<R>* __restrict__ vis_out, int T, int ntx, int N) {
-  extern __shared__ __align__(16) unsigned char smem[];
+__device__ __forceinline__ void degrid_sep_item(const int4 it, unsigned char* smem, const uint2* __restrict__ entries,
+                                                const cplx<R>* __restrict__ grids, const R* __restrict__ ktab,
+                                                int os, cplx<R>* __restrict__ vis_out, int T, int ntx, int N) {
   const int Wd = T + S;
   R* lds = reinterpret_cast<R*>(smem);
   R* lk = lds + 2 * Wd * Wd;
-  const int4 it = items[blockIdx.x];
   const TilePos tp = tile_pos(it.x, ntx);
   load_halo<R>(grids, lds, Wd, T, N, tp);
   for (int i = threadIdx.x; i < os * S; i += blockDim.x) lk[i] = ktab[i];
@@ -529,6 +584,23 @@ __global__ __launch_bounds__(degrid_threads<R>()) void k_degrid_sep(const int4* 
     }
     vis_out[en.x] = mk<R>(ar, ai);
   }
+}
+
+template <typename R, int S>
+__global__ __launch_bounds__(degrid_threads<R>()) void k_degrid_sep(const int4* __restrict__ items, const uint2* __restrict__ entries,
+                                                    const cplx<R>* __restrict__ grids, const R* __restrict__ ktab,
+                                                    int os, cplx<R>* __restrict__ vis_out, int T, int ntx, int N) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  degrid_sep_item<R, S>(items[blockIdx.x], smem, entries, grids, ktab, os, vis_out, T, ntx, N);
+}
+
+template <typename R, int S>
+__global__ __launch_bounds__(degrid_threads<R>()) void k_degrid_sep_batch(const BatchDesc* __restrict__ descs, int nd) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int b = blockIdx.x;
+  const BatchDesc& d = descs[find_desc(descs, nd, b, true)];
+  degrid_sep_item<R, S>(d.ditems[b - d.dbase], smem, d.entries, (const cplx<R>*)d.grids, (const R*)d.ktab, d.os,
+                        (cplx<R>*)d.vis_out, d.T, d.ntx, d.N);
 }
 
 // DPP moves (0 where a row is masked off) of a 32-bit or 64-bit value within the wavefront.
@@ -631,6 +703,49 @@ __global__ void k_zero_rows(const uint32_t* __restrict__ rows, int stride, int64
 __global__ void k_zero_f64(double* __restrict__ p, int64_t n) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0.0;
+}
+
+// Batched forms of the small per-plan steps around the batched gridder and degridder.
+// Sums of weights of every plan set to 0 (block d: plan d).
+__global__ void k_batch_zero_sumwt(const BatchDesc* __restrict__ descs) {
+  const BatchDesc& d = descs[blockIdx.x];
+  for (int i = threadIdx.x; i < d.nsw; i += blockDim.x) d.sumwt[i] = 0.0;
+}
+
+// Split tiles of every plan summed in slot order, as k_reduce_tiles: block (d * G + g, y) takes
+// split tiles g, g + G, ... of plan d, pixels y * 256 + threadIdx.x.
+template <typename R>
+__global__ void k_batch_reduce_tiles(const BatchDesc* __restrict__ descs, int G) {
+  const BatchDesc& d = descs[blockIdx.x / G];
+  const int gi = blockIdx.x % G;
+  const int T = d.T;
+  const int i = blockIdx.y * blockDim.x + threadIdx.x;
+  if (i >= T * T || !d.multi) return;
+  const uint32_t count = *d.nmulti;
+  const size_t tt = (size_t)T * T;
+  const cplx<R>* partial = (const cplx<R>*)d.partial;
+  cplx<R>* grids = (cplx<R>*)d.grids;
+  for (uint32_t mi = gi; mi < count; mi += G) {
+    const int4 m = d.multi[mi];
+    const TilePos tp = tile_pos(m.x, d.ntx);
+    const cplx<R>* src = partial + (size_t)m.y * tt + i;
+    R re = R(0), im = R(0);
+    for (int k = 0; k < m.z; ++k) {
+      const cplx<R> a = src[(size_t)k * tt];
+      re += a.x;
+      im += a.y;
+    }
+    grids[(size_t)tp.plane * d.N * d.N + (size_t)(tp.ty * T + i / T) * d.N + (size_t)tp.tx * T + (i % T)] =
+        mk<R>(re, im);
+  }
+}
+
+// Rows in no plane of every plan predicted as 0 (block d: plan d).
+template <typename R>
+__global__ void k_batch_zero_rows(const BatchDesc* __restrict__ descs) {
+  const BatchDesc& d = descs[blockIdx.x];
+  cplx<R>* vis = (cplx<R>*)d.vis_out;
+  for (int64_t i = threadIdx.x; i < d.norphans; i += blockDim.x) vis[d.orphans[4 * i]] = mk<R>(R(0), R(0));
 }
 
 // ---------------------------------------------------------------------------
@@ -850,6 +965,240 @@ int arlmi_grid(const arlmi_plan* p, const void* d_vis, const void* d_weight, int
   if (precision == 64) return grid_impl<double>(p, d_vis, d_weight, dopsf, k, d_grids, d_sumwt, 0, s);
   set_error("precision must be 32 or 64");
   return ARLMI_E_ARG;
+}
+
+}  // extern "C"
+
+namespace arlmi {
+namespace {
+
+// One launch group: the plans of a batch that share a kernel support.
+struct BatchGroup {
+  int support = 0;
+  int first = 0, count = 0;  // descriptors [first, first + count)
+  int64_t gblocks = 0, dblocks = 0;
+  size_t gsh = 0, dsh = 0;   // dynamic LDS of the gridder / degridder launches
+};
+
+}  // namespace
+}  // namespace arlmi
+
+struct arlmi_batch {
+  int precision = 32;
+  int n = 0;
+  arlmi::BatchDesc* d_descs = nullptr;
+  std::vector<arlmi::BatchDesc> h_descs;
+  std::vector<arlmi::BatchGroup> groups;
+  int reduce_g = 1, max_tile = 0;
+  bool any_split = false, any_orphans = false;
+};
+
+namespace arlmi {
+namespace {
+
+template <typename R>
+int batch_build(arlmi_batch* b, const arlmi_batch_item* in, int n) {
+  // group by support, keeping each group's descriptors together
+  std::vector<int> order(n);
+  for (int i = 0; i < n; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int c) { return in[a].plan->g.support < in[c].plan->g.support; });
+  for (int oi = 0; oi < n; ++oi) {
+    const arlmi_batch_item& it = in[order[oi]];
+    const arlmi_plan* p = it.plan;
+    const DevGeom& g = p->g;
+    if (!b->groups.empty() && b->groups.back().support == g.support) {
+      // same group
+    } else {
+      BatchGroup grp;
+      grp.support = g.support;
+      grp.first = oi;
+      b->groups.push_back(grp);
+    }
+    BatchGroup& grp = b->groups.back();
+    cplx<R>* partial = nullptr;
+    if (int rc = partial_buffer<R>(p, &partial)) return rc;
+    BatchDesc d{};
+    d.gitems = p->d_gitems;
+    d.ditems = p->d_ditems;
+    d.entries = p->d_entries;
+    d.vis = it.d_vis;
+    d.wt = it.d_weight;
+    d.ktab = it.kernel.d_table;
+    d.grids = it.d_grids;
+    d.partial = partial;
+    d.sumwt = it.d_sumwt;
+    d.vis_out = it.d_vis_out;
+    d.multi = p->nslots > 0 ? p->d_multi : nullptr;
+    d.nmulti = p->d_misc + 4;
+    d.orphans = p->norphans > 0 ? reinterpret_cast<const uint32_t*>(p->d_recs) + 4 * p->orphan_start : nullptr;
+    d.norphans = p->norphans;
+    d.os = g.os;
+    d.T = g.tile;
+    d.ntx = g.ntx;
+    d.N = g.n;
+    d.nsw = g.nplanes * g.nchan;
+    d.gbase = (int)grp.gblocks;
+    d.ngitems = (int)p->ngitems;
+    d.dbase = (int)grp.dblocks;
+    d.nditems = (int)p->nditems;
+    grp.gblocks += p->ngitems;
+    grp.dblocks += p->nditems;
+    if (grp.gblocks > 0x7fffffff || grp.dblocks > 0x7fffffff) {
+      set_error("arlmi_batch_create: too many work items");
+      return ARLMI_E_UNSUPPORTED;
+    }
+    const int S = g.support;
+    grp.gsh = std::max(grp.gsh, sizeof(double) * 2 * ((size_t)g.tile * lds_stride(g.tile) + 64) + sizeof(R) * (size_t)g.os * S);
+    grp.dsh = std::max(grp.dsh, sizeof(R) * (2 * (size_t)(g.tile + S) * (g.tile + S) + (size_t)g.os * S));
+    grp.count++;
+    b->any_split |= p->nslots > 0;
+    b->any_orphans |= p->norphans > 0;
+    b->max_tile = std::max(b->max_tile, g.tile);
+    b->h_descs.push_back(d);
+  }
+  for (const BatchGroup& grp : b->groups) {
+    if (grp.gsh > lds_limit() || grp.dsh > lds_limit()) {
+      set_error("arlmi_batch_create: a tile of support %d exceeds LDS", grp.support);
+      return ARLMI_E_UNSUPPORTED;
+    }
+  }
+  ARLMI_HIP(hipMalloc(&b->d_descs, sizeof(BatchDesc) * n));
+  ARLMI_HIP(hipMemcpy(b->d_descs, b->h_descs.data(), sizeof(BatchDesc) * n, hipMemcpyHostToDevice));
+  b->reduce_g = 64;
+  return ARLMI_OK;
+}
+
+template <typename R, int S>
+int batch_grid_group(const arlmi_batch* b, const BatchGroup& grp, int dopsf, hipStream_t s) {
+  if (grp.gblocks == 0) return ARLMI_OK;
+  ARLMI_LAUNCH("k_grid_sep", (k_grid_sep_batch<R, S>), dim3((unsigned)grp.gblocks), dim3(GRID_THREADS), grp.gsh, s,
+               b->d_descs + grp.first, grp.count, dopsf);
+  ARLMI_LAUNCH_CHECK();
+  return ARLMI_OK;
+}
+
+template <typename R, int S>
+int batch_degrid_group(const arlmi_batch* b, const BatchGroup& grp, hipStream_t s) {
+  if (grp.dblocks == 0) return ARLMI_OK;
+  ARLMI_LAUNCH("k_degrid_sep", (k_degrid_sep_batch<R, S>), dim3((unsigned)grp.dblocks), dim3(degrid_threads<R>()),
+               grp.dsh, s, b->d_descs + grp.first, grp.count);
+  ARLMI_LAUNCH_CHECK();
+  return ARLMI_OK;
+}
+
+template <typename R>
+int batch_grid(const arlmi_batch* b, int dopsf, hipStream_t s) {
+  ARLMI_LAUNCH("k_batch_zero_sumwt", k_batch_zero_sumwt, dim3((unsigned)b->n), dim3(256), 0, s, b->d_descs);
+  ARLMI_LAUNCH_CHECK();
+  for (const BatchGroup& grp : b->groups) {
+    int rc = ARLMI_OK;
+    switch (grp.support) {
+      case 2: rc = batch_grid_group<R, 2>(b, grp, dopsf, s); break;
+      case 4: rc = batch_grid_group<R, 4>(b, grp, dopsf, s); break;
+      case 6: rc = batch_grid_group<R, 6>(b, grp, dopsf, s); break;
+      case 8: rc = batch_grid_group<R, 8>(b, grp, dopsf, s); break;
+      case 10: rc = batch_grid_group<R, 10>(b, grp, dopsf, s); break;
+      case 12: rc = batch_grid_group<R, 12>(b, grp, dopsf, s); break;
+      case 16: rc = batch_grid_group<R, 16>(b, grp, dopsf, s); break;
+      default: set_error("batched gridding: support %d not instantiated", grp.support); return ARLMI_E_UNSUPPORTED;
+    }
+    if (rc) return rc;
+  }
+  if (b->any_split) {
+    const int T = b->max_tile;
+    ARLMI_LAUNCH("k_reduce_tiles", k_batch_reduce_tiles<R>, dim3((unsigned)(b->n * b->reduce_g), (T * T + 255) / 256),
+                 dim3(256), 0, s, b->d_descs, b->reduce_g);
+    ARLMI_LAUNCH_CHECK();
+  }
+  return ARLMI_OK;
+}
+
+template <typename R>
+int batch_degrid(const arlmi_batch* b, hipStream_t s) {
+  if (b->any_orphans) {
+    ARLMI_LAUNCH("k_zero_rows", k_batch_zero_rows<R>, dim3((unsigned)b->n), dim3(256), 0, s, b->d_descs);
+    ARLMI_LAUNCH_CHECK();
+  }
+  for (const BatchGroup& grp : b->groups) {
+    int rc = ARLMI_OK;
+    switch (grp.support) {
+      case 2: rc = batch_degrid_group<R, 2>(b, grp, s); break;
+      case 4: rc = batch_degrid_group<R, 4>(b, grp, s); break;
+      case 6: rc = batch_degrid_group<R, 6>(b, grp, s); break;
+      case 8: rc = batch_degrid_group<R, 8>(b, grp, s); break;
+      case 10: rc = batch_degrid_group<R, 10>(b, grp, s); break;
+      case 12: rc = batch_degrid_group<R, 12>(b, grp, s); break;
+      case 16: rc = batch_degrid_group<R, 16>(b, grp, s); break;
+      default: set_error("batched degridding: support %d not instantiated", grp.support); return ARLMI_E_UNSUPPORTED;
+    }
+    if (rc) return rc;
+  }
+  return ARLMI_OK;
+}
+
+}  // namespace
+}  // namespace arlmi
+
+extern "C" {
+
+int arlmi_batch_create(const arlmi_batch_item* items, int32_t n, int precision, arlmi_batch** out) {
+  if (!items || n < 1 || !out || (precision != 32 && precision != 64)) {
+    set_error("arlmi_batch_create: bad argument");
+    return ARLMI_E_ARG;
+  }
+  *out = nullptr;
+  for (int i = 0; i < n; ++i) {
+    const arlmi_batch_item& it = items[i];
+    if (!it.plan || !it.d_grids || !it.d_sumwt) {
+      set_error("arlmi_batch_create: item %d has a NULL plan or buffer", i);
+      return ARLMI_E_ARG;
+    }
+    if (int rc = check_kernel(it.plan, &it.kernel)) return rc;
+    if (it.kernel.kind != ARLMI_KERNEL_SEPARABLE) {
+      set_error("arlmi_batch_create: item %d: only separable kernels are batched", i);
+      return ARLMI_E_UNSUPPORTED;
+    }
+    if (it.plan->flags & ARLMI_FLAG_OCCUPIED_TILES) {
+      set_error("arlmi_batch_create: item %d: plans that leave untouched tiles unwritten are not batched", i);
+      return ARLMI_E_UNSUPPORTED;
+    }
+  }
+  arlmi_batch* b = new arlmi_batch();
+  b->precision = precision;
+  b->n = n;
+  const int rc = precision == 32 ? batch_build<float>(b, items, n) : batch_build<double>(b, items, n);
+  if (rc) {
+    if (b->d_descs) (void)hipFree(b->d_descs);
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return ARLMI_OK;
+}
+
+int arlmi_batch_grid(const arlmi_batch* b, int dopsf, void* stream) {
+  if (!b) {
+    set_error("arlmi_batch_grid: NULL batch");
+    return ARLMI_E_ARG;
+  }
+  hipStream_t s = as_stream(stream);
+  return b->precision == 32 ? batch_grid<float>(b, dopsf, s) : batch_grid<double>(b, dopsf, s);
+}
+
+int arlmi_batch_degrid(const arlmi_batch* b, void* stream) {
+  if (!b) {
+    set_error("arlmi_batch_degrid: NULL batch");
+    return ARLMI_E_ARG;
+  }
+  hipStream_t s = as_stream(stream);
+  return b->precision == 32 ? batch_degrid<float>(b, s) : batch_degrid<double>(b, s);
+}
+
+void arlmi_batch_destroy(arlmi_batch* b) {
+  if (!b) return;
+  if (b->d_descs) (void)hipFree(b->d_descs);
+  delete b;
 }
 
 int arlmi_grid_add(const arlmi_plan* p, const void* d_vis, const void* d_weight, int dopsf, const arlmi_kernel* k,

@@ -179,6 +179,33 @@ int arlmi_grid(const arlmi_plan* plan, const void* d_vis, const void* d_weight, 
 int arlmi_grid_add(const arlmi_plan* plan, const void* d_vis, const void* d_weight, int dopsf,
                    const arlmi_kernel* kernel, void* d_grids, double* d_sumwt, int precision, void* stream);
 
+/* Batched gridding and degridding of several plans, one launch per kernel support (no
+ * counterpart in ARL, which images its vis lists one call at a time, imaging_serial.py:176-197,
+ * or as separate Dask tasks, imaging_arlexecute.py:213-221).  Each item names a plan with its
+ * separable kernel, gridding inputs (vis, imaging weight), grids and sums of weights (as
+ * arlmi_grid / arlmi_degrid take them) and the degridding output.  arlmi_batch_grid equals
+ * arlmi_grid (dopsf as there) on every item, arlmi_batch_degrid arlmi_degrid; work items of all
+ * plans of one support run in one launch (each block finds its plan), then the split tiles of all
+ * plans are summed in one launch.  The batch records the plans' work lists and buffer addresses
+ * when it is created: after arlmi_plan_update, or with other buffers, create it again.  Creation
+ * synchronises; the calls allocate nothing, record no memset and never synchronise (they can be
+ * captured into a HIP graph).  Plans with ARLMI_FLAG_OCCUPIED_TILES and table kernels are not
+ * batched (ARLMI_E_UNSUPPORTED). */
+typedef struct arlmi_batch arlmi_batch; /* opaque */
+typedef struct arlmi_batch_item {
+  const arlmi_plan* plan;
+  const void* d_vis;      /* [V] complex, gridding input */
+  const void* d_weight;   /* [V] real imaging weights */
+  arlmi_kernel kernel;    /* separable */
+  void* d_grids;          /* [plane][chan][N][N]: gridding output, degridding input */
+  double* d_sumwt;        /* [plane][chan] */
+  void* d_vis_out;        /* [V] complex, degridding output */
+} arlmi_batch_item;
+int arlmi_batch_create(const arlmi_batch_item* items, int32_t n, int precision, arlmi_batch** out);
+int arlmi_batch_grid(const arlmi_batch* batch, int dopsf, void* stream);
+int arlmi_batch_degrid(const arlmi_batch* batch, void* stream);
+void arlmi_batch_destroy(arlmi_batch* batch);
+
 /* degrid_visibility_from_griddata (gridding.py:233-267): d_vis_out[row] = sum CF * G
  * (no conjugation), reading the plane each row belongs to.  Rows in no plane get 0
  * (imaging_serial.py:92 zero copy); a row in two w slabs keeps the later slab's value

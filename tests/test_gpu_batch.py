@@ -64,6 +64,28 @@ def test_graph_replays_match_eager(gpu):
         torch.cuda.synchronize()
 
 
+@pytest.mark.parametrize("graph", [False, True])
+def test_persistent_batch_matches_eager(gpu, graph):
+    """One gridding and one degridding launch per kernel support for all observations
+    (arlmi_batch_grid / arlmi_batch_degrid), eager and replayed from a graph three times."""
+    import torch
+    import arlmi
+    tasks = _tasks(nobs=4, seed=34)
+    ref = _eager_reference(tasks)
+    batch = arlmi.BatchImager(tasks, graph=graph, persistent=True)
+    assert batch.batch
+    for k in range(3):
+        for img in batch.images:
+            img.fill_(float("nan"))
+        for pv in batch.predicted:
+            pv.fill_(float("nan"))
+        batch.run()
+        torch.cuda.synchronize()
+        _check(batch, ref)
+        _eager_reference(tasks[:1])
+        torch.cuda.synchronize()
+
+
 def test_autotuned_batch_matches_eager(gpu, tmp_path):
     import torch
     import arlmi
@@ -83,14 +105,15 @@ def test_autotuned_batch_matches_eager(gpu, tmp_path):
 
 def test_config3_full_batch(gpu):
     """The whole batching workload (64 observations of 10k-4M rows, N from 256 to 2048, S in
-    {6, 8, 10}, 8-64 slabs, 102.8 M rows): every observation of the captured graph against its
-    eager run, and the two cheapest observations against the oracle."""
+    {6, 8, 10}, 8-64 slabs, 102.8 M rows), gridded and degridded with one launch per support and
+    replayed from a graph: every observation against its eager run, and the two cheapest
+    observations against the oracle."""
     import torch
     import arlmi
     from oracle import arl_oracle as O
     tasks = _tasks(nobs=64, seed=3, max_rows=4_000_000)
     assert sum(t["vis"].nvis for t in tasks) > 90_000_000
-    batch = arlmi.BatchImager(tasks, graph=True)
+    batch = arlmi.BatchImager(tasks, graph=True, persistent=True)
     batch.run()
     torch.cuda.synchronize()
     for i, t in enumerate(tasks):
