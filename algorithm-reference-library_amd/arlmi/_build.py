@@ -1,8 +1,8 @@
 """Build libarlmi.so (HIP, gfx950) in place next to this package.
 
 Every .hip file under ../csrc is compiled with hipcc for gfx950 and linked into
-arlmi/libarlmi.so.  Run as ``python -m arlmi._build`` from the package root, or call
-``build()``.
+arlmi/libarlmi.so; the measurement tooling under ../probe goes into arlmi/libarlmi_probe.so.
+Run as ``python -m arlmi._build`` from the package root, or call ``build()``.
 """
 
 import os
@@ -13,9 +13,11 @@ from concurrent.futures import ThreadPoolExecutor
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG_DIR)
 CSRC = os.path.join(ROOT, "csrc")
+PROBE_SRC = os.path.join(ROOT, "probe")
 INCLUDE = os.path.join(os.path.dirname(ROOT), "include")
 OBJ_DIR = os.path.join(ROOT, "_build")
 LIB = os.path.join(PKG_DIR, "libarlmi.so")
+PROBE_LIB = os.path.join(PKG_DIR, "libarlmi_probe.so")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -30,8 +32,8 @@ FLAGS = [
 ]
 
 
-def _sources():
-    return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
+def _sources(folder=CSRC):
+    return sorted(os.path.join(folder, f) for f in os.listdir(folder) if f.endswith(".hip"))
 
 
 def _compile(src, verbose):
@@ -45,21 +47,28 @@ def _compile(src, verbose):
     return obj
 
 
-def build(verbose=False, jobs=None):
-    """Compile every HIP source and link libarlmi.so; returns the library path."""
-    os.makedirs(OBJ_DIR, exist_ok=True)
-    srcs = _sources()
-    jobs = jobs or min(len(srcs), max(1, (os.cpu_count() or 2) // 2))
-    with ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(lambda s: _compile(s, verbose), srcs))
-    tmp = LIB + ".tmp"
+def _link(objs, lib, verbose):
+    tmp = lib + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError("link failed:\n%s\n%s" % (res.stdout, res.stderr))
-    os.replace(tmp, LIB)
+    os.replace(tmp, lib)
+
+
+def build(verbose=False, jobs=None):
+    """Compile every HIP source and link libarlmi.so (and libarlmi_probe.so); returns the
+    imaging library's path."""
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    srcs = _sources()
+    psrcs = _sources(PROBE_SRC)
+    jobs = jobs or min(len(srcs) + len(psrcs), max(1, (os.cpu_count() or 2) // 2))
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(lambda s: _compile(s, verbose), srcs + psrcs))
+    _link(objs[:len(srcs)], LIB, verbose)
+    _link(objs[len(srcs):], PROBE_LIB, verbose)
     return LIB
 
 

@@ -40,7 +40,7 @@ EXPORTS = (
     "arlmi_grid_weights", "arlmi_reweight", "arlmi_block_to_rows", "arlmi_apply_gaintable", "arlmi_apply_gains_rows", "arlmi_apply_gains_rows_adjoint",
     "arlmi_phaserotate_rows", "arlmi_phaserotate_blocks", "arlmi_hogbom", "arlmi_hogbom_noalloc", "arlmi_hogbom_traffic", "arlmi_solve_gains",
     "arlmi_bda_uvdist", "arlmi_bda_average",
-    "arlmi_probe_lds_add", "arlmi_profile_enable", "arlmi_profile_collect", "arlmi_profile_reset",
+    "arlmi_profile_enable", "arlmi_profile_collect", "arlmi_profile_reset",
     "arlmi_profile_report", "arlmi_profile_only",
 )
 
@@ -174,7 +174,6 @@ def lib():
         "arlmi_bda_average": (ctypes.c_int, [ctypes.POINTER(Blocks), vp, vp, vp, i64, ctypes.POINTER(Rows), vp,
                                              ctypes.c_int, vp]),
         "arlmi_hogbom_traffic": (ctypes.c_int, [ctypes.POINTER(i64), ctypes.POINTER(i64)]),
-        "arlmi_probe_lds_add": (ctypes.c_int, [i32, i32, ctypes.POINTER(dbl), ctypes.POINTER(ctypes.c_float)]),
         "arlmi_profile_enable": (ctypes.c_int, [ctypes.c_int]),
         "arlmi_profile_collect": (ctypes.c_int, []),
         "arlmi_profile_reset": (ctypes.c_int, []),
@@ -211,3 +210,29 @@ def profile_report():
         name, ms, cnt = line.split()
         out[name] = (float(ms), int(cnt))
     return out
+
+
+PROBE_LIB_PATH = os.path.join(_PKG, "libarlmi_probe.so")
+_probe = None
+
+
+def probe_lib():
+    """The measurement library (libarlmi_probe.so: the LDS microbenchmark), loaded on first use."""
+    global _probe
+    if _probe is None:
+        if not os.path.exists(PROBE_LIB_PATH):
+            raise RuntimeError("%s is not built (run arlmi._build.build())" % PROBE_LIB_PATH)
+        p = ctypes.CDLL(PROBE_LIB_PATH)
+        p.arlmi_probe_lds_add.restype = ctypes.c_int
+        p.arlmi_probe_lds_add.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
+                                          ctypes.POINTER(ctypes.c_float)]
+        p.arlmi_probe_last_error.restype = ctypes.c_char_p
+        p.arlmi_probe_last_error.argtypes = []
+        _probe = p
+    return _probe
+
+
+def probe_check(rc):
+    if rc != 0:
+        raise ArlmiError(rc, probe_lib().arlmi_probe_last_error().decode(errors="replace"))
+    return rc

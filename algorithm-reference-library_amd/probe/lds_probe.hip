@@ -1,10 +1,27 @@
 // LDS microbenchmark: the cost of ds_add_f64 wavefront instructions on gfx950, for the
-// gridder's access pattern and for a conflict-free one.  Used by the measurement (the peak the
-// gridder's atomic throughput is compared with), not by the imaging path.
-#include "arlmi_internal.h"
+// gridder's access pattern and for a conflict-free one.  Measurement tooling (the peak the
+// gridder's atomic throughput is compared with), built into its own library, libarlmi_probe.so,
+// and not part of the imaging library.
+#include <hip/hip_runtime.h>
 
-namespace arlmi {
+#include <cstdint>
+#include <cstdio>
+
+#include "arlmi_probe.h"
+
+namespace arlmi_probe {
 namespace {
+
+thread_local char g_err[256] = "";
+
+#define PROBE_HIP(call)                                                                    \
+  do {                                                                                     \
+    const hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess) {                                                                \
+      snprintf(g_err, sizeof(g_err), "%s: %s", #call, hipGetErrorString(e_));              \
+      return -7;                                                                           \
+    }                                                                                      \
+  } while (0)
 
 constexpr int kProbeThreads = 512;
 constexpr int kProbeTile = 4608;  // 64 rows x 72 doubles, the gridder's accumulator tile
@@ -54,31 +71,33 @@ __global__ __launch_bounds__(kProbeThreads) void k_probe_lds_add(double* out, in
 }
 
 }  // namespace
-}  // namespace arlmi
+}  // namespace arlmi_probe
 
-using namespace arlmi;
+using namespace arlmi_probe;
 
 extern "C" {
 
+const char* arlmi_probe_last_error(void) { return g_err; }
+
 int arlmi_probe_lds_add(int32_t pattern, int32_t iters, double* cycles_per_instr, float* ms_out) {
   if (pattern < 0 || pattern > 4 || iters <= 0 || !cycles_per_instr) {
-    set_error("arlmi_probe_lds_add: bad argument");
-    return ARLMI_E_ARG;
+    snprintf(g_err, sizeof(g_err), "arlmi_probe_lds_add: bad argument");
+    return -1;
   }
   int dev = 0, ncu = 0, clk_khz = 0;
-  ARLMI_HIP(hipGetDevice(&dev));
-  ARLMI_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-  ARLMI_HIP(hipDeviceGetAttribute(&clk_khz, hipDeviceAttributeClockRate, dev));
+  PROBE_HIP(hipGetDevice(&dev));
+  PROBE_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  PROBE_HIP(hipDeviceGetAttribute(&clk_khz, hipDeviceAttributeClockRate, dev));
   const int blocks = 2 * ncu;  // two work groups of 8 wavefronts per CU
   double* out = nullptr;
-  ARLMI_HIP(hipMalloc(&out, sizeof(double) * blocks));
+  PROBE_HIP(hipMalloc(&out, sizeof(double) * blocks));
   hipEvent_t a, b;
-  ARLMI_HIP(hipEventCreate(&a));
-  ARLMI_HIP(hipEventCreate(&b));
+  PROBE_HIP(hipEventCreate(&a));
+  PROBE_HIP(hipEventCreate(&b));
   const size_t sh = sizeof(double) * 2 * kProbeTile;
   float ms = 0.f;
   for (int rep = 0; rep < 2; ++rep) {  // the second launch is timed
-    ARLMI_HIP(hipEventRecord(a, nullptr));
+    PROBE_HIP(hipEventRecord(a, nullptr));
     if (pattern == 0)
       hipLaunchKernelGGL(k_probe_lds_add<0>, dim3(blocks), dim3(kProbeThreads), sh, nullptr, out, iters, 1u);
     else if (pattern == 1)
@@ -89,10 +108,10 @@ int arlmi_probe_lds_add(int32_t pattern, int32_t iters, double* cycles_per_instr
       hipLaunchKernelGGL(k_probe_lds_add<3>, dim3(blocks), dim3(kProbeThreads), sh, nullptr, out, iters, 1u);
     else
       hipLaunchKernelGGL(k_probe_lds_add<4>, dim3(blocks), dim3(kProbeThreads), sh, nullptr, out, iters, 1u);
-    ARLMI_LAUNCH_CHECK();
-    ARLMI_HIP(hipEventRecord(b, nullptr));
-    ARLMI_HIP(hipEventSynchronize(b));
-    ARLMI_HIP(hipEventElapsedTime(&ms, a, b));
+    PROBE_HIP(hipGetLastError());
+    PROBE_HIP(hipEventRecord(b, nullptr));
+    PROBE_HIP(hipEventSynchronize(b));
+    PROBE_HIP(hipEventElapsedTime(&ms, a, b));
   }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
@@ -102,7 +121,7 @@ int arlmi_probe_lds_add(int32_t pattern, int32_t iters, double* cycles_per_instr
   const double cycles = (double)ms * 1e-3 * (double)clk_khz * 1e3;
   *cycles_per_instr = cycles / instr;
   if (ms_out) *ms_out = ms;
-  return ARLMI_OK;
+  return 0;
 }
 
 }  // extern "C"

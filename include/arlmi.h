@@ -468,14 +468,6 @@ int arlmi_solve_gains(const void* d_vis, const void* d_model, const void* d_weig
  * cycle's algorithmic HBM bytes are 8 * scanned (+ 8 * scanned with a window) + 16 * subtracted. */
 int arlmi_hogbom_traffic(int64_t* scanned_pixels, int64_t* subtracted_pixels);
 
-/* LDS microbenchmark (no ARL counterpart; measurement only): cycles per LDS-add wavefront
- * instruction per CU for pattern 0 (ds_add_f64 on the gridder's 8x8 footprints at random
- * origins of a 64x72 tile), 1 (ds_add_f64 on 64 distinct consecutive addresses), 2 (pattern 0
- * with 64-bit integer adds), 3 (pattern 0 with ds_add_f32) or 4 (pattern 1 with ds_add_f32),
- * two work groups of 8 wavefronts per CU, the device clock taken from
- * hipDeviceAttributeClockRate.  Synchronises the device. */
-int arlmi_probe_lds_add(int32_t pattern, int32_t iters, double* cycles_per_instr, float* ms_out);
-
 /* Opt-in kernel timing (no ARL counterpart; measurement only): with profiling on, every library
  * kernel launch is bracketed by two HIP events on its stream.  collect() waits for them and adds
  * the elapsed times per kernel; report() writes "kernel total_ms launches" lines into buf and

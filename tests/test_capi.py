@@ -13,8 +13,8 @@ from oracle import arl_oracle as O
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def _declared():
-    text = open(os.path.join(ROOT, "include", "arlmi.h")).read()
+def _declared(header="arlmi.h"):
+    text = open(os.path.join(ROOT, "include", header)).read()
     return sorted(set(re.findall(r"\b(arlmi_[a-z0-9_]+)\s*\(", text)))
 
 
@@ -28,6 +28,17 @@ def test_library_exports_every_declared_symbol():
     assert set(_lib.EXPORTS) <= set(names)
     _lib.lib()  # every signature binds
     assert lib.arlmi_version() >= 100
+    assert not hasattr(lib, "arlmi_probe_lds_add")  # measurement tooling lives in its own library
+
+
+def test_probe_library_exports_every_declared_symbol():
+    from arlmi import _lib
+    lib = ctypes.CDLL(_lib.PROBE_LIB_PATH)
+    names = _declared("arlmi_probe.h")
+    assert names == ["arlmi_probe_last_error", "arlmi_probe_lds_add"]
+    for name in names:
+        assert hasattr(lib, name), name
+    _lib.probe_lib()
 
 
 def test_error_reporting_without_gpu():
