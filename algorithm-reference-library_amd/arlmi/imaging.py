@@ -137,10 +137,10 @@ class Imager:
         # neither written nor read)
         if mode == "2d":
             self.plan = Plan(vis, im, self.cf, _lib.MODE_2D, 1, tile=tile, drop_off_grid=drop_off_grid,
-                             sparse=sparse, chunk=chunk)
+                             sparse=sparse, chunk=chunk, private=True)
         elif mode == "wstack":
             self.plan = Plan(vis, im, self.cf, _lib.MODE_WSTACK, vis_slices, tile=tile, drop_off_grid=drop_off_grid,
-                             slab_boxes=slab_boxes, sparse=sparse, chunk=chunk)
+                             slab_boxes=slab_boxes, sparse=sparse, chunk=chunk, private=True)
         else:
             raise ValueError("mode must be '2d' or 'wstack'")
         self.mode = mode

@@ -96,7 +96,7 @@ class StreamingImager:
         # dense grids; gridding work items only for the tiles a chunk touches (the grids are zeroed
         # once per pass, and every chunk adds into them)
         plan = Plan(vis, self.im, self.cf, mode, self.vis_slices, tile=self.tile, slab_boxes=self.boxes,
-                    sparse=False, chunk=self.chunk, occupied_only=True)
+                    sparse=False, chunk=self.chunk, occupied_only=True, private=True)
         if self.mode == "wstack":
             _lib.check(_lib.lib().arlmi_plan_fix_plane_w(
                 plan.handle, self.plane_w.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(self.plane_w)))

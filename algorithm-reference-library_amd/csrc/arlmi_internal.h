@@ -170,6 +170,16 @@ int exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* t
 // Device twiddle table exp(-2 pi i k / n), k < n, cached per (n, precision).
 const void* twiddles(int n, int precision);
 
+// Grid rows in the digit-transposed order of the two-level column transform: for a plan with
+// ARLMI_FLAG_PRIVATE_GRIDS, arlmi_image_to_grids leaves row v of an n x n plane at row
+// perm_row_of(v, n, n1) with n1 = grid_perm_n1(n, precision) (0: natural order).
+int grid_perm_n1(int n, int precision);
+__host__ __device__ inline int perm_row_of(int v, int n, int n1) {
+  if (!n1) return v;
+  const int n2 = n / n1;
+  return n1 * (v % n2) + v / n2;
+}
+
 // Opt-in kernel timing (profile.hip).
 bool profile_on();
 void* profile_begin(const char* name, hipStream_t s);

@@ -360,7 +360,11 @@ struct FftArgs {
   int zero_fill;
   int tile, ntx;
   int tshift;  // log2(tile)
+  // grid rows in digit-transposed order (the two-level column transform's output): row v of a
+  // plane is stored at N1 * (v % (N / N1)) + v / (N / N1); 0 = natural order
+  int perm_n1;
 };
+
 
 // Masked-out loads read zeros from a small buffer instead of the grid (lane l reads entry
 // l & 63, 16 bytes at most per entry), so that every load of a pass is issued unconditionally
@@ -537,7 +541,7 @@ __global__ __launch_bounds__(Shape<N>::NT * Shape<N>::ROW_NB) void k_rows_invert
     const double sw = a.sumwt[(size_t)p * a.nchan + c];
     if (a.wstack && a.normalize == 1 && !(sw > 0.0)) continue;  // normalize_sumwt zeroes this slab's image
     const R wsc = weigh ? (R)sw : R(1);
-    const cplx<R>* row = grids + ((size_t)p * a.nchan + c) * N * N + (size_t)y * N;
+    const cplx<R>* row = grids + ((size_t)p * a.nchan + c) * N * N + (size_t)perm_row_of(y, N, a.perm_n1) * N;
     const uint32_t cm = a.colmask ? a.colmask[(size_t)p * a.nchan + c] : ~0u;
     const double wp = SCREENS ? a.plane_w[p] : 0.0;
     fft_pass<R, N, NB, NT, true, LAY_ROW>(
@@ -775,6 +779,116 @@ __global__ __launch_bounds__(256) void k_transpose(cplx<R>* __restrict__ data, i
   }
 }
 
+// Two-level (four-step) column transform for the sizes where one column fills LDS: length
+// N = N1 * N2, NB4 adjacent columns per work group (128-byte row segments).
+//   pass A (k_cols4_a), per column group and n1 < N1: the N2 values x[n1 + N1 n2] (rows N1 apart)
+//     -> N2-point DFT over n2 -> times W_N^(n1 k2) -> row n1 + N1 k2 (the rows it read);
+//   pass B (k_cols4_b), per column group and k2 < N2: the N1 contiguous rows N1 k2 + n1 -> N1-point
+//     DFT over n1 -> X[k2 + N2 k1], written to row N1 k2 + k1 (again the rows it read).
+// Both passes are in place; the output is in digit-transposed row order (perm_row_of), which the
+// consumers read through perm_row_of: the invert row pass, and the degridders for a plan whose grids
+// are private.  (-1)^(x+y) input and output modulations and the scale use the natural indices.
+constexpr int kCols4N1 = 64;
+template <typename R>
+__host__ __device__ constexpr int cols4_nb() {
+  return sizeof(R) == 4 ? 16 : 8;
+}
+// transforms of one column group per work group: pass A (length N2 = N / 64) and pass B (64)
+constexpr int kCols4MA = 4, kCols4MB = 8;
+
+template <typename R, int N, bool INV>
+__global__ __launch_bounds__(Shape<N / kCols4N1>::NT * cols4_nb<R>() * kCols4MA) void k_cols4_a(
+    FftArgs a, const cplx<R>* __restrict__ tw2g, const cplx<R>* __restrict__ twN) {
+  constexpr int N1 = kCols4N1, N2 = N / N1, NBC = cols4_nb<R>(), M = kCols4MA, NB = NBC * M;
+  constexpr int NT = Shape<N2>::NT;
+  extern __shared__ __align__(16) unsigned char smem[];
+  cplx<R>* lds = reinterpret_cast<cplx<R>*>(smem);
+  const cplx<R>* tw = stage_twiddles_at<R, N2>(smem, sizeof(cplx<R>) * (size_t)N2 * NB, tw2g);
+  constexpr int CG = N / NBC;  // column groups per plane
+  constexpr int GA = N1 / M;   // n1 groups
+  const int g = blockIdx.x % GA;
+  const int rest = blockIdx.x / GA;
+  const int64_t plane = rest / CG;
+  const int x0 = (rest % CG) * NBC;
+  const int b = threadIdx.x % NB, t = threadIdx.x / NB;
+  const int x = x0 + b % NBC, n1 = g * M + b / NBC;
+  cplx<R>* col = reinterpret_cast<cplx<R>*>(a.data) + plane * (int64_t)N * N + x;
+  fft_pass<R, N2, NB, NT, INV, LAY_COL>(
+      lds, b, t, tw,
+      [&](int, int, int n2) {
+        const int y = n1 + N1 * n2;
+        cplx<R> v = col[(size_t)y * N];
+        if (a.in_mod && ((y + x) & 1)) v = cmk<R>(-v.x, -v.y);
+        return v;
+      },
+      [&](int, int, int k2, cplx<R> v) {
+        cplx<R> w = twN[n1 * k2];  // n1 k2 < N
+        if (INV) w.y = -w.y;
+        col[(size_t)(n1 + N1 * k2) * N] = cmul(v, w);
+      });
+}
+
+template <typename R, int N, bool INV>
+__global__ __launch_bounds__(Shape<kCols4N1>::NT * cols4_nb<R>() * kCols4MB) void k_cols4_b(
+    FftArgs a, const cplx<R>* __restrict__ tw1g) {
+  constexpr int N1 = kCols4N1, N2 = N / N1, NBC = cols4_nb<R>(), M = kCols4MB, NB = NBC * M;
+  constexpr int NT = Shape<N1>::NT;
+  extern __shared__ __align__(16) unsigned char smem[];
+  cplx<R>* lds = reinterpret_cast<cplx<R>*>(smem);
+  const cplx<R>* tw = stage_twiddles_at<R, N1>(smem, sizeof(cplx<R>) * (size_t)N1 * NB, tw1g);
+  constexpr int CG = N / NBC;
+  constexpr int GB = N2 / M;  // k2 groups
+  const int g = blockIdx.x % GB;
+  const int rest = blockIdx.x / GB;
+  const int64_t plane = rest / CG;
+  const int x0 = (rest % CG) * NBC;
+  const int b = threadIdx.x % NB, t = threadIdx.x / NB;
+  const int x = x0 + b % NBC, k2 = g * M + b / NBC;
+  cplx<R>* col = reinterpret_cast<cplx<R>*>(a.data) + plane * (int64_t)N * N + x + (size_t)(N1 * k2) * N;
+  fft_pass<R, N1, NB, NT, INV, LAY_COL>(
+      lds, b, t, tw, [&](int, int, int n1) { return col[(size_t)n1 * N]; },
+      [&](int, int, int k1, cplx<R> v) {
+        const int vrow = k2 + N2 * k1;  // natural index of this output
+        R sc = (R)a.scale;
+        if (a.out_mod && ((vrow + x) & 1)) sc = -sc;
+        col[(size_t)k1 * N] = cmk<R>(v.x * sc, v.y * sc);
+      });
+}
+
+// Rows per plane of the two-level column transform's digit-transposed order, or 0 where the
+// columns are transformed in natural order (one-pass columns) for this size and precision.
+template <typename R, int N>
+constexpr int cols4_n1() {
+  return Shape<N>::template cols_by_transpose<R>() ? kCols4N1 : 0;
+}
+
+template <typename R, int N, bool INV>
+static int run_cols4(FftArgs c, hipStream_t s) {
+  constexpr int N1 = kCols4N1, N2 = N / N1, NB = cols4_nb<R>();
+  static_assert(N2 >= 64, "two-level column transform for N >= 4096");
+  const cplx<R>* tw1 = reinterpret_cast<const cplx<R>*>(twiddles(N1, sizeof(R) == 4 ? 32 : 64));
+  const cplx<R>* tw2 = reinterpret_cast<const cplx<R>*>(twiddles(N2, sizeof(R) == 4 ? 32 : 64));
+  const cplx<R>* twN = reinterpret_cast<const cplx<R>*>(twiddles(N, sizeof(R) == 4 ? 32 : 64));
+  if (!tw1 || !tw2 || !twN) {
+    set_error("twiddle table allocation failed");
+    return ARLMI_E_HIP;
+  }
+  if (c.tilemask) {
+    set_error("sparse grids are not supported at N = %d in this precision", N);
+    return ARLMI_E_UNSUPPORTED;
+  }
+  constexpr int MA = kCols4MA, MB = kCols4MB;
+  const int64_t groups = (int64_t)c.nbatch * (N / NB);
+  const size_t sha = sizeof(cplx<R>) * ((size_t)N2 * NB * MA + N2), shb = sizeof(cplx<R>) * ((size_t)N1 * NB * MB + N1);
+  ARLMI_LAUNCH("k_cols4_a", (k_cols4_a<R, N, INV>), dim3((unsigned)(groups * (N1 / MA))),
+               dim3(Shape<N2>::NT * NB * MA), sha, s, c, tw2, twN);
+  ARLMI_LAUNCH_CHECK();
+  ARLMI_LAUNCH("k_cols4_b", (k_cols4_b<R, N, INV>), dim3((unsigned)(groups * (N2 / MB))),
+               dim3(Shape<N1>::NT * NB * MB), shb, s, c, tw1);
+  ARLMI_LAUNCH_CHECK();
+  return ARLMI_OK;
+}
+
 // The column transforms of c.nbatch planes in place (modulation and scale as c says): NB adjacent
 // columns per block, or, where two columns and the twiddles exceed LDS, transpose + row pass +
 // transpose ((-1)^(x+y) is symmetric, so the row pass applies the same modulation).
@@ -855,7 +969,14 @@ static int run_grids_to_image(FftArgs a, hipStream_t s) {
   c.out_mod = 0;
   c.scale = 1.0;
   c.nbatch = a.nplanes * a.nchan;
-  if (int rc = run_cols<R, N, true>(c, tw, s)) return rc;
+  a.perm_n1 = 0;
+  if constexpr (Shape<N>::template cols_by_transpose<R>()) {
+    // two-level columns; the row pass reads their digit-transposed row order
+    if (int rc = run_cols4<R, N, true>(c, s)) return rc;
+    a.perm_n1 = kCols4N1;
+  } else {
+    if (int rc = run_cols<R, N, true>(c, tw, s)) return rc;
+  }
   const int rowblocks = a.nchan * (N / RNB);
   int G = 1;
   plane_groups(rowblocks, a.nplanes, &a.ppg, &G);
@@ -895,10 +1016,30 @@ static int run_image_to_grids(FftArgs a, hipStream_t s) {
   c.out_mod = 1;
   c.scale = 1.0;
   c.nbatch = a.nplanes * a.nchan;
+  if constexpr (Shape<N>::template cols_by_transpose<R>()) {
+    // grids private to a plan: two-level columns, left in digit-transposed row order for the
+    // plan's degridder (a.perm_n1 set by the caller); otherwise natural order through transposes
+    if (a.perm_n1) return run_cols4<R, N, false>(c, s);
+  }
   return run_cols<R, N, false>(c, tw, s);
 }
 
 enum class Op { FFT, TO_IMAGE, TO_GRIDS };
+
+}  // namespace arlmi
+
+// Row order of the grids arlmi_image_to_grids leaves for a plan with private grids: kCols4N1 where
+// the two-level column transform is used (size and precision), 0 for natural order.
+int arlmi::grid_perm_n1(int n, int precision) {
+  const bool p32 = precision == 32;
+  switch (n) {
+    case 4096: return p32 ? cols4_n1<float, 4096>() : cols4_n1<double, 4096>();
+    case 8192: return p32 ? cols4_n1<float, 8192>() : cols4_n1<double, 8192>();
+    default: return 0;
+  }
+}
+
+namespace arlmi {
 
 template <typename R>
 static int dispatch_n(Op op, int n, FftArgs a, int direction, hipStream_t s) {
@@ -1038,6 +1179,8 @@ int arlmi_image_to_grids(const arlmi_plan* p, const void* d_model, const double*
     a.tshift = __builtin_ctz((unsigned)p->g.tile);
     a.ntx = p->g.ntx;
   }
+  // grids only this plan's degridder reads may keep the two-level column transform's row order
+  if (p->flags & ARLMI_FLAG_PRIVATE_GRIDS) a.perm_n1 = grid_perm_n1(p->g.n, precision);
   return precision == 32 ? dispatch_n<float>(Op::TO_GRIDS, p->g.n, a, 0, as_stream(stream))
                          : dispatch_n<double>(Op::TO_GRIDS, p->g.n, a, 0, as_stream(stream));
 }

@@ -336,6 +336,7 @@ struct BatchDesc {
   int64_t norphans;
   int os, T, ntx, N, nsw;
   int gbase, ngitems, dbase, nditems;
+  int perm_n1;  // row order of the grids the degridder reads (perm_row_of)
 };
 
 // The descriptor whose [base, base + count) block range holds global block b (descriptors in
@@ -505,7 +506,8 @@ __global__ __launch_bounds__(GRID_THREADS) void k_grid_tab(const int4* __restric
 // in LDS.  When every thread has at most MAXL values to move, all of its loads are issued before
 // any LDS store, so a block waits for one HBM latency, not one per value.
 template <typename R, int MAXL = 16>
-__device__ inline void load_halo(const cplx<R>* __restrict__ grids, R* lds, int Wd, int T, int N, const TilePos& tp) {
+__device__ inline void load_halo(const cplx<R>* __restrict__ grids, R* lds, int Wd, int T, int N, const TilePos& tp,
+                                 int perm_n1) {
   const cplx<R>* g = grids + (size_t)tp.plane * N * N;
   const int y0 = tp.ty * T, x0 = tp.tx * T;
   const int n = Wd * Wd;
@@ -518,7 +520,7 @@ __device__ inline void load_halo(const cplx<R>* __restrict__ grids, R* lds, int 
       if (i < n) {
         const int y = i / Wd, x = i - (i / Wd) * Wd;
         const int gy = y0 + y, gx = x0 + x;
-        if (gy < N && gx < N) v[j] = g[(size_t)gy * N + gx];
+        if (gy < N && gx < N) v[j] = g[(size_t)perm_row_of(gy, N, perm_n1) * N + gx];
       }
     }
 #pragma unroll
@@ -535,7 +537,7 @@ __device__ inline void load_halo(const cplx<R>* __restrict__ grids, R* lds, int 
     const int y = i / Wd, x = i - (i / Wd) * Wd;
     const int gy = y0 + y, gx = x0 + x;
     cplx<R> v = mk<R>(R(0), R(0));
-    if (gy < N && gx < N) v = g[(size_t)gy * N + gx];
+    if (gy < N && gx < N) v = g[(size_t)perm_row_of(gy, N, perm_n1) * N + gx];
     lds[2 * i] = v.x;
     lds[2 * i + 1] = v.y;
   }
@@ -552,12 +554,13 @@ constexpr int degrid_threads() {
 template <typename R, int S>
 __device__ __forceinline__ void degrid_sep_item(const int4 it, unsigned char* smem, const uint2* __restrict__ entries,
                                                 const cplx<R>* __restrict__ grids, const R* __restrict__ ktab,
-                                                int os, cplx<R>* __restrict__ vis_out, int T, int ntx, int N) {
+                                                int os, cplx<R>* __restrict__ vis_out, int T, int ntx, int N,
+                                                int perm_n1) {
   const int Wd = T + S;
   R* lds = reinterpret_cast<R*>(smem);
   R* lk = lds + 2 * Wd * Wd;
   const TilePos tp = tile_pos(it.x, ntx);
-  load_halo<R>(grids, lds, Wd, T, N, tp);
+  load_halo<R>(grids, lds, Wd, T, N, tp, perm_n1);
   for (int i = threadIdx.x; i < os * S; i += blockDim.x) lk[i] = ktab[i];
   __syncthreads();
   for (int e = threadIdx.x; e < it.z; e += blockDim.x) {
@@ -589,9 +592,10 @@ __device__ __forceinline__ void degrid_sep_item(const int4 it, unsigned char* sm
 template <typename R, int S>
 __global__ __launch_bounds__(degrid_threads<R>()) void k_degrid_sep(const int4* __restrict__ items, const uint2* __restrict__ entries,
                                                     const cplx<R>* __restrict__ grids, const R* __restrict__ ktab,
-                                                    int os, cplx<R>* __restrict__ vis_out, int T, int ntx, int N) {
+                                                    int os, cplx<R>* __restrict__ vis_out, int T, int ntx, int N,
+                                                    int perm_n1) {
   extern __shared__ __align__(16) unsigned char smem[];
-  degrid_sep_item<R, S>(items[blockIdx.x], smem, entries, grids, ktab, os, vis_out, T, ntx, N);
+  degrid_sep_item<R, S>(items[blockIdx.x], smem, entries, grids, ktab, os, vis_out, T, ntx, N, perm_n1);
 }
 
 template <typename R, int S>
@@ -600,7 +604,7 @@ __global__ __launch_bounds__(degrid_threads<R>()) void k_degrid_sep_batch(const 
   const int b = blockIdx.x;
   const BatchDesc& d = descs[find_desc(descs, nd, b, true)];
   degrid_sep_item<R, S>(d.ditems[b - d.dbase], smem, d.entries, (const cplx<R>*)d.grids, (const R*)d.ktab, d.os,
-                        (cplx<R>*)d.vis_out, d.T, d.ntx, d.N);
+                        (cplx<R>*)d.vis_out, d.T, d.ntx, d.N, d.perm_n1);
 }
 
 // DPP moves (0 where a row is masked off) of a 32-bit or 64-bit value within the wavefront.
@@ -644,14 +648,14 @@ __global__ __launch_bounds__(DEGRID_TAB_THREADS) void k_degrid_tab(const int4* _
                                                                   const cplx<R>* __restrict__ grids,
                                                                   const cplx<R>* __restrict__ tab, int os, int S,
                                                                   cplx<R>* __restrict__ vis_out, int T, int ntx,
-                                                                  int N) {
+                                                                  int N, int perm_n1) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int E = 64 / L;  // entries per wavefront step
   const int Wd = T + S;
   R* lds = reinterpret_cast<R*>(smem);
   const int4 it = items[blockIdx.x];
   const TilePos tp = tile_pos(it.x, ntx);
-  load_halo<R, 32>(grids, lds, Wd, T, N, tp);
+  load_halo<R, 32>(grids, lds, Wd, T, N, tp, perm_n1);
   __syncthreads();
   const int lane = __lane_id(), wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
   const int slot = lane / L, a = lane % L;
@@ -787,6 +791,12 @@ static int launch_grid_sep(const arlmi_plan* p, const void* vis, const void* wt,
   return ARLMI_OK;
 }
 
+// Row order of the grids a plan's degridder reads (what arlmi_image_to_grids leaves for it).
+template <typename R>
+static int degrid_perm(const arlmi_plan* p) {
+  return (p->flags & ARLMI_FLAG_PRIVATE_GRIDS) ? grid_perm_n1(p->g.n, sizeof(R) == 4 ? 32 : 64) : 0;
+}
+
 template <typename R, int S>
 static int launch_degrid_sep(const arlmi_plan* p, const void* grids, const void* ktab, void* vis_out, hipStream_t s) {
   const DevGeom& g = p->g;
@@ -799,7 +809,8 @@ static int launch_degrid_sep(const arlmi_plan* p, const void* grids, const void*
   if (p->nditems == 0) return ARLMI_OK;
   ARLMI_LAUNCH("k_degrid_sep", (k_degrid_sep<R, S>), dim3((unsigned)p->nditems), dim3(degrid_threads<R>()), sh, s,
                p->d_ditems,
-               p->d_entries, (const cplx<R>*)grids, (const R*)ktab, g.os, (cplx<R>*)vis_out, g.tile, g.ntx, g.n);
+               p->d_entries, (const cplx<R>*)grids, (const R*)ktab, g.os, (cplx<R>*)vis_out, g.tile, g.ntx, g.n,
+               degrid_perm<R>(p));
   ARLMI_LAUNCH_CHECK();
   return ARLMI_OK;
 }
@@ -906,7 +917,7 @@ static int degrid_impl(const arlmi_plan* p, const void* grids, const arlmi_kerne
 #define ARLMI_DEGRID_TAB(L)                                                                                       \
   ARLMI_LAUNCH("k_degrid_tab", (k_degrid_tab<R, L>), dim3((unsigned)p->nditems), dim3(DEGRID_TAB_THREADS), sh, s, \
                p->d_ditems, p->d_entries, ez, (const cplx<R>*)grids, tab, g.os, g.support, (cplx<R>*)vis_out,    \
-               g.tile, g.ntx, g.n)
+               g.tile, g.ntx, g.n, degrid_perm<R>(p))
   const int S = g.support;
   if (S <= 2)
     ARLMI_DEGRID_TAB(2);
@@ -1042,6 +1053,7 @@ int batch_build(arlmi_batch* b, const arlmi_batch_item* in, int n) {
     d.ngitems = (int)p->ngitems;
     d.dbase = (int)grp.dblocks;
     d.nditems = (int)p->nditems;
+    d.perm_n1 = (p->flags & ARLMI_FLAG_PRIVATE_GRIDS) ? grid_perm_n1(g.n, sizeof(R) == 4 ? 32 : 64) : 0;
     grp.gblocks += p->ngitems;
     grp.dblocks += p->nditems;
     if (grp.gblocks > 0x7fffffff || grp.dblocks > 0x7fffffff) {

@@ -1099,8 +1099,8 @@ def bench_stream(args):
         torch.cuda.synchronize()
         L = arlmi._lib.lib()
         L.arlmi_profile_reset()
-        L.arlmi_profile_only(("k_grid_sep,k_degrid_sep,k_cols_plain,k_rows_invert,k_rows_predict,k_rows_plain,"
-                              "k_transpose,k_p1_scatter,k_p2_scatter").encode())
+        L.arlmi_profile_only(("k_grid_sep,k_degrid_sep,k_cols_plain,k_cols4_a,k_cols4_b,k_rows_invert,k_rows_predict,"
+                              "k_rows_plain,k_transpose,k_p1_scatter,k_p2_count,k_p2_scatter").encode())
         L.arlmi_profile_enable(1)
         torch.cuda.synchronize()
         t1 = time.perf_counter()

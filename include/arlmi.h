@@ -60,6 +60,12 @@ extern "C" {
  * in ARL): the gridding work items cover only tiles that rows touch.  For chunks added into grids
  * the caller zeroed once (arlmi_grid_add); the FFT-side calls read and write the grids densely. */
 #define ARLMI_FLAG_OCCUPIED_TILES 4u
+/* The plan's grids pass only between its own calls (no counterpart in ARL): arlmi_image_to_grids
+ * may then leave the grid rows in the order of the library's two-level column transform (at 8192
+ * in single and 4096 / 8192 in double precision, where one grid column and the twiddles fill the
+ * LDS), which arlmi_degrid on the same plan reads.  The grids of such a plan are not in ARL's
+ * layout between those two calls. */
+#define ARLMI_FLAG_PRIVATE_GRIDS 8u
 
 #define ARLMI_KERNEL_SEPARABLE 0 /* real separable table k[os][S]: CF[yo][xo][a][b] = k[yo][a]*k[xo][b] */
 #define ARLMI_KERNEL_TABLE 1     /* full complex table CF[nz][os][os][S][S] (ARL layout, chan 0, pol 0) */
