@@ -28,6 +28,7 @@ FLAG_DROP_OFF_GRID = 1
 FLAG_SPARSE_GRIDS = 2
 FLAG_OCCUPIED_TILES = 4
 FLAG_PRIVATE_GRIDS = 8
+FLAG_DETERMINISTIC = 16
 KERNEL_SEPARABLE = 0
 KERNEL_TABLE = 1
 

@@ -126,10 +126,12 @@ class Imager:
       mode "2d":     planes are the grid w planes (one by default), as invert_2d / predict_2d;
       mode "wstack": vis_slices w slabs with ARL's slab rule, screens and slab-mean w shift,
                      as the w-stack list workflows (imaging_serial.py:41-216).
+    deterministic=True grids in 64-bit fixed point: repeated inverts of the same rows give
+    bitwise-identical images and sums of weights (predict is deterministic in either mode).
     """
 
     def __init__(self, vis, im, gcfcf=None, mode="2d", vis_slices=1, slab_boxes=None, tile=0,
-                 drop_off_grid=False, sparse=True, chunk=0, **kwargs):
+                 drop_off_grid=False, sparse=True, chunk=0, deterministic=False, **kwargs):
         self.gcf, self.cf = _gcfcf(im, gcfcf, kwargs)
         self.im = im
         self.precision = vis.precision
@@ -137,10 +139,11 @@ class Imager:
         # neither written nor read)
         if mode == "2d":
             self.plan = Plan(vis, im, self.cf, _lib.MODE_2D, 1, tile=tile, drop_off_grid=drop_off_grid,
-                             sparse=sparse, chunk=chunk, private=True)
+                             sparse=sparse, chunk=chunk, private=True, deterministic=deterministic)
         elif mode == "wstack":
             self.plan = Plan(vis, im, self.cf, _lib.MODE_WSTACK, vis_slices, tile=tile, drop_off_grid=drop_off_grid,
-                             slab_boxes=slab_boxes, sparse=sparse, chunk=chunk, private=True)
+                             slab_boxes=slab_boxes, sparse=sparse, chunk=chunk, private=True,
+                             deterministic=deterministic)
         else:
             raise ValueError("mode must be '2d' or 'wstack'")
         self.mode = mode

@@ -104,12 +104,13 @@ class Plan:
     sparse: grids hold only the tiles rows touch (ARLMI_FLAG_SPARSE_GRIDS); private: grids pass only
     between this plan's calls, so predict may leave them in the library's own row order
     (ARLMI_FLAG_PRIVATE_GRIDS); both for grids that only pass between grid, grids_to_image,
-    image_to_grids and degrid.
+    image_to_grids and degrid.  deterministic: grids and sums of weights accumulated in 64-bit fixed
+    point, bitwise identical between repeated calls (ARLMI_FLAG_DETERMINISTIC).
     """
 
     def __init__(self, vis, im, cf, mode=_lib.MODE_2D, nplanes=1, nw_grid=1, wstep_grid=1e15, tile=0,
                  drop_off_grid=False, slab_boxes=None, use_freq=None, sparse=False, chunk=0, occupied_only=False,
-                 private=False):
+                 private=False, deterministic=False):
         self.L = _lib.lib()
         self.mode = mode
         self.nplanes = nplanes
@@ -117,7 +118,8 @@ class Plan:
             tile = table_tile(im.data.shape[-1], cf.support, 32 if vis.data["vis"].dtype == torch.complex64 else 64)
         self.geom = make_geom(im, cf, mode, nplanes, nw_grid=nw_grid, wstep_grid=wstep_grid, tile=tile)
         self.flags = ((_lib.FLAG_DROP_OFF_GRID if drop_off_grid else 0) | (_lib.FLAG_SPARSE_GRIDS if sparse else 0) |
-                      (_lib.FLAG_OCCUPIED_TILES if occupied_only else 0) | (_lib.FLAG_PRIVATE_GRIDS if private else 0))
+                      (_lib.FLAG_OCCUPIED_TILES if occupied_only else 0) | (_lib.FLAG_PRIVATE_GRIDS if private else 0) |
+                      (_lib.FLAG_DETERMINISTIC if deterministic else 0))
         self.handle = ctypes.c_void_p()
         self.npixel = self.geom.npixel
         self.nchan = self.geom.nchan

@@ -35,11 +35,12 @@ class StreamingImager:
     chunks: a list of Visibility objects, or a callable returning an iterator over them (called
     once per pass, so chunks may be produced on the fly).  mode "wstack" (vis_slices slabs, ARL's
     slab rule over the whole set) or "2d".  The image is centred on the visibilities' phase
-    centre (no image-centre shift).
+    centre (no image-centre shift).  deterministic=True grids every chunk in 64-bit fixed point, so
+    that repeated passes give bitwise-identical images.
     """
 
     def __init__(self, im, gcfcf=None, mode="wstack", vis_slices=1, precision=32, tile=0, chunk=0, keep_plans=False,
-                 **kwargs):
+                 deterministic=False, **kwargs):
         if mode not in ("2d", "wstack"):
             raise ValueError("mode must be '2d' or 'wstack'")
         self.gcf, self.cf = _gcfcf(im, gcfcf, kwargs)
@@ -55,6 +56,8 @@ class StreamingImager:
         # keep_plans: one plan per chunk, kept between calls (invert rebuilds them for the chunks'
         # uvw, predict reuses them), at the cost of every chunk's plan buffers
         self.keep_plans = keep_plans
+        # deterministic: every chunk gridded in 64-bit fixed point (bitwise-reproducible passes)
+        self.deterministic = deterministic
         self.plans = {}
         self.grids = None
         self.boxes = None
@@ -96,7 +99,8 @@ class StreamingImager:
         # dense grids; gridding work items only for the tiles a chunk touches (the grids are zeroed
         # once per pass, and every chunk adds into them)
         plan = Plan(vis, self.im, self.cf, mode, self.vis_slices, tile=self.tile, slab_boxes=self.boxes,
-                    sparse=False, chunk=self.chunk, occupied_only=True, private=True)
+                    sparse=False, chunk=self.chunk, occupied_only=True, private=True,
+                    deterministic=self.deterministic)
         if self.mode == "wstack":
             _lib.check(_lib.lib().arlmi_plan_fix_plane_w(
                 plan.handle, self.plane_w.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(self.plane_w)))

@@ -180,6 +180,39 @@ __host__ __device__ inline int perm_row_of(int v, int n, int n1) {
   return n1 * (v % n2) + v / n2;
 }
 
+// Fixed-point accumulation of ARLMI_FLAG_DETERMINISTIC plans: d_det holds [0..7] the call's
+// constants (2^qw and 2^-qw of the sums of weights, the single-tap bound X), then [nsw] 64-bit
+// integer sums of weights, then DET_BLOCKS x 4 bound partials of the call.
+constexpr int DET_BLOCKS = 512;
+inline int64_t det_words(int64_t nsw) { return 8 + nsw + 4 * (int64_t)DET_BLOCKS; }
+
+// round(x * scale) as a 64-bit integer, for a power-of-two scale and |x * scale| <= 2^51: the
+// fused multiply-add with 1.5 * 2^52 lands in [2^52, 2^53), where the doubles are the integers, and
+// its bit pattern minus that of 1.5 * 2^52 is the rounded integer (ties to even).  Integer sums of
+// such values are exact in any order.
+constexpr long long DET_MAGIC_BITS = 0x4338000000000000ll;  // the bits of 1.5 * 2^52
+__device__ __forceinline__ long long det_round(double x, double scale) {
+  return __double_as_longlong(__builtin_fma(x, scale, 0x1.8p52)) - DET_MAGIC_BITS;
+}
+
+// Exponent e with y < 2^e for y > 0 finite; -1000 for 0.
+__device__ inline int det_exp(double y) {
+  if (!(y > 0.0)) return -1000;
+  int e;
+  (void)frexp(y, &e);
+  return e;
+}
+
+// The scale exponent q for integer sums of terms |x| <= bound_one whose sums stay within
+// bound_sum: bound_sum 2^q < 2^60 (64-bit sums cannot overflow) and bound_one 2^q < 2^50 (inside
+// det_round's range); 0 when a bound is not finite.
+__device__ inline int det_q(double bound_sum, double bound_one) {
+  if (!isfinite(bound_sum) || !isfinite(bound_one)) return 0;
+  const int a = 60 - det_exp(bound_sum), b = 50 - det_exp(bound_one);
+  const int q = a < b ? a : b;
+  return q < -1000 ? -1000 : (q > 1000 ? 1000 : q);
+}
+
 // Opt-in kernel timing (profile.hip).
 bool profile_on();
 void* profile_begin(const char* name, hipStream_t s);
@@ -236,6 +269,8 @@ struct arlmi_plan {
   int4* d_multi = nullptr;               // {tile, first slot, slots, 0} for split tiles
   void* d_partial = nullptr;             // partial tiles of split tiles (grid precision)
   void* d_img_part = nullptr;            // per-plane-group partial images of the invert row pass
+  double* d_det = nullptr;               // ARLMI_FLAG_DETERMINISTIC: scales, integer sums of weights,
+                                         // bound partials (det_words)
   int64_t cap_img_part_bytes = 0;
   int64_t cap_h1 = 0, cap_s1 = 0, cap_h2 = 0, cap_s2 = 0, cap_scan = 0, cap_recs = 0, cap_blk2 = 0, cap_entries = 0, cap_z = 0,
           cap_gitems = 0, cap_ditems = 0, cap_partial_bytes = 0;

@@ -20,11 +20,6 @@
 namespace arlmi {
 
 template <typename R>
-__device__ inline void lds_add(R* p, R v) {
-  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-
-template <typename R>
 __device__ inline cplx<R> mk(R x, R y) {
   cplx<R> c;
   c.x = x;
@@ -61,6 +56,128 @@ __device__ inline void block_add_double(double v, double* dst, double* scratch) 
     if (t != 0.0) unsafeAtomicAdd(dst, t);
   }
 }
+
+// ---------------------------------------------------------------------------
+// fixed-point accumulation (ARLMI_FLAG_DETERMINISTIC)
+//
+// A tap's product x becomes the integer round(x 2^q) (det_round); the LDS tiles, partial tiles
+// and sums of weights then add 64-bit integers, exact in any order.  q is chosen per tile and
+// call: q = det_q(X n_t, X), where X = Kmax * M bounds every single tap and X n_t every grid value
+// of tile t and every partial sum of one (n_t entries of the tile), with M the largest
+// m = |Re vis w| + |Im vis w| (w for a PSF) over the rows and Kmax = (max |k|)^2 over a separable
+// table or max (|Re c| + |Im c|) over a full one.  M and Kmax are maxima and n_t an integer count,
+// so q is the same on every build of the plan whatever the order of its entries.  The sums of
+// weights take qw from sum |w| and max |w| over the rows.
+// ---------------------------------------------------------------------------
+// One accumulator add: a float64 LDS add, or (DET) the fixed-point integer add.
+template <bool DET>
+__device__ __forceinline__ void acc_add(double* p, double v, double scale) {
+  if constexpr (DET) {
+    __hip_atomic_fetch_add(reinterpret_cast<long long*>(p), det_round(v, scale), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
+  } else {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+}
+
+__device__ inline void block_add_i64(long long v, unsigned long long* dst, long long* scratch) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  const int wid = threadIdx.x >> 6;
+  if (__lane_id() == 0) scratch[wid] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long t = 0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += scratch[i];
+    if (t != 0) atomicAdd(dst, (unsigned long long)t);
+  }
+}
+
+// Sum (op 0) or maximum (op 1) over the block in a fixed order; every thread gets it.
+template <int OP>
+__device__ inline double block_fold(double v, double* scratch) {
+  for (int o = 32; o > 0; o >>= 1) {
+    const double u = __shfl_xor(v, o);
+    v = OP == 0 ? v + u : fmax(v, u);
+  }
+  __syncthreads();
+  if (__lane_id() == 0) scratch[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = scratch[0];
+  for (int i = 1; i < (int)(blockDim.x >> 6); ++i) t = OP == 0 ? t + scratch[i] : fmax(t, scratch[i]);
+  return t;
+}
+
+// Per-block partials of one call (DET_BLOCKS blocks of 256 threads, rows and kernel taps in grid
+// stride): sum |w_r|, max |w_r|, max m_r, max tap magnitude, at part[k * DET_BLOCKS + b].
+template <typename R>
+__global__ __launch_bounds__(256) void k_det_bounds(const cplx<R>* __restrict__ vis, const R* __restrict__ wt,
+                                                    int64_t nrows, int dopsf, const R* __restrict__ ktab,
+                                                    int64_t nk, int complex_tab, double* __restrict__ part) {
+  __shared__ double scratch[4];
+  double sw = 0.0, mw = 0.0, mm = 0.0, km = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (int64_t r = t0; r < nrows; r += stride) {
+    const R w = wt[r];
+    const double aw = fabs((double)w);
+    sw += aw;
+    mw = fmax(mw, aw);
+    if (!dopsf) {
+      const cplx<R> v = vis[r];
+      const R vr = v.x * w, vi = v.y * w;  // as the gridders form them
+      mm = fmax(mm, fabs((double)vr) + fabs((double)vi));
+    }
+  }
+  if (dopsf) mm = mw;
+  for (int64_t i = t0; i < nk; i += stride) {
+    if (complex_tab) {
+      const cplx<R> c = reinterpret_cast<const cplx<R>*>(ktab)[i];
+      km = fmax(km, fabs((double)c.x) + fabs((double)c.y));
+    } else {
+      km = fmax(km, fabs((double)ktab[i]));
+    }
+  }
+  const double v[4] = {block_fold<0>(sw, scratch), block_fold<1>(mw, scratch), block_fold<1>(mm, scratch),
+                       block_fold<1>(km, scratch)};
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 4; ++k) part[(size_t)k * DET_BLOCKS + blockIdx.x] = v[k];
+}
+
+// One block of DET_BLOCKS threads: folds the partials in a fixed order into det[0] = 2^qw,
+// det[1] = 2^-qw, det[2] = X = Kmax * M, and zeroes the integer sums of weights det[8 .. 8 + nsw).
+__global__ __launch_bounds__(DET_BLOCKS) void k_det_scale(double* __restrict__ det, int64_t nsw, int complex_tab) {
+  __shared__ double scratch[DET_BLOCKS / 64];
+  const double* part = det + 8 + nsw;
+  const int t = threadIdx.x;
+  const double sw = block_fold<0>(part[t], scratch), mw = block_fold<1>(part[DET_BLOCKS + t], scratch),
+               mm = block_fold<1>(part[2 * DET_BLOCKS + t], scratch),
+               km = block_fold<1>(part[3 * DET_BLOCKS + t], scratch);
+  if (t == 0) {
+    const int qw = det_q(sw, mw);
+    det[0] = ldexp(1.0, qw);
+    det[1] = ldexp(1.0, -qw);
+    det[2] = (complex_tab ? km : km * km) * mm;
+  }
+  long long* swi = reinterpret_cast<long long*>(det + 8);
+  for (int64_t i = t; i < nsw; i += blockDim.x) swi[i] = 0;
+}
+
+// The scale exponent of tile t (every item of a tile, and its partial-tile sum, use the same).
+__device__ inline int det_tile_q(const double* __restrict__ det, const uint32_t* __restrict__ tcnt, int t) {
+  return det_q(det[2] * (double)tcnt[t], det[2]);
+}
+
+// sumwt[i] += the integer sum of weights i times 2^-qw.
+__global__ void k_det_sumwt(const double* __restrict__ det, double* __restrict__ sumwt, int64_t nsw) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < nsw) sumwt[i] += (double)reinterpret_cast<const long long*>(det + 8)[i] * det[1];
+}
+
+// The state of a fixed-point call the gridders read.
+struct DetArgs {
+  const double* det;     // call constants (det[0..7]) and integer sums of weights
+  const uint32_t* tcnt;  // [ntiles] entries per tile
+};
 
 // ---------------------------------------------------------------------------
 // gridding, separable real kernel (PSWF), compile-time support
@@ -118,8 +235,32 @@ __device__ inline void store_tile(const double* acc, int WS, int T, int N, const
   }
 }
 
-__device__ inline void lds_add_f64(double* p, double v) {
-  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+// store_tile for fixed-point accumulators: values 2^-q times the integers; partial tiles keep
+// the integers (summed exactly by k_reduce_tiles_det).
+template <typename R>
+__device__ inline void store_tile_det(const long long* acc, int WS, int T, int N, const TilePos& tp, int slot,
+                                      cplx<R>* __restrict__ grids, longlong2* __restrict__ partial, int im_off,
+                                      int accum, double inv_scale) {
+  cplx<R>* g = grids + (size_t)tp.plane * N * N + (size_t)(tp.ty * T) * N + (size_t)tp.tx * T;
+  longlong2* d = partial + (size_t)(slot > 0 ? slot - 1 : 0) * T * T;
+  for (int i = threadIdx.x; i < T * T; i += blockDim.x) {
+    const int y = i / T, x = i - (i / T) * T;
+    const int o = y * WS + x;
+    if (slot == 0) {
+      double re = (double)acc[o] * inv_scale, im = (double)acc[im_off + o] * inv_scale;
+      if (accum) {
+        const cplx<R> old = g[(size_t)y * N + x];
+        re += (double)old.x;
+        im += (double)old.y;
+      }
+      g[(size_t)y * N + x] = mk<R>((R)re, (R)im);
+    } else {
+      longlong2 v;
+      v.x = acc[o];
+      v.y = acc[im_off + o];
+      d[i] = v;
+    }
+  }
 }
 
 // Sums the partial tiles of every split tile, in slot order, into the grid.
@@ -167,6 +308,37 @@ __global__ void k_reduce_tiles(const int4* __restrict__ multi, const uint32_t* _
   }
 }
 
+// k_reduce_tiles for fixed-point partial tiles: integer sums, then 2^-q times the total.
+template <typename R>
+__global__ void k_reduce_tiles_det(const int4* __restrict__ multi, const uint32_t* __restrict__ nmulti,
+                                   const longlong2* __restrict__ partial, cplx<R>* __restrict__ grids, int T, int ntx,
+                                   int N, int accum, const DetArgs da) {
+  const int i = blockIdx.y * blockDim.x + threadIdx.x;
+  if (i >= T * T) return;
+  const uint32_t count = *nmulti;
+  for (uint32_t mi = blockIdx.x; mi < count; mi += gridDim.x) {
+    const int4 m = multi[mi];
+    const double inv_scale = ldexp(1.0, -det_tile_q(da.det, da.tcnt, m.x));
+    const TilePos tp = tile_pos(m.x, ntx);
+    const size_t tt = (size_t)T * T;
+    const longlong2* src = partial + (size_t)m.y * tt + i;
+    long long re = 0, im = 0;
+    for (int k = 0; k < m.z; ++k) {
+      const longlong2 a = src[(size_t)k * tt];
+      re += a.x;
+      im += a.y;
+    }
+    cplx<R>* dst = grids + (size_t)tp.plane * N * N + (size_t)(tp.ty * T + i / T) * N + (size_t)tp.tx * T + (i % T);
+    double r = (double)re * inv_scale, q = (double)im * inv_scale;
+    if (accum) {
+      const cplx<R> old = *dst;
+      r += (double)old.x;
+      q += (double)old.y;
+    }
+    *dst = mk<R>((R)r, (R)q);
+  }
+}
+
 // Per-entry values a wavefront broadcasts while it walks its 64 entries:
 //   sa: footprint origin as an accumulator index, cy*WS + cx + SA_BIAS, and bit 31 set when
 //       the whole S x S footprint lies inside the tile (no masking needed);
@@ -190,10 +362,10 @@ __device__ __forceinline__ void sep_taps(const R* lk, uint32_t ko, const int* ta
 // of an edge footprint outside the tile add 0 to this lane's own slot of the trash row:
 // both paths issue the same two LDS adds per tap, so the wait counts the compiler
 // derives for the software-pipelined kernel reads stay exact.
-template <typename R, int S, int TPL>
+template <typename R, int S, int TPL, bool DET>
 __device__ __forceinline__ void sep_add(double* acc, int T, int TW, int lane, uint32_t sa, uint32_t pk,
                                         const int* ta, const int* tb, const int* loff, const R* ky, const R* kx,
-                                        R jr, R ji) {
+                                        R jr, R ji, double scale) {
   constexpr int TAPS = S * S;
   const int sb = (int)(sa & ~SA_INNER) - SA_BIAS;
   if (sa & SA_INNER) {
@@ -202,8 +374,8 @@ __device__ __forceinline__ void sep_add(double* acc, int T, int TW, int lane, ui
       const bool ok = TAPS % 64 == 0 || lane + 64 * g < TAPS;
       const R c = ok ? ky[g] * kx[g] : R(0);
       const int o = ok ? sb + loff[g] : TW + lane;
-      lds_add_f64(acc + o, (double)(c * jr));
-      lds_add_f64(acc + (TW + 64) + o, (double)(c * ji));
+      acc_add<DET>(acc + o, (double)(c * jr), scale);
+      acc_add<DET>(acc + (TW + 64) + o, (double)(c * ji), scale);
     }
   } else {
     const int cx = entry_lx(pk) - S, cy = entry_ly(pk) - S;
@@ -214,19 +386,22 @@ __device__ __forceinline__ void sep_add(double* acc, int T, int TW, int lane, ui
           (TAPS % 64 == 0 || lane + 64 * g < TAPS) && (unsigned)x < (unsigned)T && (unsigned)y < (unsigned)T;
       const R c = ok ? ky[g] * kx[g] : R(0);
       const int o = ok ? sb + loff[g] : TW + lane;
-      lds_add_f64(acc + o, (double)(c * jr));
-      lds_add_f64(acc + (TW + 64) + o, (double)(c * ji));
+      acc_add<DET>(acc + o, (double)(c * jr), scale);
+      acc_add<DET>(acc + (TW + 64) + o, (double)(c * ji), scale);
     }
   }
 }
 
 // One work item of the separable gridder (the body of k_grid_sep and of its batched form).
-template <typename R, int S>
+// DET: fixed-point accumulation with the scales of det (ARLMI_FLAG_DETERMINISTIC); partial then
+// holds integer tiles (longlong2) and the sums of weights go to det's integer sums.
+template <typename R, int S, bool DET>
 __device__ __forceinline__ void grid_sep_item(const int4 it, unsigned char* smem, double* red,
                                               const uint2* __restrict__ entries, const cplx<R>* __restrict__ vis,
                                               const R* __restrict__ wt, int dopsf, const R* __restrict__ ktab, int os,
                                               cplx<R>* __restrict__ grids, cplx<R>* __restrict__ partial,
-                                              double* __restrict__ sumwt, int T, int ntx, int N, int accum) {
+                                              double* __restrict__ sumwt, int T, int ntx, int N, int accum,
+                                              const DetArgs da) {
   constexpr int TAPS = S * S;
   constexpr int TPL = (TAPS + 63) / 64;  // taps per lane
   const int WS = lds_stride(T);
@@ -246,7 +421,15 @@ __device__ __forceinline__ void grid_sep_item(const int4 it, unsigned char* smem
     tb[g] = t < TAPS ? t % S : 0;
     loff[g] = ta[g] * WS + tb[g];
   }
+  double scale = 0.0, inv_scale = 0.0, wscale = 0.0;
+  if constexpr (DET) {
+    const int q = det_tile_q(da.det, da.tcnt, it.x);
+    scale = ldexp(1.0, q);
+    inv_scale = ldexp(1.0, -q);
+    wscale = da.det[0];
+  }
   double swt = 0.0;
+  long long swi = 0;
   // each wave takes one contiguous share of the item's entries, so every wave of the group
   // has work even for small items
   const int per = __builtin_amdgcn_readfirstlane((it.z + nwave - 1) / nwave);
@@ -266,7 +449,10 @@ __device__ __forceinline__ void grid_sep_item(const int4 it, unsigned char* smem
         vr = v.x * w;
         vi = v.y * w;
       }
-      if (pk & ENTRY_START) swt += (double)w;
+      if (pk & ENTRY_START) {
+        if constexpr (DET) swi += det_round((double)w, wscale);
+        else swt += (double)w;
+      }
     }
     const int cx = entry_lx(pk) - S, cy = entry_ly(pk) - S;
     const bool inner = cx >= 0 && cy >= 0 && cx + S <= T && cy + S <= T;
@@ -282,39 +468,46 @@ __device__ __forceinline__ void grid_sep_item(const int4 it, unsigned char* smem
     sep_taps<R, TPL>(lk, bcast(kov, 0), ta, tb, kya, kxa);
 #pragma unroll
     for (int g = 0; g < TPL; ++g) {
-      lds_add_f64(acc + TW + lane, 0.0);
-      lds_add_f64(acc + (TW + 64) + TW + lane, 0.0);
+      acc_add<DET>(acc + TW + lane, 0.0, scale);
+      acc_add<DET>(acc + (TW + 64) + TW + lane, 0.0, scale);
     }
     sep_taps<R, TPL>(lk, bcast(kov, 1), ta, tb, kyb, kxb);
     for (int j = 0;; j += 2) {
-      sep_add<R, S, TPL>(acc, T, TW, lane, bcast(sav, j), bcast(pk, j), ta, tb, loff, kya, kxa, bcast(vr, j),
-                         bcast(vi, j));
+      sep_add<R, S, TPL, DET>(acc, T, TW, lane, bcast(sav, j), bcast(pk, j), ta, tb, loff, kya, kxa, bcast(vr, j),
+                              bcast(vi, j), scale);
       if (j + 1 >= cnt) break;
       sep_taps<R, TPL>(lk, bcast(kov, (j + 2) & 63), ta, tb, kya, kxa);
-      sep_add<R, S, TPL>(acc, T, TW, lane, bcast(sav, j + 1), bcast(pk, j + 1), ta, tb, loff, kyb, kxb,
-                         bcast(vr, j + 1), bcast(vi, j + 1));
+      sep_add<R, S, TPL, DET>(acc, T, TW, lane, bcast(sav, j + 1), bcast(pk, j + 1), ta, tb, loff, kyb, kxb,
+                              bcast(vr, j + 1), bcast(vi, j + 1), scale);
       if (j + 2 >= cnt) break;
       sep_taps<R, TPL>(lk, bcast(kov, (j + 3) & 63), ta, tb, kyb, kxb);
     }
   }
   __syncthreads();
   const TilePos tp = tile_pos(it.x, ntx);
-  store_tile<R>(acc, WS, T, N, tp, it.w, grids, partial, TW + 64, accum);
-  block_add_double(swt, &sumwt[tp.plane], red);
+  if constexpr (DET) {
+    store_tile_det<R>(reinterpret_cast<const long long*>(acc), WS, T, N, tp, it.w, grids,
+                      reinterpret_cast<longlong2*>(partial), TW + 64, accum, inv_scale);
+    block_add_i64(swi, reinterpret_cast<unsigned long long*>(const_cast<double*>(da.det) + 8) + tp.plane,
+                  reinterpret_cast<long long*>(red));
+  } else {
+    store_tile<R>(acc, WS, T, N, tp, it.w, grids, partial, TW + 64, accum);
+    block_add_double(swt, &sumwt[tp.plane], red);
+  }
 }
 
-template <typename R, int S>
+template <typename R, int S, bool DET>
 __global__ __launch_bounds__(GRID_THREADS) void k_grid_sep(const int4* __restrict__ items,
                                                            const uint2* __restrict__ entries,
                                                            const cplx<R>* __restrict__ vis, const R* __restrict__ wt,
                                                            int dopsf, const R* __restrict__ ktab, int os,
                                                            cplx<R>* __restrict__ grids, cplx<R>* __restrict__ partial,
                                                            double* __restrict__ sumwt, int T, int ntx, int N,
-                                                           int accum) {
+                                                           int accum, const DetArgs da) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double red[GRID_THREADS / 64];
-  grid_sep_item<R, S>(items[blockIdx.x], smem, red, entries, vis, wt, dopsf, ktab, os, grids, partial, sumwt, T, ntx,
-                      N, accum);
+  grid_sep_item<R, S, DET>(items[blockIdx.x], smem, red, entries, vis, wt, dopsf, ktab, os, grids, partial, sumwt, T,
+                           ntx, N, accum, da);
 }
 
 // Per-plan arguments of the batched separable kernels: the plan's work items are blocks
@@ -360,8 +553,9 @@ __global__ __launch_bounds__(GRID_THREADS) void k_grid_sep_batch(const BatchDesc
   __shared__ double red[GRID_THREADS / 64];
   const int b = blockIdx.x;
   const BatchDesc& d = descs[find_desc(descs, nd, b, false)];
-  grid_sep_item<R, S>(d.gitems[b - d.gbase], smem, red, d.entries, (const cplx<R>*)d.vis, (const R*)d.wt, dopsf,
-                      (const R*)d.ktab, d.os, (cplx<R>*)d.grids, (cplx<R>*)d.partial, d.sumwt, d.T, d.ntx, d.N, 0);
+  grid_sep_item<R, S, false>(d.gitems[b - d.gbase], smem, red, d.entries, (const cplx<R>*)d.vis, (const R*)d.wt,
+                             dopsf, (const R*)d.ktab, d.os, (cplx<R>*)d.grids, (cplx<R>*)d.partial, d.sumwt, d.T,
+                             d.ntx, d.N, 0, DetArgs{});
 }
 
 // ---------------------------------------------------------------------------
@@ -382,10 +576,10 @@ __device__ __forceinline__ void tab_taps(const cplx<R>* __restrict__ tab, uint32
 
 // conj(c) * (jr + i ji) for this lane's taps of one entry, into the interleaved accumulators;
 // same masking and trash-row scheme as sep_add.
-template <typename R, int MT>
+template <typename R, int MT, bool DET>
 __device__ __forceinline__ void tab_add(double* acc, int T, int TW, int S, int taps, int lane, uint32_t sa,
                                         uint32_t pk, const int* ta, const int* tb, const int* loff,
-                                        const cplx<R>* k, R jr, R ji) {
+                                        const cplx<R>* k, R jr, R ji, double scale) {
   const int sb = (int)(sa & ~SA_INNER) - SA_BIAS;
   const bool inner = (sa & SA_INNER) != 0;
   const int cx = entry_lx(pk) - S, cy = entry_ly(pk) - S;
@@ -395,15 +589,15 @@ __device__ __forceinline__ void tab_add(double* acc, int T, int TW, int S, int t
     const bool ok = lane + 64 * q < taps && (inner || ((unsigned)x < (unsigned)T && (unsigned)y < (unsigned)T));
     const int o = ok ? sb + loff[q] : TW + lane;
     const R cr = ok ? k[q].x : R(0), ci = ok ? k[q].y : R(0);
-    lds_add_f64(acc + o, (double)(cr * jr + ci * ji));
-    lds_add_f64(acc + (TW + 64) + o, (double)(cr * ji - ci * jr));
+    acc_add<DET>(acc + o, (double)(cr * jr + ci * ji), scale);
+    acc_add<DET>(acc + (TW + 64) + o, (double)(cr * ji - ci * jr), scale);
   }
 }
 
 // Gridding with a full complex kernel table (AW / box / any CF).  MT = taps per lane held in
 // registers (S*S <= 64*MT); the next entry's kernel values are loaded from L2 while the current
 // entry's adds run.  MT = 0: any support, taps loaded and added one group of 64 at a time.
-template <typename R, int MT>
+template <typename R, int MT, bool DET>
 __global__ __launch_bounds__(GRID_THREADS) void k_grid_tab(const int4* __restrict__ items,
                                                            const uint2* __restrict__ entries,
                                                            const uint16_t* __restrict__ entry_z,
@@ -411,7 +605,7 @@ __global__ __launch_bounds__(GRID_THREADS) void k_grid_tab(const int4* __restric
                                                            int dopsf, const cplx<R>* __restrict__ tab, int os, int S,
                                                            cplx<R>* __restrict__ grids, cplx<R>* __restrict__ partial,
                                                            double* __restrict__ sumwt, int T, int ntx, int N,
-                                                           int accum) {
+                                                           int accum, const DetArgs da) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double red[GRID_THREADS / 64];
   const int WS = lds_stride(T);
@@ -432,7 +626,15 @@ __global__ __launch_bounds__(GRID_THREADS) void k_grid_tab(const int4* __restric
     tb[q] = t < taps ? t % S : 0;
     loff[q] = ta[q] * WS + tb[q];
   }
+  double scale = 0.0, inv_scale = 0.0, wscale = 0.0;
+  if constexpr (DET) {
+    const int q = det_tile_q(da.det, da.tcnt, it.x);
+    scale = ldexp(1.0, q);
+    inv_scale = ldexp(1.0, -q);
+    wscale = da.det[0];
+  }
   double swt = 0.0;
+  long long swi = 0;
   const int per = __builtin_amdgcn_readfirstlane((it.z + nwave - 1) / nwave);
   const int e0 = min(it.z, wave * per), e1 = min(it.z, e0 + per);
   for (int base = e0; base < e1; base += 64) {
@@ -451,7 +653,10 @@ __global__ __launch_bounds__(GRID_THREADS) void k_grid_tab(const int4* __restric
         vr = v.x * w;
         vi = v.y * w;
       }
-      if (pk & ENTRY_START) swt += (double)w;
+      if (pk & ENTRY_START) {
+        if constexpr (DET) swi += det_round((double)w, wscale);
+        else swt += (double)w;
+      }
     }
     const int cx = entry_lx(pk) - S, cy = entry_ly(pk) - S;
     const bool inner = cx >= 0 && cy >= 0 && cx + S <= T && cy + S <= T;
@@ -463,12 +668,12 @@ __global__ __launch_bounds__(GRID_THREADS) void k_grid_tab(const int4* __restric
       tab_taps<R, MT>(tab, bcast(kbv, 0), taps, lane, ka);
       tab_taps<R, MT>(tab, bcast(kbv, 1), taps, lane, kb);
       for (int j = 0;; j += 2) {
-        tab_add<R, MT>(acc, T, TW, S, taps, lane, bcast(sav, j), bcast(pk, j), ta, tb, loff, ka, bcast(vr, j),
-                       bcast(vi, j));
+        tab_add<R, MT, DET>(acc, T, TW, S, taps, lane, bcast(sav, j), bcast(pk, j), ta, tb, loff, ka,
+                            bcast(vr, j), bcast(vi, j), scale);
         if (j + 1 >= cnt) break;
         tab_taps<R, MT>(tab, bcast(kbv, (j + 2) & 63), taps, lane, ka);
-        tab_add<R, MT>(acc, T, TW, S, taps, lane, bcast(sav, j + 1), bcast(pk, j + 1), ta, tb, loff, kb,
-                       bcast(vr, j + 1), bcast(vi, j + 1));
+        tab_add<R, MT, DET>(acc, T, TW, S, taps, lane, bcast(sav, j + 1), bcast(pk, j + 1), ta, tb, loff, kb,
+                            bcast(vr, j + 1), bcast(vi, j + 1), scale);
         if (j + 2 >= cnt) break;
         tab_taps<R, MT>(tab, bcast(kbv, (j + 3) & 63), taps, lane, kb);
       }
@@ -485,8 +690,8 @@ __global__ __launch_bounds__(GRID_THREADS) void k_grid_tab(const int4* __restric
           if ((unsigned)x < (unsigned)T && (unsigned)y < (unsigned)T) {
             const cplx<R> c = k[t];
             const int o = sb + a * WS + b;
-            lds_add_f64(acc + o, (double)(c.x * jr + c.y * ji));
-            lds_add_f64(acc + (TW + 64) + o, (double)(c.x * ji - c.y * jr));
+            acc_add<DET>(acc + o, (double)(c.x * jr + c.y * ji), scale);
+            acc_add<DET>(acc + (TW + 64) + o, (double)(c.x * ji - c.y * jr), scale);
           }
         }
       }
@@ -494,8 +699,15 @@ __global__ __launch_bounds__(GRID_THREADS) void k_grid_tab(const int4* __restric
   }
   __syncthreads();
   const TilePos tp = tile_pos(it.x, ntx);
-  store_tile<R>(acc, WS, T, N, tp, it.w, grids, partial, TW + 64, accum);
-  block_add_double(swt, &sumwt[tp.plane], red);
+  if constexpr (DET) {
+    store_tile_det<R>(reinterpret_cast<const long long*>(acc), WS, T, N, tp, it.w, grids,
+                      reinterpret_cast<longlong2*>(partial), TW + 64, accum, inv_scale);
+    block_add_i64(swi, reinterpret_cast<unsigned long long*>(const_cast<double*>(da.det) + 8) + tp.plane,
+                  reinterpret_cast<long long*>(red));
+  } else {
+    store_tile<R>(acc, WS, T, N, tp, it.w, grids, partial, TW + 64, accum);
+    block_add_double(swt, &sumwt[tp.plane], red);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -763,7 +975,9 @@ static size_t lds_limit() { return 160 * 1024; }
 template <typename R>
 static int partial_buffer(const arlmi_plan* cp, cplx<R>** out) {
   arlmi_plan* p = const_cast<arlmi_plan*>(cp);
-  const int64_t need = std::max<int64_t>(1, p->nslots) * p->g.tile * p->g.tile * (int64_t)sizeof(cplx<R>);
+  // fixed-point plans keep integer partial tiles (two 64-bit words per value)
+  const int64_t bytes = (p->flags & ARLMI_FLAG_DETERMINISTIC) ? (int64_t)sizeof(longlong2) : (int64_t)sizeof(cplx<R>);
+  const int64_t need = std::max<int64_t>(1, p->nslots) * p->g.tile * p->g.tile * bytes;
   if (need > p->cap_partial_bytes || !p->d_partial) {
     if (p->d_partial) ARLMI_HIP(hipFree(p->d_partial));
     p->d_partial = nullptr;
@@ -776,7 +990,8 @@ static int partial_buffer(const arlmi_plan* cp, cplx<R>** out) {
 
 template <typename R, int S>
 static int launch_grid_sep(const arlmi_plan* p, const void* vis, const void* wt, int dopsf, const void* ktab,
-                           void* grids, cplx<R>* partial, double* sumwt, int accum, hipStream_t s) {
+                           void* grids, cplx<R>* partial, double* sumwt, int accum, const DetArgs* det,
+                           hipStream_t s) {
   const DevGeom& g = p->g;
   const size_t sh = sizeof(double) * 2 * ((size_t)g.tile * lds_stride(g.tile) + 64) + sizeof(R) * (size_t)g.os * S;
   if (sh > lds_limit()) {
@@ -784,9 +999,14 @@ static int launch_grid_sep(const arlmi_plan* p, const void* vis, const void* wt,
     return ARLMI_E_UNSUPPORTED;
   }
   if (p->ngitems == 0) return ARLMI_OK;
-  ARLMI_LAUNCH("k_grid_sep", (k_grid_sep<R, S>), dim3((unsigned)p->ngitems), dim3(GRID_THREADS), sh, s, p->d_gitems,
-               p->d_entries, (const cplx<R>*)vis, (const R*)wt, dopsf, (const R*)ktab, g.os, (cplx<R>*)grids, partial,
-               sumwt, g.tile, g.ntx, g.n, accum);
+  if (det)
+    ARLMI_LAUNCH("k_grid_sep", (k_grid_sep<R, S, true>), dim3((unsigned)p->ngitems), dim3(GRID_THREADS), sh, s,
+                 p->d_gitems, p->d_entries, (const cplx<R>*)vis, (const R*)wt, dopsf, (const R*)ktab, g.os,
+                 (cplx<R>*)grids, partial, sumwt, g.tile, g.ntx, g.n, accum, *det);
+  else
+    ARLMI_LAUNCH("k_grid_sep", (k_grid_sep<R, S, false>), dim3((unsigned)p->ngitems), dim3(GRID_THREADS), sh, s,
+                 p->d_gitems, p->d_entries, (const cplx<R>*)vis, (const R*)wt, dopsf, (const R*)ktab, g.os,
+                 (cplx<R>*)grids, partial, sumwt, g.tile, g.ntx, g.n, accum, DetArgs{});
   ARLMI_LAUNCH_CHECK();
   return ARLMI_OK;
 }
@@ -817,17 +1037,18 @@ static int launch_degrid_sep(const arlmi_plan* p, const void* grids, const void*
 
 template <typename R>
 static int grid_launch(const arlmi_plan* p, const void* vis, const void* wt, int dopsf, const arlmi_kernel* k,
-                       void* grids, cplx<R>* partial, double* sumwt, int accum, hipStream_t s) {
+                       void* grids, cplx<R>* partial, double* sumwt, int accum, const DetArgs* det,
+                       hipStream_t s) {
   const DevGeom& g = p->g;
   if (k->kind == ARLMI_KERNEL_SEPARABLE) {
     switch (g.support) {
-      case 2: return launch_grid_sep<R, 2>(p, vis, wt, dopsf, k->d_table, grids, partial, sumwt, accum, s);
-      case 4: return launch_grid_sep<R, 4>(p, vis, wt, dopsf, k->d_table, grids, partial, sumwt, accum, s);
-      case 6: return launch_grid_sep<R, 6>(p, vis, wt, dopsf, k->d_table, grids, partial, sumwt, accum, s);
-      case 8: return launch_grid_sep<R, 8>(p, vis, wt, dopsf, k->d_table, grids, partial, sumwt, accum, s);
-      case 10: return launch_grid_sep<R, 10>(p, vis, wt, dopsf, k->d_table, grids, partial, sumwt, accum, s);
-      case 12: return launch_grid_sep<R, 12>(p, vis, wt, dopsf, k->d_table, grids, partial, sumwt, accum, s);
-      case 16: return launch_grid_sep<R, 16>(p, vis, wt, dopsf, k->d_table, grids, partial, sumwt, accum, s);
+      case 2: return launch_grid_sep<R, 2>(p, vis, wt, dopsf, k->d_table, grids, partial, sumwt, accum, det, s);
+      case 4: return launch_grid_sep<R, 4>(p, vis, wt, dopsf, k->d_table, grids, partial, sumwt, accum, det, s);
+      case 6: return launch_grid_sep<R, 6>(p, vis, wt, dopsf, k->d_table, grids, partial, sumwt, accum, det, s);
+      case 8: return launch_grid_sep<R, 8>(p, vis, wt, dopsf, k->d_table, grids, partial, sumwt, accum, det, s);
+      case 10: return launch_grid_sep<R, 10>(p, vis, wt, dopsf, k->d_table, grids, partial, sumwt, accum, det, s);
+      case 12: return launch_grid_sep<R, 12>(p, vis, wt, dopsf, k->d_table, grids, partial, sumwt, accum, det, s);
+      case 16: return launch_grid_sep<R, 16>(p, vis, wt, dopsf, k->d_table, grids, partial, sumwt, accum, det, s);
       default: break;
     }
     set_error("separable kernel support %d not instantiated; pass the full table", g.support);
@@ -843,9 +1064,16 @@ static int grid_launch(const arlmi_plan* p, const void* vis, const void* wt, int
   const uint16_t* ez = g.nz > 1 ? p->d_entry_z : nullptr;
   const cplx<R>* tab = (const cplx<R>*)k->d_table;
 #define ARLMI_GRID_TAB(MT)                                                                                        \
-  ARLMI_LAUNCH("k_grid_tab", (k_grid_tab<R, MT>), dim3((unsigned)p->ngitems), dim3(GRID_THREADS), sh, s,        \
-               p->d_gitems, p->d_entries, ez, (const cplx<R>*)vis, (const R*)wt, dopsf, tab, g.os, g.support,     \
-               (cplx<R>*)grids, partial, sumwt, g.tile, g.ntx, g.n, accum)
+  do {                                                                                                            \
+    if (det)                                                                                                      \
+      ARLMI_LAUNCH("k_grid_tab", (k_grid_tab<R, MT, true>), dim3((unsigned)p->ngitems), dim3(GRID_THREADS), sh, s, \
+                   p->d_gitems, p->d_entries, ez, (const cplx<R>*)vis, (const R*)wt, dopsf, tab, g.os, g.support, \
+                   (cplx<R>*)grids, partial, sumwt, g.tile, g.ntx, g.n, accum, *det);                             \
+    else                                                                                                          \
+      ARLMI_LAUNCH("k_grid_tab", (k_grid_tab<R, MT, false>), dim3((unsigned)p->ngitems), dim3(GRID_THREADS), sh,  \
+                   s, p->d_gitems, p->d_entries, ez, (const cplx<R>*)vis, (const R*)wt, dopsf, tab, g.os,        \
+                   g.support, (cplx<R>*)grids, partial, sumwt, g.tile, g.ntx, g.n, accum, DetArgs{});             \
+  } while (0)
   // (a variant holding 8 taps per lane gave wrong sums in the single precision build and is
   // not used: supports above 16 take the loop without prefetch)
   if (taps <= 64)
@@ -859,24 +1087,64 @@ static int grid_launch(const arlmi_plan* p, const void* vis, const void* wt, int
   return ARLMI_OK;
 }
 
+// Constants of a fixed-point call (k_det_bounds over the rows and kernel taps, k_det_scale); the
+// integer sums of weights zeroed.
+template <typename R>
+static int det_prepare(const arlmi_plan* p, const void* vis, const void* wt, int dopsf, const arlmi_kernel* k,
+                       int64_t nsw, DetArgs* da, hipStream_t s) {
+  const DevGeom& g = p->g;
+  const int complex_tab = k->kind == ARLMI_KERNEL_TABLE;
+  const int64_t nk = complex_tab ? (int64_t)k->nz * g.os * g.os * g.support * g.support : (int64_t)g.os * g.support;
+  double* part = p->d_det + 8 + nsw;
+  ARLMI_LAUNCH("k_det_bounds", k_det_bounds<R>, dim3(DET_BLOCKS), dim3(256), 0, s, (const cplx<R>*)vis, (const R*)wt,
+               p->nrows, dopsf, (const R*)k->d_table, nk, complex_tab, part);
+  ARLMI_LAUNCH_CHECK();
+  ARLMI_LAUNCH("k_det_scale", k_det_scale, dim3(1), dim3(DET_BLOCKS), 0, s, p->d_det, nsw, complex_tab);
+  ARLMI_LAUNCH_CHECK();
+  da->det = p->d_det;
+  da->tcnt = p->d_cnt;
+  return ARLMI_OK;
+}
+
 template <typename R>
 static int grid_impl(const arlmi_plan* p, const void* vis, const void* wt, int dopsf, const arlmi_kernel* k,
                      void* grids, double* sumwt, int accum, hipStream_t s) {
   const DevGeom& g = p->g;
   const int64_t nsw = (int64_t)g.nplanes * g.nchan;
+  const bool det = (p->flags & ARLMI_FLAG_DETERMINISTIC) != 0;
   if (!accum) {
     ARLMI_LAUNCH("k_zero_f64", k_zero_f64, dim3((unsigned)((nsw + 255) / 256)), dim3(256), 0, s, sumwt, nsw);
     ARLMI_LAUNCH_CHECK();
   }
+  int rc = ARLMI_OK;
+  DetArgs da{};
+  if (det) {
+    if (!p->d_det) {
+      set_error("deterministic plan without its fixed-point state");
+      return ARLMI_E_ARG;
+    }
+    rc = det_prepare<R>(p, vis, wt, dopsf, k, nsw, &da, s);
+    if (rc) return rc;
+  }
   cplx<R>* partial = nullptr;
-  int rc = partial_buffer<R>(p, &partial);
+  rc = partial_buffer<R>(p, &partial);
   if (rc) return rc;
-  rc = grid_launch<R>(p, vis, wt, dopsf, k, grids, partial, sumwt, accum, s);
+  rc = grid_launch<R>(p, vis, wt, dopsf, k, grids, partial, sumwt, accum, det ? &da : nullptr, s);
   if (rc) return rc;
   if (p->nslots > 0) {  // every split tile has at least two slots
     const int64_t gx = std::min<int64_t>(std::max<int64_t>(1, p->nslots / 2), 4096);
-    ARLMI_LAUNCH("k_reduce_tiles", k_reduce_tiles<R>, dim3((unsigned)gx, (g.tile * g.tile + 255) / 256),
-                 dim3(256), 0, s, p->d_multi, p->d_misc + 4, partial, (cplx<R>*)grids, g.tile, g.ntx, g.n, accum);
+    if (det)
+      ARLMI_LAUNCH("k_reduce_tiles", k_reduce_tiles_det<R>, dim3((unsigned)gx, (g.tile * g.tile + 255) / 256),
+                   dim3(256), 0, s, p->d_multi, p->d_misc + 4, reinterpret_cast<const longlong2*>(partial),
+                   (cplx<R>*)grids, g.tile, g.ntx, g.n, accum, da);
+    else
+      ARLMI_LAUNCH("k_reduce_tiles", k_reduce_tiles<R>, dim3((unsigned)gx, (g.tile * g.tile + 255) / 256),
+                   dim3(256), 0, s, p->d_multi, p->d_misc + 4, partial, (cplx<R>*)grids, g.tile, g.ntx, g.n, accum);
+    ARLMI_LAUNCH_CHECK();
+  }
+  if (det) {
+    ARLMI_LAUNCH("k_det_sumwt", k_det_sumwt, dim3((unsigned)((nsw + 255) / 256)), dim3(256), 0, s,
+                 (const double*)p->d_det, sumwt, nsw);
     ARLMI_LAUNCH_CHECK();
   }
   return ARLMI_OK;
@@ -1173,6 +1441,10 @@ int arlmi_batch_create(const arlmi_batch_item* items, int32_t n, int precision, 
     }
     if (it.plan->flags & ARLMI_FLAG_OCCUPIED_TILES) {
       set_error("arlmi_batch_create: item %d: plans that leave untouched tiles unwritten are not batched", i);
+      return ARLMI_E_UNSUPPORTED;
+    }
+    if (it.plan->flags & ARLMI_FLAG_DETERMINISTIC) {
+      set_error("arlmi_batch_create: item %d: fixed-point (deterministic) plans are not batched", i);
       return ARLMI_E_UNSUPPORTED;
     }
   }

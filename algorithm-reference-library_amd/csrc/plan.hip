@@ -155,23 +155,45 @@ __device__ inline Members slab_members(double w, const double* boxes, int nb, do
 }
 
 // Pass A (w-stack): per-slab count and sum of w -> slab means.  Each of at most SLAB_BLOCKS
-// blocks writes its partial sums; k_slab_means adds them.
+// blocks writes its partial sums of w - centre; k_slab_means adds them.  The block sums are
+// 64-bit integer sums of (w - centre) 2^q (det_round; a member lies within half a slab width of
+// its centre, and a block adds at most rows_per_block of them), so they do not depend on the
+// order of the LDS adds and the means are the same bits on every build of the plan.
 constexpr int SLAB_BLOCKS = 2048;
 __global__ void k_slab_sums(const double* __restrict__ uvw, int64_t n, const double* __restrict__ boxes, int nb,
                             double* wsum, uint32_t* wcnt) {
   const double wslice = boxes[nb];  // the slab width follows the centres
   extern __shared__ char smem[];
-  double* ls = reinterpret_cast<double*>(smem);
-  double* lbox = ls + nb;
+  long long* ls = reinterpret_cast<long long*>(smem);
+  double* lbox = reinterpret_cast<double*>(ls + nb);
   unsigned int* lc = reinterpret_cast<unsigned int*>(lbox + nb);
   for (int i = threadIdx.x; i < nb; i += blockDim.x) {
-    ls[i] = 0.0;
+    ls[i] = 0;
     lbox[i] = boxes[i];
     lc[i] = 0u;
   }
   __syncthreads();
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const double inv_wslice = 1.0 / wslice;
+  const int64_t per_block = ((n + stride - 1) / stride) * (int64_t)blockDim.x;
+  double half = 0.5 * wslice;  // members lie within half a slab width of their centre
+  if (!(half < INFINITY)) {
+    // an unbounded slab (every row a member): bound the offsets of this block's own rows
+    __shared__ double red[32];
+    double mx = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride) {
+      const double w = uvw[3 * i + 2];
+      const Members m = slab_members(w, lbox, nb, wslice, inv_wslice);
+      for (int j = 0; j < m.n; ++j) mx = fmax(mx, fabs(w - lbox[m.k[j]]));
+    }
+    for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));
+    if (__lane_id() == 0) red[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    half = red[0];
+    for (int i = 1; i < (int)(blockDim.x >> 6); ++i) half = fmax(half, red[i]);
+  }
+  const int q = det_q((double)per_block * half, half);
+  const double scale = ldexp(1.0, q);
   for (int64_t i0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i0 < n; i0 += 4 * stride) {
     double wv[4];
 #pragma unroll
@@ -187,24 +209,27 @@ __global__ void k_slab_sums(const double* __restrict__ uvw, int64_t n, const dou
 #pragma unroll
       for (int j = 0; j < 3; ++j) {
         if (j < m.n) {
-          atomicAdd(&ls[m.k[j]], w);
+          atomicAdd(reinterpret_cast<unsigned long long*>(&ls[m.k[j]]),
+                    (unsigned long long)det_round(w - lbox[m.k[j]], scale));
           atomicAdd(&lc[m.k[j]], 1u);
         }
       }
     }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < nb; i += blockDim.x) {  // this block's partial sums
-    wsum[(size_t)blockIdx.x * nb + i] = ls[i];
+  const double inv_scale = ldexp(1.0, -q);
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {  // this block's partial sums of w - centre
+    wsum[(size_t)blockIdx.x * nb + i] = (double)ls[i] * inv_scale;
     wcnt[(size_t)blockIdx.x * nb + i] = lc[i];
   }
 }
 
-// Slab k (blockIdx.x): sums the nblk per-block partials in a fixed order (deterministic),
-// mean = sum / count (and, when tsum is given, the totals themselves).
-__global__ __launch_bounds__(256) void k_slab_means(const double* wsum, const uint32_t* wcnt, int nblk, double* wmean,
-                                                    int nb, double* tsum = nullptr,
-                                                    unsigned long long* tcnt = nullptr) {
+// Slab k (blockIdx.x): sums the nblk per-block partials of w - centre in a fixed order
+// (deterministic), mean = centre + sum / count (and, when tsum is given, the totals of w,
+// count * centre + sum).
+__global__ __launch_bounds__(256) void k_slab_means(const double* wsum, const uint32_t* wcnt, int nblk,
+                                                    const double* __restrict__ boxes, double* wmean, int nb,
+                                                    double* tsum = nullptr, unsigned long long* tcnt = nullptr) {
   __shared__ double rs[4];
   __shared__ unsigned long long rc[4];
   const int k = blockIdx.x;
@@ -226,9 +251,9 @@ __global__ __launch_bounds__(256) void k_slab_means(const double* wsum, const ui
   if (threadIdx.x == 0) {
     const double t = (rs[0] + rs[1]) + (rs[2] + rs[3]);
     const unsigned long long c = rc[0] + rc[1] + rc[2] + rc[3];
-    if (wmean) wmean[k] = c ? t / (double)c : 0.0;
+    if (wmean) wmean[k] = c ? boxes[k] + t / (double)c : 0.0;
     if (tsum) {
-      tsum[k] = t;
+      tsum[k] = (double)c * boxes[k] + t;
       tcnt[k] = c;
     }
   }
@@ -846,7 +871,7 @@ static void free_plan(arlmi_plan* p) {
                   p->d_nchd, p->d_nslot, p->d_choff, p->d_choffd, p->d_slotoff, p->d_misc, p->d_bounds,
                   p->d_hbase, p->d_h1, p->d_s1, p->d_h2, p->d_s2, p->d_scan_tmp, p->d_recs, p->d_blk2,
                   p->d_entries, p->d_entry_z, p->d_gitems, p->d_ditems, p->d_multi, p->d_partial,
-                  p->d_img_part};
+                  p->d_img_part, p->d_det};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
 }
@@ -880,6 +905,8 @@ static int plan_alloc_fixed(arlmi_plan* p) {
   // [P+1] plane bases, then [P] blocks per plane (one copy from the host)
   ARLMI_HIP(hipMalloc(&p->d_hbase, sizeof(uint32_t) * (2 * P + 1)));
   ARLMI_HIP(hipMalloc(&p->d_multi, sizeof(int4) * nt));
+  if (p->flags & ARLMI_FLAG_DETERMINISTIC)
+    ARLMI_HIP(hipMalloc(&p->d_det, sizeof(double) * (size_t)det_words((int64_t)P * p->g.nchan)));
   return ARLMI_OK;
 }
 
@@ -936,7 +963,7 @@ static int plan_build(arlmi_plan* p, const double* uvw, const double* freq, int6
       ARLMI_HIP(hipMemsetAsync(p->d_wcnt, 0, sizeof(uint32_t) * P, s));
     }
     ARLMI_LAUNCH("k_slab_means", k_slab_means, dim3((unsigned)P), dim3(256), 0, s, p->d_wsum, p->d_wcnt, blocks,
-                 p->d_plane_w, P, nullptr, nullptr);
+                 (const double*)p->d_boxes, p->d_plane_w, P, nullptr, nullptr);
     ARLMI_LAUNCH_CHECK();
     }
   }
@@ -1190,8 +1217,8 @@ int arlmi_wslab_sums(const double* d_uvw, int64_t nrows, const double* h_boxes, 
     const size_t sh = (2 * sizeof(double) + sizeof(unsigned int)) * P;
     ARLMI_LAUNCH("k_slab_sums", k_slab_sums, dim3((unsigned)blocks), dim3(256), sh, s, d_uvw, nrows, d_boxes, P, d_part,
                  d_pcnt);
-    ARLMI_LAUNCH("k_slab_means", k_slab_means, dim3((unsigned)P), dim3(256), 0, s, d_part, d_pcnt, blocks, nullptr, P,
-                 d_tsum, d_tcnt);
+    ARLMI_LAUNCH("k_slab_means", k_slab_means, dim3((unsigned)P), dim3(256), 0, s, d_part, d_pcnt, blocks,
+                 (const double*)d_boxes, nullptr, P, d_tsum, d_tcnt);
     if (hipGetLastError() != hipSuccess ||
         hipMemcpyAsync(tsum.data(), d_tsum, sizeof(double) * P, hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipMemcpyAsync(tcnt.data(), d_tcnt, sizeof(unsigned long long) * P, hipMemcpyDeviceToHost, s) != hipSuccess)

@@ -82,6 +82,8 @@ def parse():
     ap.add_argument("--niter", type=int, default=1000, help="Hogbom iterations (clean)")
     ap.add_argument("--support", type=int, default=16, help="AW kernel support (wstack-aw)")
     ap.add_argument("--nobs", type=int, default=64)
+    ap.add_argument("--deterministic", action="store_true",
+                    help="grid in 64-bit fixed point (bitwise-reproducible invert; wstack workloads)")
     ap.add_argument("--overlap", action="store_true",
                     help="after the timed steps, time them again with the predict on a second stream "
                          "concurrently with the invert (reported as overlapped_streams; the timed region "
@@ -159,7 +161,8 @@ def time_form(precision, args, dev):
                                                           support=args.support)
     else:
         gcf, cf = arlmi.create_pswf_convolutionfunction(im, oversampling=8, support=8)
-    imager = arlmi.Imager(vis, im, gcfcf=(gcf, cf), mode="wstack", vis_slices=args.slices)
+    imager = arlmi.Imager(vis, im, gcfcf=(gcf, cf), mode="wstack", vis_slices=args.slices,
+                          deterministic=args.deterministic)
     # model image: the sources at their nearest pixels ('Nearest' insertion,
     # skycomponent/operations.py:441-444)
     n = args.npixel
@@ -1375,6 +1378,7 @@ def main():
             "rows": rows,
             "rows_generated_before_uv_selection": r32["rows_total"],
             "baseline_times": 32640 * 120,
+            "gridding": "64-bit fixed point (bitwise-reproducible)" if args.deterministic else "float64 LDS adds",
             "entries": E,
             **extra,
             "kernels_fp32": kernels,

@@ -66,6 +66,17 @@ extern "C" {
  * LDS), which arlmi_degrid on the same plan reads.  The grids of such a plan are not in ARL's
  * layout between those two calls. */
 #define ARLMI_FLAG_PRIVATE_GRIDS 8u
+/* Bitwise-reproducible gridding (no counterpart in ARL, whose numpy loop adds in one fixed order):
+ * arlmi_grid / arlmi_grid_add accumulate every grid value and sum of weights as 64-bit integers,
+ * so that repeated calls on the same inputs give identical grids and sums of weights whatever the
+ * order in which the GPU issues the adds (the default float64 LDS adds vary with it in the last
+ * bits).  Each call bounds every grid value of a T x T tile by B_t = Kmax * M * n_t (Kmax the
+ * largest tap magnitude of the kernel, M the largest |Re vis w| + |Im vis w| over the rows, n_t the
+ * tile's entries) and rounds each tap's product once to a multiple of 2^-q_t with B_t 2^q_t < 2^60:
+ * the absolute rounding per tap is at most 2^-60 B_t.  The sums of weights use their own q from
+ * sum |w|.  Not batched (arlmi_batch_create returns ARLMI_E_UNSUPPORTED); non-finite visibilities
+ * or weights give unspecified grids. */
+#define ARLMI_FLAG_DETERMINISTIC 16u
 
 #define ARLMI_KERNEL_SEPARABLE 0 /* real separable table k[os][S]: CF[yo][xo][a][b] = k[yo][a]*k[xo][b] */
 #define ARLMI_KERNEL_TABLE 1     /* full complex table CF[nz][os][os][S][S] (ARL layout, chan 0, pol 0) */
@@ -195,8 +206,8 @@ int arlmi_grid_add(const arlmi_plan* plan, const void* d_vis, const void* d_weig
  * plans are summed in one launch.  The batch records the plans' work lists and buffer addresses
  * when it is created: after arlmi_plan_update, or with other buffers, create it again.  Creation
  * synchronises; the calls allocate nothing, record no memset and never synchronise (they can be
- * captured into a HIP graph).  Plans with ARLMI_FLAG_OCCUPIED_TILES and table kernels are not
- * batched (ARLMI_E_UNSUPPORTED). */
+ * captured into a HIP graph).  Plans with ARLMI_FLAG_OCCUPIED_TILES or ARLMI_FLAG_DETERMINISTIC
+ * and table kernels are not batched (ARLMI_E_UNSUPPORTED). */
 typedef struct arlmi_batch arlmi_batch; /* opaque */
 typedef struct arlmi_batch_item {
   const arlmi_plan* plan;
