@@ -339,7 +339,14 @@ __device__ inline Enum enumerate_row(const DevGeom& g, const double* __restrict_
 // ---------------------------------------------------------------------------
 
 constexpr int K1_ROWS = 4096;  // rows per pass-1 block
-constexpr int K2_RECS = 8192;  // records per pass-2 block
+constexpr int K2_RECS = 8192;  // records per pass-2 block, at least
+// ... and at least K2_HIST times the words of the block's tile histogram (2 tpp), so that with many
+// tiles per plane (128^2 at 8192^2) the per-block histograms that are written (one word per block
+// apart), scanned and read back stay small against the records themselves.  Measured on the
+// streamed 8192^2 workload (16 M-row chunks, 128 slabs): 1 -> 302 ms per step, 2 -> 297 ms, 8 -> 314
+// ms (one block per plane leaves CUs idle), 8192-record blocks alone -> 338 ms.
+constexpr int K2_HIST = 2;
+constexpr int K2_UNROLL = 4;  // record loads in flight per thread
 
 struct Rec {
   uint32_t row;
@@ -488,18 +495,28 @@ __global__ void k_p2_count(DevGeom g, const Rec* __restrict__ recs, const Blk2* 
   const int tpp = g.nchan * g.ntx * g.ntx;
   for (int i = threadIdx.x; i < 2 * tpp; i += blockDim.x) lh[i] = 0u;
   __syncthreads();
-  for (uint32_t k = bk.rs + threadIdx.x; k < bk.re; k += blockDim.x) {
-    const Rec r = recs[k];
-    int tx0, ty0, ntxt, nty, xs, ys;
-    rec_tiles(g, r, &tx0, &ty0, &ntxt, &nty, &xs, &ys);
-    const int chan = (int)((r.aux >> 16) & 0xffu);
-    const bool last = (r.aux >> 24) & 1u;
-    for (int dy = 0; dy < nty; ++dy)
-      for (int dx = 0; dx < ntxt; ++dx) {
-        const int tl = (chan * g.ntx + ty0 + dy) * g.ntx + tx0 + dx;
-        const int kind = (dx == 0 && dy == 0 && last) ? 0 : 1;
-        atomicAdd(&lh[2 * tl + kind], 1u);
-      }
+  for (uint32_t k0 = bk.rs + threadIdx.x; k0 < bk.re; k0 += K2_UNROLL * blockDim.x) {
+    Rec rr[K2_UNROLL];
+#pragma unroll
+    for (int u = 0; u < K2_UNROLL; ++u) {
+      const uint32_t k = k0 + u * blockDim.x;
+      if (k < bk.re) rr[u] = recs[k];
+    }
+#pragma unroll
+    for (int u = 0; u < K2_UNROLL; ++u) {
+      if (k0 + u * blockDim.x >= bk.re) break;
+      const Rec& r = rr[u];
+      int tx0, ty0, ntxt, nty, xs, ys;
+      rec_tiles(g, r, &tx0, &ty0, &ntxt, &nty, &xs, &ys);
+      const int chan = (int)((r.aux >> 16) & 0xffu);
+      const bool last = (r.aux >> 24) & 1u;
+      for (int dy = 0; dy < nty; ++dy)
+        for (int dx = 0; dx < ntxt; ++dx) {
+          const int tl = (chan * g.ntx + ty0 + dy) * g.ntx + tx0 + dx;
+          const int kind = (dx == 0 && dy == 0 && last) ? 0 : 1;
+          atomicAdd(&lh[2 * tl + kind], 1u);
+        }
+    }
   }
   __syncthreads();
   for (int i = threadIdx.x; i < 2 * tpp; i += blockDim.x) h2[bk.hb + (uint32_t)i * bk.nb + bk.b] = lh[i];
@@ -513,24 +530,34 @@ __global__ void k_p2_scatter(DevGeom g, const Rec* __restrict__ recs, const Blk2
   const int tpp = g.nchan * g.ntx * g.ntx;
   for (int i = threadIdx.x; i < 2 * tpp; i += blockDim.x) cur[i] = s2[bk.hb + (uint32_t)i * bk.nb + bk.b];
   __syncthreads();
-  for (uint32_t k = bk.rs + threadIdx.x; k < bk.re; k += blockDim.x) {
-    const Rec r = recs[k];
-    int tx0, ty0, ntxt, nty, xs, ys;
-    rec_tiles(g, r, &tx0, &ty0, &ntxt, &nty, &xs, &ys);
-    const int chan = (int)((r.aux >> 16) & 0xffu);
-    const bool last = (r.aux >> 24) & 1u;
-    const int xo = (int)(r.aux & 0xffu), yo = (int)((r.aux >> 8) & 0xffu);
-    for (int dy = 0; dy < nty; ++dy)
-      for (int dx = 0; dx < ntxt; ++dx) {
-        const int tx = tx0 + dx, ty = ty0 + dy;
-        const int tl = (chan * g.ntx + ty) * g.ntx + tx;
-        const bool start = dx == 0 && dy == 0;
-        const int kind = (start && last) ? 0 : 1;
-        const uint32_t pos = atomicAdd(&cur[2 * tl + kind], 1u);
-        entries[pos] = make_uint2(r.row, pack_entry(xs - tx * g.tile + g.support, ys - ty * g.tile + g.support, xo,
-                                                    yo, start, kind == 0));
-        if (entry_z) entry_z[pos] = (uint16_t)r.z;
-      }
+  for (uint32_t k0 = bk.rs + threadIdx.x; k0 < bk.re; k0 += K2_UNROLL * blockDim.x) {
+    Rec rr[K2_UNROLL];
+#pragma unroll
+    for (int u = 0; u < K2_UNROLL; ++u) {
+      const uint32_t k = k0 + u * blockDim.x;
+      if (k < bk.re) rr[u] = recs[k];
+    }
+#pragma unroll
+    for (int u = 0; u < K2_UNROLL; ++u) {
+      if (k0 + u * blockDim.x >= bk.re) break;
+      const Rec& r = rr[u];
+      int tx0, ty0, ntxt, nty, xs, ys;
+      rec_tiles(g, r, &tx0, &ty0, &ntxt, &nty, &xs, &ys);
+      const int chan = (int)((r.aux >> 16) & 0xffu);
+      const bool last = (r.aux >> 24) & 1u;
+      const int xo = (int)(r.aux & 0xffu), yo = (int)((r.aux >> 8) & 0xffu);
+      for (int dy = 0; dy < nty; ++dy)
+        for (int dx = 0; dx < ntxt; ++dx) {
+          const int tx = tx0 + dx, ty = ty0 + dy;
+          const int tl = (chan * g.ntx + ty) * g.ntx + tx;
+          const bool start = dx == 0 && dy == 0;
+          const int kind = (start && last) ? 0 : 1;
+          const uint32_t pos = atomicAdd(&cur[2 * tl + kind], 1u);
+          entries[pos] = make_uint2(r.row, pack_entry(xs - tx * g.tile + g.support, ys - ty * g.tile + g.support,
+                                                      xo, yo, start, kind == 0));
+          if (entry_z) entry_z[pos] = (uint16_t)r.z;
+        }
+    }
   }
 }
 
@@ -1011,9 +1038,10 @@ static int plan_build(arlmi_plan* p, const double* uvw, const double* freq, int6
   std::vector<uint32_t> hbase(2 * P + 1);  // plane bases, then blocks per plane
   uint32_t* nbp = hbase.data() + P + 1;
   uint64_t hb = 0;
+  const int64_t rb = std::max<int64_t>(K2_RECS, (int64_t)K2_HIST * 2 * tpp);  // records per block
   for (int pl = 0; pl < P; ++pl) {
     const uint32_t rs = bounds[pl], re = bounds[pl + 1];
-    const int nb = (int)std::max<int64_t>(1, ((int64_t)(re - rs) + K2_RECS - 1) / K2_RECS);
+    const int nb = (int)std::max<int64_t>(1, ((int64_t)(re - rs) + rb - 1) / rb);
     hbase[pl] = (uint32_t)hb;
     nbp[pl] = (uint32_t)nb;
     for (int b = 0; b < nb; ++b) {
@@ -1021,8 +1049,8 @@ static int plan_build(arlmi_plan* p, const double* uvw, const double* freq, int6
       k.plane = pl;
       k.b = b;
       k.nb = nb;
-      k.rs = rs + (uint32_t)b * K2_RECS;
-      k.re = std::min<uint32_t>(re, k.rs + K2_RECS);
+      k.rs = rs + (uint32_t)((int64_t)b * rb);
+      k.re = (uint32_t)std::min<int64_t>(re, (int64_t)k.rs + rb);
       k.hb = (uint32_t)hb;
       blks.push_back(k);
     }
