@@ -131,11 +131,47 @@ __host__ __device__ constexpr size_t lds_elems(int nb) {
 // below N, radix N / NSL) hands its outputs, element t + q*NT + r*NSL, to the caller's storer:
 // only the stages between go through LDS twice.
 
+// LDS layout of the invert row pass: padded for 16-byte values, whose stride-8 Stockham writes
+// would otherwise put every group of 8 lanes on the same banks (float64 at 2048^2: 2.26 -> 2.03 ms);
+// unpadded for 8-byte ones, where the index arithmetic costs what the conflicts do.
+template <typename R>
+__host__ __device__ constexpr int inv_lay() {
+  return sizeof(R) == 8 ? LAY_ROWPAD : LAY_ROW;
+}
+
 template <int N>
 __host__ __device__ constexpr int last_ns() {
   int ns = 1;
   while (ns * 8 < N) ns *= 8;
   return ns;
+}
+
+// Twiddle exp(-2 pi i idx / N), idx < N.  From a table holding only the first quarter period
+// (exp(-2 pi i j / N), j < N/4), idx = j + m N/4 is the table value turned by m quarter turns
+// clockwise, i.e. times (-i)^m: (x, y) -> (y, -x) for odd m, then negated for m >= 2.
+// The quarter-period table in double precision, where it frees LDS for the row passes and the
+// two-level column passes (float64: 2048^2 predict rows 1.80 -> 1.75 ms, streamed 8192^2 step
+// 630 -> 604 ms); the full table in single precision, where the turn's selects cost more than
+// the LDS they save (the invert row pass 0.70 -> 0.90 ms with a quarter table).
+template <typename R>
+__host__ __device__ constexpr bool tw_quarter() {
+  return sizeof(R) == 8;
+}
+template <typename R, int N>
+__device__ __forceinline__ cplx<R> tw_get(const cplx<R>* __restrict__ tw, int idx) {
+  if constexpr (!tw_quarter<R>()) return tw[idx];
+  constexpr int Q = N / 4;
+  static_assert((Q & (Q - 1)) == 0 && Q >= 1, "power-of-two transform length");
+  const int m = idx / Q;
+  const cplx<R> w = tw[idx & (Q - 1)];
+  cplx<R> r;
+  r.x = (m & 1) ? w.y : w.x;
+  r.y = (m & 1) ? -w.x : w.y;
+  if (m & 2) {
+    r.x = -r.x;
+    r.y = -r.y;
+  }
+  return r;
 }
 
 template <typename R, int N, int NT, int RAD, int NS, bool INV>
@@ -149,7 +185,7 @@ __device__ __forceinline__ void stage_math(cplx<R> (&v)[(N / RAD) / NT][RAD], in
     if constexpr (NS > 1) {
 #pragma unroll
       for (int r = 1; r < RAD; ++r) {
-        cplx<R> w = tw[r * k * (N / (NS * RAD))];
+        cplx<R> w = tw_get<R, N>(tw, r * k * (N / (NS * RAD)));
         if (INV) w.y = -w.y;
         v[q][r] = cmul(v[q][r], w);
       }
@@ -227,13 +263,18 @@ __device__ __forceinline__ void fft_pass(cplx<R>* lds, int b, int t, const cplx<
 
 // The twiddle table lives in LDS after the transform data (LDS loads cannot be hoisted
 // across the stage barriers, global ones can, which would pin every stage's twiddles in
-// registers at once).  Callers size dynamic LDS as data + N complex values.
+// registers at once); in double precision only its first quarter period (tw_get turns it).
+// Callers size dynamic LDS as data + tw_lds_elems<R, N>() complex values.
+template <typename R, int N>
+__host__ __device__ constexpr size_t tw_lds_elems() {
+  return tw_quarter<R>() ? (size_t)N / 4 : (size_t)N;
+}
 template <typename R, int N, bool TWG = false>
 __device__ __forceinline__ const cplx<R>* stage_twiddles_at(unsigned char* smem, size_t data_bytes,
                                                             const cplx<R>* twg) {
   if constexpr (TWG) return twg;  // the table does not fit beside the data: read it from L2
   cplx<R>* tw = reinterpret_cast<cplx<R>*>(smem + data_bytes);
-  for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = twg[i];
+  for (int i = threadIdx.x; i < (int)tw_lds_elems<R, N>(); i += blockDim.x) tw[i] = twg[i];
   return tw;
 }
 
@@ -265,11 +306,11 @@ struct Shape {
   // passes then read their twiddles from the global table (L2-resident) instead.
   template <typename R>
   static constexpr bool tw_global() {
-    return sizeof(cplx<R>) * (lds_elems<LAY_ROWPAD, N>(FNB > ROW_NB ? FNB : ROW_NB) + N) > 163840;
+    return sizeof(cplx<R>) * (lds_elems<LAY_ROWPAD, N>(FNB > ROW_NB ? FNB : ROW_NB) + tw_lds_elems<R, N>()) > 163840;
   }
   template <typename R>
   static constexpr size_t tw_lds() {
-    return tw_global<R>() ? 0 : (size_t)N;
+    return tw_global<R>() ? 0 : tw_lds_elems<R, N>();
   }
   // Only one column fits LDS beside the twiddles (complex64 at 8192, complex128 from 4096): a
   // column pass would read one 8- or 16-byte value per grid row, so the columns are transformed
@@ -508,7 +549,7 @@ __global__ __launch_bounds__(Shape<N>::NT * Shape<N>::ROW_NB) void k_rows_invert
   extern __shared__ __align__(16) unsigned char smem[];
   cplx<R>* lds = reinterpret_cast<cplx<R>*>(smem);
   const cplx<R>* tw =
-      stage_twiddles_at<R, N, Shape<N>::template tw_global<R>()>(smem, sizeof(cplx<R>) * lds_elems<LAY_ROW, N>(NB), twg);
+      stage_twiddles_at<R, N, Shape<N>::template tw_global<R>()>(smem, sizeof(cplx<R>) * lds_elems<inv_lay<R>(), N>(NB), twg);
   const cplx<R>* grids = reinterpret_cast<const cplx<R>*>(a.data);
   constexpr int RB = N / NB;  // row blocks per channel
   const int c = blockIdx.x / RB;
@@ -544,7 +585,7 @@ __global__ __launch_bounds__(Shape<N>::NT * Shape<N>::ROW_NB) void k_rows_invert
     const cplx<R>* row = grids + ((size_t)p * a.nchan + c) * N * N + (size_t)perm_row_of(y, N, a.perm_n1) * N;
     const uint32_t cm = a.colmask ? a.colmask[(size_t)p * a.nchan + c] : ~0u;
     const double wp = SCREENS ? a.plane_w[p] : 0.0;
-    fft_pass<R, N, NB, NT, true, LAY_ROW>(
+    fft_pass<R, N, NB, NT, true, inv_lay<R>()>(
         lds, b, t, tw,
         [&](int, int, int n) {
           // sparse grids: a skipped tile column reads as 0
@@ -879,7 +920,8 @@ static int run_cols4(FftArgs c, hipStream_t s) {
   }
   constexpr int MA = kCols4MA, MB = kCols4MB;
   const int64_t groups = (int64_t)c.nbatch * (N / NB);
-  const size_t sha = sizeof(cplx<R>) * ((size_t)N2 * NB * MA + N2), shb = sizeof(cplx<R>) * ((size_t)N1 * NB * MB + N1);
+  const size_t sha = sizeof(cplx<R>) * ((size_t)N2 * NB * MA + tw_lds_elems<R, N2>()),
+               shb = sizeof(cplx<R>) * ((size_t)N1 * NB * MB + tw_lds_elems<R, N1>());
   ARLMI_LAUNCH("k_cols4_a", (k_cols4_a<R, N, INV>), dim3((unsigned)(groups * (N1 / MA))),
                dim3(Shape<N2>::NT * NB * MA), sha, s, c, tw2, twN);
   ARLMI_LAUNCH_CHECK();
@@ -913,7 +955,7 @@ static int run_cols(FftArgs c, const cplx<R>* tw, hipStream_t s) {
     ARLMI_LAUNCH_CHECK();
   } else {
     constexpr int CNB = Shape<N>::template col_nb<R>();
-    const size_t csh = sizeof(cplx<R>) * (size_t)N * (CNB + 1);
+    const size_t csh = sizeof(cplx<R>) * ((size_t)N * CNB + tw_lds_elems<R, N>());
     const unsigned cblocks = (unsigned)((int64_t)c.nbatch * N / CNB);
     ARLMI_LAUNCH("k_cols_plain", (k_cols_plain<R, N, INV>), dim3(cblocks), dim3(NT * CNB), csh, s, c, tw);
     ARLMI_LAUNCH_CHECK();
@@ -963,7 +1005,7 @@ static int run_grids_to_image(FftArgs a, hipStream_t s) {
     return ARLMI_E_HIP;
   }
   constexpr int NT = Shape<N>::NT, RNB = Shape<N>::ROW_NB;
-  const size_t ish = sizeof(cplx<R>) * (lds_elems<LAY_ROW, N>(RNB) + Shape<N>::template tw_lds<R>());
+  const size_t ish = sizeof(cplx<R>) * (lds_elems<inv_lay<R>(), N>(RNB) + Shape<N>::template tw_lds<R>());
   FftArgs c = a;
   c.in_mod = 1;
   c.out_mod = 0;

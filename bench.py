@@ -1318,6 +1318,8 @@ def main():
         extra["fp64_mvis_s"] = round(r64["rows"] / (r64["ms"] * 1e-3) / 1e6, 1)
         extra["fp64_ms_per_step"] = round(r64["ms"], 3)
         extra["fp32_over_fp64"] = round(value / extra["fp64_mvis_s"], 2)
+        extra["kernels_fp64"] = {k: {"ms_per_step": round(v[0] / args.steps, 4), "launches_per_step": v[1] / args.steps}
+                                 for k, v in sorted(r64["prof"].items(), key=lambda kv: -kv[1][0])}
         a = r32["img"].numpy()
         q = N // 4
         inner = (slice(None), slice(None), slice(q, N - q), slice(q, N - q))
