@@ -678,20 +678,38 @@ __global__ __launch_bounds__(GRID_THREADS) void k_grid_tab(const int4* __restric
         tab_taps<R, MT>(tab, bcast(kbv, (j + 3) & 63), taps, lane, kb);
       }
     } else {
+      // Any support: the entry's taps in groups of 64 (group q holds taps 64 q + lane), only the
+      // groups whose footprint rows meet the tile, four groups' table reads issued before their
+      // adds (the reads come from L2, and one read then one wait per group left the loop
+      // latency-bound).  a = t / S by a multiply-high with ceil(2^32 / S), exact for t < 2^32 / S.
+      const uint32_t inv_s = 0xffffffffu / (uint32_t)S + 1u;
       for (int j = 0; j < cnt; ++j) {
         const uint32_t p = bcast(pk, j);
         const int sb = (int)(bcast(sav, j) & ~SA_INNER) - SA_BIAS;
         const cplx<R>* k = tab + bcast(kbv, j);
         const R jr = bcast(vr, j), ji = bcast(vi, j);
         const int jx = entry_lx(p) - S, jy = entry_ly(p) - S;
-        for (int t = lane; t < taps; t += 64) {
-          const int a = t / S, b = t - a * S;
-          const int x = jx + b, y = jy + a;
-          if ((unsigned)x < (unsigned)T && (unsigned)y < (unsigned)T) {
-            const cplx<R> c = k[t];
-            const int o = sb + a * WS + b;
-            acc_add<DET>(acc + o, (double)(c.x * jr + c.y * ji), scale);
-            acc_add<DET>(acc + (TW + 64) + o, (double)(c.x * ji - c.y * jr), scale);
+        const int a_lo = max(0, -jy), a_hi = min(S, T - jy);  // footprint rows inside the tile
+        const int q_lo = (a_lo * S) >> 6, q_hi = (a_hi * S + 63) >> 6;
+        for (int q0 = q_lo; q0 < q_hi; q0 += 4) {
+          cplx<R> c[4];
+          int o[4];
+          bool ok[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int t = lane + 64 * (q0 + u);
+            const int a = (int)__umulhi((uint32_t)t, inv_s), b = t - a * S;
+            const int x = jx + b, y = jy + a;
+            ok[u] = q0 + u < q_hi && t < taps && (unsigned)x < (unsigned)T && (unsigned)y < (unsigned)T;
+            o[u] = sb + a * WS + b;
+            c[u] = ok[u] ? k[t] : mk<R>(R(0), R(0));
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            if (ok[u]) {
+              acc_add<DET>(acc + o[u], (double)(c[u].x * jr + c[u].y * ji), scale);
+              acc_add<DET>(acc + (TW + 64) + o[u], (double)(c[u].x * ji - c[u].y * jr), scale);
+            }
           }
         }
       }

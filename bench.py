@@ -80,7 +80,7 @@ def parse():
                          "weighting: uniform weighting of the w-stack workload's rows on its 2048^2 grid; "
                          "stream: ~593 M rows streamed in chunks into resident 8192^2 grids (128 w slabs)")
     ap.add_argument("--niter", type=int, default=1000, help="Hogbom iterations (clean)")
-    ap.add_argument("--support", type=int, default=16, help="AW kernel support (wstack-aw)")
+    ap.add_argument("--support", type=int, default=32, help="AW kernel support (wstack-aw; the workload allows up to 32)")
     ap.add_argument("--nobs", type=int, default=64)
     ap.add_argument("--deterministic", action="store_true",
                     help="grid in 64-bit fixed point (bitwise-reproducible invert; wstack workloads)")
