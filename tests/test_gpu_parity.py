@@ -386,3 +386,33 @@ def test_wstack_with_wprojection_parity(gpu, precision):
     pv, = arlmi.predict_list_serial_workflow([vis], [mim], context="wstack", vis_slices=P, gcfcf=[(gcf, cf)])
     rpv = O.predict_list_wstack(d, model, g, ogcf, ocf, cfg, P)
     assert rel_err(pv.vis.cpu().numpy(), rpv) < TOL[precision]
+
+
+def test_wstack_w_range_assert(gpu):
+    """ARL's w-range assert (gridding.py:64-65) on the slab-mean-shifted w of a w-stack plan with a
+    one-plane kernel: a grid WW step of 1 wavelength rejects rows more than half a wavelength from
+    their slab mean.  Raised when the plan computes the slab means itself and when the caller
+    fixes them (arlmi_plan_fix_plane_w)."""
+    import ctypes
+    import numpy
+    import arlmi
+    from arlmi.plan import Plan
+    rng = numpy.random.default_rng(19)
+    n, cell, S = 256, 1e-3, 8
+    d = random_rows(rng, 3000, n, cell, S, wmax=100.0)
+    vis = to_device_vis(d, 64)
+    im, gcf, cf = _pswf(n, cell, 8, S, 64)
+    with pytest.raises(arlmi._lib.ArlmiError) as e:
+        Plan(vis, im, cf, arlmi._lib.MODE_WSTACK, 4, wstep_grid=1.0)
+    assert e.value.code == -3
+    ref = Plan(vis, im, cf, arlmi._lib.MODE_WSTACK, 4)
+    means, boxes = ref.plane_w(), ref.slab_boxes()
+    flat = dict(d)
+    flat["uvw"] = d["uvw"].copy()
+    flat["uvw"][:, 2] = boxes[0][0]  # every row on the first slab centre: no assert
+    plan = Plan(to_device_vis(flat, 64), im, cf, arlmi._lib.MODE_WSTACK, 4, wstep_grid=1.0, slab_boxes=boxes)
+    arlmi._lib.check(arlmi._lib.lib().arlmi_plan_fix_plane_w(
+        plan.handle, means.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 4))
+    with pytest.raises(arlmi._lib.ArlmiError) as e:
+        plan.update(vis, boxes)
+    assert e.value.code == -3
