@@ -563,14 +563,21 @@ __global__ __launch_bounds__(GRID_THREADS) void k_grid_sep_batch(const BatchDesc
 // ---------------------------------------------------------------------------
 
 // Complex kernel values of this lane's taps (t = lane + 64 q) of one entry, from the table
-// slice at kbase (L2-resident: the whole table is read by every tile).
+// slice at kbase (L2-resident: the whole table is read by every tile).  Every lane loads (a lane
+// past the last tap loads tap 0 and scales it by 0): with a per-lane condition on the load, the
+// compiler lowered the pipelined entry loop with per-lane exit masks and merged the load's lane
+// mask into them, so that when the last tap group was only partly filled (S = 14 with 4 taps per
+// lane, S = 22 with 8) the lanes past the last tap left the loop after its first entries and those
+// entries' later adds were lost; with unconditional loads the loop stays scalar.
 template <typename R, int MT>
 __device__ __forceinline__ void tab_taps(const cplx<R>* __restrict__ tab, uint32_t kbase, int taps, int lane,
                                          cplx<R>* k) {
 #pragma unroll
   for (int q = 0; q < MT; ++q) {
     const int t = lane + 64 * q;
-    k[q] = t < taps ? tab[(size_t)kbase + t] : mk<R>(R(0), R(0));
+    const R m = t < taps ? R(1) : R(0);
+    const cplx<R> v = tab[(size_t)kbase + (t < taps ? t : 0)];
+    k[q] = mk<R>(v.x * m, v.y * m);
   }
 }
 
@@ -615,7 +622,9 @@ __global__ __launch_bounds__(GRID_THREADS) void k_grid_tab(const int4* __restric
   if (accum && it.z == 0) return;  // an empty tile adds nothing
   for (int i = threadIdx.x; i < 2 * (TW + 64); i += blockDim.x) acc[i] = 0.0;
   __syncthreads();
-  const int lane = __lane_id(), wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+  // The wave index through readfirstlane, so that the compiler knows the entry loop's bounds to be
+  // wave-uniform and keeps the loop scalar (see tab_taps).
+  const int lane = __lane_id(), wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), nwave = blockDim.x >> 6;
   const int taps = S * S;
   constexpr int NQ = MT > 0 ? MT : 1;
   int ta[NQ], tb[NQ], loff[NQ];
@@ -1092,8 +1101,8 @@ static int grid_launch(const arlmi_plan* p, const void* vis, const void* wt, int
                    s, p->d_gitems, p->d_entries, ez, (const cplx<R>*)vis, (const R*)wt, dopsf, tab, g.os,        \
                    g.support, (cplx<R>*)grids, partial, sumwt, g.tile, g.ntx, g.n, accum, DetArgs{});             \
   } while (0)
-  // (a variant holding 8 taps per lane gave wrong sums in the single precision build and is
-  // not used: supports above 16 take the loop without prefetch)
+  // (8 taps per lane, for supports 18-22, measured slower than the grouped loop: 25.3 against 19.1
+  // ms for the S = 22 w-stack + AW workload)
   if (taps <= 64)
     ARLMI_GRID_TAB(1);
   else if (taps <= 256)

@@ -300,10 +300,13 @@ def test_awterm_kernel_parity(gpu):
 
 
 @pytest.mark.parametrize("precision", [64, 32])
-@pytest.mark.parametrize("S", [8, 16, 22, 32])
+@pytest.mark.parametrize("S", [6, 8, 10, 14, 16, 22, 32])
 def test_wprojection_grid_degrid_parity(gpu, precision, S):
     """Gridding/degridding with a complex multi-plane (AW) kernel table; the supports cover
-    every taps-per-lane variant of the table gridder (64, 256, 512 taps and the generic loop)."""
+    every taps-per-lane variant of the table gridder (up to 64 and up to 256 taps in registers, the
+    grouped loop beyond), with full and partly filled last tap groups (S = 14: 196 taps, the last
+    of four groups holding 4; a compiler lowering of the entry loop once lost the adds of the lanes
+    past the last tap there)."""
     import arlmi
     rng = numpy.random.default_rng(9)
     n, cell = 256, 5e-4
