@@ -412,7 +412,7 @@ __device__ __forceinline__ void grid_sep_item(const int4 it, unsigned char* smem
   for (int i = threadIdx.x; i < 2 * (TW + 64); i += blockDim.x) acc[i] = 0.0;
   for (int i = threadIdx.x; i < os * S; i += blockDim.x) lk[i] = ktab[i];
   __syncthreads();
-  const int lane = __lane_id(), wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+  const int lane = __lane_id(), wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), nwave = blockDim.x >> 6;
   int ta[TPL], tb[TPL], loff[TPL];
 #pragma unroll
   for (int g = 0; g < TPL; ++g) {
@@ -896,7 +896,7 @@ __global__ __launch_bounds__(DEGRID_TAB_THREADS) void k_degrid_tab(const int4* _
   const TilePos tp = tile_pos(it.x, ntx);
   load_halo<R, 32>(grids, lds, Wd, T, N, tp, perm_n1);
   __syncthreads();
-  const int lane = __lane_id(), wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+  const int lane = __lane_id(), wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), nwave = blockDim.x >> 6;
   const int slot = lane / L, a = lane % L;
   const bool col_ok = a < S;
   const int ac = col_ok ? a : 0;  // lanes past the footprint read column 0 and contribute 0
