@@ -72,6 +72,23 @@ def make_geom(im, cf, mode, nplanes, nw_grid=1, wstep_grid=1e15, tile=0):
     return g
 
 
+def _all_in_channel_zero(freq, freq0, dfreq):
+    """Whether ARL's frequency map (gridding.py:80-85) sends every row to image channel 0:
+    round((f - f0) / df + 1 - 1) == 0 with numpy's rounding, evaluated on the device in the plan's
+    own float64 operation order.  Cached on the tensor until it is modified in place."""
+    key = (freq._version, freq.data_ptr(), freq.shape[0], float(freq0), float(dfreq))
+    hit = getattr(freq, "_arlmi_chan0", None)
+    if hit is not None and hit[0] == key:
+        return hit[1]
+    if freq.shape[0] == 0:
+        ok = True
+    else:
+        p = torch.round(((freq - freq0) / dfreq + 1.0) - 1.0)
+        ok = bool(((p >= 0.0) & (p < 1.0)).all())
+    freq._arlmi_chan0 = (key, ok)
+    return ok
+
+
 def wslice_boxes(wmaxabs, vis_slices):
     """Slab centres and width of vis_wslice_iter (iterators.py:99-106)."""
     boxes = numpy.linspace(-wmaxabs, +wmaxabs, vis_slices)
@@ -133,7 +150,10 @@ class Plan:
         use = self.use_freq if self.use_freq is not None else (self.nchan > 1 or self.geom.dfreq != 0.0)
         if not use or "frequency" not in vis.data:
             return None
-        return ptr(vis.data["frequency"])
+        f = vis.data["frequency"]
+        if self.nchan == 1 and _all_in_channel_zero(f, self.geom.freq0, self.geom.dfreq):
+            return None  # every row maps to image channel 0: the plan need not read the column
+        return ptr(f)
 
     def _build(self, vis, slab_boxes, create):
         uvw = vis.data["uvw"]
