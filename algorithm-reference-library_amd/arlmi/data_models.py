@@ -157,10 +157,24 @@ class ConvolutionFunction:
     def support(self):
         return self.data.shape[5]
 
+    def check_uniform(self):
+        """The kernels read one table for every image channel and polarisation; ARL's gridders index
+        the table by the row's image channel (gridding.py:123-130, 259-265), so a table whose
+        channels or polarisations differ is refused instead of being read as channel 0.  ARL's own
+        constructors (kernels.py:19-200) give every channel the same kernel."""
+        if "uniform" in self._cache:
+            return
+        d = torch.as_tensor(self.data)
+        if d.shape[0] * d.shape[1] > 1 and not torch.equal(d, d[:1, :1].expand_as(d)):
+            raise NotImplementedError("a convolution function whose image channels or polarisations hold "
+                                      "different kernels is not supported")
+        self._cache["uniform"] = True
+
     def table(self, precision):
         """Device table in the kernels' layout for the given precision (cached)."""
         key = ("tab", precision)
         if key not in self._cache:
+            self.check_uniform()
             cdt = torch.complex64 if precision == 32 else torch.complex128
             self._cache[key] = torch.as_tensor(self.data)[0, 0].to(device=device(), dtype=cdt).contiguous()
         return self._cache[key]
@@ -170,6 +184,7 @@ class ConvolutionFunction:
         if self.separable is None:
             return None
         if key not in self._cache:
+            self.check_uniform()
             rdt = torch.float32 if precision == 32 else torch.float64
             self._cache[key] = torch.as_tensor(self.separable, dtype=torch.float64).to(
                 device=device(), dtype=rdt).contiguous()

@@ -89,3 +89,22 @@ def test_convolutionfunction_from_arrays_keeps_tables():
     _, pcf, _ = O.create_pswf_convolutionfunction(g, 4, 6)
     noisy = pcf + 1e-6 * rng.normal(size=pcf.shape)
     assert arlmi.convolutionfunction_from_arrays(im, ogcf, noisy)[1].separable is None
+
+
+def test_kernel_channels_must_agree():
+    """The gridders read one kernel table for every image channel; ARL indexes the table by the
+    row's image channel (gridding.py:123-130), so a table whose channels differ is refused rather
+    than read as channel 0, and one whose channels agree is accepted."""
+    import arlmi
+    rng = numpy.random.default_rng(5)
+    im = arlmi.create_image(npixel=64, cellsize=1e-3, frequency=[1e8, 1.1e8], channel_bandwidth=[1e7, 1e7], dev=CPU)
+    one = rng.normal(size=(1, 1, 1, 4, 4, 6, 6)) + 1j * rng.normal(size=(1, 1, 1, 4, 4, 6, 6))
+    same = numpy.repeat(one, 2, axis=0)
+    gcf = numpy.ones((2, 1, 64, 64))
+    _, cf = arlmi.convolutionfunction_from_arrays(im, gcf, same)
+    cf.check_uniform()
+    diff = same.copy()
+    diff[1, 0, 0, 2, 1, 3, 3] += 1e-9
+    _, cf2 = arlmi.convolutionfunction_from_arrays(im, gcf, diff)
+    with pytest.raises(NotImplementedError):
+        cf2.check_uniform()
