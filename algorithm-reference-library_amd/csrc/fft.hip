@@ -261,6 +261,31 @@ __device__ __forceinline__ void fft_pass(cplx<R>* lds, int b, int t, const cplx<
   }
 }
 
+// fft_pass with the first-stage inputs already in v (element t + q*NT + r*N/8 at v[q][r]); after
+// the first stage has stored its outputs to LDS, between() runs (it may refill v, e.g. with the
+// next transform's inputs, whose loads then overlap the remaining stages).
+template <typename R, int N, int NB, int NT, bool INV, int LAY, typename BT, typename ST>
+__device__ __forceinline__ void fft_pass_v(cplx<R>* lds, int b, int t, const cplx<R>* __restrict__ tw,
+                                           cplx<R> (&v)[(N / 8) / NT][8], BT&& between, ST&& st) {
+  constexpr int NSL = last_ns<N>(), RADL = N / NSL;
+  static_assert(N >= 64, "at least two stages");
+  stage_math<R, N, NT, 8, 1, INV>(v, t, tw);
+  stage_store_lds<R, N, NB, NT, 8, 1, LAY>(lds, b, t, v);
+  between();
+  __syncthreads();
+  mid_stages<R, N, NB, NT, 8, INV, LAY>(lds, b, t, tw);
+  {
+    constexpr int PER = (N / RADL) / NT;
+    cplx<R> u[PER][RADL];
+    stage_load_lds<R, N, NB, NT, RADL, LAY>(lds, b, t, u);
+    stage_math<R, N, NT, RADL, NSL, INV>(u, t, tw);
+#pragma unroll
+    for (int q = 0; q < PER; ++q)
+#pragma unroll
+      for (int r = 0; r < RADL; ++r) st(q, r, t + q * NT + r * NSL, u[q][r]);
+  }
+}
+
 // The twiddle table lives in LDS after the transform data (LDS loads cannot be hoisted
 // across the stage barriers, global ones can, which would pin every stage's twiddles in
 // registers at once); in double precision only its first quarter period (tw_get turns it).
@@ -578,33 +603,76 @@ __global__ __launch_bounds__(Shape<N>::NT * Shape<N>::ROW_NB) void k_rows_invert
   // normalize 2 (the list workflows' normalize=False): each w slab's unnormalised image weighted
   // by its sum of weights (imaging_serial.py:193-194)
   const bool weigh = a.wstack && a.normalize == 2;
-  for (int p = p0; p < p1; ++p) {
-    const double sw = a.sumwt[(size_t)p * a.nchan + c];
-    if (a.wstack && a.normalize == 1 && !(sw > 0.0)) continue;  // normalize_sumwt zeroes this slab's image
-    const R wsc = weigh ? (R)sw : R(1);
-    const cplx<R>* row = grids + ((size_t)p * a.nchan + c) * N * N + (size_t)perm_row_of(y, N, a.perm_n1) * N;
-    const uint32_t cm = a.colmask ? a.colmask[(size_t)p * a.nchan + c] : ~0u;
-    const double wp = SCREENS ? a.plane_w[p] : 0.0;
-    fft_pass<R, N, NB, NT, true, inv_lay<R>()>(
-        lds, b, t, tw,
-        [&](int, int, int n) {
+  // one plane's contribution to the accumulators, element x = t + q*NT + r*NSL of the row transform
+  auto accumulate = [&](double wp, R wsc, int q, int r, int x, cplx<R> v) {
+    if ((x + y) & 1) v = cmk<R>(-v.x, -v.y);
+    if constexpr (SCREENS) {
+      if (!((outside >> (q * RADL + r)) & 1u)) {
+        const cplx<R> s = expi_turns<R>(screen_turns(wp, qs[q][r]));
+        acc[q][r] += wsc * (s.x * v.x - s.y * v.y);
+      }
+    } else {
+      acc[q][r] += v.x;
+      acci[q][r] += v.y;
+    }
+  };
+  // normalize_sumwt zeroes the image of a slab with no positive sum of weights
+  auto skip = [&](int p) { return a.wstack && a.normalize == 1 && !(a.sumwt[(size_t)p * a.nchan + c] > 0.0); };
+  if constexpr (N >= 8192 && sizeof(R) == 4) {
+    // One 8-wave work group per CU at this size: the next plane's row is loaded into registers
+    // while the current plane's remaining stages run, so that the CU is not left idle during the
+    // row loads (with several groups per CU, as at smaller sizes, the extra registers would cost
+    // one of them).
+    constexpr int PER1 = (N / 8) / NT;
+    cplx<R> v1[PER1][8];
+    auto load_row = [&](int p) {
+      const cplx<R>* row = grids + ((size_t)p * a.nchan + c) * N * N + (size_t)perm_row_of(y, N, a.perm_n1) * N;
+      const uint32_t cm = a.colmask ? a.colmask[(size_t)p * a.nchan + c] : ~0u;
+#pragma unroll
+      for (int q = 0; q < PER1; ++q)
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+          const int n = t + q * NT + r * (N / 8);
           // sparse grids: a skipped tile column reads as 0
-          const cplx<R>* src = ((cm >> (n >> a.tshift)) & 1u) ? row + n : zero_entry<R>(a.zeros);
-          return *src;
-        },
-        [&](int q, int r, int x, cplx<R> v) {
-          if ((x + y) & 1) v = cmk<R>(-v.x, -v.y);
-          if constexpr (SCREENS) {
-            if (!((outside >> (q * RADL + r)) & 1u)) {
-              const cplx<R> s = expi_turns<R>(screen_turns(wp, qs[q][r]));
-              acc[q][r] += wsc * (s.x * v.x - s.y * v.y);
-            }
-          } else {
-            acc[q][r] += v.x;
-            acci[q][r] += v.y;
-          }
-        });
-    __syncthreads();
+          v1[q][r] = *(((cm >> (n >> a.tshift)) & 1u) ? row + n : zero_entry<R>(a.zeros));
+        }
+    };
+    int p = p0;
+    while (p < p1 && skip(p)) ++p;
+    if (p < p1) load_row(p);
+    while (p < p1) {
+      const double sw = a.sumwt[(size_t)p * a.nchan + c];
+      const R wsc = weigh ? (R)sw : R(1);
+      const double wp = SCREENS ? a.plane_w[p] : 0.0;
+      int pn = p + 1;
+      while (pn < p1 && skip(pn)) ++pn;
+      fft_pass_v<R, N, NB, NT, true, inv_lay<R>()>(
+          lds, b, t, tw, v1,
+          [&]() {
+            if (pn < p1) load_row(pn);
+          },
+          [&](int q, int r, int x, cplx<R> v) { accumulate(wp, wsc, q, r, x, v); });
+      __syncthreads();
+      p = pn;
+    }
+  } else {
+    for (int p = p0; p < p1; ++p) {
+      if (skip(p)) continue;
+      const double sw = a.sumwt[(size_t)p * a.nchan + c];
+      const R wsc = weigh ? (R)sw : R(1);
+      const cplx<R>* row = grids + ((size_t)p * a.nchan + c) * N * N + (size_t)perm_row_of(y, N, a.perm_n1) * N;
+      const uint32_t cm = a.colmask ? a.colmask[(size_t)p * a.nchan + c] : ~0u;
+      const double wp = SCREENS ? a.plane_w[p] : 0.0;
+      fft_pass<R, N, NB, NT, true, inv_lay<R>()>(
+          lds, b, t, tw,
+          [&](int, int, int n) {
+            // sparse grids: a skipped tile column reads as 0
+            const cplx<R>* src = ((cm >> (n >> a.tshift)) & 1u) ? row + n : zero_entry<R>(a.zeros);
+            return *src;
+          },
+          [&](int q, int r, int x, cplx<R> v) { accumulate(wp, wsc, q, r, x, v); });
+      __syncthreads();
+    }
   }
   const size_t rowoff = ((size_t)c * N + y) * N;
   if (G > 1) {
