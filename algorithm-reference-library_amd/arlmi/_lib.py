@@ -37,7 +37,8 @@ EXPORTS = (
     "arlmi_version", "arlmi_last_error", "arlmi_wmax", "arlmi_plan_create", "arlmi_plan_update",
     "arlmi_plan_set_chunk", "arlmi_plan_stats_get", "arlmi_plan_plane_w", "arlmi_plan_slab_boxes", "arlmi_plan_destroy", "arlmi_convolution_mapping",
     "arlmi_grid", "arlmi_grid_add", "arlmi_degrid", "arlmi_batch_create", "arlmi_batch_grid", "arlmi_batch_degrid",
-    "arlmi_batch_destroy", "arlmi_wslab_sums", "arlmi_plan_fix_plane_w",
+    "arlmi_batch_destroy", "arlmi_wslab_sums", "arlmi_plan_fix_plane_w", "arlmi_plan_or_colmask",
+    "arlmi_plan_use_colmask",
     "arlmi_grids_to_image", "arlmi_image_to_grids", "arlmi_fft2d", "arlmi_w_screen", "arlmi_dft",
     "arlmi_grid_weights", "arlmi_reweight", "arlmi_block_to_rows", "arlmi_apply_gaintable", "arlmi_apply_gains_rows", "arlmi_apply_gains_rows_adjoint",
     "arlmi_phaserotate_rows", "arlmi_phaserotate_blocks", "arlmi_hogbom", "arlmi_hogbom_noalloc", "arlmi_hogbom_traffic", "arlmi_solve_gains",
@@ -151,6 +152,8 @@ def lib():
         "arlmi_wslab_sums": (ctypes.c_int, [vp, i64, ctypes.POINTER(dbl), dbl, i32, ctypes.POINTER(dbl),
                                             ctypes.POINTER(i64), vp]),
         "arlmi_plan_fix_plane_w": (ctypes.c_int, [vp, ctypes.POINTER(dbl), i32]),
+        "arlmi_plan_or_colmask": (ctypes.c_int, [vp, vp, ctypes.POINTER(i32), vp]),
+        "arlmi_plan_use_colmask": (ctypes.c_int, [vp, vp]),
         "arlmi_degrid": (ctypes.c_int, [vp, vp, ctypes.POINTER(Kernel), vp, ctypes.c_int, vp]),
         "arlmi_grids_to_image": (ctypes.c_int, [vp, vp, vp, vp, ctypes.c_int, vp, vp, ctypes.c_int, vp]),
         "arlmi_image_to_grids": (ctypes.c_int, [vp, vp, vp, vp, ctypes.c_int, vp]),

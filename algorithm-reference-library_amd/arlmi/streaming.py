@@ -40,7 +40,7 @@ class StreamingImager:
     """
 
     def __init__(self, im, gcfcf=None, mode="wstack", vis_slices=1, precision=32, tile=0, chunk=0, keep_plans=False,
-                 deterministic=False, **kwargs):
+                 deterministic=False, band_masks=True, **kwargs):
         if mode not in ("2d", "wstack"):
             raise ValueError("mode must be '2d' or 'wstack'")
         self.gcf, self.cf = _gcfcf(im, gcfcf, kwargs)
@@ -58,6 +58,11 @@ class StreamingImager:
         self.keep_plans = keep_plans
         # deterministic: every chunk gridded in 64-bit fixed point (bitwise-reproducible passes)
         self.deterministic = deterministic
+        # band_masks: the transforms skip the column bands no chunk's rows reach (they hold zeros);
+        # the union of the chunks' bands is gathered while they are gridded (arlmi_plan_or_colmask)
+        self.band_masks = band_masks
+        self._bands = None        # device [nplanes * nchan] uint32 union of the last invert's chunks
+        self._bands_valid = False  # the kept plans still are those the union was gathered from
         self.plans = {}
         self.grids = None
         self.boxes = None
@@ -136,6 +141,8 @@ class StreamingImager:
         """Free the resident grids and the plans' buffers."""
         self.grids = None
         self._planned = None
+        self._bands = None
+        self._bands_valid = False
         for p in self.plans.values():
             p.close()
         self.plans = {}
@@ -151,6 +158,7 @@ class StreamingImager:
             self.scan(chunks)
         L = _lib.lib()
         first = True
+        self._bands_valid = False
         for i, v in enumerate(_chunks(chunks)):
             if v.precision != self.precision:
                 raise TypeError("chunk precision differs from the imager's")
@@ -159,7 +167,14 @@ class StreamingImager:
                 self._ensure_grids(v)
                 self.grids.zero_()
                 self.sumwt_planes.zero_()
+                if self.band_masks:
+                    if self._bands is None:
+                        self._bands = torch.zeros(self.plan.nplanes * self.plan.nchan, dtype=torch.int32,
+                                                  device=self.dev)
+                    self._bands.zero_()
                 first = False
+            if self.band_masks:
+                _lib.check(L.arlmi_plan_or_colmask(plan.handle, ptr(self._bands), None, stream_handle()))
             k, keep = kernel_struct(self.cf, self.precision)
             _lib.check(L.arlmi_grid_add(plan.handle, ptr(v.data["vis"]), ptr(v.data["imaging_weight"]),
                                         int(bool(dopsf)), ctypes.byref(k), ptr(self.grids), ptr(self.sumwt_planes),
@@ -169,10 +184,20 @@ class StreamingImager:
             raise ValueError("no chunks")
         n, nchan = self.plan.npixel, self.plan.nchan
         re = torch.empty((nchan, 1, n, n), dtype=_real_dtype(self.precision), device=self.dev)
-        _lib.check(L.arlmi_grids_to_image(self.plan.handle, ptr(self.grids), ptr(self.sumwt_planes),
-                                          ptr(self.gcf_dev), 1 if normalize else 2, ptr(re), None, self.precision,
-                                          stream_handle()))
+        self._use_bands(self.plan, self.band_masks)
+        try:
+            _lib.check(L.arlmi_grids_to_image(self.plan.handle, ptr(self.grids), ptr(self.sumwt_planes),
+                                              ptr(self.gcf_dev), 1 if normalize else 2, ptr(re), None,
+                                              self.precision, stream_handle()))
+        finally:
+            self._use_bands(self.plan, False)
+        # the union stays valid for predicts on the same kept plans
+        self._bands_valid = self.band_masks and self.keep_plans
         return re, self.sumwt_planes.sum(dim=0).cpu().numpy()[:, None]
+
+    def _use_bands(self, plan, on):
+        if self._bands is not None or not on:
+            _lib.check(_lib.lib().arlmi_plan_use_colmask(plan.handle, ptr(self._bands) if on else None))
 
     def predict(self, model_data, chunks, sink=None, out=None, rebuild=None):
         """Model visibilities of every chunk: gcf * conj(screen) * model, shifted FFTs once, then
@@ -186,14 +211,22 @@ class StreamingImager:
         model = model_data.to(device=self.dev, dtype=rdt).contiguous()
         if rebuild is None:  # kept plans were built for these chunks by the last invert
             rebuild = not self.keep_plans
+        # the last invert's column bands cover these chunks when its kept plans are reused
+        bands = self._bands_valid and not rebuild
+        if rebuild:
+            self._bands_valid = False
         result = [] if sink is None else None
         first = True
         for i, v in enumerate(_chunks(chunks)):
             plan = self._plan_for(v, i, rebuild)
             if first:
                 self._ensure_grids(v)
-                _lib.check(L.arlmi_image_to_grids(plan.handle, ptr(model), ptr(self.gcf_dev), ptr(self.grids),
-                                                  self.precision, stream_handle()))
+                self._use_bands(plan, bands)
+                try:
+                    _lib.check(L.arlmi_image_to_grids(plan.handle, ptr(model), ptr(self.gcf_dev), ptr(self.grids),
+                                                      self.precision, stream_handle()))
+                finally:
+                    self._use_bands(plan, False)
                 first = False
             pred = out[i] if out is not None else torch.empty_like(v.vis)
             k, keep = kernel_struct(self.cf, self.precision)

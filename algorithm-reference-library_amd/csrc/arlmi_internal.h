@@ -213,6 +213,16 @@ __device__ inline int det_q(double bound_sum, double bound_one) {
   return q < -1000 ? -1000 : (q > 1000 ? 1000 : q);
 }
 
+// log2 of the width of the column bands of an occupancy mask: the tile side, or N / 32 when a plane
+// has more than 32 tile columns (a band mask is one 32-bit word per plane-channel).
+inline int band_shift(const DevGeom& g) {
+  int t = 0, n = g.n / 32;
+  while ((1 << t) < g.tile) ++t;
+  int b = 0;
+  while ((1 << b) < n) ++b;
+  return t > b ? t : b;
+}
+
 // Opt-in kernel timing (profile.hip).
 bool profile_on();
 void* profile_begin(const char* name, hipStream_t s);
@@ -245,6 +255,7 @@ struct arlmi_plan {
   uint32_t* d_tilemask = nullptr;        // [pc][tx]: bit ty set if tile (pc, ty, tx) has entries
   uint32_t* d_colmask = nullptr;         // [pc]: bit tx set if tile column (pc, tx) has entries
   void* d_zeros = nullptr;               // 1 KiB of zeros (masked-out loads of the FFT passes)
+  const uint32_t* ext_colmask = nullptr;  // caller's column-band occupancy for the transforms (or null)
   uint32_t* d_cntd = nullptr;            // [ntiles] degrid entries per tile
   uint32_t* d_off = nullptr;             // [ntiles] first entry of a tile
   uint32_t* d_nch = nullptr;             // [ntiles] grid chunks per tile

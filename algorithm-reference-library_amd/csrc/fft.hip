@@ -540,6 +540,8 @@ __global__ __launch_bounds__(Shape<N>::NT * Shape<N>::template col_nb<R>()) void
     tm = a.tilemask[(size_t)plane * a.ntx + (x0 >> a.tshift)];  // NB divides the tile side
     if (!tm) return;
     if (!a.zero_fill) tm = ~0u;
+  } else if (a.colmask && !((a.colmask[plane] >> (x0 >> a.tshift)) & 1u)) {
+    return;  // a column band of zeros (the caller's band mask)
   }
   const int b = threadIdx.x % NB, t = threadIdx.x / NB;
   const int x = x0 + b;
@@ -921,6 +923,7 @@ __global__ __launch_bounds__(Shape<N / kCols4N1>::NT * cols4_nb<R>() * kCols4MA)
   const int x0 = (rest % CG) * NBC;
   const int b = threadIdx.x % NB, t = threadIdx.x / NB;
   const int x = x0 + b % NBC, n1 = g * M + b / NBC;
+  if (a.colmask && !((a.colmask[plane] >> (x0 >> a.tshift)) & 1u)) return;  // a column band of zeros
   cplx<R>* col = reinterpret_cast<cplx<R>*>(a.data) + plane * (int64_t)N * N + x;
   fft_pass<R, N2, NB, NT, INV, LAY_COL>(
       lds, b, t, tw,
@@ -953,6 +956,7 @@ __global__ __launch_bounds__(Shape<kCols4N1>::NT * cols4_nb<R>() * kCols4MB) voi
   const int x0 = (rest % CG) * NBC;
   const int b = threadIdx.x % NB, t = threadIdx.x / NB;
   const int x = x0 + b % NBC, k2 = g * M + b / NBC;
+  if (a.colmask && !((a.colmask[plane] >> (x0 >> a.tshift)) & 1u)) return;  // a column band of zeros
   cplx<R>* col = reinterpret_cast<cplx<R>*>(a.data) + plane * (int64_t)N * N + x + (size_t)(N1 * k2) * N;
   fft_pass<R, N1, NB, NT, INV, LAY_COL>(
       lds, b, t, tw, [&](int, int, int n1) { return col[(size_t)n1 * N]; },
@@ -1206,6 +1210,20 @@ int arlmi_fft2d(void* d_data, int32_t npixel, int64_t nbatch, int direction, int
                          : dispatch_n<double>(Op::FFT, npixel, a, direction, as_stream(stream));
 }
 
+// FftArgs for a caller's column-band mask (arlmi_plan_use_colmask): column groups of bands whose
+// bit is clear are skipped by the column passes, read as zero by the invert row pass and not
+// stored by the predict row pass.
+static void set_band_mask(const arlmi_plan* p, FftArgs* a) {
+  const int bs = band_shift(p->g);
+  a->colmask = p->ext_colmask;
+  a->tilemask = nullptr;
+  a->zeros = p->d_zeros;
+  a->zero_fill = 0;
+  a->tile = 1 << bs;
+  a->tshift = bs;
+  a->ntx = p->g.n >> bs;
+}
+
 int arlmi_grids_to_image(const arlmi_plan* p, void* d_grids, const double* d_sumwt, const double* d_gcf, int normalize,
                          void* d_img_re, void* d_img_im, int precision, void* stream) {
   if (int rc = prec_ok(precision)) return rc;
@@ -1242,6 +1260,8 @@ int arlmi_grids_to_image(const arlmi_plan* p, void* d_grids, const double* d_sum
     a.tile = p->g.tile;
     a.tshift = __builtin_ctz((unsigned)p->g.tile);
     a.ntx = p->g.ntx;
+  } else if (p->ext_colmask) {  // the caller's column bands: the others hold zeros
+    set_band_mask(p, &a);
   }
   // partial images of the plane groups live with the plan (grown on demand)
   const int n = p->g.n, nt = n >= 16 ? n / 16 : 1, rnb = nt >= 256 ? 1 : 256 / nt;  // Shape<n>::NT, ROW_NB
@@ -1288,6 +1308,8 @@ int arlmi_image_to_grids(const arlmi_plan* p, const void* d_model, const double*
     a.tile = p->g.tile;
     a.tshift = __builtin_ctz((unsigned)p->g.tile);
     a.ntx = p->g.ntx;
+  } else if (p->ext_colmask) {  // only the column bands the caller's degridding reads
+    set_band_mask(p, &a);
   }
   // grids only this plan's degridder reads may keep the two-level column transform's row order
   if (p->flags & ARLMI_FLAG_PRIVATE_GRIDS) a.perm_n1 = grid_perm_n1(p->g.n, precision);

@@ -614,6 +614,19 @@ __global__ void k_colmask(const uint32_t* tilemask, int npc, int ntx, uint32_t* 
   colmask[pc] = col;
 }
 
+// Column-band occupancy (arlmi_plan_or_colmask): one thread per (pc, tile column tx) sets bit
+// tx >> shift of colmask[pc] when the tile column holds an entry.
+__global__ void k_or_colmask(const uint32_t* __restrict__ cnt, int npc, int ntx, int shift, uint32_t* colmask) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= (int64_t)npc * ntx) return;
+  const int tx = (int)(i % ntx);
+  const int64_t pc = i / ntx;
+  const uint32_t* c = cnt + pc * ntx * ntx + tx;
+  bool any = false;
+  for (int ty = 0; ty < ntx; ++ty) any |= c[(size_t)ty * ntx] != 0u;
+  if (any) atomicOr(&colmask[pc], 1u << (tx >> shift));
+}
+
 // Work items: grid items {tile, start, count, slot+1 or 0}, degrid items {tile, start, count, 0};
 // tiles split over several grid items are listed with their first partial slot.
 // Largest t in [0, n) with off[t] <= i (off non-decreasing, off[0] = 0 <= i).
@@ -915,9 +928,10 @@ static int plan_alloc_fixed(arlmi_plan* p) {
     const int64_t npc = nt / ((int64_t)p->g.ntx * p->g.ntx);
     ARLMI_HIP(hipMalloc(&p->d_tilemask, sizeof(uint32_t) * (size_t)(npc * p->g.ntx)));
     ARLMI_HIP(hipMalloc(&p->d_colmask, sizeof(uint32_t) * (size_t)npc));
-    ARLMI_HIP(hipMalloc(&p->d_zeros, 1024));
-    ARLMI_HIP(hipMemset(p->d_zeros, 0, 1024));
   }
+  // zeros for the masked-out loads of the transforms (sparse grids, or a caller's band mask)
+  ARLMI_HIP(hipMalloc(&p->d_zeros, 1024));
+  ARLMI_HIP(hipMemset(p->d_zeros, 0, 1024));
   ARLMI_HIP(hipMalloc(&p->d_cntd, sizeof(uint32_t) * nt));
   ARLMI_HIP(hipMalloc(&p->d_off, sizeof(uint32_t) * nt));
   ARLMI_HIP(hipMalloc(&p->d_nch, sizeof(uint32_t) * nt));
@@ -1272,6 +1286,36 @@ int arlmi_plan_update(arlmi_plan* p, const double* d_uvw, const double* d_freq, 
     return ARLMI_E_ARG;
   }
   return plan_build(p, d_uvw, d_freq, nrows, h_boxes, wslice, as_stream(stream));
+}
+
+int arlmi_plan_or_colmask(const arlmi_plan* p, uint32_t* d_colmask, int32_t* band_shift_out, void* stream) {
+  if (!p || !d_colmask) {
+    set_error("arlmi_plan_or_colmask: NULL argument");
+    return ARLMI_E_ARG;
+  }
+  const DevGeom& g = p->g;
+  const int bs = band_shift(g);
+  if (band_shift_out) *band_shift_out = bs;
+  const int npc = g.nplanes * g.nchan;
+  const int64_t n = (int64_t)npc * g.ntx;
+  if (n == 0) return ARLMI_OK;
+  ARLMI_LAUNCH("k_or_colmask", k_or_colmask, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, as_stream(stream),
+               p->d_cnt, npc, g.ntx, bs - g.tshift, d_colmask);
+  ARLMI_LAUNCH_CHECK();
+  return ARLMI_OK;
+}
+
+int arlmi_plan_use_colmask(arlmi_plan* p, const uint32_t* d_colmask) {
+  if (!p) {
+    set_error("plan is NULL");
+    return ARLMI_E_ARG;
+  }
+  if (d_colmask && (p->flags & ARLMI_FLAG_SPARSE_GRIDS)) {
+    set_error("arlmi_plan_use_colmask: a plan with sparse grids uses its own occupancy");
+    return ARLMI_E_ARG;
+  }
+  p->ext_colmask = d_colmask;
+  return ARLMI_OK;
 }
 
 int arlmi_plan_set_chunk(arlmi_plan* p, int32_t chunk) {

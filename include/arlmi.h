@@ -167,6 +167,21 @@ int arlmi_wslab_sums(const double* d_uvw, int64_t nrows, const double* h_boxes, 
  * of the means of the rows each update is given.  n = 0 (or NULL) returns to the rows' own means. */
 int arlmi_plan_fix_plane_w(arlmi_plan* plan, const double* h_plane_w, int32_t n);
 
+/* Column-band occupancy for grids that several plans' rows add into (arlmi_grid_add; no
+ * counterpart in ARL, whose grids are dense numpy arrays): d_colmask[pc] (one uint32 per plane and
+ * image channel, pc = plane * nchan + chan, zeroed by the caller) gets bit b set for every band b of
+ * grid columns that holds an entry of this plan; a band is 2^band_shift columns wide, the tile side
+ * or N / 32 when a plane has more than 32 tile columns (*band_shift_out, if not NULL).  Stream-
+ * ordered. */
+int arlmi_plan_or_colmask(const arlmi_plan* plan, uint32_t* d_colmask, int32_t* band_shift_out, void* stream);
+/* The transforms of this plan (arlmi_grids_to_image, arlmi_image_to_grids) then treat the column
+ * bands whose bit is clear as zero: the column passes skip them, the invert row pass reads them as
+ * zero and the predict row pass does not store them (the degridding of the plans whose entries
+ * the mask covers reads none of them).  For grids whose other bands hold zeros (invert) or are not
+ * read (predict).  NULL returns to dense transforms.  Not for plans with ARLMI_FLAG_SPARSE_GRIDS.
+ * The mask is the caller's and must stay valid while the plan uses it. */
+int arlmi_plan_use_colmask(arlmi_plan* plan, const uint32_t* d_colmask);
+
 /* Host copy of a w-stack plan's slab centres and, after them, the slab width (nplanes + 1
  * values; iterators.py:99-106). */
 int arlmi_plan_slab_boxes(const arlmi_plan* plan, double* h_out, int32_t n);

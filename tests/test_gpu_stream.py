@@ -112,3 +112,46 @@ def test_fft_8192(gpu, direction):
         l2 = numpy.linalg.norm(out - ref) / numpy.linalg.norm(ref)
         assert l2 < (1e-6 if precision == 32 else 1e-14), (precision, l2)
         assert rel_err(out, ref) < (1e-5 if precision == 32 else 1e-13), precision
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+@pytest.mark.parametrize("n", [1024, 4096])
+def test_stream_band_masks_exact(gpu, precision, n):
+    """The transforms of a streamed pass skip the column bands no chunk reaches (their grid columns
+    are zeros): images and predicted visibilities are bit for bit those of dense transforms, at a
+    size with one band per tile column (1024^2, T = 64) and at one with 128-column bands over 64
+    tile columns (4096^2; in float64 the columns there take the two-level transform).  Gridding in
+    fixed point, so that the two passes' grids are the same bits."""
+    import torch
+    import arlmi
+    rng = numpy.random.default_rng(67 + n)
+    cell, S, P = 2.5e-4 * 1024 / n, 8, 6
+    d = random_rows(rng, 40000, n, cell, S, wmax=300.0)
+    # most rows in the inner third of the uv plane, a few in one outer corner: outer bands empty
+    d["uvw"][:, 0] *= numpy.where(numpy.arange(40000) % 50 == 0, 1.0, 1.0 / 3.0)
+    d["uvw"][:, 1] *= numpy.where(numpy.arange(40000) % 50 == 0, 1.0, 1.0 / 3.0)
+    sel = numpy.arange(40000) % 50 == 0
+    d["uvw"][sel, 0] = numpy.abs(d["uvw"][sel, 0])
+    chunks = [to_device_vis(c, precision) for c in _split(d, 3)]
+    im = device_image(n, cell, precision)
+    gcf, cf = arlmi.create_pswf_convolutionfunction(im, oversampling=8, support=S)
+    model = numpy.zeros((1, 1, n, n))
+    model[0, 0, rng.integers(n // 4, 3 * n // 4, 30), rng.integers(n // 4, 3 * n // 4, 30)] = 1.0
+    mim = device_image(n, cell, precision, data=model)
+    res = []
+    for masks in (True, False):
+        st = arlmi.StreamingImager(im, gcfcf=(gcf, cf), mode="wstack", vis_slices=P, precision=precision,
+                                   keep_plans=True, band_masks=masks, deterministic=True)
+        st.scan(chunks)
+        img, sw = st.invert(chunks)
+        pv = [p.clone() for p in st.predict(mim.data, chunks)]
+        if masks:
+            bands = st._bands.cpu().numpy().view(numpy.uint32)
+            nb = min(32, n // 64)
+            full = (1 << nb) - 1
+            assert numpy.any(bands != 0) and numpy.any(bands != full), bands  # some bands skipped
+        res.append((img.clone(), sw, pv))
+        st.release()
+    (a, swa, pa), (b, swb, pb) = res
+    assert torch.equal(a, b) and numpy.array_equal(swa, swb)
+    assert all(torch.equal(x, y) for x, y in zip(pa, pb))
