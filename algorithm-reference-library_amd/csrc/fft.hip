@@ -381,12 +381,12 @@ __device__ __forceinline__ float2 expi_turns<float>(double t) {
   const float sn = th * (1.0f + t2 * (-1.0f / 6.0f + t2 * (1.0f / 120.0f + t2 * (-1.0f / 5040.0f + t2 * (1.0f / 362880.0f)))));
   const float cs = 1.0f + t2 * (-0.5f + t2 * (1.0f / 24.0f + t2 * (-1.0f / 720.0f + t2 * (1.0f / 40320.0f))));
   const int q = ((int)q4) & 3;
-  // rotate (cs, sn) by q quarter turns
-  float c = cs, s = sn;
-  if (q == 1) { c = -sn; s = cs; }
-  else if (q == 2) { c = -cs; s = -sn; }
-  else if (q == 3) { c = sn; s = -cs; }
-  return make_float2(c, s);
+  // rotate (cs, sn) by q quarter turns, without branches: odd q swaps the pair, and the cosine is
+  // negated for q = 1, 2 and the sine for q = 2, 3 (sign-bit flips)
+  const bool swap = (q & 1) != 0;
+  const float x = swap ? sn : cs, y = swap ? cs : sn;
+  const uint32_t nc = (uint32_t)((q + 1) & 2) << 30, ns = (uint32_t)(q & 2) << 30;
+  return make_float2(__uint_as_float(__float_as_uint(x) ^ nc), __uint_as_float(__float_as_uint(y) ^ ns));
 }
 template <>
 __device__ __forceinline__ double2 expi_turns<double>(double t) {
@@ -609,10 +609,11 @@ __global__ __launch_bounds__(Shape<N>::NT * Shape<N>::ROW_NB) void k_rows_invert
   auto accumulate = [&](double wp, R wsc, int q, int r, int x, cplx<R> v) {
     if ((x + y) & 1) v = cmk<R>(-v.x, -v.y);
     if constexpr (SCREENS) {
-      if (!((outside >> (q * RADL + r)) & 1u)) {
-        const cplx<R> s = expi_turns<R>(screen_turns(wp, qs[q][r]));
-        acc[q][r] += wsc * (s.x * v.x - s.y * v.y);
-      }
+      // the screen is 0 outside the unit circle (selected, not branched on: the pixels of a row
+      // are in and out together in no useful pattern)
+      const cplx<R> s = expi_turns<R>(screen_turns(wp, qs[q][r]));
+      const R val = wsc * (s.x * v.x - s.y * v.y);
+      acc[q][r] += ((outside >> (q * RADL + r)) & 1u) ? R(0) : val;
     } else {
       acc[q][r] += v.x;
       acci[q][r] += v.y;
