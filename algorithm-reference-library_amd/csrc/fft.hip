@@ -400,6 +400,61 @@ constexpr double kMinusTwoPi = -2.0 * 3.14159265358979323846264338327950288;
 // phase -2 pi w (1 - n) of the w screen, in turns
 __device__ __forceinline__ double screen_turns(double w, double q) { return -(w * q); }
 
+// The screen phasor exp(-2 pi i w q) of a row pass, with q = 1 - n per pixel (held across the
+// planes) and w per plane.  Double precision: the float64 phase and sincospi.  Single precision:
+// w and q are each held as two floats (hi + lo, hi the float nearest the double), the product's
+// rounding error comes from a fused multiply-add (hi*hi exactly as p + e, plus the cross terms;
+// lo*lo, ~2^-48 of the phase, is left out), p is reduced modulo 1 and to an octant exactly in
+// float, e is added to the octant remainder (a rounding at |g| <= 1/8 + |e|, 2^-28 turns), and
+// sin / cos of the remainder come from the Taylor series of expi_turns: the float64 reduction's
+// accuracy without float64 arithmetic.
+template <typename R>
+struct ScreenQ {
+  double q;
+};
+template <>
+struct ScreenQ<float> {
+  float hi, lo;
+};
+template <typename R>
+__device__ __forceinline__ ScreenQ<R> screen_split(double q);
+template <>
+__device__ __forceinline__ ScreenQ<double> screen_split<double>(double q) {
+  return ScreenQ<double>{q};
+}
+template <>
+__device__ __forceinline__ ScreenQ<float> screen_split<float>(double q) {
+  const float hi = (float)q;
+  return ScreenQ<float>{hi, (float)(q - (double)hi)};
+}
+template <typename R>
+__device__ __forceinline__ cplx<R> screen_phasor(const ScreenQ<R>& w, const ScreenQ<R>& q);
+template <>
+__device__ __forceinline__ double2 screen_phasor<double>(const ScreenQ<double>& w, const ScreenQ<double>& q) {
+  return expi_turns<double>(screen_turns(w.q, q.q));
+}
+template <>
+__device__ __forceinline__ float2 screen_phasor<float>(const ScreenQ<float>& w, const ScreenQ<float>& q) {
+  // turns t = -(w q) = p + e with w negated on entry
+  const float nwh = -w.hi, nwl = -w.lo;
+  const float p = nwh * q.hi;
+  float e = __builtin_fmaf(nwh, q.hi, -p);
+  e = __builtin_fmaf(nwh, q.lo, e);
+  e = __builtin_fmaf(nwl, q.hi, e);
+  const float f = p - rintf(p);          // exact: [-1/2, 1/2]
+  const float q4 = rintf(f * 4.0f);      // quadrant -2..2
+  const float g = __builtin_fmaf(-0.25f, q4, f) + e;  // the first two terms cancel exactly
+  const float th = 6.28318530717958647692f * g;
+  const float t2 = th * th;
+  const float sn = th * (1.0f + t2 * (-1.0f / 6.0f + t2 * (1.0f / 120.0f + t2 * (-1.0f / 5040.0f + t2 * (1.0f / 362880.0f)))));
+  const float cs = 1.0f + t2 * (-0.5f + t2 * (1.0f / 24.0f + t2 * (-1.0f / 720.0f + t2 * (1.0f / 40320.0f))));
+  const int qi = ((int)q4) & 3;
+  const bool swap = (qi & 1) != 0;
+  const float x = swap ? sn : cs, y = swap ? cs : sn;
+  const uint32_t nc = (uint32_t)((qi + 1) & 2) << 30, ns = (uint32_t)(qi & 2) << 30;
+  return make_float2(__uint_as_float(__float_as_uint(x) ^ nc), __uint_as_float(__float_as_uint(y) ^ ns));
+}
+
 struct FftArgs {
   void* data;            // grids [planes][chan][N][N] (complex)
   int nbatch;            // plain mode: number of planes
@@ -487,7 +542,7 @@ __global__ __launch_bounds__(Shape<N>::FNT * Shape<N>::FNB) void k_rows_predict(
   const int b = threadIdx.x / NT, t = threadIdx.x % NT;
   const int y = (blockIdx.x % RB) * NB + b;
   R mv[PER][8];
-  double qs[PER][8];
+  ScreenQ<R> qs[PER][8];
 #pragma unroll
   for (int q = 0; q < PER; ++q)
 #pragma unroll
@@ -497,19 +552,19 @@ __global__ __launch_bounds__(Shape<N>::FNT * Shape<N>::FNB) void k_rows_predict(
       double m = (double)model[gi] * a.gcf[gi];
       if ((x + y) & 1) m = -m;
       bool outside = false;
-      qs[q][r] = a.screens ? screen_q(y, x, N, a.fov, &outside) : 0.0;
+      qs[q][r] = screen_split<R>(a.screens ? screen_q(y, x, N, a.fov, &outside) : 0.0);
       mv[q][r] = outside ? R(0) : (R)m;
     }
   const int p0 = blockIdx.y * a.ppg, p1 = min(a.nplanes, p0 + a.ppg);  // this block's plane group
   for (int p = p0; p < p1; ++p) {
-    const double wp = a.screens ? a.plane_w[p] : 0.0;
+    const ScreenQ<R> wsp = screen_split<R>(a.screens ? a.plane_w[p] : 0.0);
     cplx<R>* dst = grids + ((size_t)p * a.nchan + c) * N * N + (size_t)y * N;
     const uint32_t cm = a.colmask ? a.colmask[(size_t)p * a.nchan + c] : ~0u;
     fft_pass<R, N, NB, NT, false, LAY_ROWPAD>(
         lds, b, t, tw,
         [&](int q, int r, int) {
           if (!a.screens) return cmk<R>(mv[q][r], R(0));
-          const cplx<R> s = expi_turns<R>(screen_turns(wp, qs[q][r]));  // b; conj(b) = (s.x, -s.y)
+          const cplx<R> s = screen_phasor<R>(wsp, qs[q][r]);  // b; conj(b) = (s.x, -s.y)
           return cmk<R>(mv[q][r] * s.x, -mv[q][r] * s.y);
         },
         [&](int, int, int x, cplx<R> v) {
@@ -586,7 +641,7 @@ __global__ __launch_bounds__(Shape<N>::NT * Shape<N>::ROW_NB) void k_rows_invert
   const int p0 = blockIdx.y * a.ppg, p1 = min(a.nplanes, p0 + a.ppg);
   R acc[PER][RADL];
   R acci[SCREENS ? 1 : PER][SCREENS ? 1 : RADL];
-  double qs[SCREENS ? PER : 1][SCREENS ? RADL : 1];
+  ScreenQ<R> qs[SCREENS ? PER : 1][SCREENS ? RADL : 1];
   uint32_t outside = 0;  // bit q * RADL + r: element outside the unit circle (no screen)
   static_assert(PER * RADL <= 32, "outside mask");
 #pragma unroll
@@ -598,7 +653,7 @@ __global__ __launch_bounds__(Shape<N>::NT * Shape<N>::ROW_NB) void k_rows_invert
         acci[q][r] = R(0);
       } else {
         bool o;
-        qs[q][r] = screen_q(y, t + q * NT + r * NSL, N, a.fov, &o);
+        qs[q][r] = screen_split<R>(screen_q(y, t + q * NT + r * NSL, N, a.fov, &o));
         if (o) outside |= 1u << (q * RADL + r);
       }
     }
@@ -606,12 +661,12 @@ __global__ __launch_bounds__(Shape<N>::NT * Shape<N>::ROW_NB) void k_rows_invert
   // by its sum of weights (imaging_serial.py:193-194)
   const bool weigh = a.wstack && a.normalize == 2;
   // one plane's contribution to the accumulators, element x = t + q*NT + r*NSL of the row transform
-  auto accumulate = [&](double wp, R wsc, int q, int r, int x, cplx<R> v) {
+  auto accumulate = [&](const ScreenQ<R>& wp, R wsc, int q, int r, int x, cplx<R> v) {
     if ((x + y) & 1) v = cmk<R>(-v.x, -v.y);
     if constexpr (SCREENS) {
       // the screen is 0 outside the unit circle (selected, not branched on: the pixels of a row
       // are in and out together in no useful pattern)
-      const cplx<R> s = expi_turns<R>(screen_turns(wp, qs[q][r]));
+      const cplx<R> s = screen_phasor<R>(wp, qs[q][r]);
       const R val = wsc * (s.x * v.x - s.y * v.y);
       acc[q][r] += ((outside >> (q * RADL + r)) & 1u) ? R(0) : val;
     } else {
@@ -646,7 +701,7 @@ __global__ __launch_bounds__(Shape<N>::NT * Shape<N>::ROW_NB) void k_rows_invert
     while (p < p1) {
       const double sw = a.sumwt[(size_t)p * a.nchan + c];
       const R wsc = weigh ? (R)sw : R(1);
-      const double wp = SCREENS ? a.plane_w[p] : 0.0;
+      const ScreenQ<R> wp = screen_split<R>(SCREENS ? a.plane_w[p] : 0.0);
       int pn = p + 1;
       while (pn < p1 && skip(pn)) ++pn;
       fft_pass_v<R, N, NB, NT, true, inv_lay<R>()>(
@@ -665,7 +720,7 @@ __global__ __launch_bounds__(Shape<N>::NT * Shape<N>::ROW_NB) void k_rows_invert
       const R wsc = weigh ? (R)sw : R(1);
       const cplx<R>* row = grids + ((size_t)p * a.nchan + c) * N * N + (size_t)perm_row_of(y, N, a.perm_n1) * N;
       const uint32_t cm = a.colmask ? a.colmask[(size_t)p * a.nchan + c] : ~0u;
-      const double wp = SCREENS ? a.plane_w[p] : 0.0;
+      const ScreenQ<R> wp = screen_split<R>(SCREENS ? a.plane_w[p] : 0.0);
       fft_pass<R, N, NB, NT, true, inv_lay<R>()>(
           lds, b, t, tw,
           [&](int, int, int n) {
