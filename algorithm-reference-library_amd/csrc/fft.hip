@@ -316,6 +316,23 @@ struct Shape {
     constexpr int c = nb > 16 ? 16 : (nb < 1 ? 1 : nb);
     return c >= 16 ? 16 : c >= 8 ? 8 : c >= 4 ? 4 : c >= 2 ? 2 : 1;  // a power of two dividing N
   }
+  // Columns of the split pass (k_cols_split: one radix-2 step in registers, then two transforms of
+  // N/2 through LDS): 128-byte row segments where N/2 of them and the twiddles of N/2 fit LDS,
+  // else 64-byte ones; 0 = the plain pass.  Used from N = 1024 to 4096, where it measured faster
+  // than the plain pass of col_nb() columns (dense 2-D transforms, complex64: 1024^2 2.75 -> 2.58
+  // ms for 256 planes, 2048^2 3.08 -> 2.77 ms for 64, 4096^2 4.21 -> 3.29 ms for 16 (64-byte
+  // segments; 128-byte ones would not fit); complex128: 1024^2 2.77 -> 2.68 ms, 2048^2 3.16 ->
+  // 2.71 ms); 64-byte segments where 128-byte ones fit measured slower (complex64 at 2048^2: 2.87
+  // ms; at 1024^2: 2.79 ms, slower than the plain pass).
+  template <typename R>
+  static constexpr int split_nb() {
+    if (N < 1024 || N > 4096 || col_nb<R>() < 2) return 0;
+    constexpr size_t tw = sizeof(cplx<R>) * tw_lds_elems<R, N / 2>();
+    constexpr int nb128 = 128 / (int)sizeof(cplx<R>), nb64 = 64 / (int)sizeof(cplx<R>);
+    if ((size_t)(N / 2) * 128 + tw <= 163840) return nb128;
+    if ((size_t)(N / 2) * 64 + tw <= 163840) return nb64;
+    return 0;
+  }
   // the invert column pass also holds per-pixel accumulators: half the columns, twice the registers
   template <typename R>
   static constexpr int inv_nb() {
@@ -615,6 +632,68 @@ __global__ __launch_bounds__(Shape<N>::NT * Shape<N>::template col_nb<R>()) void
         if (a.out_mod && ((n + x) & 1)) sc = -sc;
         col[(size_t)n * N] = cmk<R>(v.x * sc, v.y * sc);
       });
+}
+
+// The column pass with half the LDS: one radix-2 decimation-in-frequency step in registers,
+// a_n = x_n + x_(n+N/2) and b_n = (x_n - x_(n+N/2)) W_N^n, then X_2m = DFT_(N/2)(a)_m and
+// X_(2m+1) = DFT_(N/2)(b)_m as two LDS-staged transforms of N/2, one after the other, while b
+// waits in registers.  NB columns per block, N/32 threads per column (32 values each).
+template <typename R, int N, bool INV, int NB>
+__global__ __launch_bounds__(Shape<N / 2>::NT * NB) void k_cols_split(FftArgs a, const cplx<R>* __restrict__ twh,
+                                                                     const cplx<R>* __restrict__ twn) {
+  constexpr int H = N / 2, NT = Shape<H>::NT, K = N / NT, KH = K / 2;
+  static_assert(K == 32 && (H / 8) / NT == 2, "32 values per thread, two first-stage butterflies");
+  extern __shared__ __align__(16) unsigned char smem[];
+  cplx<R>* lds = reinterpret_cast<cplx<R>*>(smem);
+  const cplx<R>* tw = stage_twiddles_at<R, H>(smem, sizeof(cplx<R>) * (size_t)H * NB, twh);
+  constexpr int CB = N / NB;
+  constexpr int LG = (NB * (int)sizeof(cplx<R>)) >= 128 ? 1 : 128 / (NB * (int)sizeof(cplx<R>));
+  const int bid = xcd_block(blockIdx.x, gridDim.x, LG);
+  const int64_t plane = bid / CB;
+  const int x0 = (bid % CB) * NB;
+  uint32_t tm = ~0u;
+  if (a.tilemask) {
+    tm = a.tilemask[(size_t)plane * a.ntx + (x0 >> a.tshift)];
+    if (!tm) return;
+    if (!a.zero_fill) tm = ~0u;
+  } else if (a.colmask && !((a.colmask[plane] >> (x0 >> a.tshift)) & 1u)) {
+    return;
+  }
+  const int b = threadIdx.x % NB, t = threadIdx.x / NB;
+  const int x = x0 + b;
+  cplx<R>* col = reinterpret_cast<cplx<R>*>(a.data) + plane * (int64_t)N * N + x;
+  // value k of this thread is element t + k*NT; element t + k*NT + N/2 is value k + KH
+  cplx<R> lo[KH], hi[KH];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int n = t + k * NT;
+    const cplx<R>* src = ((tm >> (n >> a.tshift)) & 1u) ? col + (size_t)n * N : zero_entry<R>(a.zeros);
+    cplx<R> v = *src;
+    if (a.in_mod && ((n + x) & 1)) v = cmk<R>(-v.x, -v.y);
+    if (k < KH) lo[k] = v;
+    else hi[k - KH] = v;
+  }
+  // butterflies: lo becomes a (first-stage layout of the half transform: element t + q*NT + r*H/8
+  // is value q + 2r), hi becomes b
+  cplx<R> va[2][8], vb[2][8];
+#pragma unroll
+  for (int k = 0; k < KH; ++k) {
+    cplx<R> w = twn[t + k * NT];
+    if (INV) w.y = -w.y;
+    va[k & 1][k >> 1] = cadd(lo[k], hi[k]);
+    vb[k & 1][k >> 1] = cmul(csub(lo[k], hi[k]), w);
+  }
+  const R sc0 = (R)a.scale;
+  auto store = [&](int n, cplx<R> v) {
+    R sc = sc0;
+    if (a.out_mod && ((n + x) & 1)) sc = -sc;
+    col[(size_t)n * N] = cmk<R>(v.x * sc, v.y * sc);
+  };
+  fft_pass_v<R, H, NB, NT, INV, LAY_COL>(
+      lds, b, t, tw, va, [] {}, [&](int, int, int m, cplx<R> v) { store(2 * m, v); });
+  __syncthreads();
+  fft_pass_v<R, H, NB, NT, INV, LAY_COL>(
+      lds, b, t, tw, vb, [] {}, [&](int, int, int m, cplx<R> v) { store(2 * m + 1, v); });
 }
 
 // Invert, after the in-place column pass: for plane group g (blockIdx.y) and the NB rows of this
@@ -1082,6 +1161,19 @@ static int run_cols(FftArgs c, const cplx<R>* tw, hipStream_t s) {
     ARLMI_LAUNCH("k_transpose", k_transpose<R>, tg, dim3(256), 0, s, (cplx<R>*)c.data, N);
     ARLMI_LAUNCH_CHECK();
   } else {
+    if constexpr (Shape<N>::template split_nb<R>() > 0) {
+      constexpr int SNB = Shape<N>::template split_nb<R>();
+      const cplx<R>* twh = reinterpret_cast<const cplx<R>*>(twiddles(N / 2, sizeof(R) == 4 ? 32 : 64));
+      if (!twh) {
+        set_error("twiddle table allocation failed");
+        return ARLMI_E_HIP;
+      }
+      const size_t sh = sizeof(cplx<R>) * ((size_t)(N / 2) * SNB + tw_lds_elems<R, N / 2>());
+      ARLMI_LAUNCH("k_cols_split", (k_cols_split<R, N, INV, SNB>), dim3((unsigned)((int64_t)c.nbatch * N / SNB)),
+                   dim3(Shape<N / 2>::NT * SNB), sh, s, c, twh, tw);
+      ARLMI_LAUNCH_CHECK();
+      return ARLMI_OK;
+    }
     constexpr int CNB = Shape<N>::template col_nb<R>();
     const size_t csh = sizeof(cplx<R>) * ((size_t)N * CNB + tw_lds_elems<R, N>());
     const unsigned cblocks = (unsigned)((int64_t)c.nbatch * N / CNB);

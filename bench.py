@@ -33,7 +33,7 @@ sys.path.insert(0, os.path.join(ROOT, "algorithm-reference-library_amd"))
 METRIC = "Mvis/s invert+predict at 2048² w-stack, FP32 vs FP64 GPU, ≤1e-5 rel err"
 # kernels timed with events inside the headline step: gridding, degridding, the plane passes and
 # the plan's two scatter passes (the shorter plan kernels run untimed)
-STEP_KERNELS = ("k_grid_sep,k_grid_tab,k_degrid_sep,k_degrid_tab,k_cols_plain,k_rows_invert,k_rows_predict,"
+STEP_KERNELS = ("k_grid_sep,k_grid_tab,k_degrid_sep,k_degrid_tab,k_cols_plain,k_cols_split,k_rows_invert,k_rows_predict,"
                 "k_p1_scatter,k_p2_scatter")
 
 # HBM bytes each kernel must move per launch (algorithmic, not measured), as functions of
@@ -43,7 +43,7 @@ def _kernel_bytes(name, N, P, V, E, prec):
     r = c // 2
     if name == "k_rows_plain":
         return 2 * c * N * N * P
-    if name == "k_cols_plain":
+    if name in ("k_cols_plain", "k_cols_split"):
         return 2 * c * N * N * P
     if name == "k_cols_invert":
         return c * N * N * P + 8 * N * N + r * N * N

@@ -122,7 +122,7 @@ def test_degrid_parity(gpu, precision):
 
 
 @pytest.mark.parametrize("precision", [64, 32])
-@pytest.mark.parametrize("n", [64, 256, 512, 2048, 4096])
+@pytest.mark.parametrize("n", [64, 256, 512, 1024, 2048, 4096])
 def test_fft_parity(gpu, precision, n):
     """ARL's shifted fft/ifft against numpy.fft on the same planes."""
     import torch
