@@ -319,11 +319,11 @@ struct Shape {
   // Columns of the split pass (k_cols_split: one radix-2 step in registers, then two transforms of
   // N/2 through LDS): 128-byte row segments where N/2 of them and the twiddles of N/2 fit LDS,
   // else 64-byte ones; 0 = the plain pass.  Used from N = 1024 to 4096, where it measured faster
-  // than the plain pass of col_nb() columns (dense 2-D transforms, complex64: 1024^2 2.75 -> 2.58
-  // ms for 256 planes, 2048^2 3.08 -> 2.77 ms for 64, 4096^2 4.21 -> 3.29 ms for 16 (64-byte
-  // segments; 128-byte ones would not fit); complex128: 1024^2 2.77 -> 2.68 ms, 2048^2 3.16 ->
-  // 2.71 ms); 64-byte segments where 128-byte ones fit measured slower (complex64 at 2048^2: 2.87
-  // ms; at 1024^2: 2.79 ms, slower than the plain pass).
+  // than the plain pass of col_nb() columns (arlmi.fft of dense planes, an input copy included,
+  // complex64: 1024^2 2.75 -> 2.58 ms for 256 planes, 2048^2 3.08 -> 2.77 ms for 64, 4096^2 4.21
+  // -> 3.29 ms for 16 (64-byte segments; 128-byte ones would not fit); complex128: 1024^2 2.77 ->
+  // 2.68 ms, 2048^2 3.16 -> 2.71 ms); 64-byte segments where 128-byte ones fit measured slower
+  // (complex64 at 2048^2: 2.87 ms; at 1024^2: 2.79 ms, slower than the plain pass).
   template <typename R>
   static constexpr int split_nb() {
     if (N < 1024 || N > 4096 || col_nb<R>() < 2) return 0;
