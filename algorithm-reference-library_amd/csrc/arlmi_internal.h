@@ -270,6 +270,7 @@ struct arlmi_plan {
   uint32_t* d_hbase = nullptr;           // [P+1] plane bases in h2, then [P] pass-2 blocks per plane
   uint32_t *d_h1 = nullptr, *d_s1 = nullptr;
   uint32_t *d_h2 = nullptr, *d_s2 = nullptr;
+  uint32_t* d_c2 = nullptr;              // [2 ntiles + 1] pass-2 bin sums, then [2 ntiles + 2] their scan
   uint32_t* d_scan_tmp = nullptr;
   void* d_recs = nullptr;                // 16-byte membership records
   void* d_blk2 = nullptr;                // pass-2 block descriptors
