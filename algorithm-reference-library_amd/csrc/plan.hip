@@ -346,10 +346,10 @@ __device__ inline Enum enumerate_row(const DevGeom& g, const double* __restrict_
 constexpr int K1_ROWS = 4096;  // rows per pass-1 block
 constexpr int K2_RECS = 8192;  // records per pass-2 block, at least
 // ... and at least K2_HIST times the words of the block's tile histogram (2 tpp), so that with many
-// tiles per plane (128^2 at 8192^2) the per-block histograms that are written (one word per block
-// apart), scanned and read back stay small against the records themselves.  Measured on the
-// streamed 8192^2 workload (16 M-row chunks, 128 slabs): 1 -> 302 ms per step, 2 -> 297 ms, 8 -> 314
-// ms (one block per plane leaves CUs idle), 8192-record blocks alone -> 338 ms.
+// tiles per plane (128^2 at 8192^2) the per-block histograms that are written, summed and read
+// back stay small against the records themselves.  Measured on the streamed 8192^2 workload (16
+// M-row chunks, 128 slabs, float32): 0 (8192-record blocks) -> 270.5 ms per step, 1 -> 256.4 ms,
+// 2 -> 255.3 ms, 4 -> 261.4 ms.
 constexpr int K2_HIST = 2;
 constexpr int K2_UNROLL = 4;  // record loads in flight per thread
 
@@ -568,76 +568,109 @@ __global__ void k_p2_scatter(DevGeom g, const Rec* __restrict__ recs, const Blk2
   }
 }
 
-// The two kernels below give one work group to 64 bins of one plane (blockIdx.y) and split the
-// plane's pass-2 blocks into 16 consecutive slices, one per 64 threads: a large plane (hundreds
-// of blocks) is then no longer one long chain of loads per bin.
-constexpr int H2_BINS = 64, H2_SLICES = 16, H2_BATCH = 8;
+// The two kernels below split the pass-2 blocks of a plane (blockIdx.y) into ns consecutive
+// slices (a power of two <= 16, chosen on the host from the largest block count of a plane) and
+// give each (group of BPT consecutive bins, slice) one thread: a large plane (hundreds of
+// blocks at 2048^2) is then no longer one long chain of loads per bin, and the many bins of
+// planes with few blocks (128^2 tiles at 8192^2, 4 bins a thread) do not each take a thread of
+// their own.  16 loads in flight per thread.
+constexpr int H2_THREADS = 1024;
 
-// The sum of h2[plane base + b * nbin + bin] over the blocks b of this thread's slice, H2_BATCH
-// independent loads in flight.
-__device__ inline uint32_t h2_slice_sum(const uint32_t* __restrict__ h, int nbin, uint32_t b0, uint32_t b1) {
-  uint32_t sum = 0;
+// Sums over the blocks [b0, b1) of this thread's slice of h[b * nbin + u], u < BPT (bins past
+// nbin read as 0), H2_BATCH blocks of loads in flight.
+template <int BPT, int H2_BATCH = 16 / BPT>
+__device__ inline void h2_slice_sum(const uint32_t* __restrict__ h, int nbin, int bin0, uint32_t b0, uint32_t b1,
+                                    uint32_t (&sum)[BPT]) {
+#pragma unroll
+  for (int u = 0; u < BPT; ++u) sum[u] = 0u;
   for (uint32_t b = b0; b < b1; b += H2_BATCH) {
-    uint32_t v[H2_BATCH];
+    uint32_t v[H2_BATCH][BPT];
 #pragma unroll
-    for (int u = 0; u < H2_BATCH; ++u) v[u] = b + u < b1 ? h[(size_t)(b + u) * nbin] : 0u;
+    for (int k = 0; k < H2_BATCH; ++k)
 #pragma unroll
-    for (int u = 0; u < H2_BATCH; ++u) sum += v[u];
+      for (int u = 0; u < BPT; ++u)
+        v[k][u] = (b + k < b1 && bin0 + u < nbin) ? h[(size_t)(b + k) * nbin + bin0 + u] : 0u;
+#pragma unroll
+    for (int k = 0; k < H2_BATCH; ++k)
+#pragma unroll
+      for (int u = 0; u < BPT; ++u) sum[u] += v[k][u];
   }
-  return sum;
 }
 
 // c2[plane * nbin + bin] = sum over the plane's pass-2 blocks of h2[plane base + block * nbin + bin]
 // (nbin = 2 tpp), then c2[P * nbin] = 0, so that the exclusive scan of c2 ends with the total.
-__global__ __launch_bounds__(H2_BINS * H2_SLICES) void k_h2_colsum(const uint32_t* __restrict__ h2,
-                                                                   const uint32_t* __restrict__ hbase,
-                                                                   const uint32_t* __restrict__ nbp, int P, int nbin,
-                                                                   uint32_t* __restrict__ c2) {
-  __shared__ uint32_t part[H2_SLICES][H2_BINS];
-  const int p = blockIdx.y;
-  const int bl = threadIdx.x % H2_BINS, j = threadIdx.x / H2_BINS;
-  const int bin = blockIdx.x * H2_BINS + bl;
-  const uint32_t nb = nbp[p], len = (nb + H2_SLICES - 1) / H2_SLICES;
+template <int BPT>
+__global__ __launch_bounds__(H2_THREADS) void k_h2_colsum(const uint32_t* __restrict__ h2,
+                                                          const uint32_t* __restrict__ hbase,
+                                                          const uint32_t* __restrict__ nbp, int P, int nbin, int ns,
+                                                          uint32_t* __restrict__ c2) {
+  __shared__ uint32_t part[BPT][H2_THREADS];
+  const int p = blockIdx.y, bg = H2_THREADS / ns;
+  const int bl = threadIdx.x % bg, j = threadIdx.x / bg;
+  const int bin0 = (blockIdx.x * bg + bl) * BPT;
+  const uint32_t nb = nbp[p], len = (nb + ns - 1) / ns;
   const uint32_t b0 = min(nb, (uint32_t)j * len), b1 = min(nb, b0 + len);
-  part[j][bl] = bin < nbin ? h2_slice_sum(h2 + hbase[p] + bin, nbin, b0, b1) : 0u;
-  __syncthreads();
-  if (j == 0 && bin < nbin) {
-    uint32_t t = 0;
+  uint32_t v[BPT];
+  h2_slice_sum<BPT>(h2 + hbase[p], nbin, bin0, b0, b1, v);
+  if (ns > 1) {
 #pragma unroll
-    for (int k = 0; k < H2_SLICES; ++k) t += part[k][bl];
-    c2[(int64_t)p * nbin + bin] = t;
+    for (int u = 0; u < BPT; ++u) part[u][threadIdx.x] = v[u];
+    __syncthreads();
+  }
+  if (j == 0) {
+    for (int k = 1; k < ns; ++k)
+#pragma unroll
+      for (int u = 0; u < BPT; ++u) v[u] += part[u][k * bg + bl];
+#pragma unroll
+    for (int u = 0; u < BPT; ++u)
+      if (bin0 + u < nbin) c2[(int64_t)p * nbin + bin0 + u] = v[u];
   }
   if (p == 0 && blockIdx.x == 0 && threadIdx.x == 0) c2[(int64_t)P * nbin] = 0u;
 }
 
 // Block cursors: s2[plane base + block * nbin + bin] = e2[plane * nbin + bin] + the counts of the
 // plane's earlier blocks in the bin (earlier slices from the slice sums, then the slice's own).
-__global__ __launch_bounds__(H2_BINS * H2_SLICES) void k_h2_prefix(const uint32_t* __restrict__ h2,
-                                                                   const uint32_t* __restrict__ hbase,
-                                                                   const uint32_t* __restrict__ nbp, int P, int nbin,
-                                                                   const uint32_t* __restrict__ e2,
-                                                                   uint32_t* __restrict__ s2) {
-  __shared__ uint32_t part[H2_SLICES][H2_BINS];
-  const int p = blockIdx.y;
-  const int bl = threadIdx.x % H2_BINS, j = threadIdx.x / H2_BINS;
-  const int bin = blockIdx.x * H2_BINS + bl;
-  const uint32_t nb = nbp[p], len = (nb + H2_SLICES - 1) / H2_SLICES;
+template <int BPT>
+__global__ __launch_bounds__(H2_THREADS) void k_h2_prefix(const uint32_t* __restrict__ h2,
+                                                          const uint32_t* __restrict__ hbase,
+                                                          const uint32_t* __restrict__ nbp, int P, int nbin, int ns,
+                                                          const uint32_t* __restrict__ e2, uint32_t* __restrict__ s2) {
+  __shared__ uint32_t part[BPT][H2_THREADS];
+  const int p = blockIdx.y, bg = H2_THREADS / ns;
+  const int bl = threadIdx.x % bg, j = threadIdx.x / bg;
+  const int bin0 = (blockIdx.x * bg + bl) * BPT;
+  const uint32_t nb = nbp[p], len = (nb + ns - 1) / ns;
   const uint32_t b0 = min(nb, (uint32_t)j * len), b1 = min(nb, b0 + len);
-  const size_t base = (size_t)hbase[p] + bin;
-  part[j][bl] = bin < nbin ? h2_slice_sum(h2 + base, nbin, b0, b1) : 0u;
-  __syncthreads();
-  if (bin >= nbin) return;
-  uint32_t run = e2[(int64_t)p * nbin + bin];
-  for (int k = 0; k < j; ++k) run += part[k][bl];
+  const uint32_t* h = h2 + hbase[p];
+  uint32_t* sp = s2 + hbase[p];
+  uint32_t run[BPT];
+  if (ns > 1) {
+    h2_slice_sum<BPT>(h, nbin, bin0, b0, b1, run);
+#pragma unroll
+    for (int u = 0; u < BPT; ++u) part[u][threadIdx.x] = run[u];
+    __syncthreads();
+  }
+  if (bin0 >= nbin) return;
+#pragma unroll
+  for (int u = 0; u < BPT; ++u) run[u] = bin0 + u < nbin ? e2[(int64_t)p * nbin + bin0 + u] : 0u;
+  for (int k = 0; k < j; ++k)
+#pragma unroll
+    for (int u = 0; u < BPT; ++u) run[u] += part[u][k * bg + bl];
+  constexpr int H2_BATCH = 16 / BPT;
   for (uint32_t b = b0; b < b1; b += H2_BATCH) {
-    uint32_t v[H2_BATCH];
+    uint32_t v[H2_BATCH][BPT];
 #pragma unroll
-    for (int u = 0; u < H2_BATCH; ++u) v[u] = b + u < b1 ? h2[base + (size_t)(b + u) * nbin] : 0u;
+    for (int k = 0; k < H2_BATCH; ++k)
 #pragma unroll
-    for (int u = 0; u < H2_BATCH; ++u) {
-      if (b + u < b1) s2[base + (size_t)(b + u) * nbin] = run;
-      run += v[u];
-    }
+      for (int u = 0; u < BPT; ++u)
+        v[k][u] = (b + k < b1 && bin0 + u < nbin) ? h[(size_t)(b + k) * nbin + bin0 + u] : 0u;
+#pragma unroll
+    for (int k = 0; k < H2_BATCH; ++k)
+#pragma unroll
+      for (int u = 0; u < BPT; ++u) {
+        if (b + k < b1 && bin0 + u < nbin) sp[(size_t)(b + k) * nbin + bin0 + u] = run[u];
+        run[u] += v[k][u];
+      }
   }
 }
 
@@ -1170,13 +1203,31 @@ static int plan_build(arlmi_plan* p, const double* uvw, const double* freq, int6
   uint32_t* c2 = p->d_c2;
   uint32_t* e2 = p->d_c2 + (nk + 1);
   if ((rc = ensure_scan_tmp(p, nk + 1))) return rc;
-  const dim3 hgrid((unsigned)((nbin + H2_BINS - 1) / H2_BINS), (unsigned)P);
-  ARLMI_LAUNCH("k_h2_colsum", k_h2_colsum, hgrid, dim3(H2_BINS * H2_SLICES), 0, s, p->d_h2, p->d_hbase,
-               p->d_hbase + P + 1, P, nbin, c2);
+  // slices of a plane's blocks: at least 16 blocks a slice for the largest plane, at most 16 slices
+  int ns = 1;
+  {
+    uint32_t nbmax = 1;
+    for (int pl = 0; pl < P; ++pl) nbmax = std::max(nbmax, nbp[pl]);
+    while (ns < 16 && (uint32_t)ns * 16 < nbmax) ns *= 2;
+  }
+  // 4 bins a thread where the bins alone would give more than 2^20 threads
+  const bool wide = nk > (1 << 20);
+  const int64_t bins_per_group = (int64_t)(H2_THREADS / ns) * (wide ? 4 : 1);
+  const dim3 hgrid((unsigned)((nbin + bins_per_group - 1) / bins_per_group), (unsigned)P);
+  if (wide)
+    ARLMI_LAUNCH("k_h2_colsum", k_h2_colsum<4>, hgrid, dim3(H2_THREADS), 0, s, p->d_h2, p->d_hbase,
+                 p->d_hbase + P + 1, P, nbin, ns, c2);
+  else
+    ARLMI_LAUNCH("k_h2_colsum", k_h2_colsum<1>, hgrid, dim3(H2_THREADS), 0, s, p->d_h2, p->d_hbase,
+                 p->d_hbase + P + 1, P, nbin, ns, c2);
   ARLMI_LAUNCH_CHECK();
   if ((rc = exclusive_scan_u32(c2, e2, nk + 1, p->d_scan_tmp, s))) return rc;
-  ARLMI_LAUNCH("k_h2_prefix", k_h2_prefix, hgrid, dim3(H2_BINS * H2_SLICES), 0, s, p->d_h2, p->d_hbase,
-               p->d_hbase + P + 1, P, nbin, e2, p->d_s2);
+  if (wide)
+    ARLMI_LAUNCH("k_h2_prefix", k_h2_prefix<4>, hgrid, dim3(H2_THREADS), 0, s, p->d_h2, p->d_hbase,
+                 p->d_hbase + P + 1, P, nbin, ns, e2, p->d_s2);
+  else
+    ARLMI_LAUNCH("k_h2_prefix", k_h2_prefix<1>, hgrid, dim3(H2_THREADS), 0, s, p->d_h2, p->d_hbase,
+                 p->d_hbase + P + 1, P, nbin, ns, e2, p->d_s2);
   ARLMI_LAUNCH_CHECK();
   ARLMI_LAUNCH("k_p2_keys", k_p2_keys, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, e2, nt, p->d_cnt,
                p->d_cntd, p->d_off);
