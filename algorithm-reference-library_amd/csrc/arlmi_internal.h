@@ -272,7 +272,8 @@ struct arlmi_plan {
   uint32_t *d_h2 = nullptr, *d_s2 = nullptr;
   uint32_t* d_c2 = nullptr;              // [2 ntiles + 1] pass-2 bin sums, then [2 ntiles + 2] their scan
   uint32_t* d_scan_tmp = nullptr;
-  void* d_recs = nullptr;                // 16-byte membership records
+  void* d_recs = nullptr;                // membership records (rec_words 32-bit words each, the row first)
+  int rec_words = 4;
   void* d_blk2 = nullptr;                // pass-2 block descriptors
   uint2* d_entries = nullptr;            // (row, packed)
   uint16_t* d_entry_z = nullptr;         // kernel w plane per entry (nz > 1)

@@ -525,8 +525,9 @@ struct BatchDesc {
   void* vis_out;
   const int4* multi;       // split tiles
   const uint32_t* nmulti;  // their number (device)
-  const uint32_t* orphans; // records of rows in no plane (4 words each), or null
+  const uint32_t* orphans; // records of rows in no plane (orphan_words words each, the row first), or null
   int64_t norphans;
+  int orphan_words;
   int os, T, ntx, N, nsw;
   int gbase, ngitems, dbase, nditems;
   int perm_n1;  // row order of the grids the degridder reads (perm_row_of)
@@ -988,7 +989,7 @@ template <typename R>
 __global__ void k_batch_zero_rows(const BatchDesc* __restrict__ descs) {
   const BatchDesc& d = descs[blockIdx.x];
   cplx<R>* vis = (cplx<R>*)d.vis_out;
-  for (int64_t i = threadIdx.x; i < d.norphans; i += blockDim.x) vis[d.orphans[4 * i]] = mk<R>(R(0), R(0));
+  for (int64_t i = threadIdx.x; i < d.norphans; i += blockDim.x) vis[d.orphans[(int64_t)d.orphan_words * i]] = mk<R>(R(0), R(0));
 }
 
 // ---------------------------------------------------------------------------
@@ -1182,7 +1183,8 @@ static int degrid_impl(const arlmi_plan* p, const void* grids, const arlmi_kerne
   const DevGeom& g = p->g;
   if (p->norphans > 0) {
     ARLMI_LAUNCH("k_zero_rows", k_zero_rows<R>, dim3((unsigned)((p->norphans + 255) / 256)), dim3(256), 0, s,
-                 reinterpret_cast<const uint32_t*>(p->d_recs) + 4 * p->orphan_start, 4, p->norphans,
+                 reinterpret_cast<const uint32_t*>(p->d_recs) + (int64_t)p->rec_words * p->orphan_start, p->rec_words,
+                 p->norphans,
                  (cplx<R>*)vis_out);
     ARLMI_LAUNCH_CHECK();
   }
@@ -1337,7 +1339,9 @@ int batch_build(arlmi_batch* b, const arlmi_batch_item* in, int n) {
     d.vis_out = it.d_vis_out;
     d.multi = p->nslots > 0 ? p->d_multi : nullptr;
     d.nmulti = p->d_misc + 4;
-    d.orphans = p->norphans > 0 ? reinterpret_cast<const uint32_t*>(p->d_recs) + 4 * p->orphan_start : nullptr;
+    d.orphans = p->norphans > 0 ? reinterpret_cast<const uint32_t*>(p->d_recs) + (int64_t)p->rec_words * p->orphan_start
+                                : nullptr;
+    d.orphan_words = p->rec_words;
     d.norphans = p->norphans;
     d.os = g.os;
     d.T = g.tile;

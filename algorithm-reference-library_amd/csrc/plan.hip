@@ -353,12 +353,55 @@ constexpr int K2_RECS = 8192;  // records per pass-2 block, at least
 constexpr int K2_HIST = 2;
 constexpr int K2_UNROLL = 4;  // record loads in flight per thread
 
+// Membership records: 16 bytes in general, 8 where the grid side is at most 4096, the
+// oversampling at most 8, and there is one image channel and one kernel w plane (pu, pv, xoff,
+// yoff and the last-membership flag then fit one word beside the row).  The first word is the row
+// in both (the degridders read the rows of orphan records with a stride of rec_words).
 struct Rec {
   uint32_t row;
   uint32_t uv;   // pu | pv << 16
   uint32_t aux;  // xoff | yoff << 8 | chan << 16 | last-membership << 24
   uint32_t z;    // kernel w plane
 };
+struct Rec8 {
+  uint32_t row;
+  uint32_t pk;  // pu | pv << 12 | xoff << 24 | yoff << 27 | last-membership << 30
+};
+template <typename RT>
+struct RecOps;
+template <>
+struct RecOps<Rec> {
+  static __device__ Rec make(uint32_t row, const RowMap& m, bool last, int z) {
+    return Rec{row, (uint32_t)m.pu | ((uint32_t)m.pv << 16),
+               (uint32_t)m.xoff | ((uint32_t)m.yoff << 8) | ((uint32_t)m.chan << 16) | (last ? (1u << 24) : 0u),
+               (uint32_t)z};
+  }
+  static __device__ Rec orphan(uint32_t row) { return Rec{row, 0u, 0u, 0u}; }
+  static __device__ int pu(const Rec& r) { return (int)(r.uv & 0xffffu); }
+  static __device__ int pv(const Rec& r) { return (int)(r.uv >> 16); }
+  static __device__ int xoff(const Rec& r) { return (int)(r.aux & 0xffu); }
+  static __device__ int yoff(const Rec& r) { return (int)((r.aux >> 8) & 0xffu); }
+  static __device__ int chan(const Rec& r) { return (int)((r.aux >> 16) & 0xffu); }
+  static __device__ bool last(const Rec& r) { return (r.aux >> 24) & 1u; }
+  static __device__ int z(const Rec& r) { return (int)r.z; }
+};
+template <>
+struct RecOps<Rec8> {
+  static __device__ Rec8 make(uint32_t row, const RowMap& m, bool last, int) {
+    return Rec8{row, (uint32_t)m.pu | ((uint32_t)m.pv << 12) | ((uint32_t)m.xoff << 24) | ((uint32_t)m.yoff << 27) |
+                         (last ? (1u << 30) : 0u)};
+  }
+  static __device__ Rec8 orphan(uint32_t row) { return Rec8{row, 0u}; }
+  static __device__ int pu(const Rec8& r) { return (int)(r.pk & 0xfffu); }
+  static __device__ int pv(const Rec8& r) { return (int)((r.pk >> 12) & 0xfffu); }
+  static __device__ int xoff(const Rec8& r) { return (int)((r.pk >> 24) & 7u); }
+  static __device__ int yoff(const Rec8& r) { return (int)((r.pk >> 27) & 7u); }
+  static __device__ int chan(const Rec8&) { return 0; }
+  static __device__ bool last(const Rec8& r) { return (r.pk >> 30) & 1u; }
+  static __device__ int z(const Rec8&) { return 0; }
+};
+// the compact records hold this plan's memberships
+inline bool compact_recs(const DevGeom& g) { return g.n <= 4096 && g.os <= 8 && g.nchan == 1 && g.nz == 1; }
 
 __global__ void k_p1_count(DevGeom g, const double* __restrict__ uvw, const double* __restrict__ freq, int64_t n,
                            const double* __restrict__ boxes, const double* __restrict__ plane_w,
@@ -415,9 +458,11 @@ __global__ void k_p1_count(DevGeom g, const double* __restrict__ uvw, const doub
   }
 }
 
+template <typename RT>
 __global__ void k_p1_scatter(DevGeom g, const double* __restrict__ uvw, const double* __restrict__ freq, int64_t n,
                              const double* __restrict__ boxes, const double* __restrict__ plane_w,
-                             uint32_t flags, const uint32_t* __restrict__ s1, int nb1, Rec* __restrict__ recs) {
+                             uint32_t flags, const uint32_t* __restrict__ s1, int nb1, RT* __restrict__ recs) {
+  using O = RecOps<RT>;
   extern __shared__ double lsh[];
   const int P = g.nplanes;
   double* lbox = lsh;
@@ -436,20 +481,14 @@ __global__ void k_p1_scatter(DevGeom g, const double* __restrict__ uvw, const do
     Enum e = enumerate_row(g, uvw, freq, row, lbox, wslice, inv_wslice, lpw, flags, &dummy);
     if (!e.valid) {
       const uint32_t pos = atomicAdd(&cur[P], 1u);
-      recs[pos] = Rec{(uint32_t)row, 0u, 0u, 0u};
+      recs[pos] = O::orphan((uint32_t)row);
       continue;
     }
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
       if (j >= e.nm) break;
       const uint32_t pos = atomicAdd(&cur[e.plane[j]], 1u);
-      Rec r;
-      r.row = (uint32_t)row;
-      r.uv = (uint32_t)e.map.pu | ((uint32_t)e.map.pv << 16);
-      r.aux = (uint32_t)e.map.xoff | ((uint32_t)e.map.yoff << 8) | ((uint32_t)e.map.chan << 16) |
-              ((j == e.nm - 1) ? (1u << 24) : 0u);
-      r.z = (uint32_t)e.zp[j];
-      recs[pos] = r;
+      recs[pos] = O::make((uint32_t)row, e.map, j == e.nm - 1, e.zp[j]);
     }
   }
 }
@@ -480,9 +519,8 @@ struct Blk2 {
   uint32_t hb;
 };
 
-__device__ inline void rec_tiles(const DevGeom& g, const Rec& r, int* tx0, int* ty0, int* ntx_t, int* nty_t,
+__device__ inline void rec_tiles(const DevGeom& g, int pu, int pv, int* tx0, int* ty0, int* ntx_t, int* nty_t,
                                  int* xs, int* ys) {
-  const int pu = (int)(r.uv & 0xffffu), pv = (int)(r.uv >> 16);
   const int half = g.support / 2;
   *xs = pu - half;
   *ys = pv - half;
@@ -493,15 +531,17 @@ __device__ inline void rec_tiles(const DevGeom& g, const Rec& r, int* tx0, int* 
   *nty_t = ((*ys + g.support - 1) >> g.tshift) - *ty0 + 1;
 }
 
-__global__ void k_p2_count(DevGeom g, const Rec* __restrict__ recs, const Blk2* __restrict__ blks,
+template <typename RT>
+__global__ void k_p2_count(DevGeom g, const RT* __restrict__ recs, const Blk2* __restrict__ blks,
                            uint32_t* __restrict__ h2) {
+  using O = RecOps<RT>;
   extern __shared__ uint32_t lh[];
   const Blk2 bk = blks[blockIdx.x];
   const int tpp = g.nchan * g.ntx * g.ntx;
   for (int i = threadIdx.x; i < 2 * tpp; i += blockDim.x) lh[i] = 0u;
   __syncthreads();
   for (uint32_t k0 = bk.rs + threadIdx.x; k0 < bk.re; k0 += K2_UNROLL * blockDim.x) {
-    Rec rr[K2_UNROLL];
+    RT rr[K2_UNROLL];
 #pragma unroll
     for (int u = 0; u < K2_UNROLL; ++u) {
       const uint32_t k = k0 + u * blockDim.x;
@@ -510,11 +550,11 @@ __global__ void k_p2_count(DevGeom g, const Rec* __restrict__ recs, const Blk2* 
 #pragma unroll
     for (int u = 0; u < K2_UNROLL; ++u) {
       if (k0 + u * blockDim.x >= bk.re) break;
-      const Rec& r = rr[u];
+      const RT& r = rr[u];
       int tx0, ty0, ntxt, nty, xs, ys;
-      rec_tiles(g, r, &tx0, &ty0, &ntxt, &nty, &xs, &ys);
-      const int chan = (int)((r.aux >> 16) & 0xffu);
-      const bool last = (r.aux >> 24) & 1u;
+      rec_tiles(g, O::pu(r), O::pv(r), &tx0, &ty0, &ntxt, &nty, &xs, &ys);
+      const int chan = O::chan(r);
+      const bool last = O::last(r);
       for (int dy = 0; dy < nty; ++dy)
         for (int dx = 0; dx < ntxt; ++dx) {
           const int tl = (chan * g.ntx + ty0 + dy) * g.ntx + tx0 + dx;
@@ -528,9 +568,11 @@ __global__ void k_p2_count(DevGeom g, const Rec* __restrict__ recs, const Blk2* 
   for (int i = threadIdx.x; i < 2 * tpp; i += blockDim.x) hrow[i] = lh[i];
 }
 
-__global__ void k_p2_scatter(DevGeom g, const Rec* __restrict__ recs, const Blk2* __restrict__ blks,
+template <typename RT>
+__global__ void k_p2_scatter(DevGeom g, const RT* __restrict__ recs, const Blk2* __restrict__ blks,
                              const uint32_t* __restrict__ s2, uint2* __restrict__ entries,
                              uint16_t* __restrict__ entry_z) {
+  using O = RecOps<RT>;
   extern __shared__ uint32_t cur[];
   const Blk2 bk = blks[blockIdx.x];
   const int tpp = g.nchan * g.ntx * g.ntx;
@@ -538,7 +580,7 @@ __global__ void k_p2_scatter(DevGeom g, const Rec* __restrict__ recs, const Blk2
   for (int i = threadIdx.x; i < 2 * tpp; i += blockDim.x) cur[i] = srow[i];
   __syncthreads();
   for (uint32_t k0 = bk.rs + threadIdx.x; k0 < bk.re; k0 += K2_UNROLL * blockDim.x) {
-    Rec rr[K2_UNROLL];
+    RT rr[K2_UNROLL];
 #pragma unroll
     for (int u = 0; u < K2_UNROLL; ++u) {
       const uint32_t k = k0 + u * blockDim.x;
@@ -547,12 +589,12 @@ __global__ void k_p2_scatter(DevGeom g, const Rec* __restrict__ recs, const Blk2
 #pragma unroll
     for (int u = 0; u < K2_UNROLL; ++u) {
       if (k0 + u * blockDim.x >= bk.re) break;
-      const Rec& r = rr[u];
+      const RT& r = rr[u];
       int tx0, ty0, ntxt, nty, xs, ys;
-      rec_tiles(g, r, &tx0, &ty0, &ntxt, &nty, &xs, &ys);
-      const int chan = (int)((r.aux >> 16) & 0xffu);
-      const bool last = (r.aux >> 24) & 1u;
-      const int xo = (int)(r.aux & 0xffu), yo = (int)((r.aux >> 8) & 0xffu);
+      rec_tiles(g, O::pu(r), O::pv(r), &tx0, &ty0, &ntxt, &nty, &xs, &ys);
+      const int chan = O::chan(r);
+      const bool last = O::last(r);
+      const int xo = O::xoff(r), yo = O::yoff(r);
       for (int dy = 0; dy < nty; ++dy)
         for (int dx = 0; dx < ntxt; ++dx) {
           const int tx = tx0 + dx, ty = ty0 + dy;
@@ -562,7 +604,7 @@ __global__ void k_p2_scatter(DevGeom g, const Rec* __restrict__ recs, const Blk2
           const uint32_t pos = atomicAdd(&cur[2 * tl + kind], 1u);
           entries[pos] = make_uint2(r.row, pack_entry(xs - tx * g.tile + g.support, ys - ty * g.tile + g.support,
                                                       xo, yo, start, kind == 0));
-          if (entry_z) entry_z[pos] = (uint16_t)r.z;
+          if (entry_z) entry_z[pos] = (uint16_t)O::z(r);
         }
     }
   }
@@ -1154,10 +1196,16 @@ static int plan_build(arlmi_plan* p, const double* uvw, const double* freq, int6
   p->nmember = misc[3];
   p->orphan_start = bounds[P];
   p->norphans = bounds[P + 1] - bounds[P];
-  if ((rc = grow_bytes(&p->d_recs, &p->cap_recs, (int64_t)sizeof(Rec) * std::max<int64_t>(1, nrec)))) return rc;
-  ARLMI_LAUNCH("k_p1_scatter", k_p1_scatter, dim3(nb1), dim3(256),
-               sizeof(double) * (2 * P + 1) + sizeof(uint32_t) * (P + 1), s, g, uvw, freq, nrows, p->d_boxes,
-               p->d_plane_w, p->flags, p->d_s1, nb1, (Rec*)p->d_recs);
+  const bool compact = compact_recs(g);
+  p->rec_words = compact ? 2 : 4;
+  if ((rc = grow_bytes(&p->d_recs, &p->cap_recs, (int64_t)4 * p->rec_words * std::max<int64_t>(1, nrec)))) return rc;
+  const size_t sh1 = sizeof(double) * (2 * P + 1) + sizeof(uint32_t) * (P + 1);
+  if (compact)
+    ARLMI_LAUNCH("k_p1_scatter", k_p1_scatter<Rec8>, dim3(nb1), dim3(256), sh1, s, g, uvw, freq, nrows, p->d_boxes,
+                 p->d_plane_w, p->flags, p->d_s1, nb1, (Rec8*)p->d_recs);
+  else
+    ARLMI_LAUNCH("k_p1_scatter", k_p1_scatter<Rec>, dim3(nb1), dim3(256), sh1, s, g, uvw, freq, nrows, p->d_boxes,
+                 p->d_plane_w, p->flags, p->d_s1, nb1, (Rec*)p->d_recs);
   ARLMI_LAUNCH_CHECK();
 
   // pass 2: plane records -> tile entries
@@ -1195,8 +1243,12 @@ static int plan_build(arlmi_plan* p, const double* uvw, const double* freq, int6
   ARLMI_HIP(hipMemcpyAsync(p->d_blk2, blks.data(), sizeof(Blk2) * blks.size(), hipMemcpyHostToDevice, s));
   ARLMI_HIP(hipMemcpyAsync(p->d_hbase, hbase.data(), sizeof(uint32_t) * (2 * P + 1), hipMemcpyHostToDevice, s));
   const size_t sh2 = sizeof(uint32_t) * 2 * (size_t)tpp;
-  ARLMI_LAUNCH("k_p2_count", k_p2_count, dim3((unsigned)blks.size()), dim3(512), sh2, s, g, (const Rec*)p->d_recs,
-               (const Blk2*)p->d_blk2, p->d_h2);
+  if (compact)
+    ARLMI_LAUNCH("k_p2_count", k_p2_count<Rec8>, dim3((unsigned)blks.size()), dim3(512), sh2, s, g,
+                 (const Rec8*)p->d_recs, (const Blk2*)p->d_blk2, p->d_h2);
+  else
+    ARLMI_LAUNCH("k_p2_count", k_p2_count<Rec>, dim3((unsigned)blks.size()), dim3(512), sh2, s, g,
+                 (const Rec*)p->d_recs, (const Blk2*)p->d_blk2, p->d_h2);
   ARLMI_LAUNCH_CHECK();
   const int nbin = 2 * tpp;
   const int64_t nk = (int64_t)P * nbin;  // == 2 nt
@@ -1239,9 +1291,14 @@ static int plan_build(arlmi_plan* p, const double* uvw, const double* freq, int6
   const int64_t ebound = std::max<int64_t>(1, p->nmember * per_axis * per_axis);
   if ((rc = grow(&p->d_entries, &p->cap_entries, ebound))) return rc;
   if (g.nz > 1 && (rc = grow(&p->d_entry_z, &p->cap_z, ebound))) return rc;
-  ARLMI_LAUNCH("k_p2_scatter", k_p2_scatter, dim3((unsigned)blks.size()), dim3(512), sh2, s, g,
-               (const Rec*)p->d_recs, (const Blk2*)p->d_blk2, p->d_s2, p->d_entries,
-               g.nz > 1 ? p->d_entry_z : nullptr);
+  if (compact)
+    ARLMI_LAUNCH("k_p2_scatter", k_p2_scatter<Rec8>, dim3((unsigned)blks.size()), dim3(512), sh2, s, g,
+                 (const Rec8*)p->d_recs, (const Blk2*)p->d_blk2, p->d_s2, p->d_entries,
+                 g.nz > 1 ? p->d_entry_z : nullptr);
+  else
+    ARLMI_LAUNCH("k_p2_scatter", k_p2_scatter<Rec>, dim3((unsigned)blks.size()), dim3(512), sh2, s, g,
+                 (const Rec*)p->d_recs, (const Blk2*)p->d_blk2, p->d_s2, p->d_entries,
+                 g.nz > 1 ? p->d_entry_z : nullptr);
   ARLMI_LAUNCH_CHECK();
 
   // work items
