@@ -267,6 +267,8 @@ struct arlmi_plan {
   uint32_t* d_misc = nullptr;            // [8]: err bits, -, dropped, nmember, nmulti
   uint32_t* d_bounds = nullptr;          // [P+2] plane segment starts in the record array, [8] misc
                                          // copy, [4] totals (gathered for single copies)
+  uint32_t* h_bounds = nullptr;          // pinned host copy of d_bounds
+  hipEvent_t ev_bounds = nullptr;        // recorded after the copy of the segment bounds
   uint32_t* d_hbase = nullptr;           // [P+1] plane bases in h2, then [P] pass-2 blocks per plane
   uint32_t *d_h1 = nullptr, *d_s1 = nullptr;
   uint32_t *d_h2 = nullptr, *d_s2 = nullptr;
@@ -274,7 +276,6 @@ struct arlmi_plan {
   uint32_t* d_scan_tmp = nullptr;
   void* d_recs = nullptr;                // membership records (rec_words 32-bit words each, the row first)
   int rec_words = 4;
-  void* d_blk2 = nullptr;                // pass-2 block descriptors
   uint2* d_entries = nullptr;            // (row, packed)
   uint16_t* d_entry_z = nullptr;         // kernel w plane per entry (nz > 1)
   int4* d_gitems = nullptr;              // {tile, start, count, partial slot + 1 or 0}
@@ -285,7 +286,7 @@ struct arlmi_plan {
   double* d_det = nullptr;               // ARLMI_FLAG_DETERMINISTIC: scales, integer sums of weights,
                                          // bound partials (det_words)
   int64_t cap_img_part_bytes = 0;
-  int64_t cap_h1 = 0, cap_s1 = 0, cap_h2 = 0, cap_s2 = 0, cap_scan = 0, cap_recs = 0, cap_blk2 = 0, cap_entries = 0, cap_z = 0,
+  int64_t cap_h1 = 0, cap_s1 = 0, cap_h2 = 0, cap_s2 = 0, cap_scan = 0, cap_recs = 0, cap_entries = 0, cap_z = 0,
           cap_gitems = 0, cap_ditems = 0, cap_partial_bytes = 0;
   // host copies
   int64_t nentries = 0, ngitems = 0, nditems = 0, nmulti = 0, nslots = 0, norphans = 0, ndropped = 0, nmember = 0;

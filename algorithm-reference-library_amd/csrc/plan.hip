@@ -461,7 +461,11 @@ __global__ void k_p1_count(DevGeom g, const double* __restrict__ uvw, const doub
 template <typename RT>
 __global__ void k_p1_scatter(DevGeom g, const double* __restrict__ uvw, const double* __restrict__ freq, int64_t n,
                              const double* __restrict__ boxes, const double* __restrict__ plane_w,
-                             uint32_t flags, const uint32_t* __restrict__ s1, int nb1, RT* __restrict__ recs) {
+                             uint32_t flags, const uint32_t* __restrict__ s1, int nb1, RT* __restrict__ recs,
+                             uint32_t cap) {
+  // records at or past cap are not written (the host launches this pass before it knows the
+  // record count, into a buffer of cap records, and launches it again into a larger one if the
+  // count exceeds cap)
   using O = RecOps<RT>;
   extern __shared__ double lsh[];
   const int P = g.nplanes;
@@ -481,14 +485,14 @@ __global__ void k_p1_scatter(DevGeom g, const double* __restrict__ uvw, const do
     Enum e = enumerate_row(g, uvw, freq, row, lbox, wslice, inv_wslice, lpw, flags, &dummy);
     if (!e.valid) {
       const uint32_t pos = atomicAdd(&cur[P], 1u);
-      recs[pos] = O::orphan((uint32_t)row);
+      if (pos < cap) recs[pos] = O::orphan((uint32_t)row);
       continue;
     }
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
       if (j >= e.nm) break;
       const uint32_t pos = atomicAdd(&cur[e.plane[j]], 1u);
-      recs[pos] = O::make((uint32_t)row, e.map, j == e.nm - 1, e.zp[j]);
+      if (pos < cap) recs[pos] = O::make((uint32_t)row, e.map, j == e.nm - 1, e.zp[j]);
     }
   }
 }
@@ -519,6 +523,66 @@ struct Blk2 {
   uint32_t hb;
 };
 
+// Pass-2 layout from the plane segment bounds (one work group): plane pl gets nbp[pl] =
+// max(1, ceil(records / rb)) blocks, hbase[pl] = nbin * (blocks of the planes before it) and
+// hbase[P] = nbin * all blocks (hbase holds [P+1] bases, then [P] block counts).
+__global__ __launch_bounds__(1024) void k_p2_layout(const uint32_t* __restrict__ bounds, int P, uint32_t rb, int nbin,
+                                                    uint32_t* __restrict__ hbase) {
+  __shared__ uint32_t lsum[1024];
+  const int per = (P + 1023) / 1024;  // planes a thread (P <= 4096)
+  const int p0 = threadIdx.x * per;
+  uint32_t loc = 0;
+  for (int k = 0; k < per; ++k) {
+    const int pl = p0 + k;
+    if (pl < P) loc += max(1u, (bounds[pl + 1] - bounds[pl] + rb - 1) / rb);
+  }
+  lsum[threadIdx.x] = loc;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {  // inclusive scan of the per-thread block counts
+    const uint32_t v = threadIdx.x >= (unsigned)off ? lsum[threadIdx.x - off] : 0u;
+    __syncthreads();
+    lsum[threadIdx.x] += v;
+    __syncthreads();
+  }
+  uint32_t run = lsum[threadIdx.x] - loc;
+  for (int k = 0; k < per; ++k) {
+    const int pl = p0 + k;
+    if (pl < P) {
+      const uint32_t nb = max(1u, (bounds[pl + 1] - bounds[pl] + rb - 1) / rb);
+      hbase[pl] = run * (uint32_t)nbin;
+      hbase[P + 1 + pl] = nb;
+      run += nb;
+    }
+  }
+  if (threadIdx.x == 1023) hbase[P] = lsum[1023] * (uint32_t)nbin;
+}
+
+// This work group's pass-2 block: the plane whose blocks hold blockIdx.x (hbase strictly
+// increases, every plane having a block), then its record range.
+__device__ inline Blk2 p2_block(const uint32_t* __restrict__ bounds, const uint32_t* __restrict__ hbase, int P,
+                                int nbin, uint32_t rb) {
+  __shared__ Blk2 sb;
+  if (threadIdx.x == 0) {
+    const uint32_t key = blockIdx.x * (uint32_t)nbin;
+    int lo = 0, hi = P - 1;  // the last plane with hbase <= key
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (hbase[mid] <= key) lo = mid;
+      else hi = mid - 1;
+    }
+    Blk2 k;
+    k.plane = lo;
+    k.hb = hbase[lo];
+    k.b = (int)((key - k.hb) / (uint32_t)nbin);
+    k.nb = (int)hbase[P + 1 + lo];
+    k.rs = bounds[lo] + (uint32_t)k.b * rb;
+    k.re = min(bounds[lo + 1], k.rs + rb);
+    sb = k;
+  }
+  __syncthreads();
+  return sb;
+}
+
 __device__ inline void rec_tiles(const DevGeom& g, int pu, int pv, int* tx0, int* ty0, int* ntx_t, int* nty_t,
                                  int* xs, int* ys) {
   const int half = g.support / 2;
@@ -532,12 +596,12 @@ __device__ inline void rec_tiles(const DevGeom& g, int pu, int pv, int* tx0, int
 }
 
 template <typename RT>
-__global__ void k_p2_count(DevGeom g, const RT* __restrict__ recs, const Blk2* __restrict__ blks,
-                           uint32_t* __restrict__ h2) {
+__global__ void k_p2_count(DevGeom g, const RT* __restrict__ recs, const uint32_t* __restrict__ bounds,
+                           const uint32_t* __restrict__ hbase, uint32_t rb, uint32_t* __restrict__ h2) {
   using O = RecOps<RT>;
   extern __shared__ uint32_t lh[];
-  const Blk2 bk = blks[blockIdx.x];
   const int tpp = g.nchan * g.ntx * g.ntx;
+  const Blk2 bk = p2_block(bounds, hbase, g.nplanes, 2 * tpp, rb);
   for (int i = threadIdx.x; i < 2 * tpp; i += blockDim.x) lh[i] = 0u;
   __syncthreads();
   for (uint32_t k0 = bk.rs + threadIdx.x; k0 < bk.re; k0 += K2_UNROLL * blockDim.x) {
@@ -569,13 +633,13 @@ __global__ void k_p2_count(DevGeom g, const RT* __restrict__ recs, const Blk2* _
 }
 
 template <typename RT>
-__global__ void k_p2_scatter(DevGeom g, const RT* __restrict__ recs, const Blk2* __restrict__ blks,
-                             const uint32_t* __restrict__ s2, uint2* __restrict__ entries,
-                             uint16_t* __restrict__ entry_z) {
+__global__ void k_p2_scatter(DevGeom g, const RT* __restrict__ recs, const uint32_t* __restrict__ bounds,
+                             const uint32_t* __restrict__ hbase, uint32_t rb, const uint32_t* __restrict__ s2,
+                             uint2* __restrict__ entries, uint16_t* __restrict__ entry_z) {
   using O = RecOps<RT>;
   extern __shared__ uint32_t cur[];
-  const Blk2 bk = blks[blockIdx.x];
   const int tpp = g.nchan * g.ntx * g.ntx;
+  const Blk2 bk = p2_block(bounds, hbase, g.nplanes, 2 * tpp, rb);
   const uint32_t* srow = s2 + bk.hb + (size_t)bk.b * 2 * tpp;
   for (int i = threadIdx.x; i < 2 * tpp; i += blockDim.x) cur[i] = srow[i];
   __syncthreads();
@@ -1061,11 +1125,13 @@ static void free_plan(arlmi_plan* p) {
   if (!p) return;
   void* ptrs[] = {p->d_boxes, p->d_plane_w, p->d_wsum, p->d_wcnt, p->d_cnt, p->d_tilemask, p->d_colmask, p->d_zeros, p->d_cntd, p->d_off, p->d_nch,
                   p->d_nchd, p->d_nslot, p->d_choff, p->d_choffd, p->d_slotoff, p->d_misc, p->d_bounds,
-                  p->d_hbase, p->d_h1, p->d_s1, p->d_h2, p->d_s2, p->d_c2, p->d_scan_tmp, p->d_recs, p->d_blk2,
+                  p->d_hbase, p->d_h1, p->d_s1, p->d_h2, p->d_s2, p->d_c2, p->d_scan_tmp, p->d_recs,
                   p->d_entries, p->d_entry_z, p->d_gitems, p->d_ditems, p->d_multi, p->d_partial,
                   p->d_img_part, p->d_det};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
+  if (p->h_bounds) (void)hipHostFree(p->h_bounds);
+  if (p->ev_bounds) (void)hipEventDestroy(p->ev_bounds);
 }
 
 static int plan_alloc_fixed(arlmi_plan* p) {
@@ -1095,6 +1161,8 @@ static int plan_alloc_fixed(arlmi_plan* p) {
   ARLMI_HIP(hipMalloc(&p->d_misc, sizeof(uint32_t) * 8));
   // [P+2] segment bounds, [8] misc copy, [4] totals
   ARLMI_HIP(hipMalloc(&p->d_bounds, sizeof(uint32_t) * (P + 14)));
+  ARLMI_HIP(hipHostMalloc(reinterpret_cast<void**>(&p->h_bounds), sizeof(uint32_t) * (P + 14), hipHostMallocDefault));
+  ARLMI_HIP(hipEventCreateWithFlags(&p->ev_bounds, hipEventDisableTiming));
   // [P+1] plane bases, then [P] blocks per plane (one copy from the host)
   ARLMI_HIP(hipMalloc(&p->d_hbase, sizeof(uint32_t) * (2 * P + 1)));
   ARLMI_HIP(hipMalloc(&p->d_multi, sizeof(int4) * nt));
@@ -1110,8 +1178,8 @@ static int ensure_scan_tmp(arlmi_plan* p, int64_t n) {
   return grow(&p->d_scan_tmp, &p->cap_scan, (int64_t)scan_tmp_words(n));
 }
 
-static int plan_build(arlmi_plan* p, const double* uvw, const double* freq, int64_t nrows, const double* h_boxes,
-                      double wslice, hipStream_t s) {
+static int plan_build_steps(arlmi_plan* p, const double* uvw, const double* freq, int64_t nrows,
+                            const double* h_boxes, double wslice, hipStream_t s) {
   DevGeom& g = p->g;
   const int P = g.nplanes;
   const int64_t nt = p->ntiles;
@@ -1178,11 +1246,33 @@ static int plan_build(arlmi_plan* p, const double* uvw, const double* freq, int6
   ARLMI_LAUNCH("k_p1_bounds", k_p1_bounds, dim3((P + 10 + 255) / 256), dim3(256), 0, s, p->d_s1, nb1, P, p->d_misc,
                p->d_bounds);
   ARLMI_LAUNCH_CHECK();
-  std::vector<uint32_t> bounds(P + 10);
-  ARLMI_HIP(hipMemcpyAsync(bounds.data(), p->d_bounds, sizeof(uint32_t) * (P + 10), hipMemcpyDeviceToHost, s));
-  ARLMI_HIP(hipStreamSynchronize(s));
+  // The segment bounds come back through pinned memory while the scatter pass runs: it is
+  // launched before they arrive, into a buffer sized for one record a row and 1/64 more (rows in
+  // two slabs are rare), and launched again into a larger one when the records are more.
+  ARLMI_HIP(hipMemcpyAsync(p->h_bounds, p->d_bounds, sizeof(uint32_t) * (P + 10), hipMemcpyDeviceToHost, s));
+  ARLMI_HIP(hipEventRecord(p->ev_bounds, s));
+  const bool compact = compact_recs(g);
+  p->rec_words = compact ? 2 : 4;
+  const size_t rec_bytes = sizeof(uint32_t) * (size_t)p->rec_words;
+  const size_t sh1 = sizeof(double) * (2 * P + 1) + sizeof(uint32_t) * (P + 1);
+  auto scatter1 = [&]() -> int {
+    const int64_t cap = std::min<int64_t>(p->cap_recs / (int64_t)rec_bytes, 0xffffffffll);
+    if (compact)
+      ARLMI_LAUNCH("k_p1_scatter", k_p1_scatter<Rec8>, dim3(nb1), dim3(256), sh1, s, g, uvw, freq, nrows, p->d_boxes,
+                   p->d_plane_w, p->flags, p->d_s1, nb1, (Rec8*)p->d_recs, (uint32_t)cap);
+    else
+      ARLMI_LAUNCH("k_p1_scatter", k_p1_scatter<Rec>, dim3(nb1), dim3(256), sh1, s, g, uvw, freq, nrows, p->d_boxes,
+                   p->d_plane_w, p->flags, p->d_s1, nb1, (Rec*)p->d_recs, (uint32_t)cap);
+    ARLMI_LAUNCH_CHECK();
+    return ARLMI_OK;
+  };
+  if ((rc = grow_bytes(&p->d_recs, &p->cap_recs, (int64_t)rec_bytes * (nrows + nrows / 64 + 1024)))) return rc;
+  if ((rc = scatter1())) return rc;
+  ARLMI_HIP(hipEventSynchronize(p->ev_bounds));
+  const std::vector<uint32_t> bounds(p->h_bounds, p->h_bounds + P + 10);
   const uint32_t* misc = bounds.data() + P + 2;
   if (misc[0]) {
+    ARLMI_HIP(hipStreamSynchronize(s));  // the scatter pass has ended before the plan can be freed
     rc = first_error_code(misc[0]);
     const char* what = rc == ARLMI_E_UV_OFF_GRID ? "uv (or kernel footprint) off the grid"
                        : rc == ARLMI_E_SUBSAMPLE ? "kernel sub-sample index equals oversampling"
@@ -1196,59 +1286,38 @@ static int plan_build(arlmi_plan* p, const double* uvw, const double* freq, int6
   p->nmember = misc[3];
   p->orphan_start = bounds[P];
   p->norphans = bounds[P + 1] - bounds[P];
-  const bool compact = compact_recs(g);
-  p->rec_words = compact ? 2 : 4;
-  if ((rc = grow_bytes(&p->d_recs, &p->cap_recs, (int64_t)4 * p->rec_words * std::max<int64_t>(1, nrec)))) return rc;
-  const size_t sh1 = sizeof(double) * (2 * P + 1) + sizeof(uint32_t) * (P + 1);
-  if (compact)
-    ARLMI_LAUNCH("k_p1_scatter", k_p1_scatter<Rec8>, dim3(nb1), dim3(256), sh1, s, g, uvw, freq, nrows, p->d_boxes,
-                 p->d_plane_w, p->flags, p->d_s1, nb1, (Rec8*)p->d_recs);
-  else
-    ARLMI_LAUNCH("k_p1_scatter", k_p1_scatter<Rec>, dim3(nb1), dim3(256), sh1, s, g, uvw, freq, nrows, p->d_boxes,
-                 p->d_plane_w, p->flags, p->d_s1, nb1, (Rec*)p->d_recs);
-  ARLMI_LAUNCH_CHECK();
-
-  // pass 2: plane records -> tile entries
-  std::vector<Blk2> blks;
-  std::vector<uint32_t> hbase(2 * P + 1);  // plane bases, then blocks per plane
-  uint32_t* nbp = hbase.data() + P + 1;
-  uint64_t hb = 0;
-  const int64_t rb = std::max<int64_t>(K2_RECS, (int64_t)K2_HIST * 2 * tpp);  // records per block
-  for (int pl = 0; pl < P; ++pl) {
-    const uint32_t rs = bounds[pl], re = bounds[pl + 1];
-    const int nb = (int)std::max<int64_t>(1, ((int64_t)(re - rs) + rb - 1) / rb);
-    hbase[pl] = (uint32_t)hb;
-    nbp[pl] = (uint32_t)nb;
-    for (int b = 0; b < nb; ++b) {
-      Blk2 k;
-      k.plane = pl;
-      k.b = b;
-      k.nb = nb;
-      k.rs = rs + (uint32_t)((int64_t)b * rb);
-      k.re = (uint32_t)std::min<int64_t>(re, (int64_t)k.rs + rb);
-      k.hb = (uint32_t)hb;
-      blks.push_back(k);
-    }
-    hb += (uint64_t)2 * tpp * nb;
+  if (nrec > p->cap_recs / (int64_t)rec_bytes) {
+    ARLMI_HIP(hipStreamSynchronize(s));  // the first scatter pass still writes the old buffer
+    if ((rc = grow_bytes(&p->d_recs, &p->cap_recs, (int64_t)rec_bytes * nrec))) return rc;
+    if ((rc = scatter1())) return rc;
   }
-  hbase[P] = (uint32_t)hb;
-  if (hb >= (1ull << 32)) {
+
+  // pass 2: plane records -> tile entries.  The block layout (k_p2_layout) is made on the device
+  // from the bounds it already holds; the host repeats the count for the launch sizes.
+  const int64_t rb = std::max<int64_t>(K2_RECS, (int64_t)K2_HIST * 2 * tpp);  // records per block
+  std::vector<uint32_t> nbp(P);
+  int64_t nblk = 0;
+  for (int pl = 0; pl < P; ++pl) {
+    nbp[pl] = (uint32_t)std::max<int64_t>(1, ((int64_t)(bounds[pl + 1] - bounds[pl]) + rb - 1) / rb);
+    nblk += nbp[pl];
+  }
+  const int64_t n2 = (int64_t)2 * tpp * nblk;
+  if (n2 >= (1ll << 32) || rb >= (1ll << 32)) {
     set_error("tile histogram too large");
     return ARLMI_E_UNSUPPORTED;
   }
-  const int64_t n2 = (int64_t)hb;
-  if ((rc = grow_bytes(&p->d_blk2, &p->cap_blk2, (int64_t)(sizeof(Blk2) * blks.size())))) return rc;
   if ((rc = grow(&p->d_h2, &p->cap_h2, n2 + 1))) return rc;
   if ((rc = grow(&p->d_s2, &p->cap_s2, n2 + 2))) return rc;
-  ARLMI_HIP(hipMemcpyAsync(p->d_blk2, blks.data(), sizeof(Blk2) * blks.size(), hipMemcpyHostToDevice, s));
-  ARLMI_HIP(hipMemcpyAsync(p->d_hbase, hbase.data(), sizeof(uint32_t) * (2 * P + 1), hipMemcpyHostToDevice, s));
+  ARLMI_LAUNCH("k_p2_layout", k_p2_layout, dim3(1), dim3(1024), 0, s, p->d_bounds, P, (uint32_t)rb, 2 * tpp,
+               p->d_hbase);
+  ARLMI_LAUNCH_CHECK();
   const size_t sh2 = sizeof(uint32_t) * 2 * (size_t)tpp;
   if (compact)
-    ARLMI_LAUNCH("k_p2_count", k_p2_count<Rec8>, dim3((unsigned)blks.size()), dim3(512), sh2, s, g,
-                 (const Rec8*)p->d_recs, (const Blk2*)p->d_blk2, p->d_h2);
+    ARLMI_LAUNCH("k_p2_count", k_p2_count<Rec8>, dim3((unsigned)nblk), dim3(512), sh2, s, g,
+                 (const Rec8*)p->d_recs, p->d_bounds, p->d_hbase, (uint32_t)rb, p->d_h2);
   else
-    ARLMI_LAUNCH("k_p2_count", k_p2_count<Rec>, dim3((unsigned)blks.size()), dim3(512), sh2, s, g,
-                 (const Rec*)p->d_recs, (const Blk2*)p->d_blk2, p->d_h2);
+    ARLMI_LAUNCH("k_p2_count", k_p2_count<Rec>, dim3((unsigned)nblk), dim3(512), sh2, s, g,
+                 (const Rec*)p->d_recs, p->d_bounds, p->d_hbase, (uint32_t)rb, p->d_h2);
   ARLMI_LAUNCH_CHECK();
   const int nbin = 2 * tpp;
   const int64_t nk = (int64_t)P * nbin;  // == 2 nt
@@ -1292,12 +1361,12 @@ static int plan_build(arlmi_plan* p, const double* uvw, const double* freq, int6
   if ((rc = grow(&p->d_entries, &p->cap_entries, ebound))) return rc;
   if (g.nz > 1 && (rc = grow(&p->d_entry_z, &p->cap_z, ebound))) return rc;
   if (compact)
-    ARLMI_LAUNCH("k_p2_scatter", k_p2_scatter<Rec8>, dim3((unsigned)blks.size()), dim3(512), sh2, s, g,
-                 (const Rec8*)p->d_recs, (const Blk2*)p->d_blk2, p->d_s2, p->d_entries,
+    ARLMI_LAUNCH("k_p2_scatter", k_p2_scatter<Rec8>, dim3((unsigned)nblk), dim3(512), sh2, s, g,
+                 (const Rec8*)p->d_recs, p->d_bounds, p->d_hbase, (uint32_t)rb, p->d_s2, p->d_entries,
                  g.nz > 1 ? p->d_entry_z : nullptr);
   else
-    ARLMI_LAUNCH("k_p2_scatter", k_p2_scatter<Rec>, dim3((unsigned)blks.size()), dim3(512), sh2, s, g,
-                 (const Rec*)p->d_recs, (const Blk2*)p->d_blk2, p->d_s2, p->d_entries,
+    ARLMI_LAUNCH("k_p2_scatter", k_p2_scatter<Rec>, dim3((unsigned)nblk), dim3(512), sh2, s, g,
+                 (const Rec*)p->d_recs, p->d_bounds, p->d_hbase, (uint32_t)rb, p->d_s2, p->d_entries,
                  g.nz > 1 ? p->d_entry_z : nullptr);
   ARLMI_LAUNCH_CHECK();
 
@@ -1345,6 +1414,15 @@ static int plan_build(arlmi_plan* p, const double* uvw, const double* freq, int6
   ARLMI_LAUNCH_CHECK();
   p->nmulti = -1;  // left on the device (d_misc[4]); the reduction kernel reads it
   return ARLMI_OK;
+}
+
+// A build that fails part way may leave kernels and copies of its own in flight (the scatter
+// pass, the staged block descriptors): they end before the caller frees or reuses the buffers.
+static int plan_build(arlmi_plan* p, const double* uvw, const double* freq, int64_t nrows, const double* h_boxes,
+                      double wslice, hipStream_t s) {
+  const int rc = plan_build_steps(p, uvw, freq, nrows, h_boxes, wslice, s);
+  if (rc) (void)hipStreamSynchronize(s);
+  return rc;
 }
 
 extern "C" {
