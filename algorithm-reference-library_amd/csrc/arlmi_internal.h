@@ -266,7 +266,7 @@ struct arlmi_plan {
   uint32_t* d_slotoff = nullptr;         // [ntiles+1]
   uint32_t* d_misc = nullptr;            // [8]: err bits, -, dropped, nmember, nmulti
   uint32_t* d_bounds = nullptr;          // [P+2] plane segment starts in the record array, [8] misc
-                                         // copy, [4] totals (gathered for single copies)
+                                         // copy, [8] totals (gathered for single copies)
   uint32_t* h_bounds = nullptr;          // pinned host copy of d_bounds
   hipEvent_t ev_bounds = nullptr;        // recorded after the copy of the segment bounds
   uint32_t* d_hbase = nullptr;           // [P+1] plane bases in h2, then [P] pass-2 blocks per plane
@@ -277,6 +277,17 @@ struct arlmi_plan {
   void* d_recs = nullptr;                // membership records (rec_words 32-bit words each, the row first)
   int rec_words = 4;
   uint2* d_entries = nullptr;            // (row, packed)
+  uint2* d_entries_alt = nullptr;        // the other buffer of the origin sort
+  bool origin_sorted = false;            // entries grouped by footprint origin in their tile segments
+  uint2* d_sort_mid = nullptr;           // origin sort: middle segments (start, entries)
+  void* d_sort_big = nullptr;            // large segments (start, entries, first block, blocks)
+  uint2* d_sort_bigblk = nullptr;        // their blocks (segment, block)
+  uint32_t* d_sort_h3 = nullptr;         // block histograms, then cursors
+  uint32_t* d_sort_ksum = nullptr;       // per large segment and key: counts over its blocks
+  uint32_t* d_sort_wg = nullptr;         // per work group of the list pass: its list sizes
+  uint32_t* d_sort_segidx = nullptr;     // large segments, the huge ones first
+  uint32_t* d_sortcnt = nullptr;         // [4] list sizes
+  int64_t cap_sort_mid = 0, cap_sort_big = 0, cap_sort_bigblk = 0, cap_sort_h3 = 0, cap_sort_ksum = 0, cap_sort_wg = 0, cap_sort_segidx = 0;
   uint16_t* d_entry_z = nullptr;         // kernel w plane per entry (nz > 1)
   int4* d_gitems = nullptr;              // {tile, start, count, partial slot + 1 or 0}
   int4* d_ditems = nullptr;              // {tile, start, count, 0}
@@ -286,7 +297,7 @@ struct arlmi_plan {
   double* d_det = nullptr;               // ARLMI_FLAG_DETERMINISTIC: scales, integer sums of weights,
                                          // bound partials (det_words)
   int64_t cap_img_part_bytes = 0;
-  int64_t cap_h1 = 0, cap_s1 = 0, cap_h2 = 0, cap_s2 = 0, cap_scan = 0, cap_recs = 0, cap_entries = 0, cap_z = 0,
+  int64_t cap_h1 = 0, cap_s1 = 0, cap_h2 = 0, cap_s2 = 0, cap_scan = 0, cap_recs = 0, cap_entries = 0, cap_entries_alt = 0, cap_z = 0,
           cap_gitems = 0, cap_ditems = 0, cap_partial_bytes = 0;
   // host copies
   int64_t nentries = 0, ngitems = 0, nditems = 0, nmulti = 0, nslots = 0, norphans = 0, ndropped = 0, nmember = 0;

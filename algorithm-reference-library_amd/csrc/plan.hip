@@ -509,11 +509,12 @@ __global__ void k_p1_bounds(const uint32_t* __restrict__ s1, int nb1, int P, con
 // The four totals the host sizes the work-item lists with, gathered for one copy.
 __global__ void k_totals(const uint32_t* __restrict__ s2, int64_t n2, const uint32_t* __restrict__ choff,
                          const uint32_t* __restrict__ choffd, const uint32_t* __restrict__ slotoff, int64_t nt,
-                         uint32_t* out) {
+                         const uint32_t* __restrict__ sortcnt, uint32_t* out) {
   out[0] = s2[n2];
   out[1] = choff[nt];
   out[2] = choffd[nt];
   out[3] = slotoff[nt];
+  for (int k = 0; k < 4; ++k) out[4 + k] = sortcnt ? sortcnt[k] : 0u;
 }
 
 // Pass-2 block descriptor: plane, block index in the plane, record range, h2 base, blocks in plane.
@@ -791,6 +792,374 @@ __global__ void k_p2_keys(const uint32_t* __restrict__ e2, int64_t nt, uint32_t*
   off[key] = a;
   cntd[key] = d - a;
   cnt[key] = e - a;
+}
+
+// Entries of every (tile, kind) segment grouped by footprint origin: a counting sort on the
+// origin (ly, lx) (W = T + S values a side, NK = W^2 keys) from the entry array into a second
+// one, so that the gridder finds the entries of one origin next to each other and adds their
+// taps in registers before one LDS add (rows of the short baselines fall on the same pixels:
+// 29.7 M rows of the 2048^2 w-stack workload have 10.1 M distinct (slab, pixel) origins, and a
+// central tile holds hundreds of entries per origin).  k_sort_copytiny copies segments of fewer
+// than SORT_MIN entries as they are, k_sort_lists lists the others: up to SORT_MID entries one work group
+// sorts in LDS (k_sort_mid); larger ones are cut into blocks of SORT_BB entries, counted per
+// block (k_sort_bcount, block-major histograms), the blocks' counts summed per key
+// (k_sort_bksum), turned into every block's cursors (k_sort_bcursor) and scattered
+// (k_sort_bscatter).
+// (Measured on the 2048^2 w-stack workload: a copy instead of the LDS sort for the middle segments
+// 7.04 -> 7.17 ms per step, SORT_MIN 256 or 1024 no different from 64, SORT_BB 4096 slower than
+// 8192.)
+constexpr int SORT_MIN = 64, SORT_MID = 4096, SORT_BB = 8192, SORT_MAXKEYS = 16384;
+// large segments of more than SORT_HUGE blocks get 16 slices of threads over their blocks in the
+// key passes, the others one
+constexpr int SORT_HUGE = 16;
+
+struct SortSeg {
+  uint32_t start, m, blk0, nblk;
+};
+
+// Segment lists of the sort without global atomics: segment q = 2 t + kind of tile t, one
+// thread a segment.  k_sort_lists_a: each work group's numbers of middle segments, large
+// segments and their blocks (wg[3][G]); k_sort_lists_b: the work groups before it summed, its
+// threads' offsets by block scans, the lists written.  cnt: [0] middle segments, [1] large
+// segments, [2] their blocks.
+constexpr int SORT_LIST_THREADS = 1024;
+__device__ inline void sort_seg(const uint32_t* __restrict__ off, const uint32_t* __restrict__ tcnt,
+                                const uint32_t* __restrict__ tcntd, int64_t q, uint32_t* s, uint32_t* m) {
+  const int64_t t = q >> 1;
+  const uint32_t nd = tcntd[t];
+  *s = (q & 1) ? off[t] + nd : off[t];
+  *m = (q & 1) ? tcnt[t] - nd : nd;
+}
+// exclusive scan over the work group (every thread calls it); *total gets the sum
+__device__ inline uint32_t block_excl_u32(uint32_t v, uint32_t* wtot, uint32_t* total) {
+  const int lane = __lane_id(), wv = threadIdx.x >> 6;
+  uint32_t inc = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t u = __shfl_up(inc, o);
+    if (lane >= o) inc += u;
+  }
+  if (lane == 63) wtot[wv] = inc;
+  __syncthreads();
+  uint32_t base = 0, tot = 0;
+  for (int k = 0; k < (int)(blockDim.x >> 6); ++k) {
+    if (k < wv) base += wtot[k];
+    tot += wtot[k];
+  }
+  __syncthreads();
+  *total = tot;
+  return base + inc - v;
+}
+__device__ inline void sort_class(uint32_t m, uint32_t* nm, uint32_t* nb, uint32_t* nbb, uint32_t* nh) {
+  *nm = (m >= (uint32_t)SORT_MIN && m <= (uint32_t)SORT_MID) ? 1u : 0u;
+  *nb = m > (uint32_t)SORT_MID ? 1u : 0u;
+  *nbb = m > (uint32_t)SORT_MID ? (m + SORT_BB - 1) / SORT_BB : 0u;
+  *nh = *nbb > (uint32_t)SORT_HUGE ? 1u : 0u;
+}
+__global__ __launch_bounds__(SORT_LIST_THREADS) void k_sort_lists_a(const uint32_t* __restrict__ off,
+                                                                    const uint32_t* __restrict__ tcnt,
+                                                                    const uint32_t* __restrict__ tcntd, int64_t nt,
+                                                                    uint32_t* __restrict__ wg) {
+  __shared__ uint32_t wtot[SORT_LIST_THREADS / 64];
+  const int64_t q = blockIdx.x * (int64_t)SORT_LIST_THREADS + threadIdx.x;
+  uint32_t s = 0, m = 0;
+  if (q < 2 * nt) sort_seg(off, tcnt, tcntd, q, &s, &m);
+  uint32_t a, b, c, h, ta, tb, tc, th;
+  sort_class(m, &a, &b, &c, &h);
+  block_excl_u32(a, wtot, &ta);
+  block_excl_u32(b, wtot, &tb);
+  block_excl_u32(c, wtot, &tc);
+  block_excl_u32(h, wtot, &th);
+  if (threadIdx.x == 0) {
+    wg[blockIdx.x] = ta;
+    wg[gridDim.x + blockIdx.x] = tb;
+    wg[2 * gridDim.x + blockIdx.x] = tc;
+    wg[3 * gridDim.x + blockIdx.x] = th;
+  }
+}
+__device__ inline uint32_t cnt_big_total(const uint32_t* __restrict__ wg, unsigned G) {
+  uint32_t t = 0;
+  for (unsigned g = 0; g < G; ++g) t += wg[G + g];
+  return t;
+}
+__global__ __launch_bounds__(SORT_LIST_THREADS) void k_sort_lists_b(const uint32_t* __restrict__ off,
+                                                                    const uint32_t* __restrict__ tcnt,
+                                                                    const uint32_t* __restrict__ tcntd, int64_t nt,
+                                                                    const uint32_t* __restrict__ wg,
+                                                                    uint2* __restrict__ mid, SortSeg* __restrict__ big,
+                                                                    uint2* __restrict__ bigblk,
+                                                                    uint32_t* __restrict__ segidx,
+                                                                    uint32_t* __restrict__ cnt) {
+  __shared__ uint32_t wtot[SORT_LIST_THREADS / 64];
+  __shared__ uint32_t base[4];
+  if (threadIdx.x < 4) {
+    uint32_t t = 0, all = 0;
+    for (unsigned g = 0; g < gridDim.x; ++g) {
+      const uint32_t v = wg[threadIdx.x * gridDim.x + g];
+      if (g < blockIdx.x) t += v;
+      all += v;
+    }
+    base[threadIdx.x] = t;
+    if (blockIdx.x == 0) cnt[threadIdx.x] = all;
+  }
+  const int64_t q = blockIdx.x * (int64_t)SORT_LIST_THREADS + threadIdx.x;
+  uint32_t s = 0, m = 0;
+  if (q < 2 * nt) sort_seg(off, tcnt, tcntd, q, &s, &m);
+  uint32_t a, b, c, h, ta, tb, tc, th;
+  sort_class(m, &a, &b, &c, &h);
+  uint32_t oa = block_excl_u32(a, wtot, &ta);
+  uint32_t ob = block_excl_u32(b, wtot, &tb);
+  uint32_t oc = block_excl_u32(c, wtot, &tc);
+  uint32_t oh = block_excl_u32(h, wtot, &th);
+  oa += base[0];
+  ob += base[1];
+  oc += base[2];
+  oh += base[3];
+  if (a) mid[oa] = make_uint2(s, m);
+  if (b) {
+    big[ob] = SortSeg{s, m, oc, c};
+    for (uint32_t k = 0; k < c; ++k) bigblk[oc + k] = make_uint2(ob, k);
+    // segment index lists for the key passes: the huge ones from the front, the others from the
+    // back (cnt[1] - 1 down)
+    const uint32_t total_big = cnt_big_total(wg, gridDim.x);
+    if (h) segidx[oh] = ob;
+    else segidx[total_big - 1 - (ob - oh)] = ob;
+  }
+}
+
+// Segments of fewer than SORT_MIN entries copied as they are: 16 lanes a segment.
+__global__ __launch_bounds__(256) void k_sort_copytiny(const uint2* __restrict__ in, uint2* __restrict__ out,
+                                                       const uint32_t* __restrict__ off,
+                                                       const uint32_t* __restrict__ tcnt,
+                                                       const uint32_t* __restrict__ tcntd, int64_t nt) {
+  const int64_t q = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 4;
+  const int l = threadIdx.x & 15;
+  if (q >= 2 * nt) return;
+  uint32_t s, m;
+  sort_seg(off, tcnt, tcntd, q, &s, &m);
+  if (m >= (uint32_t)SORT_MIN) return;
+  for (uint32_t k = l; k < m; k += 16) out[s + k] = in[s + k];
+}
+
+__device__ inline int sort_key(uint32_t pk, int W) { return entry_ly(pk) * W + entry_lx(pk); }
+
+// Exclusive scan of hist[0, nkeys) in place by a work group of NTH threads (runs of consecutive
+// keys a thread, wavefront scans, wavefront totals).
+template <int NTH>
+__device__ inline void block_scan_keys(uint32_t* hist, int nkeys, uint32_t* wtot) {
+  const int per = (nkeys + NTH - 1) / NTH;
+  const int k0 = threadIdx.x * per;
+  const int lane = __lane_id(), wv = threadIdx.x >> 6;
+  uint32_t loc = 0;
+  for (int j = 0; j < per; ++j)
+    if (k0 + j < nkeys) loc += hist[k0 + j];
+  uint32_t inc = loc;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t v = __shfl_up(inc, o);
+    if (lane >= o) inc += v;
+  }
+  if (lane == 63) wtot[wv] = inc;
+  __syncthreads();
+  uint32_t base = 0;
+  for (int k = 0; k < wv; ++k) base += wtot[k];
+  uint32_t run = base + inc - loc;
+  for (int j = 0; j < per; ++j)
+    if (k0 + j < nkeys) {
+      const uint32_t h = hist[k0 + j];
+      hist[k0 + j] = run;
+      run += h;
+    }
+  __syncthreads();
+}
+
+// One middle segment (at most SORT_MID entries) per work group: entries held in registers,
+// histogram and cursors in LDS.
+constexpr int SORT_MID_THREADS = 256;
+__global__ __launch_bounds__(SORT_MID_THREADS) void k_sort_mid(const uint2* __restrict__ in, uint2* __restrict__ out,
+                                                               const uint2* __restrict__ mid, int W) {
+  extern __shared__ uint32_t hist[];  // [W * W]
+  __shared__ uint32_t wtot[SORT_MID_THREADS / 64];
+  constexpr int PER = SORT_MID / SORT_MID_THREADS;
+  const uint2 sg = mid[blockIdx.x];
+  const uint32_t s = sg.x, m = sg.y;
+  const int nkeys = W * W;
+  for (int i = threadIdx.x; i < nkeys; i += SORT_MID_THREADS) hist[i] = 0u;
+  uint2 e[PER];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const uint32_t i = threadIdx.x + k * SORT_MID_THREADS;
+    e[k] = i < m ? in[s + i] : make_uint2(0u, 0u);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < PER; ++k)
+    if (threadIdx.x + k * SORT_MID_THREADS < m) atomicAdd(&hist[sort_key(e[k].y, W)], 1u);
+  __syncthreads();
+  block_scan_keys<SORT_MID_THREADS>(hist, nkeys, wtot);
+#pragma unroll
+  for (int k = 0; k < PER; ++k)
+    if (threadIdx.x + k * SORT_MID_THREADS < m) out[s + atomicAdd(&hist[sort_key(e[k].y, W)], 1u)] = e[k];
+}
+
+// Large segments: block histograms (block-major, NK words a block).
+constexpr int SORT_BIG_THREADS = 512;
+__global__ __launch_bounds__(SORT_BIG_THREADS) void k_sort_bcount(const uint2* __restrict__ in,
+                                                                  const SortSeg* __restrict__ big,
+                                                                  const uint2* __restrict__ bigblk, int W,
+                                                                  uint32_t* __restrict__ h3) {
+  extern __shared__ uint32_t hist[];
+  const uint2 bb = bigblk[blockIdx.x];
+  const SortSeg sg = big[bb.x];
+  const uint32_t s = sg.start + bb.y * SORT_BB, m = min((uint32_t)SORT_BB, sg.m - bb.y * SORT_BB);
+  const int nkeys = W * W;
+  for (int i = threadIdx.x; i < nkeys; i += SORT_BIG_THREADS) hist[i] = 0u;
+  __syncthreads();
+  for (uint32_t i0 = threadIdx.x; i0 < m; i0 += 4 * SORT_BIG_THREADS) {
+    uint32_t pk[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t i = i0 + u * SORT_BIG_THREADS;
+      pk[u] = i < m ? in[s + i].y : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (i0 + u * SORT_BIG_THREADS < m) atomicAdd(&hist[sort_key(pk[u], W)], 1u);
+  }
+  __syncthreads();
+  uint32_t* hrow = h3 + (size_t)blockIdx.x * nkeys;
+  for (int i = threadIdx.x; i < nkeys; i += SORT_BIG_THREADS) hrow[i] = hist[i];
+}
+
+// Per large segment (segidx[blockIdx.y]) and KCH = 1024 / NS keys (blockIdx.x): the key's count
+// over the segment's blocks, ksum[segment * NK + key]; the blocks split into NS slices of threads.
+template <int NS>
+__global__ __launch_bounds__(1024) void k_sort_bksum(const SortSeg* __restrict__ big,
+                                                     const uint32_t* __restrict__ segidx, int W,
+                                                     const uint32_t* __restrict__ h3, uint32_t* __restrict__ ksum) {
+  constexpr int SORT_KCH = 1024 / NS;
+  __shared__ uint32_t part[NS][SORT_KCH];
+  const uint32_t gi = segidx[blockIdx.y];
+  const SortSeg sg = big[gi];
+  const int nkeys = W * W;
+  const int kl = threadIdx.x % SORT_KCH, sl = threadIdx.x / SORT_KCH;
+  const int k = blockIdx.x * SORT_KCH + kl;
+  const uint32_t len = (sg.nblk + NS - 1) / NS, b0 = min(sg.nblk, sl * len), b1 = min(sg.nblk, b0 + len);
+  const uint32_t* hk = h3 + (size_t)sg.blk0 * nkeys + k;
+  uint32_t t = 0;
+  if (k < nkeys)
+    for (uint32_t b = b0; b < b1; b += 4) {
+      uint32_t v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) v[u] = b + u < b1 ? hk[(size_t)(b + u) * nkeys] : 0u;
+      t += (v[0] + v[1]) + (v[2] + v[3]);
+    }
+  part[sl][kl] = t;
+  __syncthreads();
+  if (sl == 0 && k < nkeys) {
+    uint32_t u = 0;
+#pragma unroll
+    for (int q = 0; q < NS; ++q) u += part[q][kl];
+    ksum[(size_t)gi * nkeys + k] = u;
+  }
+}
+
+// Every block's cursors in place of its counts: segment start + the counts of the segment's
+// smaller keys + the key's counts in the earlier blocks.  Same grid as k_sort_bksum.
+template <int NS>
+__global__ __launch_bounds__(1024) void k_sort_bcursor(const SortSeg* __restrict__ big,
+                                                       const uint32_t* __restrict__ segidx, int W,
+                                                       const uint32_t* __restrict__ ksum, uint32_t* __restrict__ h3) {
+  constexpr int SORT_KCH = 1024 / NS;
+  __shared__ uint32_t wtot[1024 / 64];
+  __shared__ uint32_t keyb[SORT_KCH];
+  __shared__ uint32_t part[NS][SORT_KCH];
+  const uint32_t gi = segidx[blockIdx.y];
+  const SortSeg sg = big[gi];
+  const int nkeys = W * W;
+  const uint32_t* ks = ksum + (size_t)gi * nkeys;
+  const int kbase = blockIdx.x * SORT_KCH;
+  const int kl = threadIdx.x % SORT_KCH, sl = threadIdx.x / SORT_KCH;
+  const int k = kbase + kl;
+  // the counts of the keys below this group's first key, then the group's own keys scanned
+  uint32_t below = 0;
+  for (int q = threadIdx.x; q < kbase; q += 1024) below += ks[q];
+  uint32_t tbelow, tk;
+  block_excl_u32(below, wtot, &tbelow);
+  const uint32_t own = (sl == 0 && k < nkeys) ? ks[k] : 0u;
+  const uint32_t ex = block_excl_u32(own, wtot, &tk);
+  if (sl == 0) keyb[kl] = tbelow + ex;
+  // this slice's blocks: counts summed (four loads in flight), then the slices' offsets
+  const uint32_t len = (sg.nblk + NS - 1) / NS, b0 = min(sg.nblk, sl * len), b1 = min(sg.nblk, b0 + len);
+  const uint32_t* hk = h3 + (size_t)sg.blk0 * nkeys + k;
+  uint32_t t = 0;
+  if (k < nkeys)
+    for (uint32_t b = b0; b < b1; b += 4) {
+      uint32_t v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) v[u] = b + u < b1 ? hk[(size_t)(b + u) * nkeys] : 0u;
+      t += (v[0] + v[1]) + (v[2] + v[3]);
+    }
+  part[sl][kl] = t;
+  __syncthreads();
+  if (k >= nkeys) return;
+  uint32_t run = sg.start + keyb[kl];
+  for (int q = 0; q < sl; ++q) run += part[q][kl];
+  uint32_t* hw = h3 + (size_t)sg.blk0 * nkeys + k;
+  for (uint32_t b = b0; b < b1; b += 4) {
+    uint32_t v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = b + u < b1 ? hw[(size_t)(b + u) * nkeys] : 0u;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (b + u < b1) {
+        hw[(size_t)(b + u) * nkeys] = run;
+        run += v[u];
+      }
+  }
+}
+
+// A block's entries sorted by key in LDS first, then written in that order: entries of one key
+// go to consecutive places (its global cursor on), so the writes come in runs rather than one
+// scattered 8-byte store each.  LDS: [NK] cursor - local base, [NK] local cursors, [SORT_BB] entries.
+constexpr int SORT_SCAT_THREADS = 1024;
+__global__ __launch_bounds__(SORT_SCAT_THREADS) void k_sort_bscatter(const uint2* __restrict__ in,
+                                                                     uint2* __restrict__ out,
+                                                                     const SortSeg* __restrict__ big,
+                                                                     const uint2* __restrict__ bigblk, int W,
+                                                                     const uint32_t* __restrict__ h3) {
+  extern __shared__ uint32_t sm[];
+  __shared__ uint32_t wtot[SORT_SCAT_THREADS / 64];
+  constexpr int PER = SORT_BB / SORT_SCAT_THREADS;
+  const int nkeys = W * W;
+  uint32_t* delta = sm;
+  uint32_t* lcur = sm + nkeys;
+  uint2* buf = reinterpret_cast<uint2*>(sm + 2 * nkeys);
+  const uint2 bb = bigblk[blockIdx.x];
+  const SortSeg sg = big[bb.x];
+  const uint32_t s = sg.start + bb.y * SORT_BB, m = min((uint32_t)SORT_BB, sg.m - bb.y * SORT_BB);
+  for (int i = threadIdx.x; i < nkeys; i += SORT_SCAT_THREADS) lcur[i] = 0u;
+  uint2 e[PER];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const uint32_t i = threadIdx.x + k * SORT_SCAT_THREADS;
+    e[k] = i < m ? in[s + i] : make_uint2(0u, 0u);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < PER; ++k)
+    if (threadIdx.x + k * SORT_SCAT_THREADS < m) atomicAdd(&lcur[sort_key(e[k].y, W)], 1u);
+  __syncthreads();
+  block_scan_keys<SORT_SCAT_THREADS>(lcur, nkeys, wtot);
+  const uint32_t* hrow = h3 + (size_t)blockIdx.x * nkeys;
+  for (int i = threadIdx.x; i < nkeys; i += SORT_SCAT_THREADS) delta[i] = hrow[i] - lcur[i];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < PER; ++k)
+    if (threadIdx.x + k * SORT_SCAT_THREADS < m) buf[atomicAdd(&lcur[sort_key(e[k].y, W)], 1u)] = e[k];
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < m; i += SORT_SCAT_THREADS) {
+    const uint2 v = buf[i];
+    out[delta[sort_key(v.y, W)] + i] = v;
+  }
 }
 
 // Chunk counts per tile; tiles with several grid chunks also get partial-tile slots.
@@ -1126,7 +1495,7 @@ static void free_plan(arlmi_plan* p) {
   void* ptrs[] = {p->d_boxes, p->d_plane_w, p->d_wsum, p->d_wcnt, p->d_cnt, p->d_tilemask, p->d_colmask, p->d_zeros, p->d_cntd, p->d_off, p->d_nch,
                   p->d_nchd, p->d_nslot, p->d_choff, p->d_choffd, p->d_slotoff, p->d_misc, p->d_bounds,
                   p->d_hbase, p->d_h1, p->d_s1, p->d_h2, p->d_s2, p->d_c2, p->d_scan_tmp, p->d_recs,
-                  p->d_entries, p->d_entry_z, p->d_gitems, p->d_ditems, p->d_multi, p->d_partial,
+                  p->d_entries, p->d_entries_alt, p->d_sort_mid, p->d_sort_big, p->d_sort_bigblk, p->d_sort_h3, p->d_sort_ksum, p->d_sort_wg, p->d_sort_segidx, p->d_sortcnt, p->d_entry_z, p->d_gitems, p->d_ditems, p->d_multi, p->d_partial,
                   p->d_img_part, p->d_det};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
@@ -1160,8 +1529,9 @@ static int plan_alloc_fixed(arlmi_plan* p) {
   ARLMI_HIP(hipMalloc(&p->d_slotoff, sizeof(uint32_t) * (nt + 1)));
   ARLMI_HIP(hipMalloc(&p->d_misc, sizeof(uint32_t) * 8));
   // [P+2] segment bounds, [8] misc copy, [4] totals
-  ARLMI_HIP(hipMalloc(&p->d_bounds, sizeof(uint32_t) * (P + 14)));
-  ARLMI_HIP(hipHostMalloc(reinterpret_cast<void**>(&p->h_bounds), sizeof(uint32_t) * (P + 14), hipHostMallocDefault));
+  ARLMI_HIP(hipMalloc(&p->d_bounds, sizeof(uint32_t) * (P + 18)));
+  ARLMI_HIP(hipMalloc(&p->d_sortcnt, sizeof(uint32_t) * 4));
+  ARLMI_HIP(hipHostMalloc(reinterpret_cast<void**>(&p->h_bounds), sizeof(uint32_t) * (P + 18), hipHostMallocDefault));
   ARLMI_HIP(hipEventCreateWithFlags(&p->ev_bounds, hipEventDisableTiming));
   // [P+1] plane bases, then [P] blocks per plane (one copy from the host)
   ARLMI_HIP(hipMalloc(&p->d_hbase, sizeof(uint32_t) * (2 * P + 1)));
@@ -1369,6 +1739,32 @@ static int plan_build_steps(arlmi_plan* p, const double* uvw, const double* freq
                  (const Rec*)p->d_recs, p->d_bounds, p->d_hbase, (uint32_t)rb, p->d_s2, p->d_entries,
                  g.nz > 1 ? p->d_entry_z : nullptr);
   ARLMI_LAUNCH_CHECK();
+  // entries grouped by footprint origin within their tile segments (one kernel w plane: the
+  // kernel w planes of the entries would have to move with them): segments classified here, the
+  // sorts launched once the host knows the list sizes (with the work-item totals)
+  const int sortw = g.tile + g.support;
+  p->origin_sorted = g.nz == 1 && sortw * sortw <= SORT_MAXKEYS &&
+                     sizeof(uint32_t) * 2 * sortw * sortw + sizeof(uint2) * SORT_BB <= 160 * 1024 - 1024;
+  if (p->origin_sorted) {
+    if ((rc = grow(&p->d_entries_alt, &p->cap_entries_alt, ebound))) return rc;
+    if ((rc = grow(&p->d_sort_mid, &p->cap_sort_mid, 2 * nt))) return rc;
+    const int64_t nbig = ebound / SORT_MID + 1;
+    if ((rc = grow_bytes(&p->d_sort_big, &p->cap_sort_big, (int64_t)sizeof(SortSeg) * nbig))) return rc;
+    if ((rc = grow(&p->d_sort_bigblk, &p->cap_sort_bigblk, ebound / SORT_BB + nbig))) return rc;
+    ARLMI_LAUNCH("k_sort_copytiny", k_sort_copytiny, dim3((unsigned)((2 * nt * 16 + 255) / 256)), dim3(256), 0, s,
+                 p->d_entries, p->d_entries_alt, p->d_off, p->d_cnt, p->d_cntd, nt);
+    ARLMI_LAUNCH_CHECK();
+    const unsigned lg = (unsigned)((2 * nt + SORT_LIST_THREADS - 1) / SORT_LIST_THREADS);
+    if ((rc = grow(&p->d_sort_wg, &p->cap_sort_wg, 4 * (int64_t)lg))) return rc;
+    if ((rc = grow(&p->d_sort_segidx, &p->cap_sort_segidx, nbig))) return rc;
+    ARLMI_LAUNCH("k_sort_lists_a", k_sort_lists_a, dim3(lg), dim3(SORT_LIST_THREADS), 0, s, p->d_off, p->d_cnt,
+                 p->d_cntd, nt, p->d_sort_wg);
+    ARLMI_LAUNCH_CHECK();
+    ARLMI_LAUNCH("k_sort_lists_b", k_sort_lists_b, dim3(lg), dim3(SORT_LIST_THREADS), 0, s, p->d_off, p->d_cnt,
+                 p->d_cntd, nt, p->d_sort_wg, p->d_sort_mid, (SortSeg*)p->d_sort_big, p->d_sort_bigblk,
+                 p->d_sort_segidx, p->d_sortcnt);
+    ARLMI_LAUNCH_CHECK();
+  }
 
   // work items
   ARLMI_LAUNCH("k_chunks", k_chunks, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, p->d_cnt, p->d_cntd,
@@ -1389,9 +1785,9 @@ static int plan_build_steps(arlmi_plan* p, const double* uvw, const double* freq
   if ((rc = exclusive_scan_u32(p->d_nch, p->d_choff, nt, p->d_scan_tmp, s))) return rc;
   if ((rc = exclusive_scan_u32(p->d_nchd, p->d_choffd, nt, p->d_scan_tmp, s))) return rc;
   if ((rc = exclusive_scan_u32(p->d_nslot, p->d_slotoff, nt, p->d_scan_tmp, s))) return rc;
-  uint32_t tot[4] = {0, 0, 0, 0};
+  uint32_t tot[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   ARLMI_LAUNCH("k_totals", k_totals, dim3(1), dim3(1), 0, s, e2, nk, p->d_choff, p->d_choffd, p->d_slotoff, nt,
-               p->d_bounds + P + 10);
+               p->origin_sorted ? p->d_sortcnt : nullptr, p->d_bounds + P + 10);
   ARLMI_LAUNCH_CHECK();
   ARLMI_HIP(hipMemcpyAsync(tot, p->d_bounds + P + 10, sizeof(tot), hipMemcpyDeviceToHost, s));
   ARLMI_HIP(hipStreamSynchronize(s));
@@ -1401,6 +1797,48 @@ static int plan_build_steps(arlmi_plan* p, const double* uvw, const double* freq
   if (total > ebound) {
     set_error("internal: %u entries exceed the bound %lld", total, (long long)ebound);
     return ARLMI_E_UNSUPPORTED;
+  }
+  if (p->origin_sorted) {
+    const uint32_t nmid = tot[4], nbig = tot[5], nbigblk = tot[6];
+    const int nkeys = sortw * sortw;
+    const size_t shk = sizeof(uint32_t) * (size_t)nkeys;
+    if (nmid > 0) {
+      ARLMI_LAUNCH("k_sort_mid", k_sort_mid, dim3(nmid), dim3(SORT_MID_THREADS), shk, s, p->d_entries,
+                   p->d_entries_alt, p->d_sort_mid, sortw);
+      ARLMI_LAUNCH_CHECK();
+    }
+    if (nbig > 0) {
+      if ((rc = grow(&p->d_sort_h3, &p->cap_sort_h3, (int64_t)nbigblk * nkeys))) return rc;
+      ARLMI_LAUNCH("k_sort_bcount", k_sort_bcount, dim3(nbigblk), dim3(SORT_BIG_THREADS), shk, s, p->d_entries,
+                   (const SortSeg*)p->d_sort_big, p->d_sort_bigblk, sortw, p->d_sort_h3);
+      ARLMI_LAUNCH_CHECK();
+      if ((rc = grow(&p->d_sort_ksum, &p->cap_sort_ksum, (int64_t)nbig * nkeys))) return rc;
+      const uint32_t nhuge = tot[7];
+      if (nhuge > 0) {  // segidx[0, nhuge): 16 slices
+        const dim3 kg((unsigned)((nkeys + 63) / 64), nhuge);
+        ARLMI_LAUNCH("k_sort_bksum", k_sort_bksum<16>, kg, dim3(1024), 0, s, (const SortSeg*)p->d_sort_big,
+                     p->d_sort_segidx, sortw, p->d_sort_h3, p->d_sort_ksum);
+        ARLMI_LAUNCH_CHECK();
+        ARLMI_LAUNCH("k_sort_bcursor", k_sort_bcursor<16>, kg, dim3(1024), 0, s, (const SortSeg*)p->d_sort_big,
+                     p->d_sort_segidx, sortw, p->d_sort_ksum, p->d_sort_h3);
+        ARLMI_LAUNCH_CHECK();
+      }
+      if (nbig > nhuge) {  // segidx[nhuge, nbig): one slice
+        const dim3 kg((unsigned)((nkeys + 1023) / 1024), nbig - nhuge);
+        ARLMI_LAUNCH("k_sort_bksum", k_sort_bksum<1>, kg, dim3(1024), 0, s, (const SortSeg*)p->d_sort_big,
+                     p->d_sort_segidx + nhuge, sortw, p->d_sort_h3, p->d_sort_ksum);
+        ARLMI_LAUNCH_CHECK();
+        ARLMI_LAUNCH("k_sort_bcursor", k_sort_bcursor<1>, kg, dim3(1024), 0, s, (const SortSeg*)p->d_sort_big,
+                     p->d_sort_segidx + nhuge, sortw, p->d_sort_ksum, p->d_sort_h3);
+        ARLMI_LAUNCH_CHECK();
+      }
+      ARLMI_LAUNCH("k_sort_bscatter", k_sort_bscatter, dim3(nbigblk), dim3(SORT_SCAT_THREADS),
+                   2 * shk + sizeof(uint2) * SORT_BB, s, p->d_entries,
+                   p->d_entries_alt, (const SortSeg*)p->d_sort_big, p->d_sort_bigblk, sortw, p->d_sort_h3);
+      ARLMI_LAUNCH_CHECK();
+    }
+    std::swap(p->d_entries, p->d_entries_alt);
+    std::swap(p->cap_entries, p->cap_entries_alt);
   }
   p->ngitems = tot3[0];
   p->nditems = tot3[1];
