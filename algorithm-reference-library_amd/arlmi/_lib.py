@@ -38,7 +38,7 @@ EXPORTS = (
     "arlmi_plan_set_chunk", "arlmi_plan_stats_get", "arlmi_plan_plane_w", "arlmi_plan_slab_boxes", "arlmi_plan_destroy", "arlmi_convolution_mapping",
     "arlmi_grid", "arlmi_grid_add", "arlmi_degrid", "arlmi_batch_create", "arlmi_batch_grid", "arlmi_batch_degrid",
     "arlmi_batch_destroy", "arlmi_wslab_sums", "arlmi_plan_fix_plane_w", "arlmi_plan_or_colmask",
-    "arlmi_plan_use_colmask",
+    "arlmi_plan_use_colmask", "arlmi_plan_grid_runs",
     "arlmi_grids_to_image", "arlmi_image_to_grids", "arlmi_fft2d", "arlmi_w_screen", "arlmi_dft",
     "arlmi_grid_weights", "arlmi_reweight", "arlmi_block_to_rows", "arlmi_apply_gaintable", "arlmi_apply_gains_rows", "arlmi_apply_gains_rows_adjoint",
     "arlmi_phaserotate_rows", "arlmi_phaserotate_blocks", "arlmi_hogbom", "arlmi_hogbom_noalloc", "arlmi_hogbom_traffic", "arlmi_solve_gains",
@@ -137,6 +137,7 @@ def lib():
         "arlmi_plan_update": (ctypes.c_int, [vp, vp, vp, i64, ctypes.POINTER(dbl), dbl, vp]),
         "arlmi_plan_set_chunk": (ctypes.c_int, [vp, i32]),
         "arlmi_plan_stats_get": (ctypes.c_int, [vp, ctypes.POINTER(PlanStats)]),
+        "arlmi_plan_grid_runs": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64), vp]),
         "arlmi_plan_plane_w": (ctypes.c_int, [vp, ctypes.POINTER(dbl), i32]),
         "arlmi_plan_slab_boxes": (ctypes.c_int, [vp, ctypes.POINTER(dbl), i32]),
         "arlmi_plan_destroy": (None, [vp]),

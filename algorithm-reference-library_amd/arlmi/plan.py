@@ -186,6 +186,13 @@ class Plan:
         _lib.check(self.L.arlmi_plan_stats_get(self.handle, ctypes.byref(s)))
         return {k: getattr(s, k) for k, _ in _lib.PlanStats._fields_}
 
+    def grid_runs(self, support):
+        """LDS add pairs the separable gridder of this support issues on the plan (runs of one
+        footprint origin on origin-grouped plans, else one per entry)."""
+        n = ctypes.c_int64()
+        _lib.check(self.L.arlmi_plan_grid_runs(self.handle, int(support), ctypes.byref(n), stream_handle()))
+        return int(n.value)
+
     def plane_w(self):
         out = numpy.zeros(self.nplanes)
         _lib.check(self.L.arlmi_plan_plane_w(self.handle, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),

@@ -1364,11 +1364,65 @@ static int check_kernel(const arlmi_plan* p, const arlmi_kernel* k) {
   return ARLMI_OK;
 }
 
+// The number of runs k_grid_sep_merge adds (one pair of LDS adds each), counted over the same
+// work partition: per work item and wavefront share, per 64 entries, the entries whose origin
+// differs from the next one's, the 32nd and 64th and the last.  A measurement aid.
+__global__ __launch_bounds__(GRID_THREADS) void k_count_runs(const int4* __restrict__ items,
+                                                             const uint2* __restrict__ entries,
+                                                             unsigned long long* __restrict__ runs) {
+  __shared__ unsigned long long wsum[GRID_THREADS / 64];
+  const int4 it = items[blockIdx.x];
+  const int lane = __lane_id(), wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+  const int per = (it.z + nwave - 1) / nwave;
+  const int e0 = min(it.z, wave * per), e1 = min(it.z, e0 + per);
+  unsigned long long n = 0;
+  for (int base = e0; base < e1; base += 64) {
+    const int e = base + lane;
+    const uint32_t org = e < e1 ? (entries[it.y + e].y & 0xffffu) : 0u;  // (lx, ly)
+    const int cnt = min(64, e1 - base);
+    const uint32_t nxt = (uint32_t)__shfl_down((int)org, 1);
+    const bool last = lane == cnt - 1 || (lane & 31) == 31 || (lane < cnt - 1 && nxt != org);
+    n += (unsigned long long)__popcll(__ballot(last && lane < cnt));
+  }
+  if (lane == 0) wsum[wave] = n;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < nwave; ++w) t += wsum[w];
+    atomicAdd(runs, t);
+  }
+}
+
 }  // namespace arlmi
 
 using namespace arlmi;
 
 extern "C" {
+
+int arlmi_plan_grid_runs(const arlmi_plan* p, int32_t support, int64_t* runs_out, void* stream) {
+  if (!p || !runs_out) {
+    set_error("arlmi_plan_grid_runs: NULL argument");
+    return ARLMI_E_ARG;
+  }
+  // the separable gridder adds runs on origin-grouped plans (at most 64 taps), else every entry
+  if (!p->origin_sorted || support * support > 64 || (p->flags & ARLMI_FLAG_DETERMINISTIC) || p->ngitems == 0) {
+    *runs_out = p->nentries;
+    return ARLMI_OK;
+  }
+  hipStream_t s = as_stream(stream);
+  unsigned long long* d = nullptr;
+  ARLMI_HIP(hipMallocAsync(reinterpret_cast<void**>(&d), sizeof(unsigned long long), s));
+  ARLMI_HIP(hipMemsetAsync(d, 0, sizeof(unsigned long long), s));
+  ARLMI_LAUNCH("k_count_runs", k_count_runs, dim3((unsigned)p->ngitems), dim3(GRID_THREADS), 0, s, p->d_gitems,
+               p->d_entries, d);
+  ARLMI_LAUNCH_CHECK();
+  unsigned long long h = 0;
+  ARLMI_HIP(hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, s));
+  ARLMI_HIP(hipFreeAsync(d, s));
+  ARLMI_HIP(hipStreamSynchronize(s));
+  *runs_out = (int64_t)h;
+  return ARLMI_OK;
+}
 
 int arlmi_grid(const arlmi_plan* p, const void* d_vis, const void* d_weight, int dopsf, const arlmi_kernel* k,
                void* d_grids, double* d_sumwt, int precision, void* stream) {

@@ -194,6 +194,7 @@ def time_form(precision, args, dev):
     L.arlmi_profile_only(None)
     prof = arlmi._lib.profile_report()
     st = imager.plan.stats()
+    st["grid_runs"] = imager.plan.grid_runs(support)
     overlap_ms = None
     if args.overlap and precision == 32:
         # the same steps with the predict on a second stream, after the timed region
@@ -1271,7 +1272,8 @@ def main():
                 "frac": round(ach / 8000.0, 4), "traffic": measured_traffic(name, N, P),
                 "bytes_per_launch": by, "avg_launch_ms": round(per_launch_ms, 4)}
         if name == "k_grid_sep":
-            # the gridder's real bound: LDS float64 atomics, 2 wavefront instructions per entry,
+            # the gridder's real bound: LDS float64 atomics, 2 wavefront instructions per run of
+            # entries of one footprint origin (per entry on plans without the origin grouping),
             # against the rate a microbenchmark of the same access pattern reaches on this GPU
             import ctypes
             import arlmi
@@ -1279,9 +1281,11 @@ def main():
             arlmi._lib.probe_check(arlmi._lib.probe_lib().arlmi_probe_lds_add(0, 4096, ctypes.byref(cyc), None))
             props = torch.cuda.get_device_properties(0)
             peak = props.multi_processor_count * 2.4e9 / cyc.value
-            rate = 2.0 * E / (per_launch_ms * 1e-3)
+            runs = r32["stats"].get("grid_runs", E)
+            rate = 2.0 * runs / (per_launch_ms * 1e-3)
             roof["lds_atomics"] = {"achieved": round(rate / 1e9, 2), "peak": round(peak / 1e9, 2),
                                    "unit": "G ds_add_f64 wavefront-instructions/s", "frac": round(rate / peak, 3),
+                                   "entries": E, "runs": runs,
                                    "peak_source": "arlmi_probe_lds_add, the gridder's access pattern "
                                                   "(%.2f cycles per instruction per CU)" % cyc.value}
     # Whole-step roofline: sum over phases of max(HBM, LDS, FLOP) time with the algorithmic

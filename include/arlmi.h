@@ -153,6 +153,12 @@ int arlmi_plan_update(arlmi_plan* plan, const double* d_uvw, const double* d_fre
  * entries of one tile (default 4096).  Takes effect at the next arlmi_plan_update. */
 int arlmi_plan_set_chunk(arlmi_plan* plan, int32_t chunk);
 int arlmi_plan_stats_get(const arlmi_plan* plan, arlmi_plan_stats* out);
+/* Measurement aid (no counterpart in ARL): the number of (real, imaginary) LDS add pairs the
+ * separable gridder of kernel support `support` issues on this plan.  Plans whose entries are
+ * grouped by footprint origin (one kernel w plane, at most 64 taps, not deterministic) add each
+ * run of one origin once (runs cut at 32 entries and at every 64 entries of a wavefront); other
+ * plans add every entry (the entry count is returned).  Synchronises `stream`. */
+int arlmi_plan_grid_runs(const arlmi_plan* plan, int32_t support, int64_t* runs_out, void* stream);
 /* Streaming (one vis set in chunks of rows, as ARL's list workflows see one Visibility whose rows
  * arrive in pieces; imaging_serial.py:166-197 sums the per-slab images of all its rows).  The slab
  * rule and means are those of the whole set (iterators.py:99-106 takes max|w| over every row,

@@ -95,3 +95,50 @@ def test_support_two(gpu, precision):
     out = arlmi.degrid_visibility_from_griddata(vis, gd, cf)
     rv = O.degrid_visibility_from_griddata(d, ref, g, ocf, cfg)
     assert rel_err(out.vis.cpu().numpy(), rv) < TOL[precision]
+
+
+def _clustered_rows(rng, n, cell, S, wmax):
+    """Rows of all three sizes of tile segment the plan's origin sort tells apart: 30,000 rows on
+    40 pixels of one tile (more than 4096 entries, hundreds a footprint origin), 6,000 near the
+    centre, and a few hundred spread over the grid (segments under 64 entries)."""
+    du = 1.0 / (n * cell)
+    d = random_rows(rng, 36600, n, cell, S, wmax=wmax)
+    px = rng.integers(-40, -20, (40, 2))  # pixel offsets from the centre, inside one 64-wide tile
+    pick = rng.integers(0, 40, 30000)
+    d["uvw"][:30000, 0] = (px[pick, 0] + rng.uniform(-0.45, 0.45, 30000)) * du
+    d["uvw"][:30000, 1] = (px[pick, 1] + rng.uniform(-0.45, 0.45, 30000)) * du
+    d["uvw"][30000:36000, :2] = rng.uniform(-30 * du, 30 * du, (6000, 2))
+    return d
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_origin_grouped_gridding(gpu, precision):
+    """Entries grouped by footprint origin (every segment class of the sort) and added in runs:
+    images and visibilities equal the oracle's, the gridder adds far fewer runs than entries, and
+    a deterministic plan (which adds every entry) gives the same images."""
+    import arlmi
+    rng = numpy.random.default_rng(77)
+    n, cell, S, P = 256, 1e-3, 8, 3
+    d = _clustered_rows(rng, n, cell, S, 40.0)
+    im = device_image(n, cell, precision)
+    gcf, cf = arlmi.create_pswf_convolutionfunction(im, oversampling=8, support=S)
+    vis = to_device_vis(d, precision)
+    imager = arlmi.Imager(vis, im, gcfcf=(gcf, cf), mode="wstack", vis_slices=P)
+    st = imager.plan.stats()
+    runs = imager.plan.grid_runs(S)
+    assert 0 < runs < 0.5 * st["nentries"], (runs, st["nentries"])
+    img, _ = imager.invert(vis)
+    g = oracle_geom(n, cell)
+    ogcf, ocf, cfg = O.create_pswf_convolutionfunction(g, 8, S)
+    rimg, rsw = O.invert_list_wstack(d, g, ogcf, ocf, cfg, P)
+    _check_image(img.cpu().numpy(), rimg, precision)
+    model = numpy.zeros((1, 1, n, n))
+    model[0, 0, rng.integers(40, 216, 25), rng.integers(40, 216, 25)] = rng.uniform(0.5, 2.0, 25)
+    mim = device_image(n, cell, precision, data=model)
+    pv = imager.predict(vis, mim.data)
+    rpv = O.predict_list_wstack(d, model, g, ogcf, ocf, cfg, P)
+    assert rel_err(pv.cpu().numpy(), rpv) < TOL[precision]
+    det = arlmi.Imager(vis, im, gcfcf=(gcf, cf), mode="wstack", vis_slices=P, deterministic=True)
+    assert det.plan.grid_runs(S) == det.plan.stats()["nentries"]
+    dimg, _ = det.invert(vis)
+    _check_image(dimg.cpu().numpy(), rimg, precision)
