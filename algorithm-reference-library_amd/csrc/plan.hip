@@ -876,10 +876,28 @@ __global__ __launch_bounds__(SORT_LIST_THREADS) void k_sort_lists_a(const uint32
     wg[3 * gridDim.x + blockIdx.x] = th;
   }
 }
-__device__ inline uint32_t cnt_big_total(const uint32_t* __restrict__ wg, unsigned G) {
-  uint32_t t = 0;
-  for (unsigned g = 0; g < G; ++g) t += wg[G + g];
-  return t;
+// The four per-work-group counts scanned over the G work groups of the list passes (one work
+// group): wg[r][g] becomes their exclusive prefix, cnt[r] their total.
+__global__ __launch_bounds__(SORT_LIST_THREADS) void k_sort_lists_scan(uint32_t* __restrict__ wg, int G,
+                                                                       uint32_t* __restrict__ cnt) {
+  __shared__ uint32_t wtot[SORT_LIST_THREADS / 64];
+  const int per = (G + SORT_LIST_THREADS - 1) / SORT_LIST_THREADS;
+  const int g0 = threadIdx.x * per;
+  for (int r = 0; r < 4; ++r) {
+    uint32_t* row = wg + (size_t)r * G;
+    uint32_t loc = 0;
+    for (int j = 0; j < per; ++j)
+      if (g0 + j < G) loc += row[g0 + j];
+    uint32_t total;
+    uint32_t run = block_excl_u32(loc, wtot, &total);
+    for (int j = 0; j < per; ++j)
+      if (g0 + j < G) {
+        const uint32_t v = row[g0 + j];
+        row[g0 + j] = run;
+        run += v;
+      }
+    if (threadIdx.x == 0) cnt[r] = total;
+  }
 }
 __global__ __launch_bounds__(SORT_LIST_THREADS) void k_sort_lists_b(const uint32_t* __restrict__ off,
                                                                     const uint32_t* __restrict__ tcnt,
@@ -891,16 +909,7 @@ __global__ __launch_bounds__(SORT_LIST_THREADS) void k_sort_lists_b(const uint32
                                                                     uint32_t* __restrict__ cnt) {
   __shared__ uint32_t wtot[SORT_LIST_THREADS / 64];
   __shared__ uint32_t base[4];
-  if (threadIdx.x < 4) {
-    uint32_t t = 0, all = 0;
-    for (unsigned g = 0; g < gridDim.x; ++g) {
-      const uint32_t v = wg[threadIdx.x * gridDim.x + g];
-      if (g < blockIdx.x) t += v;
-      all += v;
-    }
-    base[threadIdx.x] = t;
-    if (blockIdx.x == 0) cnt[threadIdx.x] = all;
-  }
+  if (threadIdx.x < 4) base[threadIdx.x] = wg[threadIdx.x * gridDim.x + blockIdx.x];  // scanned
   const int64_t q = blockIdx.x * (int64_t)SORT_LIST_THREADS + threadIdx.x;
   uint32_t s = 0, m = 0;
   if (q < 2 * nt) sort_seg(off, tcnt, tcntd, q, &s, &m);
@@ -920,7 +929,7 @@ __global__ __launch_bounds__(SORT_LIST_THREADS) void k_sort_lists_b(const uint32
     for (uint32_t k = 0; k < c; ++k) bigblk[oc + k] = make_uint2(ob, k);
     // segment index lists for the key passes: the huge ones from the front, the others from the
     // back (cnt[1] - 1 down)
-    const uint32_t total_big = cnt_big_total(wg, gridDim.x);
+    const uint32_t total_big = cnt[1];
     if (h) segidx[oh] = ob;
     else segidx[total_big - 1 - (ob - oh)] = ob;
   }
@@ -1751,14 +1760,14 @@ static int plan_build_steps(arlmi_plan* p, const double* uvw, const double* freq
     const int64_t nbig = ebound / SORT_MID + 1;
     if ((rc = grow_bytes(&p->d_sort_big, &p->cap_sort_big, (int64_t)sizeof(SortSeg) * nbig))) return rc;
     if ((rc = grow(&p->d_sort_bigblk, &p->cap_sort_bigblk, ebound / SORT_BB + nbig))) return rc;
-    ARLMI_LAUNCH("k_sort_copytiny", k_sort_copytiny, dim3((unsigned)((2 * nt * 16 + 255) / 256)), dim3(256), 0, s,
-                 p->d_entries, p->d_entries_alt, p->d_off, p->d_cnt, p->d_cntd, nt);
-    ARLMI_LAUNCH_CHECK();
     const unsigned lg = (unsigned)((2 * nt + SORT_LIST_THREADS - 1) / SORT_LIST_THREADS);
     if ((rc = grow(&p->d_sort_wg, &p->cap_sort_wg, 4 * (int64_t)lg))) return rc;
     if ((rc = grow(&p->d_sort_segidx, &p->cap_sort_segidx, nbig))) return rc;
     ARLMI_LAUNCH("k_sort_lists_a", k_sort_lists_a, dim3(lg), dim3(SORT_LIST_THREADS), 0, s, p->d_off, p->d_cnt,
                  p->d_cntd, nt, p->d_sort_wg);
+    ARLMI_LAUNCH_CHECK();
+    ARLMI_LAUNCH("k_sort_lists_scan", k_sort_lists_scan, dim3(1), dim3(SORT_LIST_THREADS), 0, s, p->d_sort_wg, (int)lg,
+                 p->d_sortcnt);
     ARLMI_LAUNCH_CHECK();
     ARLMI_LAUNCH("k_sort_lists_b", k_sort_lists_b, dim3(lg), dim3(SORT_LIST_THREADS), 0, s, p->d_off, p->d_cnt,
                  p->d_cntd, nt, p->d_sort_wg, p->d_sort_mid, (SortSeg*)p->d_sort_big, p->d_sort_bigblk,
@@ -1800,6 +1809,17 @@ static int plan_build_steps(arlmi_plan* p, const double* uvw, const double* freq
   }
   if (p->origin_sorted) {
     const uint32_t nmid = tot[4], nbig = tot[5], nbigblk = tot[6];
+    // the segments under SORT_MIN entries: copied one by one where segments are few against the
+    // entries, else with the whole array (the sorts then overwrite their segments): at 8192^2,
+    // 4 M segments of 4 entries on average, the per-segment copy took 3 ms a plan
+    if (2 * nt <= (int64_t)total / 8) {
+      ARLMI_LAUNCH("k_sort_copytiny", k_sort_copytiny, dim3((unsigned)((2 * nt * 16 + 255) / 256)), dim3(256), 0, s,
+                   p->d_entries, p->d_entries_alt, p->d_off, p->d_cnt, p->d_cntd, nt);
+      ARLMI_LAUNCH_CHECK();
+    } else if (total > 0) {
+      ARLMI_HIP(hipMemcpyAsync(p->d_entries_alt, p->d_entries, sizeof(uint2) * (size_t)total, hipMemcpyDeviceToDevice,
+                               s));
+    }
     const int nkeys = sortw * sortw;
     const size_t shk = sizeof(uint32_t) * (size_t)nkeys;
     if (nmid > 0) {
