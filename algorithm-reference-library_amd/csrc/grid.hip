@@ -632,6 +632,7 @@ struct BatchDesc {
   int os, T, ntx, N, nsw;
   int gbase, ngitems, dbase, nditems;
   int perm_n1;  // row order of the grids the degridder reads (perm_row_of)
+  int runs;     // entries grouped by footprint origin: the gridder adds runs (k_grid_sep_merge)
 };
 
 // The descriptor whose [base, base + count) block range holds global block b (descriptors in
@@ -655,6 +656,14 @@ __global__ __launch_bounds__(GRID_THREADS) void k_grid_sep_batch(const BatchDesc
   __shared__ double red[GRID_THREADS / 64];
   const int b = blockIdx.x;
   const BatchDesc& d = descs[find_desc(descs, nd, b, false)];
+  if constexpr (S * S <= 64) {
+    if (d.runs) {
+      grid_sep_item_merge<R, S>(d.gitems[b - d.gbase], smem, red, d.entries, (const cplx<R>*)d.vis, (const R*)d.wt,
+                                dopsf, (const R*)d.ktab, d.os, (cplx<R>*)d.grids, (cplx<R>*)d.partial, d.sumwt, d.T,
+                                d.ntx, d.N, 0);
+      return;
+    }
+  }
   grid_sep_item<R, S, false>(d.gitems[b - d.gbase], smem, red, d.entries, (const cplx<R>*)d.vis, (const R*)d.wt,
                              dopsf, (const R*)d.ktab, d.os, (cplx<R>*)d.grids, (cplx<R>*)d.partial, d.sumwt, d.T,
                              d.ntx, d.N, 0, DetArgs{});
@@ -1494,6 +1503,7 @@ int batch_build(arlmi_batch* b, const arlmi_batch_item* in, int n) {
     d.gitems = p->d_gitems;
     d.ditems = p->d_ditems;
     d.entries = p->d_entries;
+    d.runs = p->origin_sorted ? 1 : 0;
     d.vis = it.d_vis;
     d.wt = it.d_weight;
     d.ktab = it.kernel.d_table;

@@ -1749,10 +1749,12 @@ static int plan_build_steps(arlmi_plan* p, const double* uvw, const double* freq
                  g.nz > 1 ? p->d_entry_z : nullptr);
   ARLMI_LAUNCH_CHECK();
   // entries grouped by footprint origin within their tile segments (one kernel w plane: the
-  // kernel w planes of the entries would have to move with them): segments classified here, the
-  // sorts launched once the host knows the list sizes (with the work-item totals)
+  // kernel w planes of the entries would have to move with them; not for deterministic plans,
+  // whose gridder adds every entry and runs slower on grouped entries, consecutive adds to the
+  // same addresses): segments classified here, the sorts launched once the host knows the list
+  // sizes (with the work-item totals)
   const int sortw = g.tile + g.support;
-  p->origin_sorted = g.nz == 1 && sortw * sortw <= SORT_MAXKEYS &&
+  p->origin_sorted = g.nz == 1 && !(p->flags & ARLMI_FLAG_DETERMINISTIC) && sortw * sortw <= SORT_MAXKEYS &&
                      sizeof(uint32_t) * 2 * sortw * sortw + sizeof(uint2) * SORT_BB <= 160 * 1024 - 1024;
   if (p->origin_sorted) {
     if ((rc = grow(&p->d_entries_alt, &p->cap_entries_alt, ebound))) return rc;
