@@ -19,7 +19,9 @@ from helpers import rel_err, random_rows, to_device_vis, oracle_geom, device_ima
 pytestmark = pytest.mark.gpu
 
 TOL = {32: 1e-5, 64: 1e-10}
-FULL = {32: 5e-3, 64: 1e-8}
+# full field: float32 transform rounding times ARL's grid correction, which reaches ~1e3-1e4 at
+# the corners of these small images; measured 6.6e-4 to 1.1e-3 over the float32 cases here
+FULL = {32: 2.5e-3, 64: 1e-8}
 
 
 def assert_image(a, b, precision):
