@@ -894,7 +894,8 @@ __device__ inline void load_halo(const cplx<R>* __restrict__ grids, R* lds, int 
 }
 
 // Work-group size of the degridders (one entry per thread; an item holds up to 4096 entries):
-// measured best at 512 in single and 1024 in double precision.
+// measured best at 512 in single and 1024 in double precision (with origin-grouped entries too:
+// 1024 threads in single precision take 0.82 against 0.75 ms on the 2048^2 w-stack workload).
 template <typename R>
 constexpr int degrid_threads() {
   return sizeof(R) == 4 ? 512 : 1024;
